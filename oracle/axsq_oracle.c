@@ -1,0 +1,1317 @@
+/*
+ * axsq_oracle.c -- CPU restatement of Axom's BVH + SignedDistance SEQ path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the parity checker and the timed CPU
+ * baseline; the HIP product path never calls it. Every function cites the
+ * reference file:line it follows (paths relative to the reference's
+ * src/axom/). Compile with -ffp-contract=off and without fast-math.
+ *
+ * Parity pins (what checks each function): see tests/test_oracle_*.py --
+ * reference known answers (spin_bvh.cpp, spin_morton.cpp,
+ * primal_closest_point.cpp, quest_signed_distance*.cpp), brute force,
+ * closed forms and tree invariants.
+ */
+#include "axsq_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_BARRIER (-2000000000) /* bvh_traverse.hpp:80 */
+#define ORC_STACK 64              /* bvh_traverse.hpp:79 */
+
+/* ------------------------------------------------------------------------ */
+/* primal subset                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* core/utilities/Utilities.hpp:43-46 (abs) and :318-321 (isNearlyEqual). */
+static inline int near_eq(double a, double b, double thresh)
+{
+  double d = a - b;
+  d = (d < 0.0) ? -d : d;
+  return d <= thresh;
+}
+
+/* primal/operators/detail/fuzzy_comparators.hpp:26-44 */
+static inline int is_gt(double x, double y, double eps) { return (x > y) && !near_eq(x, y, eps); }
+static inline int is_lt(double x, double y, double eps) { return (x < y) && !near_eq(x, y, eps); }
+static inline int is_leq(double x, double y, double eps) { return !is_gt(x, y, eps); }
+static inline int is_geq(double x, double y, double eps) { return !is_lt(x, y, eps); }
+
+/* primal/geometry/Vector.hpp:543-552: ((0 + u0 v0) + u1 v1) + u2 v2 */
+static inline double dot3(const double u[3], const double v[3])
+{
+  double r = 0.0;
+  r += u[0] * v[0];
+  r += u[1] * v[1];
+  r += u[2] * v[2];
+  return r;
+}
+
+/* primal/geometry/Vector.hpp:564-571 via core/numerics/Determinants.hpp:31-38 */
+static inline void cross3(const double u[3], const double v[3], double out[3])
+{
+  out[0] = u[1] * v[2] - v[1] * u[2];
+  out[1] = v[0] * u[2] - u[0] * v[2];
+  out[2] = u[0] * v[1] - v[0] * u[1];
+}
+
+/* primal/geometry/Vector.hpp:477-490; division is multiplication by the
+ * reciprocal (core/NumericArray.hpp:510-514); PRIMAL_TINY = 1e-50. */
+void orc_unit_vector(const double v[3], double out[3])
+{
+  const double len_sq = dot3(v, v);
+  if(len_sq >= 1e-50)
+  {
+    const double inv = 1. / sqrt(len_sq);
+    out[0] = v[0] * inv;
+    out[1] = v[1] * inv;
+    out[2] = v[2] * inv;
+  }
+  else
+  {
+    out[0] = 1.0;
+    out[1] = 0.0;
+    out[2] = 0.0;
+  }
+}
+
+/* primal/geometry/Ray.hpp:122-128: the Ray stores direction.unitVector(). */
+void orc_ray_normalize(const double d[3], double out[3]) { orc_unit_vector(d, out); }
+
+/* primal/geometry/BoundingBox.hpp:451-462 */
+int orc_box_valid(const orc_box* b)
+{
+  for(int d = 0; d < 3; ++d)
+    if(b->min[d] > b->max[d]) return 0;
+  return 1;
+}
+
+/* BoundingBox.hpp:72-73,588-592 */
+static inline void box_clear(orc_box* b)
+{
+  for(int d = 0; d < 3; ++d)
+  {
+    b->min[d] = DBL_MAX;
+    b->max[d] = -DBL_MAX;
+  }
+}
+
+/* BoundingBox.hpp:466-484 */
+static inline void box_add_point(orc_box* b, const double p[3])
+{
+  for(int d = 0; d < 3; ++d)
+  {
+    if(p[d] < b->min[d]) b->min[d] = p[d];
+    if(p[d] > b->max[d]) b->max[d] = p[d];
+  }
+}
+
+/* BoundingBox.hpp:486-508: an invalid receiver is overwritten, an invalid
+ * argument is ignored by a valid receiver. */
+void orc_box_add_box(orc_box* b, const orc_box* o)
+{
+  if(orc_box_valid(b))
+  {
+    if(orc_box_valid(o))
+    {
+      box_add_point(b, o->min);
+      box_add_point(b, o->max);
+    }
+  }
+  else
+  {
+    *b = *o;
+  }
+}
+
+/* BoundingBox.hpp:547-561 with Point::midpoint (Point.hpp:279-290) and
+ * range() = max - min; fixBounds at :575-585. */
+void orc_box_scale(orc_box* b, double sf)
+{
+  if(!orc_box_valid(b)) return;
+  const double s = sf * 0.5;
+  double mid[3], r[3];
+  for(int d = 0; d < 3; ++d)
+  {
+    mid[d] = 0.5 * (b->min[d] + b->max[d]);
+    r[d] = (b->max[d] - b->min[d]) * s;
+  }
+  for(int d = 0; d < 3; ++d)
+  {
+    b->min[d] = mid[d] - r[d];
+    b->max[d] = mid[d] + r[d];
+    if(b->min[d] > b->max[d])
+    {
+      const double t = b->min[d];
+      b->min[d] = b->max[d];
+      b->max[d] = t;
+    }
+  }
+}
+
+/* BoundingBox.hpp:389-401 */
+static inline int box_contains(const orc_box* b, const double p[3])
+{
+  for(int d = 0; d < 3; ++d)
+    if(p[d] < b->min[d] || p[d] > b->max[d]) return 0;
+  return 1;
+}
+
+/* BoundingBox.hpp:433-447 + intersect_bounding_box_impl.hpp:34-41;
+ * q is the query box (this), b the tree box (otherBB). */
+static inline int box_intersects(const orc_box* q, const orc_box* b)
+{
+  for(int d = 0; d < 3; ++d)
+    if(!(q->max[d] >= b->min[d] && q->min[d] <= b->max[d])) return 0;
+  return 1;
+}
+
+/* primal/operators/squared_distance.hpp:77-102 (Point, Box) and :62-66. */
+double orc_sqdist_point_box(const double p[3], const orc_box* b)
+{
+  if(!orc_box_valid(b)) return DBL_MAX;
+  if(box_contains(b, p)) return 0;
+  double v[3];
+  for(int d = 0; d < 3; ++d)
+  {
+    /* core/utilities/Utilities.hpp:147-150 clampVal */
+    const double c = (p[d] < b->min[d]) ? b->min[d] : (p[d] > b->max[d]) ? b->max[d] : p[d];
+    v[d] = c - p[d];
+  }
+  return dot3(v, v);
+}
+
+static inline double sqdist_pp(const double a[3], const double b[3])
+{
+  double v[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  return dot3(v, v);
+}
+
+/* primal/operators/detail/intersect_ray_impl.hpp:150-187 (per-axis slab),
+ * utilities max/min at core/utilities/Utilities.hpp:80-96. */
+static inline int ray_slab(double x0, double n, double bmin, double bmax, double* tmin,
+                           double* tmax, double eps)
+{
+  if(near_eq(n, 0.0, eps))
+  {
+    return ((x0 < bmin) || (x0 > bmax)) ? 0 : 1;
+  }
+  const double invn = 1.0 / n;
+  double t1 = (bmin - x0) * invn;
+  double t2 = (bmax - x0) * invn;
+  if(t1 > t2)
+  {
+    const double t = t1;
+    t1 = t2;
+    t2 = t;
+  }
+  *tmin = (t1 < *tmin) ? *tmin : t1;
+  *tmax = (t2 < *tmax) ? t2 : *tmax;
+  return (*tmin > *tmax) ? 0 : 1;
+}
+
+/* intersect_ray_impl.hpp:321-351: tmin starts at DBL_MIN (a ray, not a line),
+ * the && short-circuits the remaining axes. */
+int orc_ray_box(const double o[3], const double dir[3], const orc_box* b, double eps)
+{
+  double tmin = DBL_MIN, tmax = DBL_MAX;
+  int hit = 1;
+  for(int d = 0; d < 3; ++d)
+    hit = hit && ray_slab(o[d], dir[d], b->min[d], b->max[d], &tmin, &tmax, eps);
+  return hit;
+}
+
+/* primal/operators/closest_point.hpp:162-290 (Ericson RTCD 5.1.5) with the
+ * degenerate-edge guard at :215. Returns the location code:
+ * 0,1,2 vertex A,B,C; -1 edge AB; -3 edge AC; -2 edge BC; 3 face. */
+int orc_closest_point(const double P[3], const double A[3], const double B[3],
+                      const double C[3], double EPS, double cp[3])
+{
+  double ab[3], ac[3], ap[3], bp[3], cpv[3];
+  for(int d = 0; d < 3; ++d)
+  {
+    ab[d] = B[d] - A[d];
+    ac[d] = C[d] - A[d];
+    ap[d] = P[d] - A[d];
+  }
+  const double d1 = dot3(ab, ap);
+  const double d2 = dot3(ac, ap);
+  if(is_leq(d1, 0.0, EPS) && is_leq(d2, 0.0, EPS))
+  {
+    memcpy(cp, A, 3 * sizeof(double));
+    return 0;
+  }
+  for(int d = 0; d < 3; ++d) bp[d] = P[d] - B[d];
+  const double d3 = dot3(ab, bp);
+  const double d4 = dot3(ac, bp);
+  if(is_geq(d3, 0.0, EPS) && is_leq(d4, d3, EPS))
+  {
+    memcpy(cp, B, 3 * sizeof(double));
+    return 1;
+  }
+  const double vc = d1 * d4 - d3 * d2;
+  if(is_leq(vc, 0.0, EPS) && is_geq(d1, 0.0, EPS) && is_leq(d3, 0.0, EPS) &&
+     !near_eq(d1, d3, EPS))
+  {
+    const double v = d1 / (d1 - d3);
+    for(int d = 0; d < 3; ++d) cp[d] = A[d] + ab[d] * v;
+    return -1;
+  }
+  for(int d = 0; d < 3; ++d) cpv[d] = P[d] - C[d];
+  const double d5 = dot3(ab, cpv);
+  const double d6 = dot3(ac, cpv);
+  if(is_geq(d6, 0.0, EPS) && is_leq(d5, d6, EPS))
+  {
+    memcpy(cp, C, 3 * sizeof(double));
+    return 2;
+  }
+  const double vb = d5 * d2 - d1 * d6;
+  if(is_leq(vb, 0.0, EPS) && is_geq(d2, 0.0, EPS) && is_leq(d6, 0.0, EPS))
+  {
+    const double w = d2 / (d2 - d6);
+    for(int d = 0; d < 3; ++d) cp[d] = A[d] + ac[d] * w;
+    return -3;
+  }
+  const double va = d3 * d6 - d5 * d4;
+  if(is_leq(va, 0.0, EPS) && is_geq(d4 - d3, 0.0, EPS) && is_geq(d5 - d6, 0.0, EPS))
+  {
+    const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    for(int d = 0; d < 3; ++d) cp[d] = B[d] + (C[d] - B[d]) * w;
+    return -2;
+  }
+  const double denom = 1.0 / (va + vb + vc);
+  const double v = vb * denom;
+  const double w = vc * denom;
+  for(int d = 0; d < 3; ++d)
+  {
+    const double n = ab[d] * v + ac[d] * w;
+    cp[d] = A[d] + n;
+  }
+  return 3;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BVH build: spin/internal/linear_bvh/build_radix_tree.hpp                 */
+/* ------------------------------------------------------------------------ */
+
+/* 10-bit spread of spin/MortonIndex.hpp:151-159,369-470 restated with the
+ * classic shift/mask sequence for 3-D, 10 bits per axis. */
+static inline uint32_t spread10(uint32_t v)
+{
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+uint32_t orc_morton3(int32_t x, int32_t y, int32_t z)
+{
+  return spread10((uint32_t)x) | (spread10((uint32_t)y) << 1) | (spread10((uint32_t)z) << 2);
+}
+
+/* build_radix_tree.hpp:48-65 (morton32_encode) on the normalised centroid
+ * computed at :154-175. */
+static inline uint32_t box_mcode(const orc_box* b, const double minc[3], const double inv[3])
+{
+  int32_t q[3];
+  for(int d = 0; d < 3; ++d)
+  {
+    const double c = 0.5 * (b->min[d] + b->max[d]);
+    const double u = (c - minc[d]) * inv[d];
+    q[d] = (int32_t)fmin(fmax(u * 1024.0, 0.0), 1023.0);
+  }
+  return orc_morton3(q[0], q[1], q[2]);
+}
+
+/* build_radix_tree.hpp:265-287 */
+static inline int32_t karras_delta(int32_t a, int32_t b, int32_t inner, const uint32_t* mc)
+{
+  const int oor = (b < 0 || b > inner);
+  const int32_t bb = oor ? 0 : b;
+  uint32_t ex = mc[a] ^ mc[bb];
+  const int tie = (ex == 0);
+  if(tie) ex = (uint32_t)a ^ (uint32_t)bb;
+  int32_t cnt = ex ? __builtin_clz(ex) : 32;
+  if(tie) cnt += 32;
+  return oor ? -1 : cnt;
+}
+
+/* Stable LSD radix sort of (key, idx); equals std::stable_sort by key
+ * (build_radix_tree.hpp:243-260) because both give the unique
+ * (key, original index) lexicographic order. */
+static void stable_sort_codes(uint32_t* keys, int32_t* idx, int32_t n)
+{
+  uint32_t* k2 = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
+  int32_t* i2 = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  for(int shift = 0; shift < 32; shift += 8)
+  {
+    size_t cnt[257];
+    memset(cnt, 0, sizeof(cnt));
+    for(int32_t i = 0; i < n; ++i) cnt[((keys[i] >> shift) & 0xffu) + 1]++;
+    for(int b = 0; b < 256; ++b) cnt[b + 1] += cnt[b];
+    for(int32_t i = 0; i < n; ++i)
+    {
+      const size_t dst = cnt[(keys[i] >> shift) & 0xffu]++;
+      k2[dst] = keys[i];
+      i2[dst] = idx[i];
+    }
+    memcpy(keys, k2, sizeof(uint32_t) * (size_t)n);
+    memcpy(idx, i2, sizeof(int32_t) * (size_t)n);
+  }
+  free(k2);
+  free(i2);
+}
+
+orc_bvh* orc_bvh_new(const double* boxes_in, int32_t n_in, double scale)
+{
+  orc_bvh* t = (orc_bvh*)calloc(1, sizeof(orc_bvh));
+  /* spin/BVH.hpp:441-464: pad 0 or 1 boxes to 2 with an invalid box. */
+  const int32_t n = (n_in <= 1) ? 2 : n_in;
+  const int32_t inner = n - 1;
+  t->n = n;
+  t->mcodes = (uint32_t*)malloc(sizeof(uint32_t) * n);
+  t->leafs = (int32_t*)malloc(sizeof(int32_t) * n);
+  t->lchild = (int32_t*)malloc(sizeof(int32_t) * inner);
+  t->rchild = (int32_t*)malloc(sizeof(int32_t) * inner);
+  t->parents = (int32_t*)malloc(sizeof(int32_t) * (2 * n - 1));
+  t->inner_aabbs = (orc_box*)malloc(sizeof(orc_box) * inner);
+  t->leaf_aabbs = (orc_box*)malloc(sizeof(orc_box) * n);
+  t->nodes = (orc_box*)malloc(sizeof(orc_box) * 2 * inner);
+  t->children = (int32_t*)malloc(sizeof(int32_t) * 2 * inner);
+
+  /* transform_boxes :85-99 */
+  orc_box* aabb = (orc_box*)malloc(sizeof(orc_box) * n);
+  for(int32_t i = 0; i < n; ++i)
+  {
+    if(n_in > 1 || (n_in == 1 && i == 0))
+      memcpy(&aabb[i], boxes_in + 6 * (size_t)i, sizeof(orc_box));
+    else
+      box_clear(&aabb[i]);
+    orc_box_scale(&aabb[i], scale);
+  }
+
+  /* reduce :134-142 (SEQ path: addBox from an invalid box) */
+  box_clear(&t->bounds);
+  for(int32_t i = 0; i < n; ++i) orc_box_add_box(&t->bounds, &aabb[i]);
+
+  /* get_mcodes :146-176; 1.f/extent promotes to double, thresh 1e-8. */
+  double minc[3], inv[3];
+  for(int d = 0; d < 3; ++d)
+  {
+    const double ext = t->bounds.max[d] - t->bounds.min[d];
+    minc[d] = t->bounds.min[d];
+    inv[d] = near_eq(ext, 0.0, 1.0e-8) ? 0.0 : 1.0 / ext;
+  }
+  for(int32_t i = 0; i < n; ++i)
+  {
+    t->mcodes[i] = box_mcode(&aabb[i], minc, inv);
+    t->leafs[i] = i;
+  }
+
+  /* sort_mcodes + reorder :243-260, :606 */
+  stable_sort_codes(t->mcodes, t->leafs, n);
+  for(int32_t i = 0; i < n; ++i) t->leaf_aabbs[i] = aabb[t->leafs[i]];
+  free(aabb);
+
+  /* build_tree :290-384 (Karras 2012) */
+  const uint32_t* mc = t->mcodes;
+  for(int32_t i = 0; i < inner; ++i)
+  {
+    const int32_t d =
+      (karras_delta(i, i + 1, inner, mc) - karras_delta(i, i - 1, inner, mc)) < 0 ? -1 : 1;
+    const int32_t dmin = karras_delta(i, i - d, inner, mc);
+    int32_t lmax = 2;
+    while(karras_delta(i, i + lmax * d, inner, mc) > dmin) lmax *= 2;
+    int32_t l = 0;
+    for(int32_t s = lmax / 2; s >= 1; s /= 2)
+      if(karras_delta(i, i + (l + s) * d, inner, mc) > dmin) l += s;
+    const int32_t j = i + l * d;
+    const int32_t dnode = karras_delta(i, j, inner, mc);
+    int32_t s = 0;
+    double div = 2.0;
+    int32_t step = (int32_t)ceil((double)(float)l / div);
+    for(;;)
+    {
+      if(karras_delta(i, i + (s + step) * d, inner, mc) > dnode) s += step;
+      if(step == 1) break;
+      div *= 2;
+      step = (int32_t)ceil((double)(float)l / div);
+    }
+    const int32_t split = i + s * d + (d < 0 ? d : 0);
+    const int32_t lo = (i < j) ? i : j;
+    const int32_t hi = (i < j) ? j : i;
+    if(lo == split)
+    {
+      t->parents[split + inner] = i;
+      t->lchild[i] = split + inner;
+    }
+    else
+    {
+      t->parents[split] = i;
+      t->lchild[i] = split;
+    }
+    if(hi == split + 1)
+    {
+      t->parents[split + inner + 1] = i;
+      t->rchild[i] = split + inner + 1;
+    }
+    else
+    {
+      t->parents[split + 1] = i;
+      t->rchild[i] = split + 1;
+    }
+    if(i == 0) t->parents[0] = -1;
+  }
+
+  /* propagate_aabbs :505-576 in SEQ leaf order */
+  int32_t* counters = (int32_t*)calloc((size_t)inner, sizeof(int32_t));
+  for(int32_t i = 0; i < inner; ++i) box_clear(&t->inner_aabbs[i]);
+  for(int32_t i = 0; i < n; ++i)
+  {
+    orc_box acc = t->leaf_aabbs[i];
+    int32_t last = inner + i;
+    int32_t cur = t->parents[inner + i];
+    while(cur != -1)
+    {
+      const int32_t old = counters[cur]++;
+      if(old == 0) break;
+      const int32_t lc = t->lchild[cur], rc = t->rchild[cur];
+      const int32_t other = (lc == last) ? rc : lc;
+      const orc_box ob = (other >= inner) ? t->leaf_aabbs[other - inner] : t->inner_aabbs[other];
+      orc_box_add_box(&acc, &ob);
+      t->inner_aabbs[cur] = acc;
+      last = cur;
+      cur = t->parents[cur];
+    }
+  }
+  free(counters);
+
+  /* emit_bvh_parents, spin/policy/LinearBVH.hpp:225-264 */
+  for(int32_t i = 0; i < inner; ++i)
+  {
+    int32_t lc = t->lchild[i], rc = t->rchild[i];
+    orc_box lb, rb;
+    if(lc >= inner)
+    {
+      lb = t->leaf_aabbs[lc - inner];
+      lc = -(lc - inner + 1);
+    }
+    else
+    {
+      lb = t->inner_aabbs[lc];
+      lc *= 2;
+    }
+    if(rc >= inner)
+    {
+      rb = t->leaf_aabbs[rc - inner];
+      rc = -(rc - inner + 1);
+    }
+    else
+    {
+      rb = t->inner_aabbs[rc];
+      rc *= 2;
+    }
+    t->nodes[2 * i] = lb;
+    t->nodes[2 * i + 1] = rb;
+    t->children[2 * i] = lc;
+    t->children[2 * i + 1] = rc;
+  }
+  return t;
+}
+
+void orc_bvh_free(orc_bvh* t)
+{
+  if(!t) return;
+  free(t->mcodes);
+  free(t->leafs);
+  free(t->lchild);
+  free(t->rchild);
+  free(t->parents);
+  free(t->inner_aabbs);
+  free(t->leaf_aabbs);
+  free(t->nodes);
+  free(t->children);
+  free(t);
+}
+
+int32_t orc_bvh_size(const orc_bvh* t) { return t->n; }
+
+void orc_bvh_export(const orc_bvh* t, double* bounds6, uint32_t* mcodes, int32_t* leafs,
+                    int32_t* lchild, int32_t* rchild, int32_t* parents, double* inner_aabbs,
+                    double* nodes, int32_t* children)
+{
+  const int32_t n = t->n, inner = n - 1;
+  if(bounds6) memcpy(bounds6, &t->bounds, sizeof(orc_box));
+  if(mcodes) memcpy(mcodes, t->mcodes, sizeof(uint32_t) * n);
+  if(leafs) memcpy(leafs, t->leafs, sizeof(int32_t) * n);
+  if(lchild) memcpy(lchild, t->lchild, sizeof(int32_t) * inner);
+  if(rchild) memcpy(rchild, t->rchild, sizeof(int32_t) * inner);
+  if(parents) memcpy(parents, t->parents, sizeof(int32_t) * (2 * n - 1));
+  if(inner_aabbs) memcpy(inner_aabbs, t->inner_aabbs, sizeof(orc_box) * inner);
+  if(nodes) memcpy(nodes, t->nodes, sizeof(orc_box) * 2 * inner);
+  if(children) memcpy(children, t->children, sizeof(int32_t) * 2 * inner);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Traversal: spin/internal/linear_bvh/bvh_traverse.hpp:66-154               */
+/* ------------------------------------------------------------------------ */
+
+typedef int (*pred_fn)(const void* prim, const orc_box* b, void* ctx);
+typedef void (*leaf_fn)(int32_t pos, const int32_t* leafs, void* ctx);
+typedef int (*pref_fn)(const orc_box* l, const orc_box* r, const void* prim);
+
+/* Returns the number of inner-node visits; -1 on stack overflow (the
+ * reference has no bound check, bvh_traverse.hpp:79-120). */
+static inline int32_t traverse(const orc_bvh* t, const void* prim, pred_fn B, void* pctx,
+                               leaf_fn A, void* actx, pref_fn Comp)
+{
+  int32_t todo[ORC_STACK];
+  int32_t sp = 0;
+  int32_t visits = 0;
+  todo[0] = ORC_BARRIER;
+  int32_t found = 0;
+  int32_t cur = 0;
+  while(cur != ORC_BARRIER)
+  {
+    while(cur >= 0)
+    {
+      const orc_box* lb = &t->nodes[cur];
+      const orc_box* rb = &t->nodes[cur + 1];
+      ++visits;
+      const int in_l = orc_box_valid(lb) ? B(prim, lb, pctx) : 0;
+      const int in_r = orc_box_valid(rb) ? B(prim, rb, pctx) : 0;
+      int32_t lc = t->children[cur];
+      int32_t rc = t->children[cur + 1];
+      const int swap = Comp(lb, rb, prim);
+      if(!in_l && !in_r)
+      {
+        cur = todo[sp];
+        sp--;
+      }
+      else
+      {
+        cur = in_l ? lc : rc;
+        if(in_l && in_r)
+        {
+          if(swap)
+          {
+            const int32_t tmp = cur;
+            cur = rc;
+            rc = tmp;
+          }
+          sp++;
+          if(sp >= ORC_STACK) return -1;
+          todo[sp] = rc;
+        }
+      }
+      if(cur < 0 && !(found < 0))
+      {
+        found = cur;
+        if(cur != ORC_BARRIER)
+        {
+          cur = todo[sp];
+          sp--;
+        }
+      }
+    }
+    while(found < 0 && found != ORC_BARRIER)
+    {
+      A(-found - 1, t->leafs, actx);
+      found = cur;
+      if(cur < 0 && cur != ORC_BARRIER)
+      {
+        cur = todo[sp];
+        sp--;
+      }
+    }
+    found = 0;
+  }
+  return visits;
+}
+
+static int no_pref(const orc_box* l, const orc_box* r, const void* p)
+{
+  (void)l;
+  (void)r;
+  (void)p;
+  return 0;
+}
+
+/* LinearBVH.hpp:75-82: near-first by squared distance to the centroids; an
+ * invalid right box never wins. */
+static int point_pref(const orc_box* l, const orc_box* r, const void* prim)
+{
+  const double* p = (const double*)prim;
+  double cl[3], cr[3];
+  for(int d = 0; d < 3; ++d) cl[d] = 0.5 * (l->min[d] + l->max[d]);
+  const double dl = sqdist_pp(p, cl);
+  double dr = DBL_MAX;
+  if(orc_box_valid(r))
+  {
+    for(int d = 0; d < 3; ++d) cr[d] = 0.5 * (r->min[d] + r->max[d]);
+    dr = sqdist_pp(p, cr);
+  }
+  return dl > dr;
+}
+
+typedef struct
+{
+  double tol;
+} find_ctx;
+
+static int pred_point(const void* prim, const orc_box* b, void* ctx)
+{
+  (void)ctx;
+  return box_contains(b, (const double*)prim);
+}
+static int pred_box(const void* prim, const orc_box* b, void* ctx)
+{
+  (void)ctx;
+  return box_intersects((const orc_box*)prim, b);
+}
+static int pred_ray(const void* prim, const orc_box* b, void* ctx)
+{
+  const double* r = (const double*)prim;
+  return orc_ray_box(r, r + 3, b, ((find_ctx*)ctx)->tol);
+}
+
+typedef struct
+{
+  int32_t count;
+  int32_t* out;
+} collect_ctx;
+
+static void leaf_count(int32_t pos, const int32_t* leafs, void* ctx)
+{
+  collect_ctx* c = (collect_ctx*)ctx;
+  if(c->out) c->out[c->count] = leafs[pos];
+  c->count++;
+}
+
+typedef struct
+{
+  const orc_bvh* t;
+  int kind;
+  const double* prims;
+  double tol;
+  int32_t* counts;
+  const int32_t* offsets;
+  int32_t* cands;
+  int64_t begin, end;
+  int overflow;
+} find_job;
+
+static void* find_worker(void* arg)
+{
+  find_job* j = (find_job*)arg;
+  const int stride = (j->kind == 0) ? 3 : 6;
+  pred_fn B = (j->kind == 0) ? pred_point : (j->kind == 1) ? pred_box : pred_ray;
+  find_ctx fc = {j->tol};
+  for(int64_t i = j->begin; i < j->end; ++i)
+  {
+    collect_ctx cc = {0, j->cands ? j->cands + j->offsets[i] : NULL};
+    if(traverse(j->t, j->prims + stride * i, B, &fc, leaf_count, &cc, no_pref) < 0)
+      j->overflow = 1;
+    if(!j->cands) j->counts[i] = cc.count;
+  }
+  return NULL;
+}
+
+static void run_jobs(void* (*fn)(void*), void* jobs, size_t job_size, int nthreads)
+{
+  if(nthreads <= 1)
+  {
+    fn(jobs);
+    return;
+  }
+  pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+  for(int k = 0; k < nthreads; ++k) pthread_create(&th[k], NULL, fn, (char*)jobs + k * job_size);
+  for(int k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
+  free(th);
+}
+
+/* LinearBVH.hpp:273-402: counts, exclusive-scan offsets, then candidates in
+ * traversal (DFS) order per query. */
+int64_t orc_bvh_find(const orc_bvh* t, int kind, const double* prims, int64_t nq, double tol,
+                     int32_t* counts, int32_t* offsets, int32_t* cands, int nthreads)
+{
+  if(nthreads < 1) nthreads = 1;
+  find_job* jobs = (find_job*)calloc((size_t)nthreads, sizeof(find_job));
+  for(int k = 0; k < nthreads; ++k)
+  {
+    jobs[k].t = t;
+    jobs[k].kind = kind;
+    jobs[k].prims = prims;
+    jobs[k].tol = tol;
+    jobs[k].counts = counts;
+    jobs[k].offsets = offsets;
+    jobs[k].cands = cands;
+    jobs[k].begin = nq * k / nthreads;
+    jobs[k].end = nq * (k + 1) / nthreads;
+  }
+  int overflow = 0;
+  int64_t total = 0;
+  if(!cands)
+  {
+    run_jobs(find_worker, jobs, sizeof(find_job), nthreads);
+    for(int k = 0; k < nthreads; ++k) overflow |= jobs[k].overflow;
+    for(int64_t i = 0; i < nq; ++i)
+    {
+      offsets[i] = (int32_t)total;
+      total += counts[i];
+    }
+  }
+  else
+  {
+    run_jobs(find_worker, jobs, sizeof(find_job), nthreads);
+    for(int k = 0; k < nthreads; ++k) overflow |= jobs[k].overflow;
+    for(int64_t i = 0; i < nq; ++i) total += counts[i];
+  }
+  free(jobs);
+  return overflow ? -1 : total;
+}
+
+int64_t orc_bvh_brute(const orc_bvh* t, int kind, const double* prims, int64_t nq, double tol,
+                      int32_t* counts, int32_t* cands, int64_t cap)
+{
+  const int32_t n = t->n;
+  /* scaled boxes in original-id order */
+  orc_box* orig = (orc_box*)malloc(sizeof(orc_box) * n);
+  for(int32_t p = 0; p < n; ++p) orig[t->leafs[p]] = t->leaf_aabbs[p];
+  const int stride = (kind == 0) ? 3 : 6;
+  find_ctx fc = {tol};
+  int64_t total = 0;
+  for(int64_t i = 0; i < nq; ++i)
+  {
+    const double* pr = prims + stride * i;
+    int32_t c = 0;
+    for(int32_t b = 0; b < n; ++b)
+    {
+      if(!orc_box_valid(&orig[b])) continue;
+      const int hit = (kind == 0) ? pred_point(pr, &orig[b], NULL)
+        : (kind == 1)             ? pred_box(pr, &orig[b], NULL)
+                                  : pred_ray(pr, &orig[b], &fc);
+      if(hit)
+      {
+        if(cands && total < cap) cands[total] = b;
+        total++;
+        c++;
+      }
+    }
+    counts[i] = c;
+  }
+  free(orig);
+  return total;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SignedDistance: quest/SignedDistance.hpp:411-763                          */
+/* ------------------------------------------------------------------------ */
+
+struct orc_sdf
+{
+  int32_t nverts, ntris;
+  double *x, *y, *z;
+  int32_t* conn;
+  int watertight, compute_sign;
+  orc_box domain; /* m_boxDomain */
+  orc_bvh* bvh;
+};
+
+orc_sdf* orc_sdf_new(const double* x, const double* y, const double* z, int32_t nverts,
+                     const int32_t* conn, int32_t ntris, int watertight, int compute_sign)
+{
+  orc_sdf* s = (orc_sdf*)calloc(1, sizeof(orc_sdf));
+  s->nverts = nverts;
+  s->ntris = ntris;
+  s->x = (double*)malloc(sizeof(double) * nverts);
+  s->y = (double*)malloc(sizeof(double) * nverts);
+  s->z = (double*)malloc(sizeof(double) * nverts);
+  s->conn = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)ntris);
+  memcpy(s->x, x, sizeof(double) * nverts);
+  memcpy(s->y, y, sizeof(double) * nverts);
+  memcpy(s->z, z, sizeof(double) * nverts);
+  memcpy(s->conn, conn, sizeof(int32_t) * 3 * (size_t)ntris);
+  s->watertight = watertight;
+  s->compute_sign = compute_sign;
+  /* setMesh :470-475 (SEQ: addPoint over nodes) */
+  box_clear(&s->domain);
+  for(int32_t i = 0; i < nverts; ++i)
+  {
+    const double p[3] = {x[i], y[i], z[i]};
+    box_add_point(&s->domain, p);
+  }
+  /* getCellBoundingBox :609-633 */
+  double* boxes = (double*)malloc(sizeof(orc_box) * (size_t)ntris);
+  for(int32_t c = 0; c < ntris; ++c)
+  {
+    orc_box b;
+    box_clear(&b);
+    for(int k = 0; k < 3; ++k)
+    {
+      const int32_t v = conn[3 * (size_t)c + k];
+      const double p[3] = {x[v], y[v], z[v]};
+      box_add_point(&b, p);
+    }
+    memcpy(boxes + 6 * (size_t)c, &b, sizeof(orc_box));
+  }
+  /* m_bvh.initialize(boxes, ncells) with the default scale 1.000123
+   * (spin/BVH.hpp:410) */
+  s->bvh = orc_bvh_new(boxes, ntris, 1.000123);
+  free(boxes);
+  return s;
+}
+
+void orc_sdf_free(orc_sdf* s)
+{
+  if(!s) return;
+  free(s->x);
+  free(s->y);
+  free(s->z);
+  free(s->conn);
+  orc_bvh_free(s->bvh);
+  free(s);
+}
+
+const orc_bvh* orc_sdf_bvh(const orc_sdf* s) { return s->bvh; }
+
+void orc_sdf_bounds(const orc_sdf* s, double lo[3], double hi[3])
+{
+  memcpy(lo, s->domain.min, 3 * sizeof(double));
+  memcpy(hi, s->domain.max, 3 * sizeof(double));
+}
+
+/* MinCandidate, SignedDistance.hpp:159-175. Location kinds follow
+ * getClosestPointLocType :119-137: 0 vertex, 1 edge, 2 face, -1 unset. */
+typedef struct
+{
+  double minSqDist;
+  double minPt[3];
+  int minType;
+  int minLoc;
+  int32_t minElem;
+  double sumN[3];
+  int minCount;
+  const orc_sdf* s;
+  const double* q;
+  int32_t leaves;
+} min_cand;
+
+static inline int loc_type(int loc) { return (loc < 0) ? 1 : (loc < 3) ? 0 : 2; }
+
+static inline void tri_points(const orc_sdf* s, int32_t c, double A[3], double B[3], double C[3])
+{
+  const int32_t* v = s->conn + 3 * (size_t)c;
+  A[0] = s->x[v[0]];
+  A[1] = s->y[v[0]];
+  A[2] = s->z[v[0]];
+  B[0] = s->x[v[1]];
+  B[1] = s->y[v[1]];
+  B[2] = s->z[v[1]];
+  C[0] = s->x[v[2]];
+  C[1] = s->y[v[2]];
+  C[2] = s->z[v[2]];
+}
+
+/* Triangle::normal, Triangle.hpp:99-102 */
+static inline void tri_normal(const double A[3], const double B[3], const double C[3], double n[3])
+{
+  const double u[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+  const double v[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+  cross3(u, v, n);
+}
+
+/* Triangle::angle, Triangle.hpp:385-400 */
+static inline double tri_angle(const double* P[3], int idx)
+{
+  const double* pt = P[idx];
+  const double* p1 = P[(idx + 1) % 3];
+  const double* p2 = P[(idx + 2) % 3];
+  const double a[3] = {p1[0] - pt[0], p1[1] - pt[1], p1[2] - pt[2]};
+  const double b[3] = {p2[0] - pt[0], p2[1] - pt[1], p2[2] - pt[2]};
+  double ua[3], ub[3];
+  orc_unit_vector(a, ua);
+  orc_unit_vector(b, ub);
+  double dp = dot3(ua, ub);
+  dp = (dp < -1.0) ? -1.0 : (dp > 1.0) ? 1.0 : dp;
+  return acos(dp);
+}
+
+/* checkCandidate, SignedDistance.hpp:637-737 (triangles only). */
+static void check_candidate(min_cand* m, int32_t cell)
+{
+  const double EPS = 1e-12;
+  double A[3], B[3], C[3], cp[3];
+  tri_points(m->s, cell, A, B, C);
+  const int loc = orc_closest_point(m->q, A, B, C, EPS, cp);
+  const double sq = sqdist_pp(m->q, cp);
+  const int type = loc_type(loc);
+  const int shared = (type != 2);
+  int upd = 0;
+  const int cn = m->s->compute_sign;
+  if(sq < m->minSqDist)
+  {
+    const int clear = !shared || (m->minType != type) || !near_eq(sqdist_pp(cp, m->minPt), 0., EPS);
+    m->minSqDist = sq;
+    memcpy(m->minPt, cp, sizeof(cp));
+    m->minType = type;
+    m->minLoc = loc;
+    m->minElem = cell;
+    if(cn && clear)
+    {
+      m->sumN[0] = m->sumN[1] = m->sumN[2] = 0.0;
+      m->minCount = 0;
+    }
+    upd = cn && shared;
+  }
+  else
+  {
+    upd = cn && shared && (m->minType == type) && near_eq(sqdist_pp(cp, m->minPt), 0., EPS);
+  }
+  if(upd)
+  {
+    ++m->minCount;
+    double n[3], un[3];
+    tri_normal(A, B, C, n);
+    if(type == 1)
+    {
+      orc_unit_vector(n, un);
+      for(int d = 0; d < 3; ++d) m->sumN[d] += un[d];
+    }
+    else if(type == 0)
+    {
+      /* Triangle::degenerate, Triangle.hpp:327-330: area = 0.5*|n| */
+      const double area = 0.5 * sqrt(dot3(n, n));
+      if(!near_eq(area, 0.0, 1.0e-12))
+      {
+        const double* P[3] = {A, B, C};
+        const double alpha = tri_angle(P, loc);
+        orc_unit_vector(n, un);
+        for(int d = 0; d < 3; ++d) m->sumN[d] += un[d] * alpha;
+      }
+    }
+  }
+}
+
+static void sdf_leaf(int32_t pos, const int32_t* leafs, void* ctx)
+{
+  min_cand* m = (min_cand*)ctx;
+  m->leaves++;
+  check_candidate(m, leafs[pos]);
+}
+
+/* traversePredicate, SignedDistance.hpp:576-578 */
+static int sdf_pred(const void* prim, const orc_box* b, void* ctx)
+{
+  const min_cand* m = (const min_cand*)ctx;
+  return orc_sqdist_point_box((const double*)prim, b) <= m->minSqDist;
+}
+
+/* getSurfaceNormal :741-749 */
+static inline void surface_normal(const min_cand* m, double n[3])
+{
+  if(m->minType == 2)
+  {
+    double A[3], B[3], C[3];
+    tri_points(m->s, m->minElem, A, B, C);
+    tri_normal(A, B, C, n);
+  }
+  else
+  {
+    memcpy(n, m->sumN, 3 * sizeof(double));
+  }
+}
+
+typedef struct
+{
+  const orc_sdf* s;
+  const double* q;
+  double *dist, *cp, *normal;
+  int32_t *elem, *loc, *visits, *leaves;
+  int64_t begin, end;
+  int overflow;
+} sdf_job;
+
+static void* sdf_worker(void* arg)
+{
+  sdf_job* j = (sdf_job*)arg;
+  const orc_sdf* s = j->s;
+  for(int64_t i = j->begin; i < j->end; ++i)
+  {
+    const double* q = j->q + 3 * i;
+    min_cand m;
+    memset(&m, 0, sizeof(m));
+    m.minSqDist = DBL_MAX;
+    m.minType = -1;
+    m.minLoc = 3;
+    m.minElem = -1;
+    m.s = s;
+    m.q = q;
+    const int32_t v = traverse(s->bvh, q, sdf_pred, &m, sdf_leaf, &m, point_pref);
+    if(v < 0) j->overflow = 1;
+    /* computeDistances :580-603 */
+    double sgn = 1.0;
+    if(s->compute_sign)
+    {
+      if(!(s->watertight && !box_contains(&s->domain, m.minPt)))
+      {
+        double n[3];
+        surface_normal(&m, n);
+        const double r[3] = {q[0] - m.minPt[0], q[1] - m.minPt[1], q[2] - m.minPt[2]};
+        sgn = (dot3(r, n) >= 0.0) ? 1.0 : -1.0;
+      }
+    }
+    j->dist[i] = sqrt(m.minSqDist) * sgn;
+    if(j->cp) memcpy(j->cp + 3 * i, m.minPt, 3 * sizeof(double));
+    if(j->normal)
+    {
+      double n[3];
+      surface_normal(&m, n);
+      orc_unit_vector(n, j->normal + 3 * i);
+    }
+    if(j->elem) j->elem[i] = m.minElem;
+    if(j->loc) j->loc[i] = m.minLoc;
+    if(j->visits) j->visits[i] = v;
+    if(j->leaves) j->leaves[i] = m.leaves;
+  }
+  return NULL;
+}
+
+int orc_sdf_eval(const orc_sdf* s, const double* q, int64_t nq, double* dist, double* cp,
+                 double* normal, int32_t* elem, int32_t* loc, int32_t* visits, int32_t* leaves,
+                 int nthreads)
+{
+  if(nthreads < 1) nthreads = 1;
+  sdf_job* jobs = (sdf_job*)calloc((size_t)nthreads, sizeof(sdf_job));
+  for(int k = 0; k < nthreads; ++k)
+  {
+    sdf_job* j = &jobs[k];
+    j->s = s;
+    j->q = q;
+    j->dist = dist;
+    j->cp = cp;
+    j->normal = normal;
+    j->elem = elem;
+    j->loc = loc;
+    j->visits = visits;
+    j->leaves = leaves;
+    j->begin = nq * k / nthreads;
+    j->end = nq * (k + 1) / nthreads;
+  }
+  run_jobs(sdf_worker, jobs, sizeof(sdf_job), nthreads);
+  int overflow = 0;
+  for(int k = 0; k < nthreads; ++k) overflow |= jobs[k].overflow;
+  free(jobs);
+  return overflow ? -1 : 0;
+}
+
+typedef struct
+{
+  const orc_sdf* s;
+  const double* q;
+  double* sq;
+  int32_t* elem;
+  int64_t begin, end;
+} brute_job;
+
+static void* brute_worker(void* arg)
+{
+  brute_job* j = (brute_job*)arg;
+  const orc_sdf* s = j->s;
+  for(int64_t i = j->begin; i < j->end; ++i)
+  {
+    const double* q = j->q + 3 * i;
+    double best = DBL_MAX;
+    int32_t be = -1;
+    for(int32_t c = 0; c < s->ntris; ++c)
+    {
+      double A[3], B[3], C[3], cp[3];
+      tri_points(s, c, A, B, C);
+      orc_closest_point(q, A, B, C, 1e-12, cp);
+      const double d = sqdist_pp(q, cp);
+      if(d < best)
+      {
+        best = d;
+        be = c;
+      }
+    }
+    j->sq[i] = best;
+    if(j->elem) j->elem[i] = be;
+  }
+  return NULL;
+}
+
+int orc_sdf_brute(const orc_sdf* s, const double* q, int64_t nq, double* sqdist, int32_t* elem,
+                  int nthreads)
+{
+  if(nthreads < 1) nthreads = 1;
+  brute_job* jobs = (brute_job*)calloc((size_t)nthreads, sizeof(brute_job));
+  for(int k = 0; k < nthreads; ++k)
+  {
+    jobs[k].s = s;
+    jobs[k].q = q;
+    jobs[k].sq = sqdist;
+    jobs[k].elem = elem;
+    jobs[k].begin = nq * k / nthreads;
+    jobs[k].end = nq * (k + 1) / nthreads;
+  }
+  run_jobs(brute_worker, jobs, sizeof(brute_job), nthreads);
+  free(jobs);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint (no reference counterpart).                                      */
+/*   d(sd)/dq   = g * sgn * (q - cp)/|q - cp|                               */
+/*   d(sd)/dV_k = -w_k * d(sd)/dq, w = barycentric weights of cp            */
+/* Deterministic order: per triangle, per corner, queries in ascending index */
+/* (partial sums from 0.0); per vertex, partials in ascending triangle id.  */
+/* When q lies on the surface (|q-cp| = 0) the unit surface normal stands   */
+/* in for the direction.                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Barycentric weights of the closest point for each location code of
+ * closest_point.hpp:162-290 (recomputes the same v/w the forward uses). */
+static void cp_weights(const double P[3], const double A[3], const double B[3], const double C[3],
+                       double cp[3], double w[3], int* locp)
+{
+  const int loc = orc_closest_point(P, A, B, C, 1e-12, cp);
+  double ab[3], ac[3], ap[3], bp[3], cpv[3];
+  for(int d = 0; d < 3; ++d)
+  {
+    ab[d] = B[d] - A[d];
+    ac[d] = C[d] - A[d];
+    ap[d] = P[d] - A[d];
+    bp[d] = P[d] - B[d];
+    cpv[d] = P[d] - C[d];
+  }
+  const double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  const double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  const double d5 = dot3(ab, cpv), d6 = dot3(ac, cpv);
+  w[0] = w[1] = w[2] = 0.0;
+  switch(loc)
+  {
+  case 0: w[0] = 1.0; break;
+  case 1: w[1] = 1.0; break;
+  case 2: w[2] = 1.0; break;
+  case -1:
+  {
+    const double v = d1 / (d1 - d3);
+    w[0] = 1.0 - v;
+    w[1] = v;
+    break;
+  }
+  case -3:
+  {
+    const double ww = d2 / (d2 - d6);
+    w[0] = 1.0 - ww;
+    w[2] = ww;
+    break;
+  }
+  case -2:
+  {
+    const double ww = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    w[1] = 1.0 - ww;
+    w[2] = ww;
+    break;
+  }
+  default:
+  {
+    const double vc = d1 * d4 - d3 * d2;
+    const double vb = d5 * d2 - d1 * d6;
+    const double va = d3 * d6 - d5 * d4;
+    const double denom = 1.0 / (va + vb + vc);
+    const double v = vb * denom;
+    const double ww = vc * denom;
+    w[0] = 1.0 - v - ww;
+    w[1] = v;
+    w[2] = ww;
+    break;
+  }
+  }
+  *locp = loc;
+}
+
+static int cmp_pair(const void* a, const void* b)
+{
+  const int64_t* x = (const int64_t*)a;
+  const int64_t* y = (const int64_t*)b;
+  return (x[0] < y[0]) ? -1 : (x[0] > y[0]) ? 1 : (x[1] < y[1]) ? -1 : (x[1] > y[1]);
+}
+
+int orc_sdf_backward(const orc_sdf* s, const double* q, int64_t nq, const double* dist,
+                     const int32_t* elem, const float* gout, double* grad_q, double* grad_v)
+{
+  const int32_t nt = s->ntris, nv = s->nverts;
+  /* per-query direction u = sgn*(q-cp)/|q-cp| scaled by g */
+  double* dir = (double*)malloc(sizeof(double) * 3 * (size_t)nq);
+  double* wts = (double*)malloc(sizeof(double) * 3 * (size_t)nq);
+  for(int64_t i = 0; i < nq; ++i)
+  {
+    const double* P = q + 3 * i;
+    const int32_t c = elem[i];
+    double A[3], B[3], C[3], cp[3], w[3];
+    int loc;
+    tri_points(s, c, A, B, C);
+    cp_weights(P, A, B, C, cp, w, &loc);
+    const double sgn = signbit(dist[i]) ? -1.0 : 1.0;
+    double r[3] = {P[0] - cp[0], P[1] - cp[1], P[2] - cp[2]};
+    const double len = sqrt(dot3(r, r));
+    double u[3];
+    if(len > 0.0)
+    {
+      const double inv = 1.0 / len;
+      for(int d = 0; d < 3; ++d) u[d] = r[d] * inv;
+    }
+    else
+    {
+      double n[3];
+      tri_normal(A, B, C, n);
+      orc_unit_vector(n, u);
+    }
+    const double g = (double)gout[i] * sgn;
+    for(int d = 0; d < 3; ++d)
+    {
+      dir[3 * i + d] = g * u[d];
+      wts[3 * i + d] = w[d];
+    }
+    if(grad_q)
+      for(int d = 0; d < 3; ++d) grad_q[3 * i + d] = dir[3 * i + d];
+  }
+  if(grad_v)
+  {
+    /* queries grouped by triangle, ascending query index */
+    int64_t* pairs = (int64_t*)malloc(sizeof(int64_t) * 2 * (size_t)nq);
+    for(int64_t i = 0; i < nq; ++i)
+    {
+      pairs[2 * i] = elem[i];
+      pairs[2 * i + 1] = i;
+    }
+    qsort(pairs, (size_t)nq, 2 * sizeof(int64_t), cmp_pair);
+    double* part = (double*)calloc((size_t)nt * 9, sizeof(double));
+    for(int64_t k = 0; k < nq; ++k)
+    {
+      const int64_t c = pairs[2 * k], i = pairs[2 * k + 1];
+      for(int corner = 0; corner < 3; ++corner)
+        for(int d = 0; d < 3; ++d)
+          part[9 * c + 3 * corner + d] += -(wts[3 * i + corner] * dir[3 * i + d]);
+    }
+    for(int64_t v = 0; v < 3 * (int64_t)nv; ++v) grad_v[v] = 0.0;
+    for(int32_t c = 0; c < nt; ++c)
+      for(int corner = 0; corner < 3; ++corner)
+      {
+        const int32_t v = s->conn[3 * (size_t)c + corner];
+        for(int d = 0; d < 3; ++d) grad_v[3 * (size_t)v + d] += part[9 * (size_t)c + 3 * corner + d];
+      }
+    free(part);
+    free(pairs);
+  }
+  free(dir);
+  free(wts);
+  return 0;
+}

@@ -1,0 +1,261 @@
+"""ctypes front end of the CPU oracle (oracle/lib/liborc.so).
+
+TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module, as the checker or the timed CPU
+baseline. The product path (axom_amd) never imports it.
+
+The C library restates the reference's sequential path; see axsq_oracle.c
+for the file:line each function follows.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "liborc.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+def build():
+    """Compile liborc.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_bvh_new.restype = ctypes.c_void_p
+        L.orc_bvh_new.argtypes = [_dp, ctypes.c_int32, ctypes.c_double]
+        L.orc_bvh_free.argtypes = [ctypes.c_void_p]
+        L.orc_bvh_size.restype = ctypes.c_int32
+        L.orc_bvh_size.argtypes = [ctypes.c_void_p]
+        L.orc_bvh_export.argtypes = [ctypes.c_void_p, _dp, _u32p, _i32p, _i32p, _i32p,
+                                     _i32p, _dp, _dp, _i32p]
+        L.orc_bvh_find.restype = ctypes.c_int64
+        L.orc_bvh_find.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64,
+                                   ctypes.c_double, _i32p, _i32p, _i32p, ctypes.c_int]
+        L.orc_bvh_brute.restype = ctypes.c_int64
+        L.orc_bvh_brute.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64,
+                                    ctypes.c_double, _i32p, _i32p, ctypes.c_int64]
+        L.orc_sdf_new.restype = ctypes.c_void_p
+        L.orc_sdf_new.argtypes = [_dp, _dp, _dp, ctypes.c_int32, _i32p, ctypes.c_int32,
+                                  ctypes.c_int, ctypes.c_int]
+        L.orc_sdf_free.argtypes = [ctypes.c_void_p]
+        L.orc_sdf_bvh.restype = ctypes.c_void_p
+        L.orc_sdf_bvh.argtypes = [ctypes.c_void_p]
+        L.orc_sdf_bounds.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_sdf_eval.restype = ctypes.c_int
+        L.orc_sdf_eval.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64, _dp, _dp, _dp,
+                                   _i32p, _i32p, _i32p, _i32p, ctypes.c_int]
+        L.orc_sdf_brute.restype = ctypes.c_int
+        L.orc_sdf_brute.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64, _dp, _i32p,
+                                    ctypes.c_int]
+        L.orc_sdf_backward.restype = ctypes.c_int
+        L.orc_sdf_backward.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64, _dp, _i32p,
+                                       _f32p, _dp, _dp]
+        L.orc_morton3.restype = ctypes.c_uint32
+        L.orc_morton3.argtypes = [ctypes.c_int32] * 3
+        L.orc_closest_point.restype = ctypes.c_int
+        L.orc_closest_point.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_double, _dp]
+        L.orc_sqdist_point_box.restype = ctypes.c_double
+        L.orc_sqdist_point_box.argtypes = [_dp, _dp]
+        L.orc_ray_box.restype = ctypes.c_int
+        L.orc_ray_box.argtypes = [_dp, _dp, _dp, ctypes.c_double]
+        L.orc_box_scale.argtypes = [_dp, ctypes.c_double]
+        L.orc_unit_vector.argtypes = [_dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---------------------------------------------------------------- primal ----
+def morton3(x, y, z):
+    return int(lib().orc_morton3(int(x), int(y), int(z)))
+
+
+def closest_point(p, a, b, c, eps=1e-12):
+    """Returns (cp, loc) following primal::closest_point."""
+    cp = np.zeros(3)
+    loc = lib().orc_closest_point(_p(f64(p), _dp), _p(f64(a), _dp), _p(f64(b), _dp),
+                                  _p(f64(c), _dp), eps, _p(cp, _dp))
+    return cp, loc
+
+
+def sqdist_point_box(p, box6):
+    return lib().orc_sqdist_point_box(_p(f64(p), _dp), _p(f64(box6), _dp))
+
+
+def ray_box(o, d, box6, eps):
+    return bool(lib().orc_ray_box(_p(f64(o), _dp), _p(f64(d), _dp), _p(f64(box6), _dp), eps))
+
+
+def box_scale(box6, sf):
+    b = f64(box6).copy()
+    lib().orc_box_scale(_p(b, _dp), sf)
+    return b
+
+
+def unit_vector(v):
+    out = np.zeros(3)
+    lib().orc_unit_vector(_p(f64(v), _dp), _p(out, _dp))
+    return out
+
+
+# ------------------------------------------------------------------- BVH ----
+KIND = {"points": 0, "boxes": 1, "rays": 2}
+
+
+class OracleBVH:
+    """spin::BVH<3, SEQ_EXEC, double> restated (build + find*)."""
+
+    def __init__(self, boxes, scale=1.000123, _handle=None, _owner=None):
+        if _handle is not None:
+            self.h = _handle
+            self._owner = _owner
+        else:
+            b = f64(boxes).reshape(-1, 6)
+            self._boxes = b
+            self.h = lib().orc_bvh_new(_p(b, _dp), b.shape[0], scale)
+            self._owner = None
+
+    def __del__(self):
+        if getattr(self, "_owner", 1) is None and getattr(self, "h", None):
+            lib().orc_bvh_free(self.h)
+            self.h = None
+
+    @property
+    def n(self):
+        return lib().orc_bvh_size(self.h)
+
+    def export(self):
+        n = self.n
+        inner = n - 1
+        out = dict(
+            bounds=np.zeros(6), mcodes=np.zeros(n, np.uint32), leafs=np.zeros(n, np.int32),
+            lchild=np.zeros(inner, np.int32), rchild=np.zeros(inner, np.int32),
+            parents=np.zeros(2 * n - 1, np.int32), inner_aabbs=np.zeros((inner, 6)),
+            nodes=np.zeros((2 * inner, 6)), children=np.zeros(2 * inner, np.int32))
+        lib().orc_bvh_export(self.h, _p(out["bounds"], _dp), _p(out["mcodes"], _u32p),
+                             _p(out["leafs"], _i32p), _p(out["lchild"], _i32p),
+                             _p(out["rchild"], _i32p), _p(out["parents"], _i32p),
+                             _p(out["inner_aabbs"], _dp), _p(out["nodes"], _dp),
+                             _p(out["children"], _i32p))
+        return out
+
+    def find(self, kind, prims, tol=np.finfo(np.float64).eps, nthreads=1):
+        """Returns (offsets, counts, candidates) like LinearBVH::findCandidatesImpl."""
+        k = KIND[kind]
+        stride = 3 if k == 0 else 6
+        pr = f64(prims).reshape(-1, stride)
+        nq = pr.shape[0]
+        counts = np.zeros(nq, np.int32)
+        offsets = np.zeros(nq, np.int32)
+        total = lib().orc_bvh_find(self.h, k, _p(pr, _dp), nq, tol, _p(counts, _i32p),
+                                   _p(offsets, _i32p), None, nthreads)
+        if total < 0:
+            raise RuntimeError("oracle traversal stack overflow")
+        cands = np.zeros(max(total, 1), np.int32)
+        lib().orc_bvh_find(self.h, k, _p(pr, _dp), nq, tol, _p(counts, _i32p),
+                           _p(offsets, _i32p), _p(cands, _i32p), nthreads)
+        return offsets, counts, cands[:total]
+
+    def brute(self, kind, prims, tol=np.finfo(np.float64).eps):
+        k = KIND[kind]
+        stride = 3 if k == 0 else 6
+        pr = f64(prims).reshape(-1, stride)
+        nq = pr.shape[0]
+        counts = np.zeros(nq, np.int32)
+        total = lib().orc_bvh_brute(self.h, k, _p(pr, _dp), nq, tol, _p(counts, _i32p), None, 0)
+        cands = np.zeros(max(total, 1), np.int32)
+        lib().orc_bvh_brute(self.h, k, _p(pr, _dp), nq, tol, _p(counts, _i32p),
+                            _p(cands, _i32p), total)
+        return counts, cands[:total]
+
+
+# -------------------------------------------------------- SignedDistance ----
+class OracleSDF:
+    """quest::SignedDistance<3, SEQ_EXEC> restated, plus the adjoint."""
+
+    def __init__(self, verts, tris, watertight=True, compute_sign=True):
+        v = f64(verts).reshape(-1, 3)
+        t = i32(tris).reshape(-1, 3)
+        self.nverts = v.shape[0]
+        self.ntris = t.shape[0]
+        x, y, z = f64(v[:, 0]), f64(v[:, 1]), f64(v[:, 2])
+        self.h = lib().orc_sdf_new(_p(x, _dp), _p(y, _dp), _p(z, _dp), self.nverts,
+                                   _p(t, _i32p), self.ntris, int(watertight), int(compute_sign))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_sdf_free(self.h)
+            self.h = None
+
+    def bvh(self):
+        return OracleBVH(None, _handle=lib().orc_sdf_bvh(self.h), _owner=self)
+
+    def bounds(self):
+        lo, hi = np.zeros(3), np.zeros(3)
+        lib().orc_sdf_bounds(self.h, _p(lo, _dp), _p(hi, _dp))
+        return lo, hi
+
+    def eval(self, q, nthreads=1, want_cp=True, want_normal=True, want_stats=False):
+        q = f64(q).reshape(-1, 3)
+        nq = q.shape[0]
+        dist = np.zeros(nq)
+        cp = np.zeros((nq, 3)) if want_cp else None
+        nrm = np.zeros((nq, 3)) if want_normal else None
+        elem = np.zeros(nq, np.int32)
+        loc = np.zeros(nq, np.int32)
+        visits = np.zeros(nq, np.int32) if want_stats else None
+        leaves = np.zeros(nq, np.int32) if want_stats else None
+        rc = lib().orc_sdf_eval(self.h, _p(q, _dp), nq, _p(dist, _dp), _p(cp, _dp),
+                                _p(nrm, _dp), _p(elem, _i32p), _p(loc, _i32p),
+                                _p(visits, _i32p), _p(leaves, _i32p), nthreads)
+        if rc != 0:
+            raise RuntimeError("oracle traversal stack overflow")
+        out = dict(dist=dist, cp=cp, normal=nrm, elem=elem, loc=loc)
+        if want_stats:
+            out["visits"] = visits
+            out["leaves"] = leaves
+        return out
+
+    def brute(self, q, nthreads=1):
+        q = f64(q).reshape(-1, 3)
+        nq = q.shape[0]
+        sq = np.zeros(nq)
+        elem = np.zeros(nq, np.int32)
+        lib().orc_sdf_brute(self.h, _p(q, _dp), nq, _p(sq, _dp), _p(elem, _i32p), nthreads)
+        return sq, elem
+
+    def backward(self, q, dist, elem, gout):
+        q = f64(q).reshape(-1, 3)
+        nq = q.shape[0]
+        gq = np.zeros((nq, 3))
+        gv = np.zeros((self.nverts, 3))
+        lib().orc_sdf_backward(self.h, _p(q, _dp), nq, _p(f64(dist), _dp), _p(i32(elem), _i32p),
+                               _p(np.ascontiguousarray(gout, np.float32), _f32p),
+                               _p(gq, _dp), _p(gv, _dp))
+        return gq, gv
