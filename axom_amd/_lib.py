@@ -1,0 +1,93 @@
+"""ctypes binding of libaxsq.so (the C-ABI declared in include/axsq.h).
+
+The library is the only compute path: if it is missing, or no GPU is
+visible when a compute call is made, these functions raise.
+"""
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libaxsq.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "axsq.h")
+
+AXSQ_OK = 0
+AXSQ_FP64 = 0
+AXSQ_FP32_TRAVERSAL = 1
+QUERY_POINTS, QUERY_BOXES, QUERY_RAYS = 0, 1, 2
+
+ERRORS = {-1: "AXSQ_ERR_INVALID_ARG", -2: "AXSQ_ERR_HIP", -3: "AXSQ_ERR_NOMEM",
+          -4: "AXSQ_ERR_STACK_OVERFLOW", -5: "AXSQ_ERR_STATE"}
+
+_lib = None
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+_dbl = ctypes.c_double
+_dbl3 = ctypes.c_double * 3
+
+
+class AxsqError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_SIGS = {
+    "axsq_last_error": (ctypes.c_char_p, []),
+    "axsq_version": (_i32, []),
+    "axsq_free": (_i32, [_vp]),
+    "axsq_bvh_build": (_i32, [ctypes.POINTER(_vp), _vp, _i64, _dbl, _i32, _vp]),
+    "axsq_bvh_destroy": (_i32, [_vp]),
+    "axsq_bvh_bounds": (_i32, [_vp, _dbl3, _dbl3]),
+    "axsq_bvh_num_leaves": (_i64, [_vp]),
+    "axsq_bvh_export": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "axsq_bvh_find_count": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp,
+                                   ctypes.POINTER(_i64), _vp]),
+    "axsq_bvh_find_fill": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp, _vp]),
+    "axsq_bvh_find": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp,
+                             ctypes.POINTER(_vp), ctypes.POINTER(_i64), _vp]),
+    "axsq_sdf_create": (_i32, [ctypes.POINTER(_vp), _vp, _vp, _vp, _i64, _vp, _i64, _i32, _i32,
+                               _i32, _vp]),
+    "axsq_sdf_destroy": (_i32, [_vp]),
+    "axsq_sdf_mesh_bounds": (_i32, [_vp, _dbl3, _dbl3]),
+    "axsq_sdf_update_vertices": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "axsq_sdf_eval": (_i32, [_vp, _vp, _vp, _vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "axsq_sdf_backward": (_i32, [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "axsq_prof_enable": (_i32, [_i32]),
+    "axsq_prof_reset": (_i32, []),
+    "axsq_prof_get": (_i32, [ctypes.c_char_p, ctypes.POINTER(_i64), ctypes.POINTER(_dbl)]),
+    "axsq_sdf_stats_enable": (_i32, [_vp, _i32]),
+    "axsq_sdf_stats_get": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+}
+
+
+def header_symbols():
+    """Names of the entry points declared in include/axsq.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(axsq_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    """Loads libaxsq.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run axom_amd/build.py (or "
+                               "__graft_entry__.build()) first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != AXSQ_OK:
+        msg = lib().axsq_last_error()
+        raise AxsqError(rc, msg.decode() if msg else "")
+    return rc

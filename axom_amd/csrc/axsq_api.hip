@@ -1,0 +1,562 @@
+// axsq_api.hip -- the extern "C" entry points declared in include/axsq.h.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "axsq_internal.hpp"
+
+namespace axsq {
+
+int find_launch(const axsq_bvh* b, int kind, bool fill, const double* x, const double* y,
+                const double* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
+                const int32_t* offsets, int32_t* cands, hipStream_t s);
+int bvh_export_launch(const axsq_bvh* b, double* inner_nodes, int32_t* children,
+                      int32_t* leaf_nodes, uint32_t* mcodes, hipStream_t s);
+int sdf_check_conn(const axsq_sdf* h, hipStream_t s);
+int sdf_setup(axsq_sdf* h, hipStream_t s);
+int sdf_build_v2t(axsq_sdf* h, hipStream_t s);
+int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const double* z,
+                    int64_t stride, int64_t nq, int fp_mode, double* dist, double* cp,
+                    double* normal, int32_t* elem, bool stats, hipStream_t s);
+int sdf_backward_launch(const axsq_sdf* h, const double* x, const double* y, const double* z,
+                        int64_t stride, int64_t nq, const double* dist, const int32_t* elem,
+                        const float* gout, double* grad_q, double* grad_v, hipStream_t s);
+
+// ---- errors -------------------------------------------------------------------
+static thread_local std::string t_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  t_err = buf;
+}
+void clear_error() { t_err.clear(); }
+
+// ---- device buffers / workspace ---------------------------------------------
+int DevBuf::ensure(size_t b) {
+  if (b <= bytes && p) return AXSQ_OK;
+  if (p) hipFree(p);
+  p = nullptr;
+  bytes = 0;
+  size_t want = b + b / 4 + 4096;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    p = nullptr;
+    set_error("workspace hipMalloc of %zu bytes failed: %s", want, hipGetErrorString(e));
+    return AXSQ_ERR_NOMEM;
+  }
+  bytes = want;
+  return AXSQ_OK;
+}
+void DevBuf::release() {
+  if (p) hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+void Workspace::release() {
+  k0.release();
+  v0.release();
+  k1.release();
+  v1.release();
+  hist.release();
+  tiles.release();
+  misc.release();
+  flag.release();
+}
+
+namespace {
+struct ThreadWorkspaces {
+  std::vector<Workspace> per_device;
+  // Device memory is left to the runtime at thread exit: freeing here could
+  // run after the HIP runtime has been torn down.
+};
+thread_local ThreadWorkspaces t_ws;
+}  // namespace
+
+Workspace& workspace() {
+  int dev = 0;
+  hipGetDevice(&dev);
+  if ((int)t_ws.per_device.size() <= dev) t_ws.per_device.resize(dev + 1);
+  Workspace& w = t_ws.per_device[dev];
+  w.device = dev;
+  return w;
+}
+
+// ---- profiling -------------------------------------------------------------
+namespace {
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+struct Pending {
+  hipEvent_t e0, e1;
+  std::string name;
+};
+std::vector<Pending> g_pending;
+std::map<std::string, std::pair<int64_t, double>> g_totals;
+}  // namespace
+
+bool prof_on() { return g_prof_on; }
+
+ProfRange::ProfRange(const char* n, hipStream_t st) : name(n), s(st) {
+  if (!g_prof_on) return;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    e0 = e1 = nullptr;
+    return;
+  }
+  hipEventRecord(e0, s);
+}
+
+void ProfRange::stop() {
+  if (!e0) return;
+  hipEventRecord(e1, s);
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_pending.push_back({e0, e1, name});
+  e0 = e1 = nullptr;
+}
+
+void prof_flush() {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  std::vector<Pending> keep;
+  for (auto& p : g_pending) {
+    if (hipEventQuery(p.e1) != hipSuccess) {
+      keep.push_back(p);
+      continue;
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, p.e0, p.e1);
+    auto& t = g_totals[p.name];
+    t.first += 1;
+    t.second += ms;
+    hipEventDestroy(p.e0);
+    hipEventDestroy(p.e1);
+  }
+  g_pending.swap(keep);
+}
+
+// Waits for the stream, folds profiling ranges, and turns a traversal stack
+// overflow flagged by a kernel into an error code.
+static int finish(int32_t* status, hipStream_t s) {
+  AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+  prof_flush();
+  if (status) {
+    int32_t h = 0;
+    AXSQ_CHECK_HIP(hipMemcpy(&h, status, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (h) {
+      int32_t z = 0;
+      AXSQ_CHECK_HIP(hipMemcpy(status, &z, sizeof(int32_t), hipMemcpyHostToDevice));
+      if (h & 1) {
+        set_error("traversal stopped: stack deeper than %d entries or malformed tree", kStack);
+        return AXSQ_ERR_STACK_OVERFLOW;
+      }
+      set_error("element id out of range in the adjoint input");
+      return AXSQ_ERR_INVALID_ARG;
+    }
+  }
+  return AXSQ_OK;
+}
+
+static bool valid_kind(int k) {
+  return k == AXSQ_QUERY_POINTS || k == AXSQ_QUERY_BOXES || k == AXSQ_QUERY_RAYS;
+}
+
+static int check_find_args(const axsq_bvh* b, int kind, const double* x, const double* y,
+                           const double* z, int64_t stride, int64_t nq) {
+  if (!b || !b->nodes) {
+    set_error("BVH handle is NULL or not built");
+    return AXSQ_ERR_STATE;
+  }
+  if (!valid_kind(kind)) {
+    set_error("unknown query kind %d", kind);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (nq < 0 || nq > 0x7fffffffll) {
+    set_error("query count %lld out of range", (long long)nq);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (nq > 0 && !x) {
+    set_error("query pointer is NULL");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (kind == AXSQ_QUERY_POINTS) {
+    if (nq > 0 && (!y || !z)) {
+      set_error("point queries need x, y and z pointers");
+      return AXSQ_ERR_INVALID_ARG;
+    }
+    if (stride < 1) {
+      set_error("point stride must be >= 1");
+      return AXSQ_ERR_INVALID_ARG;
+    }
+  } else if (stride < 6) {
+    set_error("box/ray queries need a stride of at least 6 doubles");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  return AXSQ_OK;
+}
+
+}  // namespace axsq
+
+using namespace axsq;
+
+extern "C" {
+
+const char* axsq_last_error(void) { return t_err.c_str(); }
+
+int axsq_version(void) { return 100; }
+
+int axsq_free(void* p) {
+  if (p) AXSQ_CHECK_HIP(hipFree(p));
+  return AXSQ_OK;
+}
+
+// ---- BVH ------------------------------------------------------------------
+int axsq_bvh_build(axsq_bvh** out, const double* boxes, int64_t n, double scale, int fp_mode,
+                   void* stream) {
+  clear_error();
+  if (!out) {
+    set_error("axsq_bvh_build: out is NULL");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  if (fp_mode != AXSQ_FP64 && fp_mode != AXSQ_FP32_TRAVERSAL) {
+    set_error("axsq_bvh_build: unknown fp_mode %d", fp_mode);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  axsq_bvh* b = new axsq_bvh();
+  hipGetDevice(&b->device);
+  const int rc = bvh_build_into(b, boxes, n, scale, fp_mode, (hipStream_t)stream);
+  if (rc != AXSQ_OK) {
+    bvh_release(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return AXSQ_OK;
+}
+
+int axsq_bvh_destroy(axsq_bvh* b) {
+  if (!b) return AXSQ_OK;
+  bvh_release(b);
+  delete b;
+  return AXSQ_OK;
+}
+
+int axsq_bvh_bounds(const axsq_bvh* b, double lo[3], double hi[3]) {
+  clear_error();
+  if (!b || !lo || !hi) {
+    set_error("axsq_bvh_bounds: NULL argument");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = b->bounds[d];
+    hi[d] = b->bounds[3 + d];
+  }
+  return AXSQ_OK;
+}
+
+int64_t axsq_bvh_num_leaves(const axsq_bvh* b) { return b ? b->n : 0; }
+
+int axsq_bvh_export(const axsq_bvh* b, double* inner_nodes, int32_t* children,
+                    int32_t* leaf_nodes, uint32_t* mcodes, void* stream) {
+  clear_error();
+  if (!b || !b->nodes) {
+    set_error("axsq_bvh_export: BVH not built");
+    return AXSQ_ERR_STATE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  AXSQ_TRY(bvh_export_launch(b, inner_nodes, children, leaf_nodes, mcodes, s));
+  return finish(nullptr, s);
+}
+
+int axsq_bvh_find_count(const axsq_bvh* b, int kind, const double* x, const double* y,
+                        const double* z, int64_t stride, int64_t nq, double tol,
+                        int32_t* offsets, int32_t* counts, int64_t* total, void* stream) {
+  clear_error();
+  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq));
+  if (!total || (nq > 0 && (!offsets || !counts))) {
+    set_error("axsq_bvh_find_count: NULL output");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  *total = 0;
+  if (nq == 0) return AXSQ_OK;
+  Workspace& ws = workspace();
+  AXSQ_TRY(find_launch(b, kind, false, x, y, z, stride, nq, tol, counts, nullptr, nullptr, s));
+  AXSQ_TRY(ws.misc.ensure(sizeof(int64_t) * 2));
+  int64_t* tot_dev = ws.misc.as<int64_t>() + 1;
+  AXSQ_TRY(scan_exclusive_i32(counts, offsets, nq, tot_dev, ws, s));
+  int64_t t = 0;
+  AXSQ_CHECK_HIP(hipMemcpyAsync(&t, tot_dev, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  AXSQ_TRY(finish(b->status, s));
+  if (t > 0x7fffffffll) {
+    set_error("candidate total %lld exceeds the int32 index range", (long long)t);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  *total = t;
+  return AXSQ_OK;
+}
+
+int axsq_bvh_find_fill(const axsq_bvh* b, int kind, const double* x, const double* y,
+                       const double* z, int64_t stride, int64_t nq, double tol,
+                       const int32_t* offsets, int32_t* candidates, void* stream) {
+  clear_error();
+  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq));
+  if (nq > 0 && !offsets) {
+    set_error("axsq_bvh_find_fill: offsets is NULL");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (nq == 0) return AXSQ_OK;
+  hipStream_t s = (hipStream_t)stream;
+  AXSQ_TRY(find_launch(b, kind, true, x, y, z, stride, nq, tol, nullptr, offsets, candidates, s));
+  return finish(b->status, s);
+}
+
+int axsq_bvh_find(const axsq_bvh* b, int kind, const double* x, const double* y, const double* z,
+                  int64_t stride, int64_t nq, double tol, int32_t* offsets, int32_t* counts,
+                  int32_t** candidates, int64_t* total, void* stream) {
+  clear_error();
+  if (!candidates || !total) {
+    set_error("axsq_bvh_find: NULL output");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  *candidates = nullptr;
+  AXSQ_TRY(axsq_bvh_find_count(b, kind, x, y, z, stride, nq, tol, offsets, counts, total,
+                               stream));
+  int32_t* c = nullptr;
+  hipError_t e = hipMalloc((void**)&c, sizeof(int32_t) * (size_t)(*total > 0 ? *total : 1));
+  if (e != hipSuccess) {
+    set_error("candidate allocation failed: %s", hipGetErrorString(e));
+    return AXSQ_ERR_NOMEM;
+  }
+  if (*total > 0) {
+    const int rc = axsq_bvh_find_fill(b, kind, x, y, z, stride, nq, tol, offsets, c, stream);
+    if (rc != AXSQ_OK) {
+      hipFree(c);
+      return rc;
+    }
+  }
+  *candidates = c;
+  return AXSQ_OK;
+}
+
+// ---- SignedDistance -------------------------------------------------------
+int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const double* z,
+                    int64_t nverts, const int32_t* tris, int64_t ntris, int watertight,
+                    int compute_sign, int fp_mode, void* stream) {
+  clear_error();
+  if (!out) {
+    set_error("axsq_sdf_create: out is NULL");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  if (!x || !y || !z || !tris || nverts < 3 || ntris < 1 || nverts > 0x7fffffffll ||
+      ntris > 0x3fffffffll) {
+    set_error("axsq_sdf_create: need a triangle mesh with >= 3 vertices and >= 1 triangle");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (fp_mode != AXSQ_FP64 && fp_mode != AXSQ_FP32_TRAVERSAL) {
+    set_error("axsq_sdf_create: unknown fp_mode %d", fp_mode);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  axsq_sdf* h = new axsq_sdf();
+  hipGetDevice(&h->device);
+  h->nverts = nverts;
+  h->ntris = ntris;
+  h->watertight = watertight ? 1 : 0;
+  h->compute_sign = compute_sign ? 1 : 0;
+  h->fp_mode = fp_mode;
+  h->bvh = new axsq_bvh();
+  h->bvh->device = h->device;
+  const char* env = getenv("AXSQ_SORT_QUERIES");
+  h->sort_queries = !(env && env[0] == '0');
+  int rc = AXSQ_OK;
+  auto fail = [&](int code) {
+    axsq_sdf_destroy(h);
+    return code;
+  };
+  if (hipMalloc((void**)&h->vx, sizeof(double) * nverts) != hipSuccess ||
+      hipMalloc((void**)&h->vy, sizeof(double) * nverts) != hipSuccess ||
+      hipMalloc((void**)&h->vz, sizeof(double) * nverts) != hipSuccess ||
+      hipMalloc((void**)&h->conn, sizeof(int32_t) * 3 * ntris) != hipSuccess ||
+      hipMalloc((void**)&h->stats_dev, sizeof(int64_t) * 2) != hipSuccess) {
+    set_error("axsq_sdf_create: out of device memory");
+    return fail(AXSQ_ERR_NOMEM);
+  }
+  if (hipMemcpyAsync(h->vx, x, sizeof(double) * nverts, hipMemcpyDefault, s) != hipSuccess ||
+      hipMemcpyAsync(h->vy, y, sizeof(double) * nverts, hipMemcpyDefault, s) != hipSuccess ||
+      hipMemcpyAsync(h->vz, z, sizeof(double) * nverts, hipMemcpyDefault, s) != hipSuccess ||
+      hipMemcpyAsync(h->conn, tris, sizeof(int32_t) * 3 * ntris, hipMemcpyDefault, s) !=
+          hipSuccess) {
+    set_error("axsq_sdf_create: mesh copy failed");
+    return fail(AXSQ_ERR_HIP);
+  }
+  rc = sdf_check_conn(h, s);
+  if (rc != AXSQ_OK) return fail(rc);
+  rc = sdf_setup(h, s);
+  if (rc != AXSQ_OK) return fail(rc);
+  rc = sdf_build_v2t(h, s);
+  if (rc != AXSQ_OK) return fail(rc);
+  *out = h;
+  return AXSQ_OK;
+}
+
+int axsq_sdf_destroy(axsq_sdf* h) {
+  if (!h) return AXSQ_OK;
+  hipFree(h->vx);
+  hipFree(h->vy);
+  hipFree(h->vz);
+  hipFree(h->conn);
+  hipFree(h->tris);
+  hipFree(h->v2t_start);
+  hipFree(h->v2t_end);
+  hipFree(h->v2t_slot);
+  hipFree(h->stats_dev);
+  if (h->bvh) {
+    bvh_release(h->bvh);
+    delete h->bvh;
+  }
+  delete h;
+  return AXSQ_OK;
+}
+
+int axsq_sdf_mesh_bounds(const axsq_sdf* h, double lo[3], double hi[3]) {
+  clear_error();
+  if (!h || !lo || !hi) {
+    set_error("axsq_sdf_mesh_bounds: NULL argument");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = h->domain[d];
+    hi[d] = h->domain[3 + d];
+  }
+  return AXSQ_OK;
+}
+
+int axsq_sdf_update_vertices(axsq_sdf* h, const double* x, const double* y, const double* z,
+                             void* stream) {
+  clear_error();
+  if (!h || !x || !y || !z) {
+    set_error("axsq_sdf_update_vertices: NULL argument");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const size_t nb = sizeof(double) * h->nverts;
+  AXSQ_CHECK_HIP(hipMemcpyAsync(h->vx, x, nb, hipMemcpyDefault, s));
+  AXSQ_CHECK_HIP(hipMemcpyAsync(h->vy, y, nb, hipMemcpyDefault, s));
+  AXSQ_CHECK_HIP(hipMemcpyAsync(h->vz, z, nb, hipMemcpyDefault, s));
+  return sdf_setup(h, s);
+}
+
+int axsq_sdf_eval(const axsq_sdf* h, const double* x, const double* y, const double* z,
+                  int64_t stride, int64_t nq, int fp_mode, double* dist, double* cp,
+                  double* normal, int32_t* elem, void* stream) {
+  clear_error();
+  if (!h) {
+    set_error("axsq_sdf_eval: handle is NULL");
+    return AXSQ_ERR_STATE;
+  }
+  if (nq < 0 || nq > 0x7fffffffll || stride < 1) {
+    set_error("axsq_sdf_eval: bad query count %lld or stride %lld", (long long)nq,
+              (long long)stride);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (nq > 0 && (!x || !y || !z || !dist)) {
+    set_error("axsq_sdf_eval: NULL query or distance pointer");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (fp_mode != AXSQ_FP64 && fp_mode != AXSQ_FP32_TRAVERSAL) {
+    set_error("axsq_sdf_eval: unknown fp_mode %d", fp_mode);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (fp_mode == AXSQ_FP32_TRAVERSAL && !h->bvh->nodes32) {
+    set_error("axsq_sdf_eval: FP32 traversal needs a handle created with AXSQ_FP32_TRAVERSAL");
+    return AXSQ_ERR_STATE;
+  }
+  if (nq == 0) return AXSQ_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const bool stats = h->stats && fp_mode == AXSQ_FP64;
+  if (stats) AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(h->stats_dev), 0, 4, s));
+  AXSQ_TRY(sdf_eval_launch(h, x, y, z, stride, nq, fp_mode, dist, cp, normal, elem, stats, s));
+  AXSQ_TRY(finish(h->bvh->status, s));
+  if (stats) {
+    int64_t st[2];
+    AXSQ_CHECK_HIP(hipMemcpy(st, h->stats_dev, sizeof(st), hipMemcpyDeviceToHost));
+    const_cast<axsq_sdf*>(h)->last_visits = st[0];
+    const_cast<axsq_sdf*>(h)->last_leaves = st[1];
+  }
+  return AXSQ_OK;
+}
+
+int axsq_sdf_backward(const axsq_sdf* h, const double* x, const double* y, const double* z,
+                      int64_t stride, int64_t nq, const double* dist, const int32_t* elem,
+                      const float* gout, double* grad_q, double* grad_v, void* stream) {
+  clear_error();
+  if (!h) {
+    set_error("axsq_sdf_backward: handle is NULL");
+    return AXSQ_ERR_STATE;
+  }
+  if (nq < 0 || nq > 0x7fffffffll || stride < 1) {
+    set_error("axsq_sdf_backward: bad query count or stride");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (nq == 0) {
+    if (grad_v) AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(grad_v), 0, 6 * h->nverts, s));
+    return finish(nullptr, s);
+  }
+  if (!x || !y || !z || !dist || !elem || !gout) {
+    set_error("axsq_sdf_backward: NULL input");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  AXSQ_TRY(sdf_backward_launch(h, x, y, z, stride, nq, dist, elem, gout, grad_q, grad_v, s));
+  return finish(h->bvh->status, s);
+}
+
+// ---- measurement ---------------------------------------------------------
+int axsq_prof_enable(int on) {
+  g_prof_on = on != 0;
+  return AXSQ_OK;
+}
+
+int axsq_prof_reset(void) {
+  prof_flush();
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_totals.clear();
+  return AXSQ_OK;
+}
+
+int axsq_prof_get(const char* name, int64_t* launches, double* total_ms) {
+  clear_error();
+  if (!name || !launches || !total_ms) {
+    set_error("axsq_prof_get: NULL argument");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  prof_flush();
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  auto it = g_totals.find(name);
+  *launches = (it == g_totals.end()) ? 0 : it->second.first;
+  *total_ms = (it == g_totals.end()) ? 0.0 : it->second.second;
+  return AXSQ_OK;
+}
+
+int axsq_sdf_stats_enable(axsq_sdf* h, int on) {
+  if (!h) return AXSQ_ERR_INVALID_ARG;
+  h->stats = on != 0;
+  return AXSQ_OK;
+}
+
+int axsq_sdf_stats_get(const axsq_sdf* h, int64_t* visits, int64_t* leaves) {
+  if (!h || !visits || !leaves) return AXSQ_ERR_INVALID_ARG;
+  *visits = h->last_visits;
+  *leaves = h->last_leaves;
+  return AXSQ_OK;
+}
+
+}  // extern "C"

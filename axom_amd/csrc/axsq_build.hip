@@ -1,0 +1,447 @@
+// axsq_build.hip -- linear BVH construction on gfx950.
+//
+// Pipeline (reference: src/axom/spin/internal/linear_bvh/build_radix_tree.hpp:579-611
+// and src/axom/spin/policy/LinearBVH.hpp:193-269):
+//   k_transform   scale every box about its centroid (+ padding of 0/1 boxes)
+//                 and reduce block-wise the bounds of the valid boxes
+//   k_bounds      finish the bounds; Morton normalisation constants
+//   k_mcodes      30-bit Morton code of each scaled-box centroid
+//   radix sort    stable LSD sort of (code, id)
+//   k_reorder     scaled leaf boxes in leaf (sorted) order
+//   k_karras      Karras 2012 hierarchy over the sorted codes
+//   k_refit       bottom-up boxes; the second thread to reach a node goes on
+//                 (agent-scope acquire/release counters)
+//   k_emit        128-byte FP64 nodes (+ 64-byte FP32 nodes, rounded outward)
+#include "axsq_internal.hpp"
+
+namespace axsq {
+
+namespace {
+
+struct BuildParams {
+  double bounds[6];
+  double minc[3];
+  double inv[3];
+};
+
+__device__ __forceinline__ void load_box(const double* __restrict__ p, double b[6]) {
+  const double2* q = reinterpret_cast<const double2*>(p);
+  if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+    const double2 a = q[0], c = q[1], d = q[2];
+    b[0] = a.x; b[1] = a.y; b[2] = c.x; b[3] = c.y; b[4] = d.x; b[5] = d.y;
+  } else {
+    for (int k = 0; k < 6; ++k) b[k] = p[k];
+  }
+}
+
+__device__ __forceinline__ void store_box(double* __restrict__ p, const double b[6]) {
+  double2* q = reinterpret_cast<double2*>(p);
+  q[0] = make_double2(b[0], b[1]);
+  q[1] = make_double2(b[2], b[3]);
+  q[2] = make_double2(b[4], b[5]);
+}
+
+__device__ __forceinline__ void invalid_box(double b[6]) {
+  b[0] = b[1] = b[2] = kDblMax;
+  b[3] = b[4] = b[5] = -kDblMax;
+}
+
+// Per block: scaled boxes out + min/max of the valid ones. A block with no
+// valid box reports an invalid partial.
+__global__ __launch_bounds__(kBlock) void k_transform(const double* __restrict__ in, int64_t n_in,
+                                                       int64_t n, double scale,
+                                                       double* __restrict__ out,
+                                                       double* __restrict__ partials) {
+  __shared__ double s_red[6][kBlock];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[6];
+  invalid_box(acc);
+  if (i < n) {
+    double b[6];
+    if (n_in > 1 || (n_in == 1 && i == 0))
+      load_box(in + 6 * i, b);
+    else
+      invalid_box(b);
+    box_scale(b, scale);
+    store_box(out + 6 * i, b);
+    if (box_valid(b))
+      for (int k = 0; k < 6; ++k) acc[k] = b[k];
+  }
+  for (int k = 0; k < 6; ++k) s_red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      for (int k = 0; k < 3; ++k) {
+        const double a = s_red[k][threadIdx.x], b = s_red[k][threadIdx.x + off];
+        s_red[k][threadIdx.x] = (b < a) ? b : a;
+        const double c = s_red[3 + k][threadIdx.x], d = s_red[3 + k][threadIdx.x + off];
+        s_red[3 + k][threadIdx.x] = (d > c) ? d : c;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; ++k) partials[6 * (int64_t)blockIdx.x + k] = s_red[k][0];
+}
+
+// Union of the valid scaled boxes; with none valid, the SEQ reduce
+// (addBox from an invalid box, build_radix_tree.hpp:134-142) leaves the last
+// box. Then inv_extent as in get_mcodes (:154-163).
+__global__ __launch_bounds__(1024) void k_bounds(const double* __restrict__ partials,
+                                                  int64_t nparts, const double* __restrict__ boxes,
+                                                  int64_t n, BuildParams* __restrict__ prm) {
+  __shared__ double s_red[6][1024];
+  double acc[6];
+  invalid_box(acc);
+  for (int64_t p = threadIdx.x; p < nparts; p += 1024) {
+    for (int k = 0; k < 3; ++k) {
+      const double a = partials[6 * p + k];
+      if (a < acc[k]) acc[k] = a;
+      const double b = partials[6 * p + 3 + k];
+      if (b > acc[3 + k]) acc[3 + k] = b;
+    }
+  }
+  for (int k = 0; k < 6; ++k) s_red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      for (int k = 0; k < 3; ++k) {
+        const double a = s_red[k][threadIdx.x], b = s_red[k][threadIdx.x + off];
+        s_red[k][threadIdx.x] = (b < a) ? b : a;
+        const double c = s_red[3 + k][threadIdx.x], d = s_red[3 + k][threadIdx.x + off];
+        s_red[3 + k][threadIdx.x] = (d > c) ? d : c;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double b[6];
+    for (int k = 0; k < 6; ++k) b[k] = s_red[k][0];
+    if (!box_valid(b)) load_box(boxes + 6 * (n - 1), b);
+    for (int k = 0; k < 6; ++k) prm->bounds[k] = b[k];
+    for (int d = 0; d < 3; ++d) {
+      const double ext = b[3 + d] - b[d];
+      prm->minc[d] = b[d];
+      prm->inv[d] = near_eq(ext, 0.0, 1.0e-8) ? 0.0 : 1.0 / ext;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mcodes(const double* __restrict__ boxes, int64_t n,
+                                                    const BuildParams* __restrict__ prm,
+                                                    uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double b[6];
+  load_box(boxes + 6 * i, b);
+  double c[3], minc[3], inv[3];
+  for (int d = 0; d < 3; ++d) {
+    c[d] = 0.5 * (b[d] + b[3 + d]);
+    minc[d] = prm->minc[d];
+    inv[d] = prm->inv[d];
+  }
+  keys[i] = morton_of(c, minc, inv);
+  vals[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reorder(const double* __restrict__ boxes,
+                                                     const uint32_t* __restrict__ perm,
+                                                     const uint32_t* __restrict__ codes,
+                                                     int64_t n, double* __restrict__ leaf_boxes,
+                                                     int32_t* __restrict__ leaf_ids,
+                                                     uint32_t* __restrict__ mcodes) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t src = perm[p];
+  double b[6];
+  load_box(boxes + 6 * (int64_t)src, b);
+  store_box(leaf_boxes + 6 * p, b);
+  leaf_ids[p] = (int32_t)src;
+  mcodes[p] = codes[p];
+}
+
+// build_tree, build_radix_tree.hpp:290-384
+__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ mc, int32_t inner,
+                                                    int32_t* __restrict__ lchild,
+                                                    int32_t* __restrict__ rchild,
+                                                    int32_t* __restrict__ parents) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= inner) return;
+  const int32_t d =
+      (karras_delta(i, i + 1, inner, mc) - karras_delta(i, i - 1, inner, mc)) < 0 ? -1 : 1;
+  const int32_t dmin = karras_delta(i, i - d, inner, mc);
+  int32_t lmax = 2;
+  while (karras_delta(i, i + lmax * d, inner, mc) > dmin) lmax *= 2;
+  int32_t l = 0;
+  for (int32_t s = lmax / 2; s >= 1; s /= 2)
+    if (karras_delta(i, i + (l + s) * d, inner, mc) > dmin) l += s;
+  const int32_t j = i + l * d;
+  const int32_t dnode = karras_delta(i, j, inner, mc);
+  int32_t s = 0;
+  double div = 2.0;
+  int32_t step = (int32_t)ceil((double)(float)l / div);
+  for (;;) {
+    if (karras_delta(i, i + (s + step) * d, inner, mc) > dnode) s += step;
+    if (step == 1) break;
+    div *= 2;
+    step = (int32_t)ceil((double)(float)l / div);
+  }
+  const int32_t split = i + s * d + (d < 0 ? d : 0);
+  const int32_t lo = (i < j) ? i : j;
+  const int32_t hi = (i < j) ? j : i;
+  if (lo == split) {
+    parents[split + inner] = i;
+    lchild[i] = split + inner;
+  } else {
+    parents[split] = i;
+    lchild[i] = split;
+  }
+  if (hi == split + 1) {
+    parents[split + inner + 1] = i;
+    rchild[i] = split + inner + 1;
+  } else {
+    parents[split + 1] = i;
+    rchild[i] = split + 1;
+  }
+  if (i == 0) parents[0] = -1;
+}
+
+// propagate_aabbs, build_radix_tree.hpp:505-576. The counter increment is an
+// agent-scope acq_rel RMW: the first thread's box stores are released before
+// its increment, and the second thread acquires them with its increment, so
+// no volatile polling is needed. Every thread finishes after at most
+// depth-many steps.
+__global__ __launch_bounds__(kBlock) void k_refit(const double* __restrict__ leaf_boxes,
+                                                   int32_t inner,
+                                                   const int32_t* __restrict__ lchild,
+                                                   const int32_t* __restrict__ rchild,
+                                                   const int32_t* __restrict__ parents,
+                                                   int32_t* counters, double* inner_boxes) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i > inner) return;
+  double acc[6];
+  load_box(leaf_boxes + 6 * (int64_t)i, acc);
+  int32_t last = inner + i;
+  int32_t cur = parents[inner + i];
+  while (cur != -1) {
+    const int32_t old = __hip_atomic_fetch_add(&counters[cur], 1, __ATOMIC_ACQ_REL,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) break;
+    const int32_t lc = lchild[cur], rc = rchild[cur];
+    const int32_t other = (lc == last) ? rc : lc;
+    double ob[6];
+    if (other >= inner)
+      load_box(leaf_boxes + 6 * (int64_t)(other - inner), ob);
+    else
+      for (int k = 0; k < 6; ++k)
+        ob[k] = __hip_atomic_load(&inner_boxes[6 * (int64_t)other + k], __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+    box_add_box(acc, ob);
+    for (int k = 0; k < 6; ++k)
+      __hip_atomic_store(&inner_boxes[6 * (int64_t)cur + k], acc[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    last = cur;
+    cur = parents[cur];
+  }
+}
+
+__device__ __forceinline__ float f_down(double x) { return __double2float_rd(x); }
+__device__ __forceinline__ float f_up(double x) { return __double2float_ru(x); }
+
+// emit_bvh_parents, LinearBVH.hpp:225-264, into 128-byte nodes (child links
+// are node indices, not doubled offsets) and, for mixed precision, 64-byte
+// FP32 nodes whose min corners round down and max corners round up.
+__global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf_boxes,
+                                                  const double* __restrict__ inner_boxes,
+                                                  int32_t inner,
+                                                  const int32_t* __restrict__ lchild,
+                                                  const int32_t* __restrict__ rchild,
+                                                  Node64* __restrict__ nodes,
+                                                  Node32* __restrict__ nodes32) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= inner) return;
+  Node64 nd;
+  int32_t lc = lchild[i], rc = rchild[i];
+  if (lc >= inner) {
+    load_box(leaf_boxes + 6 * (int64_t)(lc - inner), nd.l);
+    lc = -(lc - inner + 1);
+  } else {
+    load_box(inner_boxes + 6 * (int64_t)lc, nd.l);
+  }
+  if (rc >= inner) {
+    load_box(leaf_boxes + 6 * (int64_t)(rc - inner), nd.r);
+    rc = -(rc - inner + 1);
+  } else {
+    load_box(inner_boxes + 6 * (int64_t)rc, nd.r);
+  }
+  nd.lc = lc;
+  nd.rc = rc;
+  for (int k = 0; k < 6; ++k) nd.pad[k] = 0;
+  nodes[i] = nd;
+  if (nodes32) {
+    Node32 h;
+    for (int k = 0; k < 3; ++k) {
+      h.l[k] = f_down(nd.l[k]);
+      h.l[3 + k] = f_up(nd.l[3 + k]);
+      h.r[k] = f_down(nd.r[k]);
+      h.r[3 + k] = f_up(nd.r[3 + k]);
+    }
+    h.lc = lc;
+    h.rc = rc;
+    h.pad[0] = h.pad[1] = 0;
+    nodes32[i] = h;
+  }
+}
+
+// Reference layout (LinearBVH.hpp:170-187,257-262): child boxes of inner
+// node i at 2i and 2i+1, child links doubled for inner children.
+__global__ __launch_bounds__(kBlock) void k_export(const Node64* __restrict__ nodes, int32_t inner,
+                                                    double* __restrict__ boxes,
+                                                    int32_t* __restrict__ children) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= inner) return;
+  const Node64 nd = nodes[i];
+  if (boxes) {
+    for (int k = 0; k < 6; ++k) {
+      boxes[12 * (int64_t)i + k] = nd.l[k];
+      boxes[12 * (int64_t)i + 6 + k] = nd.r[k];
+    }
+  }
+  if (children) {
+    children[2 * (int64_t)i] = nd.lc >= 0 ? 2 * nd.lc : nd.lc;
+    children[2 * (int64_t)i + 1] = nd.rc >= 0 ? 2 * nd.rc : nd.rc;
+  }
+}
+
+}  // namespace
+
+int bvh_export_launch(const axsq_bvh* b, double* inner_nodes, int32_t* children,
+                      int32_t* leaf_nodes, uint32_t* mcodes, hipStream_t s) {
+  const int32_t inner = (int32_t)(b->n - 1);
+  if (inner_nodes || children) {
+    k_export<<<grid_for(inner), kBlock, 0, s>>>(b->nodes, inner, inner_nodes, children);
+    AXSQ_CHECK_HIP(hipGetLastError());
+  }
+  if (leaf_nodes)
+    AXSQ_CHECK_HIP(hipMemcpyAsync(leaf_nodes, b->leaf_ids, sizeof(int32_t) * b->n,
+                                  hipMemcpyDeviceToDevice, s));
+  if (mcodes)
+    AXSQ_CHECK_HIP(hipMemcpyAsync(mcodes, b->mcodes, sizeof(uint32_t) * b->n,
+                                  hipMemcpyDeviceToDevice, s));
+  return AXSQ_OK;
+}
+
+void bvh_release(axsq_bvh* b) {
+  if (!b) return;
+  hipFree(b->nodes);
+  hipFree(b->nodes32);
+  hipFree(b->leaf_boxes);
+  hipFree(b->leaf_ids);
+  hipFree(b->mcodes);
+  hipFree(b->status);
+  b->nodes = nullptr;
+  b->nodes32 = nullptr;
+  b->leaf_boxes = nullptr;
+  b->leaf_ids = nullptr;
+  b->mcodes = nullptr;
+  b->status = nullptr;
+}
+
+template <class T>
+static int dmalloc(T** p, size_t count) {
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
+  if (e != hipSuccess) {
+    set_error("hipMalloc of %zu bytes failed: %s", sizeof(T) * count, hipGetErrorString(e));
+    return AXSQ_ERR_NOMEM;
+  }
+  return AXSQ_OK;
+}
+
+int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale, int fp_mode,
+                   hipStream_t s) {
+  if (n_in < 0 || n_in > 0x3fffffff) {
+    set_error("axsq_bvh_build: box count %lld out of range", (long long)n_in);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (n_in > 0 && !boxes) {
+    set_error("axsq_bvh_build: boxes is NULL");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  const int64_t n = (n_in <= 1) ? 2 : n_in;
+  const int32_t inner = (int32_t)(n - 1);
+  Workspace& ws = workspace();
+
+  // (re)allocate the persistent arrays when the size changes
+  if (b->n != n || (fp_mode == AXSQ_FP32_TRAVERSAL && !b->nodes32) || !b->nodes) {
+    bvh_release(b);
+    AXSQ_TRY(dmalloc(&b->nodes, (size_t)inner));
+    if (fp_mode == AXSQ_FP32_TRAVERSAL) AXSQ_TRY(dmalloc(&b->nodes32, (size_t)inner));
+    AXSQ_TRY(dmalloc(&b->leaf_boxes, 6 * (size_t)n));
+    AXSQ_TRY(dmalloc(&b->leaf_ids, (size_t)n));
+    AXSQ_TRY(dmalloc(&b->mcodes, (size_t)n));
+    AXSQ_TRY(dmalloc(&b->status, 1));
+  }
+  b->n = n;
+  b->fp_mode = fp_mode;
+  b->scale = scale;
+
+  // scratch: scaled boxes (6n doubles), keys/vals x2, tree arrays, partials
+  const int64_t nparts = grid_for(n);
+  AXSQ_TRY(ws.k0.ensure(sizeof(uint32_t) * n));
+  AXSQ_TRY(ws.v0.ensure(sizeof(uint32_t) * n));
+  AXSQ_TRY(ws.k1.ensure(sizeof(uint32_t) * n));
+  AXSQ_TRY(ws.v1.ensure(sizeof(uint32_t) * n));
+  // flag buffer holds: scaled boxes | partials | params | lchild | rchild |
+  // parents | counters | inner boxes
+  const size_t off_part = sizeof(double) * 6 * n;
+  const size_t off_prm = off_part + sizeof(double) * 6 * nparts;
+  const size_t off_l = (off_prm + sizeof(BuildParams) + 255) & ~size_t(255);
+  const size_t off_r = off_l + sizeof(int32_t) * inner;
+  const size_t off_p = off_r + sizeof(int32_t) * inner;
+  const size_t off_c = off_p + sizeof(int32_t) * (2 * n - 1);
+  const size_t off_ib = (off_c + sizeof(int32_t) * inner + 255) & ~size_t(255);
+  const size_t total = off_ib + sizeof(double) * 6 * inner;
+  AXSQ_TRY(ws.flag.ensure(total));
+  char* base = ws.flag.as<char>();
+  double* scaled = reinterpret_cast<double*>(base);
+  double* partials = reinterpret_cast<double*>(base + off_part);
+  BuildParams* prm = reinterpret_cast<BuildParams*>(base + off_prm);
+  int32_t* lch = reinterpret_cast<int32_t*>(base + off_l);
+  int32_t* rch = reinterpret_cast<int32_t*>(base + off_r);
+  int32_t* par = reinterpret_cast<int32_t*>(base + off_p);
+  int32_t* cnt = reinterpret_cast<int32_t*>(base + off_c);
+  double* ib = reinterpret_cast<double*>(base + off_ib);
+
+  ProfRange pr("bvh_build", s);
+  k_transform<<<(unsigned)nparts, kBlock, 0, s>>>(boxes, n_in, n, scale, scaled, partials);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  k_bounds<<<1, 1024, 0, s>>>(partials, nparts, scaled, n, prm);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  uint32_t *keys = ws.k0.as<uint32_t>(), *vals = ws.v0.as<uint32_t>();
+  k_mcodes<<<grid_for(n), kBlock, 0, s>>>(scaled, n, prm, keys, vals);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  uint32_t *sk, *sv;
+  AXSQ_TRY(radix_sort_pairs(keys, vals, ws.k1.as<uint32_t>(), ws.v1.as<uint32_t>(), n, 30, ws, s,
+                            &sk, &sv));
+  k_reorder<<<grid_for(n), kBlock, 0, s>>>(scaled, sv, sk, n, b->leaf_boxes, b->leaf_ids,
+                                           b->mcodes);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  k_karras<<<grid_for(inner), kBlock, 0, s>>>(b->mcodes, inner, lch, rch, par);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  AXSQ_TRY(fill_i32(cnt, 0, inner, s));
+  k_refit<<<grid_for(n), kBlock, 0, s>>>(b->leaf_boxes, inner, lch, rch, par, cnt, ib);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  k_emit<<<grid_for(inner), kBlock, 0, s>>>(b->leaf_boxes, ib, inner, lch, rch, b->nodes,
+                                            b->nodes32);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  pr.stop();
+  AXSQ_CHECK_HIP(hipMemcpyAsync(b->bounds, prm->bounds, sizeof(double) * 6,
+                                hipMemcpyDeviceToHost, s));
+  AXSQ_TRY(fill_i32(b->status, 0, 1, s));
+  AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+  prof_flush();
+  return AXSQ_OK;
+}
+
+}  // namespace axsq

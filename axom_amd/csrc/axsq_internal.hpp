@@ -1,0 +1,132 @@
+// axsq_internal.hpp -- host-side plumbing shared by the libaxsq translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "axsq.h"
+#include "axsq_device.hpp"
+
+namespace axsq {
+
+// ---- errors ---------------------------------------------------------------
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+
+#define AXSQ_CHECK_HIP(call)                                                   \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::axsq::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                   \
+      return AXSQ_ERR_HIP;                                                     \
+    }                                                                          \
+  } while (0)
+
+#define AXSQ_TRY(expr)       \
+  do {                       \
+    int rc_ = (expr);        \
+    if (rc_ != AXSQ_OK) return rc_; \
+  } while (0)
+
+// ---- device buffers -------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t b);  // grows (never shrinks); contents are not preserved
+  void release();
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+};
+
+// Scratch owned by the calling host thread (per device), reused across calls.
+struct Workspace {
+  DevBuf k0, v0, k1, v1, hist, tiles, misc, flag;
+  int device = -1;
+  void release();
+};
+Workspace& workspace();  // for the current device
+
+// ---- profiling (HIP events on the launching stream) -----------------------
+bool prof_on();
+struct ProfRange {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const char* name = nullptr;
+  hipStream_t s = nullptr;
+  ProfRange(const char* n, hipStream_t st);
+  void stop();
+};
+void prof_flush();  // after a stream sync: folds finished ranges into totals
+
+// ---- primitives (axsq_prims.hip) ------------------------------------------
+constexpr int kBlock = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;
+
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+  int64_t g = (n + block - 1) / block;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
+int fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
+// Exclusive scan of n int32 counts into out; *total_dev (int64) receives the sum.
+int scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total_dev,
+                       Workspace& ws, hipStream_t s);
+// Stable ascending sort of (keys, vals) by the low `bits` bits of the key.
+// On return the sorted data is in (*keys_out, *vals_out), which point at
+// either the input arrays or the alternates.
+int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                     int64_t n, int bits, Workspace& ws, hipStream_t s, uint32_t** keys_out,
+                     uint32_t** vals_out);
+
+inline int bits_for(uint64_t max_value) {
+  int b = 1;
+  while (b < 32 && (max_value >> b) != 0) ++b;
+  return b;
+}
+
+}  // namespace axsq
+
+// ---- handles -------------------------------------------------------------
+struct axsq_bvh {
+  int64_t n = 0;        // leaves after padding
+  int fp_mode = AXSQ_FP64;
+  double scale = 1.000123;
+  double bounds[6];     // host copy of the root box
+  int device = 0;
+  axsq::Node64* nodes = nullptr;    // n-1 FP64 nodes, root = 0
+  axsq::Node32* nodes32 = nullptr;  // n-1 FP32 nodes (mixed precision)
+  double* leaf_boxes = nullptr;     // n scaled leaf boxes in leaf order (AoS)
+  int32_t* leaf_ids = nullptr;      // n: leaf position -> original id
+  uint32_t* mcodes = nullptr;       // n sorted Morton codes
+  int32_t* status = nullptr;        // device flag: traversal stack overflow
+};
+
+struct axsq_sdf {
+  int64_t nverts = 0, ntris = 0;
+  int watertight = 1, compute_sign = 1, fp_mode = AXSQ_FP64;
+  int device = 0;
+  double* vx = nullptr;
+  double* vy = nullptr;
+  double* vz = nullptr;
+  int32_t* conn = nullptr;
+  double domain[6];                 // host copy of the mesh bounds
+  axsq_bvh* bvh = nullptr;
+  axsq::TriRec* tris = nullptr;     // triangles in BVH leaf order
+  int32_t* v2t_start = nullptr;     // per vertex: first incident slot in v2t_slot
+  int32_t* v2t_end = nullptr;
+  int32_t* v2t_slot = nullptr;      // incident (3*tri + corner), ascending
+  bool stats = false;
+  int64_t* stats_dev = nullptr;     // [visits, leaves]
+  int64_t last_visits = 0, last_leaves = 0;
+  bool sort_queries = true;
+};
+
+namespace axsq {
+int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n, double scale, int fp_mode,
+                   hipStream_t s);
+void bvh_release(axsq_bvh* b);
+}  // namespace axsq

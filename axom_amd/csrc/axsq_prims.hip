@@ -1,0 +1,279 @@
+// axsq_prims.hip -- scan and stable radix sort for gfx950 (wave64).
+//
+// Radix sort: least-significant-digit first, 8-bit digits, three kernels per
+// pass (tile histogram, exclusive scan of the digit-major histogram, stable
+// scatter). Inside a tile of 256 threads x 16 keys, keys are ranked in global
+// order: sub-tile k covers [base + 256k, base + 256k + 256) and each 64-lane
+// wave finds the lanes that share its digit with 8 ballots, so equal keys
+// keep their input order and the result equals a stable sort.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "axsq_internal.hpp"
+
+namespace axsq {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const unsigned lane = __lane_id();
+  return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+// Inclusive scan across one wave with shuffles.
+template <class T>
+__device__ __forceinline__ T wave_inclusive(T v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    T y = __shfl_up(v, off, kWave);
+    if (lane >= off) v += y;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+template <class T>
+__device__ __forceinline__ T block_exclusive(T v, T* s_wave, T* total) {
+  const int lane = __lane_id();
+  const int wid = threadIdx.x / kWave;
+  T inc = wave_inclusive(v);
+  if (lane == kWave - 1) s_wave[wid] = inc;
+  __syncthreads();
+  T base = 0, sum = 0;
+#pragma unroll
+  for (int w = 0; w < kWavesPerBlock; ++w) {
+    const T x = s_wave[w];
+    if (w < wid) base += x;
+    sum += x;
+  }
+  if (total) *total = sum;
+  __syncthreads();
+  return base + inc - v;
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---- scan -------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_tile_sums(const T* __restrict__ in, int64_t n,
+                                                       int64_t* __restrict__ sums) {
+  __shared__ int64_t s_wave[kWavesPerBlock];
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  int64_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int64_t i = base + k * kBlock + threadIdx.x;
+    if (i < n) acc += (int64_t)in[i];
+  }
+  int64_t tot;
+  block_exclusive<int64_t>(acc, s_wave, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// One block scans the tile sums (in place, exclusive) and writes the total.
+__global__ __launch_bounds__(1024) void k_scan_tile_sums(int64_t* __restrict__ sums,
+                                                          int64_t ntiles,
+                                                          int64_t* __restrict__ total) {
+  __shared__ int64_t s_part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (ntiles + 1023) / 1024;
+  const int64_t b = t * per;
+  const int64_t e = (b + per < ntiles) ? b + per : ntiles;
+  int64_t acc = 0;
+  for (int64_t i = b; i < e; ++i) acc += sums[i];
+  s_part[t] = acc;
+  __syncthreads();
+  // Hillis-Steele over 1024 partials
+  for (int off = 1; off < 1024; off <<= 1) {
+    int64_t y = (t >= off) ? s_part[t - off] : 0;
+    __syncthreads();
+    s_part[t] += y;
+    __syncthreads();
+  }
+  int64_t run = s_part[t] - acc;
+  for (int64_t i = b; i < e; ++i) {
+    const int64_t x = sums[i];
+    sums[i] = run;
+    run += x;
+  }
+  if (t == 1023 && total) *total = s_part[1023];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const T* __restrict__ in,
+                                                        T* __restrict__ out, int64_t n,
+                                                        const int64_t* __restrict__ sums) {
+  __shared__ T s_tile[kTile];
+  __shared__ T s_wave[kWavesPerBlock];
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int64_t i = base + k * kBlock + threadIdx.x;
+    s_tile[k * kBlock + threadIdx.x] = (i < n) ? in[i] : T(0);
+  }
+  __syncthreads();
+  T v[kItems];
+  T acc = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    v[k] = s_tile[threadIdx.x * kItems + k];
+    acc += v[k];
+  }
+  T excl = block_exclusive<T>(acc, s_wave, nullptr);
+  const T tile_base = (T)sums[blockIdx.x];
+  T run = tile_base + excl;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    s_tile[threadIdx.x * kItems + k] = run;
+    run += v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int64_t i = base + k * kBlock + threadIdx.x;
+    if (i < n) out[i] = s_tile[k * kBlock + threadIdx.x];
+  }
+}
+
+// ---- radix sort -------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys,
+                                                        int64_t n, int shift, int nbits,
+                                                        uint32_t* __restrict__ hist,
+                                                        int64_t nblocks) {
+  __shared__ uint32_t s_h[256];
+  s_h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t mask = (1u << nbits) - 1u;
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int64_t i = base + k * kBlock + threadIdx.x;
+    if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n, int shift,
+    int nbits, const uint32_t* __restrict__ hist_scanned, int64_t nblocks,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_wcnt[kWavesPerBlock][256];
+  const int t = threadIdx.x;
+  const int wid = t / kWave;
+  const uint64_t lt = lanemask_lt();
+  const uint32_t mask = (1u << nbits) - 1u;
+  s_base[t] = hist_scanned[(int64_t)t * nblocks + blockIdx.x];
+#pragma unroll
+  for (int w = 0; w < kWavesPerBlock; ++w) s_wcnt[w][t] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+  for (int k = 0; k < kItems; ++k) {
+    const int64_t i = base + k * kBlock + t;
+    const bool valid = i < n;
+    uint32_t key = 0, val = 0, d = 0;
+    if (valid) {
+      key = keys[i];
+      val = vals[i];
+      d = (key >> shift) & mask;
+    }
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < nbits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(valid && bit);
+      peers &= bit ? bb : ~bb;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    const bool leader = valid && ((peers & lt) == 0);
+    if (leader) s_wcnt[wid][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = s_base[d] + rank;
+      for (int w = 0; w < wid; ++w) pos += s_wcnt[w][d];
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) {
+      add += s_wcnt[w][t];
+      s_wcnt[w][t] = 0;
+    }
+    s_base[t] += add;
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+int fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
+  if (n <= 0) return AXSQ_OK;
+  k_fill_i32<<<grid_for(n), kBlock, 0, s>>>(p, v, n);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  return AXSQ_OK;
+}
+
+int scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total_dev,
+                       Workspace& ws, hipStream_t s) {
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  AXSQ_TRY(ws.tiles.ensure(sizeof(int64_t) * (size_t)(ntiles < 1 ? 1 : ntiles)));
+  int64_t* sums = ws.tiles.as<int64_t>();
+  if (n > 0) {
+    k_tile_sums<int32_t><<<(unsigned)ntiles, kBlock, 0, s>>>(in, n, sums);
+    AXSQ_CHECK_HIP(hipGetLastError());
+  }
+  k_scan_tile_sums<<<1, 1024, 0, s>>>(sums, ntiles, total_dev);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  if (n > 0) {
+    k_scan_tiles<int32_t><<<(unsigned)ntiles, kBlock, 0, s>>>(in, out, n, sums);
+    AXSQ_CHECK_HIP(hipGetLastError());
+  }
+  return AXSQ_OK;
+}
+
+int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                     int64_t n, int bits, Workspace& ws, hipStream_t s, uint32_t** keys_out,
+                     uint32_t** vals_out) {
+  *keys_out = keys;
+  *vals_out = vals;
+  if (n <= 1) return AXSQ_OK;
+  if (n > (int64_t)0x7fffffffll) {
+    set_error("radix sort: %lld keys exceed the int32 position range", (long long)n);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  const int64_t nb = (n + kTile - 1) / kTile;
+  const int64_t nh = 256 * nb;
+  AXSQ_TRY(ws.hist.ensure(sizeof(uint32_t) * (size_t)nh));
+  uint32_t* hist = ws.hist.as<uint32_t>();
+  AXSQ_TRY(ws.misc.ensure(sizeof(int64_t) * 2));
+  uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
+  for (int shift = 0; shift < bits; shift += 8) {
+    const int nbits = (bits - shift < 8) ? bits - shift : 8;
+    k_radix_hist<<<(unsigned)nb, kBlock, 0, s>>>(ki, n, shift, nbits, hist, nb);
+    AXSQ_CHECK_HIP(hipGetLastError());
+    AXSQ_TRY(scan_exclusive_i32((const int32_t*)hist, (int32_t*)hist, nh,
+                                ws.misc.as<int64_t>(), ws, s));
+    k_radix_scatter<<<(unsigned)nb, kBlock, 0, s>>>(ki, vi, n, shift, nbits, hist, nb, ko, vo);
+    AXSQ_CHECK_HIP(hipGetLastError());
+    uint32_t* t = ki;
+    ki = ko;
+    ko = t;
+    t = vi;
+    vi = vo;
+    vo = t;
+  }
+  *keys_out = ki;
+  *vals_out = vi;
+  return AXSQ_OK;
+}
+
+}  // namespace axsq

@@ -1,0 +1,165 @@
+// axsq_query.hip -- BVH candidate search (findPoints / findBoundingBoxes / findRays).
+//
+// Reference: LinearBVH::findCandidatesImpl (src/axom/spin/policy/LinearBVH.hpp:273-402),
+// GPU form: count pass, exclusive scan, fill pass (:296-367). One lane per
+// query; candidates of query i are written in traversal order at offsets[i].
+//
+// FP32 traversal: nodes hold FP32 boxes rounded outward, so every predicate
+// evaluated on them passes whenever it passes on the FP64 box (points and
+// boxes as FP32 intervals of the exact query; rays with the FP64 slab test,
+// which is monotone under box growth). Each reached leaf is confirmed on its
+// FP64 box, so the candidate lists are identical to the FP64 traversal's,
+// order included.
+#include "axsq_internal.hpp"
+#include "axsq_traverse.hpp"
+
+namespace axsq {
+
+namespace {
+
+struct FindArgs {
+  const Node64* nodes;
+  const Node32* nodes32;
+  const double* leaf_boxes;
+  const int32_t* leaf_ids;
+  const double* x;
+  const double* y;
+  const double* z;
+  int64_t stride;
+  int64_t nq;
+  int32_t ninner;
+  double tol;
+  int32_t* counts;
+  const int32_t* offsets;
+  int32_t* cands;
+  int32_t* status;
+};
+
+// Predicates of BVH.hpp:499-501 (contains), :558-560 (intersectsWith),
+// :529-532 (intersect_ray with the BVH tolerance).
+template <int KIND>
+__device__ __forceinline__ bool pred64(const double* prim, double tol, const double b[6]) {
+  if (KIND == AXSQ_QUERY_POINTS) {
+    return !(prim[0] < b[0] || prim[0] > b[3] || prim[1] < b[1] || prim[1] > b[4] ||
+             prim[2] < b[2] || prim[2] > b[5]);
+  } else if (KIND == AXSQ_QUERY_BOXES) {
+    return (prim[3] >= b[0] && prim[0] <= b[3]) && (prim[4] >= b[1] && prim[1] <= b[4]) &&
+           (prim[5] >= b[2] && prim[2] <= b[5]);
+  } else {
+    return ray_box(prim, b, tol);
+  }
+}
+
+// Superset predicates on outward-rounded FP32 boxes. lo/hi bracket the query
+// (points: the point; boxes: min rounded down / max rounded up).
+template <int KIND>
+__device__ __forceinline__ bool pred32(const double* prim, const float* lo, const float* hi,
+                                       double tol, const float b[6]) {
+  if (KIND == AXSQ_QUERY_RAYS) {
+    const double bd[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
+    return ray_box(prim, bd, tol);
+  } else {
+    return (hi[0] >= b[0] && lo[0] <= b[3]) && (hi[1] >= b[1] && lo[1] <= b[4]) &&
+           (hi[2] >= b[2] && lo[2] <= b[5]);
+  }
+}
+
+template <int KIND, bool FILL, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nq) return;
+  double prim[6];
+  if (KIND == AXSQ_QUERY_POINTS) {
+    prim[0] = a.x[i * a.stride];
+    prim[1] = a.y[i * a.stride];
+    prim[2] = a.z[i * a.stride];
+    prim[3] = prim[4] = prim[5] = 0.0;
+  } else {
+    for (int k = 0; k < 6; ++k) prim[k] = a.x[i * a.stride + k];
+  }
+  int32_t count = 0;
+  int32_t offset = FILL ? a.offsets[i] : 0;
+  auto emit = [&](int32_t pos) {
+    if (FILL) a.cands[offset++] = a.leaf_ids[pos];
+    else ++count;
+  };
+  auto no_pref = [](const auto*, const auto*) { return false; };
+  int32_t v;
+  if (!MIXED) {
+    v = traverse<Node64, double>(
+        a.nodes, a.ninner, [&](const double* b) { return pred64<KIND>(prim, a.tol, b); }, no_pref, emit);
+  } else {
+    float lo[3], hi[3];
+    const int hoff = (KIND == AXSQ_QUERY_BOXES) ? 3 : 0;
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = __double2float_rd(prim[d]);
+      hi[d] = __double2float_ru(prim[hoff + d]);
+    }
+    v = traverse<Node32, float>(
+        a.nodes32, a.ninner, [&](const float* b) { return pred32<KIND>(prim, lo, hi, a.tol, b); }, no_pref,
+        [&](int32_t pos) {
+          double lb[6];
+          const double2* p = reinterpret_cast<const double2*>(a.leaf_boxes + 6 * (int64_t)pos);
+          const double2 c0 = p[0], c1 = p[1], c2 = p[2];
+          lb[0] = c0.x; lb[1] = c0.y; lb[2] = c1.x; lb[3] = c1.y; lb[4] = c2.x; lb[5] = c2.y;
+          if (valid6(lb) && pred64<KIND>(prim, a.tol, lb)) emit(pos);
+        });
+  }
+  if (v < 0) atomicOr(a.status, 1);
+  if (!FILL) a.counts[i] = count;
+}
+
+template <bool FILL, bool MIXED>
+hipError_t launch_kind(int kind, const FindArgs& a, hipStream_t s) {
+  const unsigned g = grid_for(a.nq);
+  switch (kind) {
+    case AXSQ_QUERY_POINTS:
+      k_find<AXSQ_QUERY_POINTS, FILL, MIXED><<<g, kBlock, 0, s>>>(a);
+      break;
+    case AXSQ_QUERY_BOXES:
+      k_find<AXSQ_QUERY_BOXES, FILL, MIXED><<<g, kBlock, 0, s>>>(a);
+      break;
+    default:
+      k_find<AXSQ_QUERY_RAYS, FILL, MIXED><<<g, kBlock, 0, s>>>(a);
+      break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int find_launch(const axsq_bvh* b, int kind, bool fill, const double* x, const double* y,
+                const double* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
+                const int32_t* offsets, int32_t* cands, hipStream_t s) {
+  FindArgs a;
+  a.nodes = b->nodes;
+  a.nodes32 = b->nodes32;
+  a.leaf_boxes = b->leaf_boxes;
+  a.leaf_ids = b->leaf_ids;
+  a.x = x;
+  a.y = y;
+  a.z = z;
+  a.stride = stride;
+  a.nq = nq;
+  a.ninner = (int32_t)(b->n - 1);
+  a.tol = tol;
+  a.counts = counts;
+  a.offsets = offsets;
+  a.cands = cands;
+  a.status = b->status;
+  const bool mixed = (b->fp_mode == AXSQ_FP32_TRAVERSAL) && b->nodes32;
+  ProfRange pr(fill ? "bvh_find_fill" : "bvh_find_count", s);
+  hipError_t e;
+  if (fill)
+    e = mixed ? launch_kind<true, true>(kind, a, s) : launch_kind<true, false>(kind, a, s);
+  else
+    e = mixed ? launch_kind<false, true>(kind, a, s) : launch_kind<false, false>(kind, a, s);
+  pr.stop();
+  if (e != hipSuccess) {
+    set_error("find kernel launch failed: %s", hipGetErrorString(e));
+    return AXSQ_ERR_HIP;
+  }
+  return AXSQ_OK;
+}
+
+}  // namespace axsq

@@ -1,0 +1,688 @@
+// axsq_sdf.hip -- signed distance to a triangle surface on gfx950.
+//
+// Forward: SignedDistance::computeDistances (src/axom/quest/SignedDistance.hpp:529-605)
+// -- one lane per query runs the nearer-first traversal with the stateful
+// pruning predicate squared_distance(p, box) <= minSqDist, checks each leaf
+// triangle (checkCandidate :637-737), then decides the sign (:580-592,
+// :741-763). Queries are processed in Morton order of their position (a
+// permutation applied on input and undone on output) so neighbouring lanes
+// walk neighbouring parts of the tree.
+//
+// FP32 traversal: node boxes are FP32, rounded outward. The box test is
+// sum_d gap_d^2 <= t, with gap_d taken against the FP32 interval
+// [rd(q_d), ru(q_d)] of the exact query and
+// t = ru(minSqDist * (1 + 1e-6) + 1e-36). Each FP32 operation in
+// round-to-nearest is off by at most 2^-24 relative (plus a subnormal
+// absolute error far below 1e-36); five of them stay far below 1e-6, and
+// the FP32 box contains the FP64 box, so every node the FP64 predicate keeps
+// is also kept: the nearest triangle is always checked. Triangle tests, the
+// closest point, the pseudo-normal and the sign are FP64.
+//
+// Adjoint (no reference counterpart): see axsq_sdf_backward in axsq.h.
+#include "axsq_internal.hpp"
+#include "axsq_traverse.hpp"
+
+namespace axsq {
+
+namespace {
+
+struct SdfArgs {
+  const Node64* nodes;
+  const Node32* nodes32;
+  const TriRec* tris;
+  const double* qx;
+  const double* qy;
+  const double* qz;
+  int64_t qs;
+  int64_t nq;
+  int32_t ninner;
+  const uint32_t* order;
+  double dom[6];
+  int watertight;
+  int compute_sign;
+  double* dist;
+  double* cp;
+  double* normal;
+  int32_t* elem;
+  int32_t* status;
+  int64_t* stats;
+};
+
+__device__ __forceinline__ void load_tri(const TriRec* __restrict__ tris, int32_t pos, double T[9]) {
+  const double2* p = reinterpret_cast<const double2*>(tris + pos);
+  const double2 a = p[0], b = p[1], c = p[2], d = p[3];
+  T[0] = a.x; T[1] = a.y; T[2] = b.x; T[3] = b.y; T[4] = c.x; T[5] = c.y;
+  T[6] = d.x; T[7] = d.y;
+  T[8] = reinterpret_cast<const double*>(p)[8];
+}
+
+// MinCandidate (SignedDistance.hpp:159-175) plus the leaf position of the winner.
+struct MinCand {
+  double sq;
+  double p[3];
+  double sn[3];
+  int type;  // -1 unset, 0 vertex, 1 edge, 2 face (getClosestPointLocType :119-137)
+  int32_t pos;
+};
+
+// checkCandidate, SignedDistance.hpp:637-737 (triangle cells)
+__device__ __forceinline__ void check_candidate(MinCand& m, const double q[3], const double T[9],
+                                                int32_t pos, bool compute_normal) {
+  constexpr double EPS = 1e-12;
+  double cp[3], wdummy[3];
+  const int loc = closest_point<false>(q, T, EPS, cp, wdummy);
+  const double sq = dot3(cp[0] - q[0], cp[1] - q[1], cp[2] - q[2], cp[0] - q[0], cp[1] - q[1],
+                         cp[2] - q[2]);
+  const int type = (loc < 0) ? 1 : (loc < 3) ? 0 : 2;
+  const bool shared = (type != 2);
+  bool upd;
+  if (sq < m.sq) {
+    const double e0 = m.p[0] - cp[0], e1 = m.p[1] - cp[1], e2 = m.p[2] - cp[2];
+    const bool clear = !shared || (m.type != type) || !near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., EPS);
+    m.sq = sq;
+    m.p[0] = cp[0];
+    m.p[1] = cp[1];
+    m.p[2] = cp[2];
+    m.type = type;
+    m.pos = pos;
+    if (compute_normal && clear) {
+      m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
+    }
+    upd = compute_normal && shared;
+  } else {
+    const double e0 = m.p[0] - cp[0], e1 = m.p[1] - cp[1], e2 = m.p[2] - cp[2];
+    upd = compute_normal && shared && (m.type == type) &&
+          near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., EPS);
+  }
+  if (upd) {
+    double n[3], un[3];
+    tri_normal(T, n);
+    if (type == 1) {
+      unit3(n, un);
+      m.sn[0] += un[0];
+      m.sn[1] += un[1];
+      m.sn[2] += un[2];
+    } else {
+      // Triangle::degenerate (Triangle.hpp:327-330): area = 0.5*|n|
+      const double area = 0.5 * sqrt(dot3(n[0], n[1], n[2], n[0], n[1], n[2]));
+      if (!near_eq(area, 0.0, 1.0e-12)) {
+        const double alpha = tri_angle(T, loc);
+        unit3(n, un);
+        m.sn[0] += un[0] * alpha;
+        m.sn[1] += un[1] * alpha;
+        m.sn[2] += un[2] * alpha;
+      }
+    }
+  }
+}
+
+template <bool MIXED, bool STATS>
+__global__ __launch_bounds__(kBlock) void k_sdf_eval(SdfArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nq) return;
+  const int64_t i = a.order ? (int64_t)a.order[t] : t;
+  const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
+  MinCand m;
+  m.sq = kDblMax;
+  m.p[0] = m.p[1] = m.p[2] = 0.0;
+  m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
+  m.type = -1;
+  m.pos = 0;
+  const bool cn = a.compute_sign != 0;
+  int32_t leaves = 0;
+  auto leaf = [&](int32_t pos) {
+    double T[9];
+    load_tri(a.tris, pos, T);
+    check_candidate(m, q, T, pos, cn);
+    if (STATS) ++leaves;
+  };
+  int32_t v;
+  if (!MIXED) {
+    // LinearBVH.hpp:75-82: nearer child (by centroid distance) first
+    auto pref = [&](const double* l, const double* r) {
+      const double cl0 = 0.5 * (l[0] + l[3]), cl1 = 0.5 * (l[1] + l[4]), cl2 = 0.5 * (l[2] + l[5]);
+      const double cr0 = 0.5 * (r[0] + r[3]), cr1 = 0.5 * (r[1] + r[4]), cr2 = 0.5 * (r[2] + r[5]);
+      const double dl = dot3(cl0 - q[0], cl1 - q[1], cl2 - q[2], cl0 - q[0], cl1 - q[1], cl2 - q[2]);
+      const double dr = dot3(cr0 - q[0], cr1 - q[1], cr2 - q[2], cr0 - q[0], cr1 - q[1], cr2 - q[2]);
+      return dl > dr;
+    };
+    v = traverse<Node64, double>(
+        a.nodes, a.ninner, [&](const double* b) { return sqdist_box(q[0], q[1], q[2], b) <= m.sq; }, pref,
+        leaf);
+  } else {
+    float qlo[3], qhi[3], qn[3];
+    for (int d = 0; d < 3; ++d) {
+      qlo[d] = __double2float_rd(q[d]);
+      qhi[d] = __double2float_ru(q[d]);
+      qn[d] = (float)q[d];
+    }
+    double last_sq = -1.0;
+    float thr = 0.0f;
+    auto pred = [&](const float* b) {
+      if (m.sq != last_sq) {
+        // relative slack covers the FP32 round-to-nearest error; the absolute
+        // 1e-36 covers errors of subnormal intermediate values
+        last_sq = m.sq;
+        thr = __double2float_ru(m.sq * (1.0 + 1e-6) + 1e-36);
+      }
+      float acc = 0.0f;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float g = fmaxf(fmaxf(b[d] - qhi[d], qlo[d] - b[3 + d]), 0.0f);
+        acc += g * g;
+      }
+      return acc <= thr;
+    };
+    auto pref = [&](const float* l, const float* r) {
+      float dl = 0.0f, dr = 0.0f;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float el = 0.5f * (l[d] + l[3 + d]) - qn[d];
+        const float er = 0.5f * (r[d] + r[3 + d]) - qn[d];
+        dl += el * el;
+        dr += er * er;
+      }
+      return dl > dr;
+    };
+    v = traverse<Node32, float>(a.nodes32, a.ninner, pred, pref, leaf);
+  }
+  if (v < 0) atomicOr(a.status, 1);
+  if (STATS) {
+    atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)(v < 0 ? 0 : v));
+    atomicAdd((unsigned long long*)&a.stats[1], (unsigned long long)leaves);
+  }
+  // surface normal (getSurfaceNormal :741-749) and sign (:580-592)
+  double n[3];
+  double T[9];
+  load_tri(a.tris, m.pos, T);
+  if (m.type == 2) {
+    tri_normal(T, n);
+  } else {
+    n[0] = m.sn[0];
+    n[1] = m.sn[1];
+    n[2] = m.sn[2];
+  }
+  double sgn = 1.0;
+  if (cn) {
+    const bool inside_dom = !(m.p[0] < a.dom[0] || m.p[0] > a.dom[3] || m.p[1] < a.dom[1] ||
+                              m.p[1] > a.dom[4] || m.p[2] < a.dom[2] || m.p[2] > a.dom[5]);
+    if (!(a.watertight && !inside_dom)) {
+      const double dp = dot3(q[0] - m.p[0], q[1] - m.p[1], q[2] - m.p[2], n[0], n[1], n[2]);
+      sgn = (dp >= 0.0) ? 1.0 : -1.0;
+    }
+  }
+  a.dist[i] = sqrt(m.sq) * sgn;
+  if (a.cp) {
+    a.cp[3 * i] = m.p[0];
+    a.cp[3 * i + 1] = m.p[1];
+    a.cp[3 * i + 2] = m.p[2];
+  }
+  if (a.normal) {
+    double un[3];
+    unit3(n, un);
+    a.normal[3 * i] = un[0];
+    a.normal[3 * i + 1] = un[1];
+    a.normal[3 * i + 2] = un[2];
+  }
+  if (a.elem) a.elem[i] = reinterpret_cast<const int32_t*>(a.tris + m.pos)[18];
+}
+
+// Morton code of each query in the mesh's bounding box (ordering only).
+__global__ __launch_bounds__(kBlock) void k_query_codes(const double* __restrict__ qx,
+                                                         const double* __restrict__ qy,
+                                                         const double* __restrict__ qz,
+                                                         int64_t qs, int64_t nq, double lo0,
+                                                         double lo1, double lo2, double s0,
+                                                         double s1, double s2,
+                                                         uint32_t* __restrict__ keys,
+                                                         uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const double c[3] = {qx[i * qs], qy[i * qs], qz[i * qs]};
+  const double minc[3] = {lo0, lo1, lo2};
+  const double inv[3] = {s0, s1, s2};
+  keys[i] = morton_of(c, minc, inv);
+  vals[i] = (uint32_t)i;
+}
+
+// ---- mesh set-up -------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_vert_bounds(const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ z, int64_t n,
+                                                         double* __restrict__ partials) {
+  __shared__ double s_red[6][kBlock];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[6] = {kDblMax, kDblMax, kDblMax, -kDblMax, -kDblMax, -kDblMax};
+  if (i < n) {
+    acc[0] = acc[3] = x[i];
+    acc[1] = acc[4] = y[i];
+    acc[2] = acc[5] = z[i];
+  }
+  for (int k = 0; k < 6; ++k) s_red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      for (int k = 0; k < 3; ++k) {
+        const double a = s_red[k][threadIdx.x], b = s_red[k][threadIdx.x + off];
+        s_red[k][threadIdx.x] = (b < a) ? b : a;
+        const double c = s_red[3 + k][threadIdx.x], d = s_red[3 + k][threadIdx.x + off];
+        s_red[3 + k][threadIdx.x] = (d > c) ? d : c;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; ++k) partials[6 * (int64_t)blockIdx.x + k] = s_red[k][0];
+}
+
+__global__ __launch_bounds__(1024) void k_final_bounds(const double* __restrict__ partials,
+                                                        int64_t np, double* __restrict__ out) {
+  __shared__ double s_red[6][1024];
+  double acc[6] = {kDblMax, kDblMax, kDblMax, -kDblMax, -kDblMax, -kDblMax};
+  for (int64_t p = threadIdx.x; p < np; p += 1024)
+    for (int k = 0; k < 3; ++k) {
+      const double a = partials[6 * p + k];
+      if (a < acc[k]) acc[k] = a;
+      const double b = partials[6 * p + 3 + k];
+      if (b > acc[3 + k]) acc[3 + k] = b;
+    }
+  for (int k = 0; k < 6; ++k) s_red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int off = 512; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int k = 0; k < 3; ++k) {
+        const double a = s_red[k][threadIdx.x], b = s_red[k][threadIdx.x + off];
+        s_red[k][threadIdx.x] = (b < a) ? b : a;
+        const double c = s_red[3 + k][threadIdx.x], d = s_red[3 + k][threadIdx.x + off];
+        s_red[3 + k][threadIdx.x] = (d > c) ? d : c;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 6; ++k) out[k] = s_red[k][0];
+}
+
+// getCellBoundingBox (SignedDistance.hpp:609-633): addPoint of the 3 vertices.
+__global__ __launch_bounds__(kBlock) void k_tri_boxes(const double* __restrict__ x,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ z,
+                                                       const int32_t* __restrict__ conn,
+                                                       int64_t nt, double* __restrict__ boxes) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nt) return;
+  double b[6] = {kDblMax, kDblMax, kDblMax, -kDblMax, -kDblMax, -kDblMax};
+  for (int k = 0; k < 3; ++k) {
+    const int32_t v = conn[3 * c + k];
+    const double p[3] = {x[v], y[v], z[v]};
+    for (int d = 0; d < 3; ++d) {
+      if (p[d] < b[d]) b[d] = p[d];
+      if (p[d] > b[3 + d]) b[3 + d] = p[d];
+    }
+  }
+  for (int k = 0; k < 6; ++k) boxes[6 * c + k] = b[k];
+}
+
+// One record per BVH leaf (nleaves >= nt: a single triangle is padded with an
+// invalid leaf that traversal never reaches; its record is zero).
+__global__ __launch_bounds__(kBlock) void k_gather_tris(const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ z,
+                                                         const int32_t* __restrict__ conn,
+                                                         const int32_t* __restrict__ leaf_ids,
+                                                         int64_t nt, int64_t nleaves,
+                                                         TriRec* __restrict__ tris) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nleaves) return;
+  const int32_t c = leaf_ids[p];
+  TriRec r;
+  if (c < 0 || c >= nt) {
+    for (int k = 0; k < 9; ++k) r.v[k] = 0.0;
+    r.id = -1;
+    r.pad = 0;
+    tris[p] = r;
+    return;
+  }
+  for (int k = 0; k < 3; ++k) {
+    const int32_t v = conn[3 * (int64_t)c + k];
+    r.v[3 * k] = x[v];
+    r.v[3 * k + 1] = y[v];
+    r.v[3 * k + 2] = z[v];
+  }
+  r.id = c;
+  r.pad = 0;
+  tris[p] = r;
+}
+
+// boundaries of sorted keys: start[key] = first index, end[key] = one past
+// last. A key >= nseg is not written and raises bit 2 of *status.
+__global__ __launch_bounds__(kBlock) void k_segments(const uint32_t* __restrict__ keys, int64_t n,
+                                                      int64_t nseg, int32_t* __restrict__ start,
+                                                      int32_t* __restrict__ end,
+                                                      int32_t* __restrict__ status) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t k = keys[p];
+  if ((int64_t)k >= nseg) {
+    atomicOr(status, 2);
+    return;
+  }
+  if (p == 0 || keys[p - 1] != k) start[k] = (int32_t)p;
+  if (p == n - 1 || keys[p + 1] != k) end[k] = (int32_t)(p + 1);
+}
+
+// Every connectivity entry must name a vertex; bit 4 of *status otherwise.
+__global__ __launch_bounds__(kBlock) void k_check_conn(const int32_t* __restrict__ conn, int64_t m,
+                                                        int64_t nv, int32_t* __restrict__ status) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m) return;
+  const int32_t v = conn[s];
+  if (v < 0 || (int64_t)v >= nv) atomicOr(status, 4);
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota_conn(const int32_t* __restrict__ conn, int64_t m,
+                                                       uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m) return;
+  keys[s] = (uint32_t)conn[s];
+  vals[s] = (uint32_t)s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_u32(const uint32_t* __restrict__ in, int64_t n,
+                                                      int32_t* __restrict__ out) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) out[s] = (int32_t)in[s];
+}
+
+// ---- adjoint -----------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_elem_keys(const int32_t* __restrict__ elem, int64_t nq,
+                                                       uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  keys[i] = (uint32_t)elem[i];
+  vals[i] = (uint32_t)i;
+}
+
+// One lane per triangle: its queries in ascending index order. Writes the
+// query gradients and the per-corner partial sums (9 doubles per triangle).
+__global__ __launch_bounds__(kBlock) void k_bwd_tri(
+    const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
+    const int32_t* __restrict__ conn, int64_t nt, const int32_t* __restrict__ seg_start,
+    const int32_t* __restrict__ seg_end, const uint32_t* __restrict__ qidx,
+    const double* __restrict__ qx, const double* __restrict__ qy, const double* __restrict__ qz,
+    int64_t qs, const double* __restrict__ dist, const float* __restrict__ gout,
+    double* __restrict__ grad_q, double* __restrict__ part) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nt) return;
+  double T[9];
+  for (int k = 0; k < 3; ++k) {
+    const int32_t v = conn[3 * c + k];
+    T[3 * k] = vx[v];
+    T[3 * k + 1] = vy[v];
+    T[3 * k + 2] = vz[v];
+  }
+  double acc[9];
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  const int32_t b = seg_start[c], e = seg_end[c];
+  if (b < e) {
+    double nrm[3], un[3];
+    tri_normal(T, nrm);
+    unit3(nrm, un);
+    for (int32_t p = b; p < e; ++p) {
+      const int64_t i = (int64_t)qidx[p];
+      const double P[3] = {qx[i * qs], qy[i * qs], qz[i * qs]};
+      double cp[3], w[3];
+      closest_point<true>(P, T, 1e-12, cp, w);
+      const double sgn = signbit(dist[i]) ? -1.0 : 1.0;
+      const double r0 = P[0] - cp[0], r1 = P[1] - cp[1], r2 = P[2] - cp[2];
+      const double len = sqrt(dot3(r0, r1, r2, r0, r1, r2));
+      double u0, u1, u2;
+      if (len > 0.0) {
+        const double inv = 1.0 / len;
+        u0 = r0 * inv;
+        u1 = r1 * inv;
+        u2 = r2 * inv;
+      } else {
+        u0 = un[0];
+        u1 = un[1];
+        u2 = un[2];
+      }
+      const double g = (double)gout[i] * sgn;
+      const double d0 = g * u0, d1 = g * u1, d2 = g * u2;
+      if (grad_q) {
+        grad_q[3 * i] = d0;
+        grad_q[3 * i + 1] = d1;
+        grad_q[3 * i + 2] = d2;
+      }
+      for (int k = 0; k < 3; ++k) {
+        acc[3 * k] += -(w[k] * d0);
+        acc[3 * k + 1] += -(w[k] * d1);
+        acc[3 * k + 2] += -(w[k] * d2);
+      }
+    }
+  }
+  for (int k = 0; k < 9; ++k) part[9 * c + k] = acc[k];
+}
+
+// One lane per vertex: sum of its incident corner partials, ascending triangle.
+__global__ __launch_bounds__(kBlock) void k_bwd_vert(const int32_t* __restrict__ vstart,
+                                                      const int32_t* __restrict__ vend,
+                                                      const int32_t* __restrict__ slots,
+                                                      int64_t nv, const double* __restrict__ part,
+                                                      double* __restrict__ grad_v) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  const int32_t b = vstart[v], e = vend[v];
+  for (int32_t k = b; k < e; ++k) {
+    const int64_t s = slots[k];  // 3*tri + corner
+    const double* p = part + 3 * s;
+    a0 += p[0];
+    a1 += p[1];
+    a2 += p[2];
+  }
+  grad_v[3 * v] = a0;
+  grad_v[3 * v + 1] = a1;
+  grad_v[3 * v + 2] = a2;
+}
+
+template <class T>
+int dmalloc(T** p, size_t count) {
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
+  if (e != hipSuccess) {
+    set_error("hipMalloc of %zu bytes failed: %s", sizeof(T) * count, hipGetErrorString(e));
+    return AXSQ_ERR_NOMEM;
+  }
+  return AXSQ_OK;
+}
+
+}  // namespace
+
+// Rejects connectivity that names a vertex outside [0, nverts).
+int sdf_check_conn(const axsq_sdf* h, hipStream_t s) {
+  int32_t* flag = nullptr;
+  AXSQ_TRY(dmalloc(&flag, 1));
+  int rc = fill_i32(flag, 0, 1, s);
+  if (rc == AXSQ_OK) {
+    k_check_conn<<<grid_for(3 * h->ntris), kBlock, 0, s>>>(h->conn, 3 * h->ntris, h->nverts, flag);
+    int32_t hf = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&hf, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error("connectivity check failed to run");
+      rc = AXSQ_ERR_HIP;
+    } else if (hf) {
+      set_error("triangle connectivity names a vertex outside [0, %lld)", (long long)h->nverts);
+      rc = AXSQ_ERR_INVALID_ARG;
+    }
+  }
+  (void)hipFree(flag);
+  return rc;
+}
+
+// Mesh bounds, triangle boxes, BVH, leaf-ordered triangles.
+int sdf_setup(axsq_sdf* h, hipStream_t s) {
+  Workspace& ws = workspace();
+  const int64_t nv = h->nverts, nt = h->ntris;
+  const int64_t np = grid_for(nv);
+  AXSQ_TRY(ws.tiles.ensure(sizeof(double) * 6 * (size_t)(np + 1)));
+  double* part = ws.tiles.as<double>();
+  k_vert_bounds<<<(unsigned)np, kBlock, 0, s>>>(h->vx, h->vy, h->vz, nv, part);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  k_final_bounds<<<1, 1024, 0, s>>>(part, np, part + 6 * np);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  AXSQ_CHECK_HIP(hipMemcpyAsync(h->domain, part + 6 * np, 6 * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+  AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+  // triangle boxes live in a temporary allocation: the BVH build uses the
+  // whole workspace for its own scratch
+  double* boxes = nullptr;
+  AXSQ_TRY(dmalloc(&boxes, 6 * (size_t)nt));
+  k_tri_boxes<<<grid_for(nt), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, nt, boxes);
+  hipError_t e = hipGetLastError();
+  int rc = (e == hipSuccess) ? AXSQ_OK : AXSQ_ERR_HIP;
+  if (rc == AXSQ_OK) rc = bvh_build_into(h->bvh, boxes, nt, 1.000123, h->fp_mode, s);
+  hipFree(boxes);
+  if (e != hipSuccess) set_error("k_tri_boxes: %s", hipGetErrorString(e));
+  AXSQ_TRY(rc);
+  const int64_t nl = h->bvh->n;
+  if (!h->tris) AXSQ_TRY(dmalloc(&h->tris, (size_t)nl));
+  k_gather_tris<<<grid_for(nl), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, h->bvh->leaf_ids,
+                                                nt, nl, h->tris);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+  return AXSQ_OK;
+}
+
+// vertex -> incident (3*tri + corner) slots, ascending, by a stable sort of
+// the connectivity.
+int sdf_build_v2t(axsq_sdf* h, hipStream_t s) {
+  Workspace& ws = workspace();
+  const int64_t m = 3 * h->ntris;
+  AXSQ_TRY(ws.k0.ensure(sizeof(uint32_t) * m));
+  AXSQ_TRY(ws.v0.ensure(sizeof(uint32_t) * m));
+  AXSQ_TRY(ws.k1.ensure(sizeof(uint32_t) * m));
+  AXSQ_TRY(ws.v1.ensure(sizeof(uint32_t) * m));
+  if (!h->v2t_slot) {
+    AXSQ_TRY(dmalloc(&h->v2t_slot, (size_t)m));
+    AXSQ_TRY(dmalloc(&h->v2t_start, (size_t)h->nverts));
+    AXSQ_TRY(dmalloc(&h->v2t_end, (size_t)h->nverts));
+  }
+  k_iota_conn<<<grid_for(m), kBlock, 0, s>>>(h->conn, m, ws.k0.as<uint32_t>(),
+                                             ws.v0.as<uint32_t>());
+  AXSQ_CHECK_HIP(hipGetLastError());
+  uint32_t *sk, *sv;
+  AXSQ_TRY(radix_sort_pairs(ws.k0.as<uint32_t>(), ws.v0.as<uint32_t>(), ws.k1.as<uint32_t>(),
+                            ws.v1.as<uint32_t>(), m, bits_for((uint64_t)h->nverts), ws, s, &sk,
+                            &sv));
+  AXSQ_TRY(fill_i32(h->v2t_start, 0, h->nverts, s));
+  AXSQ_TRY(fill_i32(h->v2t_end, 0, h->nverts, s));
+  k_segments<<<grid_for(m), kBlock, 0, s>>>(sk, m, h->nverts, h->v2t_start, h->v2t_end,
+                                          h->bvh->status);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  k_copy_u32<<<grid_for(m), kBlock, 0, s>>>(sv, m, h->v2t_slot);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+  return AXSQ_OK;
+}
+
+int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const double* z,
+                    int64_t stride, int64_t nq, int fp_mode, double* dist, double* cp,
+                    double* normal, int32_t* elem, bool stats, hipStream_t s) {
+  Workspace& ws = workspace();
+  const axsq_bvh* b = h->bvh;
+  SdfArgs a;
+  a.nodes = b->nodes;
+  a.nodes32 = b->nodes32;
+  a.tris = h->tris;
+  a.qx = x;
+  a.qy = y;
+  a.qz = z;
+  a.qs = stride;
+  a.nq = nq;
+  a.ninner = (int32_t)(b->n - 1);
+  a.order = nullptr;
+  for (int k = 0; k < 6; ++k) a.dom[k] = h->domain[k];
+  a.watertight = h->watertight;
+  a.compute_sign = h->compute_sign;
+  a.dist = dist;
+  a.cp = cp;
+  a.normal = normal;
+  a.elem = elem;
+  a.status = b->status;
+  a.stats = h->stats_dev;
+  if (h->sort_queries && nq > 4096) {
+    AXSQ_TRY(ws.k0.ensure(sizeof(uint32_t) * nq));
+    AXSQ_TRY(ws.v0.ensure(sizeof(uint32_t) * nq));
+    AXSQ_TRY(ws.k1.ensure(sizeof(uint32_t) * nq));
+    AXSQ_TRY(ws.v1.ensure(sizeof(uint32_t) * nq));
+    double inv[3];
+    for (int d = 0; d < 3; ++d) {
+      const double ext = h->domain[3 + d] - h->domain[d];
+      inv[d] = ext > 0.0 ? 1.0 / ext : 0.0;
+    }
+    ProfRange pq("sdf_query_order", s);
+    k_query_codes<<<grid_for(nq), kBlock, 0, s>>>(x, y, z, stride, nq, h->domain[0],
+                                                  h->domain[1], h->domain[2], inv[0], inv[1],
+                                                  inv[2], ws.k0.as<uint32_t>(),
+                                                  ws.v0.as<uint32_t>());
+    AXSQ_CHECK_HIP(hipGetLastError());
+    uint32_t *sk, *sv;
+    AXSQ_TRY(radix_sort_pairs(ws.k0.as<uint32_t>(), ws.v0.as<uint32_t>(), ws.k1.as<uint32_t>(),
+                              ws.v1.as<uint32_t>(), nq, 30, ws, s, &sk, &sv));
+    pq.stop();
+    a.order = sv;
+  }
+  const bool mixed = (fp_mode == AXSQ_FP32_TRAVERSAL);
+  ProfRange pr(mixed ? "sdf_eval_fp32" : "sdf_eval_fp64", s);
+  if (mixed) {
+    k_sdf_eval<true, false><<<grid_for(nq), kBlock, 0, s>>>(a);
+  } else if (stats) {
+    k_sdf_eval<false, true><<<grid_for(nq), kBlock, 0, s>>>(a);
+  } else {
+    k_sdf_eval<false, false><<<grid_for(nq), kBlock, 0, s>>>(a);
+  }
+  pr.stop();
+  AXSQ_CHECK_HIP(hipGetLastError());
+  return AXSQ_OK;
+}
+
+int sdf_backward_launch(const axsq_sdf* h, const double* x, const double* y, const double* z,
+                        int64_t stride, int64_t nq, const double* dist, const int32_t* elem,
+                        const float* gout, double* grad_q, double* grad_v, hipStream_t s) {
+  Workspace& ws = workspace();
+  const int64_t nt = h->ntris;
+  AXSQ_TRY(ws.k0.ensure(sizeof(uint32_t) * nq));
+  AXSQ_TRY(ws.v0.ensure(sizeof(uint32_t) * nq));
+  AXSQ_TRY(ws.k1.ensure(sizeof(uint32_t) * nq));
+  AXSQ_TRY(ws.v1.ensure(sizeof(uint32_t) * nq));
+  // triangle segments (2*nt int32) and partials (9*nt doubles)
+  const size_t seg_bytes = (sizeof(int32_t) * 2 * nt + 255) & ~size_t(255);
+  AXSQ_TRY(ws.flag.ensure(seg_bytes + sizeof(double) * 9 * nt));
+  int32_t* seg_start = ws.flag.as<int32_t>();
+  int32_t* seg_end = seg_start + nt;
+  double* part = reinterpret_cast<double*>(ws.flag.as<char>() + seg_bytes);
+  ProfRange pr("sdf_backward", s);
+  k_elem_keys<<<grid_for(nq), kBlock, 0, s>>>(elem, nq, ws.k0.as<uint32_t>(),
+                                              ws.v0.as<uint32_t>());
+  AXSQ_CHECK_HIP(hipGetLastError());
+  uint32_t *sk, *sv;
+  AXSQ_TRY(radix_sort_pairs(ws.k0.as<uint32_t>(), ws.v0.as<uint32_t>(), ws.k1.as<uint32_t>(),
+                            ws.v1.as<uint32_t>(), nq, bits_for((uint64_t)nt), ws, s, &sk, &sv));
+  AXSQ_TRY(fill_i32(seg_start, 0, 2 * nt, s));
+  k_segments<<<grid_for(nq), kBlock, 0, s>>>(sk, nq, nt, seg_start, seg_end, h->bvh->status);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  k_bwd_tri<<<grid_for(nt), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, nt, seg_start, seg_end,
+                                            sv, x, y, z, stride, dist, gout, grad_q, part);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  if (grad_v) {
+    k_bwd_vert<<<grid_for(h->nverts), kBlock, 0, s>>>(h->v2t_start, h->v2t_end, h->v2t_slot,
+                                                      h->nverts, part, grad_v);
+    AXSQ_CHECK_HIP(hipGetLastError());
+  }
+  pr.stop();
+  return AXSQ_OK;
+}
+
+}  // namespace axsq

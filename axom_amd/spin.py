@@ -1,0 +1,179 @@
+"""Host mirror of axom::spin::BVH<3, MI355X_EXEC, double> over libaxsq.
+
+Method names, argument meaning and defaults follow src/axom/spin/BVH.hpp:129-419
+so that code and tests written against the reference read the same. Data
+lives in torch CUDA tensors (device memory); the compute is the HIP library.
+
+Layouts: boxes are (n, 6) float64 {min xyz, max xyz} (BoundingBox<double,3>);
+points are (n, 3) float64 (Point<double,3>) or an (x, y, z) tuple of (n,)
+float64 tensors (ZipPoint); rays are (n, 6) float64 {origin, unit direction}
+as built by make_rays (Ray's constructor normalises, Ray.hpp:122-128).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+BVH_BUILD_OK = 0      # spin/BVH.hpp:39-43
+BVH_BUILD_FAILED = -1
+
+FP_MODES = {"fp64": L.AXSQ_FP64, "fp32": L.AXSQ_FP32_TRAVERSAL,
+            L.AXSQ_FP64: L.AXSQ_FP64, L.AXSQ_FP32_TRAVERSAL: L.AXSQ_FP32_TRAVERSAL}
+
+
+def stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev(t, dtype, name, cols=None):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if cols is not None and (t.dim() != 2 or t.shape[1] != cols):
+        raise ValueError(f"{name} must have shape (n, {cols})")
+    return t.contiguous()
+
+
+def point_args(points):
+    """(x, y, z, stride, n, keepalive) for AoS (n,3) or SoA (x,y,z) points."""
+    if isinstance(points, (tuple, list)):
+        x, y, z = (_dev(c, torch.float64, "points") for c in points)
+        if not (x.shape == y.shape == z.shape) or x.dim() != 1:
+            raise ValueError("SoA points need three (n,) tensors of equal length")
+        return _ptr(x), _ptr(y), _ptr(z), 1, x.shape[0], (x, y, z)
+    p = _dev(points, torch.float64, "points", 3)
+    base = p.data_ptr()
+    return (ctypes.c_void_p(base), ctypes.c_void_p(base + 8), ctypes.c_void_p(base + 16), 3,
+            p.shape[0], (p,))
+
+
+def make_rays(origins, directions):
+    """Rays with directions normalised as primal::Ray does (Vector::unitVector,
+    Vector.hpp:477-490: multiply by 1/sqrt of ((0 + x*x) + y*y) + z*z)."""
+    o = _dev(origins, torch.float64, "origins", 3)
+    d = _dev(directions, torch.float64, "directions", 3)
+    len_sq = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    inv = 1.0 / torch.sqrt(len_sq)
+    u = d * inv[:, None]
+    tiny = len_sq < 1e-50
+    if bool(tiny.any()):
+        u[tiny] = torch.tensor([1.0, 0.0, 0.0], dtype=torch.float64, device=u.device)
+    return torch.cat([o, u], 1).contiguous()
+
+
+class BVH:
+    """axom::spin::BVH<3, MI355X_EXEC, double> (spin/BVH.hpp:129-419)."""
+
+    DEFAULT_SCALE_FACTOR = 1.000123                         # BVH.hpp:410
+    DEFAULT_TOLERANCE = float(np.finfo(np.float64).eps)     # BVH.hpp:411-412
+
+    def __init__(self, boxes=None, numItems=None, tolerance=DEFAULT_TOLERANCE,
+                 scaleFactor=DEFAULT_SCALE_FACTOR, fp_mode="fp64"):
+        self._h = None
+        self._tol = float(tolerance)
+        self._scale = float(scaleFactor)
+        self._fp = FP_MODES[fp_mode]
+        if boxes is not None:
+            self.initialize(boxes, numItems)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            L.lib().axsq_bvh_destroy(self._h)
+            self._h = None
+
+    # -- BVH.hpp:246-247, 426-477
+    def initialize(self, boxes, numItems=None):
+        b = _dev(boxes, torch.float64, "boxes", 6)
+        n = b.shape[0] if numItems is None else int(numItems)
+        if n > b.shape[0]:
+            raise ValueError("numItems exceeds the number of boxes")
+        if self._h:
+            L.lib().axsq_bvh_destroy(self._h)
+            self._h = None
+        h = ctypes.c_void_p()
+        L.check(L.lib().axsq_bvh_build(ctypes.byref(h), _ptr(b), n, self._scale, self._fp,
+                                       stream_ptr()))
+        self._h = h
+        return BVH_BUILD_OK
+
+    def isInitialized(self):
+        return bool(self._h)
+
+    def setScaleFactor(self, s):
+        self._scale = float(s)
+
+    def getScaleFactor(self):
+        return self._scale
+
+    def setTolerance(self, eps):
+        self._tol = float(eps)
+
+    def getTolerance(self):
+        return self._tol
+
+    def getBounds(self):
+        """(lo, hi) of the root box; an invalid box before initialize()."""
+        if not self._h:
+            return (np.full(3, np.finfo(np.float64).max), np.full(3, -np.finfo(np.float64).max))
+        lo, hi = L._dbl3(), L._dbl3()
+        L.check(L.lib().axsq_bvh_bounds(self._h, lo, hi))
+        return np.array(lo[:]), np.array(hi[:])
+
+    def numLeaves(self):
+        return int(L.lib().axsq_bvh_num_leaves(self._h)) if self._h else 0
+
+    def _find(self, kind, offsets, counts, n, x, y, z, stride):
+        if not self._h:
+            raise RuntimeError("BVH is not initialized")
+        off = _dev(offsets, torch.int32, "offsets")
+        cnt = _dev(counts, torch.int32, "counts")
+        # LinearBVH.hpp:284-285
+        if off.numel() != n or cnt.numel() != n:
+            raise ValueError("offsets/counts length not equal to the number of queries")
+        total = ctypes.c_int64()
+        s = stream_ptr()
+        L.check(L.lib().axsq_bvh_find_count(self._h, kind, x, y, z, stride, n, self._tol,
+                                            _ptr(off), _ptr(cnt), ctypes.byref(total), s))
+        cand = torch.empty(max(total.value, 0), dtype=torch.int32, device=off.device)
+        if total.value > 0:
+            L.check(L.lib().axsq_bvh_find_fill(self._h, kind, x, y, z, stride, n, self._tol,
+                                               _ptr(off), _ptr(cand), s))
+        if off.data_ptr() != offsets.data_ptr():
+            offsets.copy_(off)
+        if cnt.data_ptr() != counts.data_ptr():
+            counts.copy_(cnt)
+        return cand
+
+    # -- BVH.hpp:341-346, 482-505
+    def findPoints(self, offsets, counts, numPts, points):
+        x, y, z, stride, n, keep = point_args(points)
+        return self._find(L.QUERY_POINTS, offsets, counts, int(numPts), x, y, z, stride)
+
+    # -- BVH.hpp:367-372, 510-536 (rays from make_rays)
+    def findRays(self, offsets, counts, numRays, rays):
+        r = _dev(rays, torch.float64, "rays", 6)
+        return self._find(L.QUERY_RAYS, offsets, counts, int(numRays), _ptr(r), None, None, 6)
+
+    # -- BVH.hpp:393-398, 541-568
+    def findBoundingBoxes(self, offsets, counts, numBoxes, boxes):
+        b = _dev(boxes, torch.float64, "boxes", 6)
+        return self._find(L.QUERY_BOXES, offsets, counts, int(numBoxes), _ptr(b), None, None, 6)
+
+    def export(self):
+        """Tree arrays in the reference's LinearBVH layout (device tensors)."""
+        n = self.numLeaves()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        inner = torch.empty((2 * (n - 1), 6), dtype=torch.float64, device=dev)
+        children = torch.empty(2 * (n - 1), dtype=torch.int32, device=dev)
+        leafs = torch.empty(n, dtype=torch.int32, device=dev)
+        mcodes = torch.empty(n, dtype=torch.int32, device=dev)
+        L.check(L.lib().axsq_bvh_export(self._h, _ptr(inner), _ptr(children), _ptr(leafs),
+                                        _ptr(mcodes), stream_ptr()))
+        return dict(nodes=inner, children=children, leafs=leafs, mcodes=mcodes)

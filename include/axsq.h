@@ -1,0 +1,175 @@
+/*
+ * axsq.h -- C-ABI of the MI355X (gfx950) spatial-query library libaxsq.so.
+ *
+ * This is the drop-in boundary for Axom's spatial-query hot path:
+ *   spin::BVH<3, ExecSpace>::initialize / findPoints / findBoundingBoxes /
+ *   findRays / getBounds, and quest::SignedDistance<3, ExecSpace>
+ *   construction + computeDistances, plus the signed-distance adjoint.
+ *
+ * Conventions
+ *   - Every pointer to bulk data is a DEVICE pointer (hipMalloc'd or managed),
+ *     the same contract the reference states for GPU exec spaces
+ *     (src/axom/spin/BVH.hpp:198-203,238-241).
+ *   - Boxes are primal::BoundingBox<double,3> laid out as 6 doubles
+ *     {min x,y,z, max x,y,z} (48 B, the reference's AoS layout).
+ *   - Point queries are given as three coordinate pointers and an element
+ *     stride in doubles: AoS Point<double,3> arrays pass (p, p+1, p+2, 3),
+ *     SoA ZipPoint-style arrays pass (x, y, z, 1).
+ *   - Every call returns AXSQ_OK (0) or a negative status; the text of the
+ *     last error on the calling thread is returned by axsq_last_error().
+ *     Nothing aborts (the reference aborts through SLIC_ERROR instead).
+ *   - Calls are stream-ordered on `stream` (a hipStream_t, NULL = default
+ *     stream) and return after their results are complete, matching the
+ *     reference's synchronous HIP_EXEC (core/execution/internal/hip_exec.hpp:38-85).
+ *   - IndexType is int32 (core/Types.hpp:63-65 default).
+ */
+#ifndef AXSQ_H
+#define AXSQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AXSQ_OK 0
+#define AXSQ_ERR_INVALID_ARG (-1)
+#define AXSQ_ERR_HIP (-2)
+#define AXSQ_ERR_NOMEM (-3)
+#define AXSQ_ERR_STACK_OVERFLOW (-4)
+#define AXSQ_ERR_STATE (-5)
+
+/* Precision of the tree traversal. AXSQ_FP64 is the reference's plain form;
+ * AXSQ_FP32_TRAVERSAL walks FP32 nodes whose boxes are rounded outward and
+ * finishes every leaf test in FP64. */
+#define AXSQ_FP64 0
+#define AXSQ_FP32_TRAVERSAL 1
+
+/* Query kinds of axsq_bvh_find*. */
+#define AXSQ_QUERY_POINTS 0 /* BoundingBox::contains(Point), BVH.hpp:499-501 */
+#define AXSQ_QUERY_BOXES 1  /* BoundingBox::intersectsWith, BVH.hpp:558-560 */
+#define AXSQ_QUERY_RAYS 2   /* intersect_ray(Ray, Box, ip, tol), BVH.hpp:529-532 */
+
+typedef struct axsq_bvh axsq_bvh;
+typedef struct axsq_sdf axsq_sdf;
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* axsq_last_error(void);
+/* Library version as 10000*major + 100*minor + patch. */
+int axsq_version(void);
+/* Frees device memory returned by the library (candidate arrays). */
+int axsq_free(void* device_ptr);
+
+/* ---- BVH ----------------------------------------------------------------
+ * Builds a linear BVH over n boxes.
+ * Replaces spin::BVH<3,Exec,double>::initialize (src/axom/spin/BVH.hpp:426-477)
+ * -> LinearBVH::buildImpl (src/axom/spin/policy/LinearBVH.hpp:193-269)
+ * -> build_radix_tree (src/axom/spin/internal/linear_bvh/build_radix_tree.hpp:579-611).
+ * scale: BVH::setScaleFactor (BVH.hpp:269, default 1.000123).
+ * fp_mode: AXSQ_FP64, or AXSQ_FP32_TRAVERSAL to also emit FP32 nodes.
+ * n may be 0 or 1 (padded to 2 leaves with an invalid box, BVH.hpp:441-464). */
+int axsq_bvh_build(axsq_bvh** out, const double* boxes, int64_t n, double scale,
+                   int fp_mode, void* stream);
+int axsq_bvh_destroy(axsq_bvh* bvh);
+
+/* Root box of the BVH: BVH::getBounds (BVH.hpp:297-308). Host outputs. */
+int axsq_bvh_bounds(const axsq_bvh* bvh, double lo[3], double hi[3]);
+/* Number of leaves after padding. */
+int64_t axsq_bvh_num_leaves(const axsq_bvh* bvh);
+
+/* Copies the tree out in the reference's own layout (device outputs, any may
+ * be NULL): inner_nodes 2(n-1) boxes, children 2(n-1) int32 (2*child for an
+ * inner child, -(pos+1) for a leaf), leaf_nodes n int32, mcodes n uint32
+ * (LinearBVH.hpp:170-187, RadixTree.hpp:28-63). */
+int axsq_bvh_export(const axsq_bvh* bvh, double* inner_nodes, int32_t* children,
+                    int32_t* leaf_nodes, uint32_t* mcodes, void* stream);
+
+/* Candidate search, two-phase form with caller-owned candidate storage.
+ * Replaces LinearBVH::findCandidatesImpl (LinearBVH.hpp:273-402) behind
+ * BVH::findPoints / findBoundingBoxes / findRays (BVH.hpp:482-568).
+ *   kind AXSQ_QUERY_POINTS: prims = x pointer, y/z/stride as in the header notes
+ *   kind AXSQ_QUERY_BOXES : prims = AoS boxes (6 doubles each), y=z=NULL, stride 6
+ *   kind AXSQ_QUERY_RAYS  : prims = AoS rays {origin xyz, unit direction xyz},
+ *                           y=z=NULL, stride 6; tol = BVH::setTolerance value
+ * Phase 1 (axsq_bvh_find_count) writes counts[nq], offsets[nq] (exclusive
+ * scan) and *total. Phase 2 (axsq_bvh_find_fill) writes candidates[total]:
+ * for query i, ids at [offsets[i], offsets[i]+counts[i]) in traversal order. */
+int axsq_bvh_find_count(const axsq_bvh* bvh, int kind, const double* x, const double* y,
+                        const double* z, int64_t stride, int64_t nq, double tol,
+                        int32_t* offsets, int32_t* counts, int64_t* total, void* stream);
+int axsq_bvh_find_fill(const axsq_bvh* bvh, int kind, const double* x, const double* y,
+                       const double* z, int64_t stride, int64_t nq, double tol,
+                       const int32_t* offsets, int32_t* candidates, void* stream);
+
+/* One-call form matching BVH::find* semantics: counts and offsets as above;
+ * the library allocates *candidates (device, *total entries; release with
+ * axsq_free), as the reference's BVH allocates the candidate array
+ * (LinearBVH.hpp:338-341). */
+int axsq_bvh_find(const axsq_bvh* bvh, int kind, const double* x, const double* y,
+                  const double* z, int64_t stride, int64_t nq, double tol, int32_t* offsets,
+                  int32_t* counts, int32_t** candidates, int64_t* total, void* stream);
+
+/* ---- SignedDistance -----------------------------------------------------
+ * Replaces quest::SignedDistance<3,Exec>(mesh, isWatertight, computeSign)
+ * (src/axom/quest/SignedDistance.hpp:411-504): mesh bounds, per-triangle
+ * boxes and a BVH with the default scale 1.000123. The handle COPIES the
+ * mesh (the reference keeps a pointer to the mint mesh instead).
+ * x,y,z: device SoA vertex coordinates (nverts); tris: device int32
+ * connectivity (3*ntris), as mint::UnstructuredMesh<SINGLE_SHAPE> holds them
+ * (mint/mesh/UnstructuredMesh.hpp:775-783,1256). */
+int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const double* z,
+                    int64_t nverts, const int32_t* tris, int64_t ntris, int watertight,
+                    int compute_sign, int fp_mode, void* stream);
+int axsq_sdf_destroy(axsq_sdf* sdf);
+/* Mesh bounding box (signed_distance_get_mesh_bounds,
+ * quest/interface/signed_distance.cpp:468-488). Host outputs. */
+int axsq_sdf_mesh_bounds(const axsq_sdf* sdf, double lo[3], double hi[3]);
+/* Replaces the vertex coordinates (same connectivity) and rebuilds the BVH:
+ * the rebuild step of a shape-fitting loop. No reference counterpart beyond
+ * SignedDistance::setMesh (SignedDistance.hpp:428-504). */
+int axsq_sdf_update_vertices(axsq_sdf* sdf, const double* x, const double* y,
+                             const double* z, void* stream);
+
+/* Signed distances for nq query points.
+ * Replaces SignedDistance::computeDistances (SignedDistance.hpp:529-605) and,
+ * through it, signed_distance_evaluate (quest/interface/signed_distance.cpp:432-465).
+ * fp_mode: AXSQ_FP64 (plain) or AXSQ_FP32_TRAVERSAL (needs a handle created
+ * with AXSQ_FP32_TRAVERSAL). Outputs (device, NULL to skip):
+ *   dist[nq]        signed distance (sign carried also for zero distances)
+ *   cp[3*nq]        closest surface point (AoS)
+ *   normal[3*nq]    unit (pseudo-)normal at the closest point (AoS)
+ *   elem[nq]        winning triangle id (input of axsq_sdf_backward)         */
+int axsq_sdf_eval(const axsq_sdf* sdf, const double* x, const double* y, const double* z,
+                  int64_t stride, int64_t nq, int fp_mode, double* dist, double* cp,
+                  double* normal, int32_t* elem, void* stream);
+
+/* Adjoint of axsq_sdf_eval (no reference counterpart):
+ *   grad_q[3*nq]     = g_i * sgn_i * (q_i - cp_i)/|q_i - cp_i|        (AoS)
+ *   grad_v[3*nverts] = - sum over queries of w_k * that, w = barycentric
+ *                      weights of cp_i on triangle elem_i              (AoS)
+ * gout[nq] is the upstream gradient (float). Sums are deterministic: per
+ * triangle and corner in ascending query order, then per vertex in
+ * ascending triangle order, so repeated calls are bitwise identical.
+ * Either output may be NULL. */
+int axsq_sdf_backward(const axsq_sdf* sdf, const double* x, const double* y, const double* z,
+                      int64_t stride, int64_t nq, const double* dist, const int32_t* elem,
+                      const float* gout, double* grad_q, double* grad_v, void* stream);
+
+/* ---- measurement --------------------------------------------------------
+ * When enabled, the library records HIP events around its main kernels on
+ * the stream they run on. axsq_prof_get returns, for kernel `name`, the
+ * number of launches and their summed duration in milliseconds. */
+int axsq_prof_enable(int on);
+int axsq_prof_reset(void);
+int axsq_prof_get(const char* name, int64_t* launches, double* total_ms);
+
+/* Per-query traversal statistics of the last FP64 axsq_sdf_eval when
+ * enabled: summed inner-node visits and leaf (triangle) checks. */
+int axsq_sdf_stats_enable(axsq_sdf* sdf, int on);
+int axsq_sdf_stats_get(const axsq_sdf* sdf, int64_t* visits, int64_t* leaves);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
