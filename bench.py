@@ -1,0 +1,236 @@
+"""Benchmark of the MI355X spatial-query path (one JSON line on stdout).
+
+Headline workload (the plain quest::SignedDistance FP64 configuration):
+icosphere level 6 (40,962 vertices, 81,920 triangles, r = 1, each vertex
+scaled by 1 + U[-1e-3, 1e-3]) and 4,000,000 queries U[-1.5,1.5]^3.
+One step = signed-distance forward (distance, closest point, winning
+element) + adjoint (query and vertex gradients for an N(0,1) FP32 upstream
+gradient) over all queries, inputs resident in HBM.
+
+  value            FP64-plain forward+backward queries/s
+  fp32_traversal   the same step with FP32 traversal (FP64 finish), and its
+                   error against the FP64 result
+  bvh_build        BVH build Mprims/s over the 81,920 triangles and over the
+                   100,000-box findPoints workload
+  find_points      findPoints queries/s (100,000 boxes, 1,000,000 points)
+  roofline         dominant kernel (FP64 SDF forward) timed with HIP events on
+                   its stream; algorithmic bytes per query
+                   = 24 (query) + 8 (distance) + 24 (closest point) + 4 (element)
+                   + V*104 (inner-node visits: 2 x 48 B boxes + 2 x 4 B links)
+                   + L*84 (leaf checks: 3 x 24 B vertices + 3 x 4 B indices),
+                   V, L = mean visits / leaf checks of the reference traversal
+                   (the FP64 kernel follows it exactly; counted on the GPU)
+  cpu_baseline     the CPU oracle (restatement of the reference's SEQ path),
+                   forward+backward on the first 1,000,000 queries, timed on
+                   this host's cores
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("SDF queries/s fwd+bwd, FP32-trav vs FP64 plain @≤1e-6 rel err; "
+          "BVH build Mprims/s")
+HBM_PEAK_GBS = 8000.0
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "sdf_eval_fp64_icosphere6_4M.traffic.json")
+
+
+def parse():
+    p = argparse.ArgumentParser(description=__doc__,
+                                formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--queries", type=int, default=4_000_000)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-extras", action="store_true",
+                   help="only the headline step (for profiler runs)")
+    p.add_argument("--calib-copy", action="store_true",
+                   help="also run one 1 GiB device copy (counter calibration)")
+    p.add_argument("--cpu-sample", type=int, default=1_000_000)
+    return p.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(every GPU box here has one GPU)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from axom_amd import _lib, quest, spin, synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    L = _lib.lib()
+
+    log("generating inputs")
+    v, t, q = synth.workload2(nq=args.queries)
+    g = synth.normal_f32(synth.SEED0 + 2, 40, q.shape[0])
+    nq = q.shape[0]
+    vd = torch.as_tensor(v, device=dev)
+    td = torch.as_tensor(t, device=dev)
+    qd = torch.as_tensor(q, device=dev)
+    gd = torch.as_tensor(g, device=dev)
+    dist = torch.empty(nq, dtype=torch.float64, device=dev)
+    cp = torch.empty((nq, 3), dtype=torch.float64, device=dev)
+    elem = torch.empty(nq, dtype=torch.int32, device=dev)
+
+    sdf = quest.SignedDistance(vd, td, isWatertight=True, computeSign=True, fp_mode="fp32")
+
+    def step(mode):
+        sdf.computeDistances(nq, qd, outSgnDist=dist, outClosestPts=cp, outElem=elem,
+                             fp_mode=mode)
+        return sdf.backward(qd, dist, elem, gd)
+
+    def timed(mode, steps, warmup):
+        for _ in range(warmup):
+            step(mode)
+        torch.cuda.synchronize()
+        L.axsq_prof_reset()
+        L.axsq_prof_enable(1)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(mode)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        L.axsq_prof_enable(0)
+        return el
+
+    def prof(name):
+        n, ms = ctypes.c_int64(), ctypes.c_double()
+        _lib.check(L.axsq_prof_get(name.encode(), ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    log("timing FP64 forward+backward")
+    el64 = timed("fp64", args.steps, args.warmup)
+    n_eval, ms_eval = prof("sdf_eval_fp64")
+    n_bwd, ms_bwd = prof("sdf_backward")
+    n_ord, ms_ord = prof("sdf_query_order")
+    d64 = dist.clone()
+    qps = nq * args.steps / el64
+    out = {
+        "metric": METRIC, "value": qps, "unit": "queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1000.0 * el64 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "icosphere level 6 (40,962 verts, 81,920 tris, r=1, +-1e-3 "
+                               "radial noise), %d queries U[-1.5,1.5]^3; plain FP64 "
+                               "SignedDistance forward (dist, cp, elem) + adjoint" % nq,
+                   "queries": nq, "triangles": int(t.shape[0]), "parallelism": "single GPU"},
+        "kernels_ms_per_step": {"sdf_eval_fp64": ms_eval / max(n_eval, 1),
+                                "sdf_query_order": ms_ord / max(n_ord, 1),
+                                "sdf_backward": ms_bwd / max(n_bwd, 1)},
+    }
+
+    # roofline of the dominant kernel: visits / leaf checks of the traversal
+    sdf.enableStats(True)
+    sdf.computeDistances(nq, qd, outSgnDist=dist, outClosestPts=cp, outElem=elem, fp_mode="fp64")
+    visits, leaves = sdf.lastStats()
+    sdf.enableStats(False)
+    vbar, lbar = visits / nq, leaves / nq
+    bytes_per_query = 24 + 8 + 24 + 4 + 104.0 * vbar + 84.0 * lbar
+    t_eval = ms_eval / max(n_eval, 1) / 1000.0
+    achieved = bytes_per_query * nq / t_eval / 1e9
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        traffic = json.load(open(PROFILE_TRAFFIC)).get("hbm_bytes_per_launch")
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                       "kernel": "k_sdf_eval<FP64>", "avg_launch_ms": t_eval * 1e3,
+                       "bytes_per_query": bytes_per_query, "mean_inner_visits": vbar,
+                       "mean_leaf_checks": lbar}
+
+    if not args.no_extras:
+        log("timing FP32-traversal forward+backward")
+        el32 = timed("fp32", args.steps, args.warmup)
+        n32, ms32 = prof("sdf_eval_fp32")
+        d32 = dist.clone()
+        rel = (d32 - d64).abs() / d64.abs().clamp_min(1e-300)
+        out["fp32_traversal"] = {
+            "value": nq * args.steps / el32, "unit": "queries/s",
+            "ms_per_step": 1000.0 * el32 / args.steps,
+            "sdf_eval_fp32_ms": ms32 / max(n32, 1),
+            "max_rel_err_vs_fp64": float(rel.max()),
+            "rms_rel_err_vs_fp64": float(torch.sqrt((rel * rel).mean())),
+            "sign_mismatches": int((torch.signbit(d32) != torch.signbit(d64)).sum())}
+
+        log("timing BVH builds and findPoints")
+        tb = torch.empty((t.shape[0], 6), dtype=torch.float64, device=dev)
+        a, b, c = vd[td[:, 0].long()], vd[td[:, 1].long()], vd[td[:, 2].long()]
+        tb[:, :3] = torch.minimum(torch.minimum(a, b), c)
+        tb[:, 3:] = torch.maximum(torch.maximum(a, b), c)
+        boxes1, pts1 = synth.workload1()
+        bx1 = torch.as_tensor(boxes1, device=dev)
+        builds = {}
+        for name, bx in (("icosphere6_tris", tb), ("boxes_100k", bx1)):
+            bvh = spin.BVH(bx)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            reps = 20
+            for _ in range(reps):
+                bvh.initialize(bx)
+            torch.cuda.synchronize()
+            el = (time.perf_counter() - t0) / reps
+            builds[name] = {"prims": int(bx.shape[0]), "ms": el * 1e3,
+                            "mprims_per_s": bx.shape[0] / el / 1e6}
+        out["bvh_build"] = builds
+        bvh = spin.BVH(bx1)
+        p1 = torch.as_tensor(pts1, device=dev)
+        off = torch.empty(p1.shape[0], dtype=torch.int32, device=dev)
+        cnt = torch.empty(p1.shape[0], dtype=torch.int32, device=dev)
+        bvh.findPoints(off, cnt, p1.shape[0], p1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            cand = bvh.findPoints(off, cnt, p1.shape[0], p1)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / 10
+        out["find_points"] = {"queries": int(p1.shape[0]), "boxes": int(bx1.shape[0]),
+                              "ms": el * 1e3, "queries_per_s": p1.shape[0] / el,
+                              "candidates": int(cand.numel())}
+
+    if args.calib_copy:
+        src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calib_copy_bytes"] = {"read": src.numel() * 8, "write": dst.numel() * 8}
+
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        ncpu = min(os.cpu_count() or 1, 16)
+        ns = min(args.cpu_sample, nq)
+        log(f"CPU baseline: oracle on {ns} queries, {ncpu} threads")
+        o = orc.OracleSDF(v, t)
+        t0 = time.perf_counter()
+        r = o.eval(q[:ns], nthreads=ncpu, want_normal=False)
+        o.backward(q[:ns], r["dist"], r["elem"], g[:ns])
+        el = time.perf_counter() - t0
+        same = bool(np.array_equal(r["dist"], d64[:ns].cpu().numpy()))
+        out["cpu_baseline"] = {"value": ns / el, "unit": "queries/s", "cores": ncpu,
+                               "kind": "port",
+                               "sample": f"first {ns} of the {nq} queries, forward (dist, cp, "
+                                         "elem) + adjoint, FP64",
+                               "seconds": el, "matches_gpu_bitwise": same}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
