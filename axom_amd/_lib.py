@@ -8,7 +8,7 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libaxsq.so")
+LIB_PATH = os.environ.get("AXSQ_LIB_PATH") or os.path.join(HERE, "lib", "libaxsq.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "axsq.h")
 
 AXSQ_OK = 0

@@ -32,31 +32,50 @@ def _hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src):
-    out = os.path.join(OBJ, src.replace(".hip", ".o"))
-    cmd = [_hipcc()] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", out]
+def _compile(src, defines=(), objdir=OBJ):
+    out = os.path.join(objdir, src.replace(".hip", ".o"))
+    cmd = [_hipcc()] + FLAGS + [f"-D{d}" for d in defines] + ["-c", os.path.join(CSRC, src),
+                                                              "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
     return out, r.stderr
 
 
-def build(verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
-    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+def build(verbose=False, defines=(), lib=LIB, objdir=OBJ):
+    """Compiles every source (with extra -D `defines`) and links `lib`."""
+    os.makedirs(objdir, exist_ok=True)
+    os.makedirs(os.path.dirname(lib), exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(lambda src: _compile(src, defines, objdir), SOURCES))
     for _, err in results:
         if verbose and err.strip():
             sys.stderr.write(err)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-o", tmp] + [o for o, _ in results]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
+
+
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_facade.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_facade")
+
+
+def build_cpp_tests():
+    """Host-only C++ test of include/axom_mi355x.hpp, linked to libaxsq.so."""
+    cmd = [_hipcc(), "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC,
+           "-o", CPP_TEST_BIN + ".tmp", "-L", os.path.dirname(LIB), "-laxsq",
+           "-Wl,-rpath,$ORIGIN/../../axom_amd/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ facade test build failed:\n{r.stderr}")
+    os.replace(CPP_TEST_BIN + ".tmp", CPP_TEST_BIN)
+    return CPP_TEST_BIN
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose="-v" in sys.argv))
+    print(build_cpp_tests())

@@ -70,6 +70,8 @@ void Workspace::release() {
   tiles.release();
   misc.release();
   flag.release();
+  susp.release();
+  cnt.release();
 }
 
 namespace {
@@ -376,6 +378,11 @@ int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const doub
   h->bvh->device = h->device;
   const char* env = getenv("AXSQ_SORT_QUERIES");
   h->sort_queries = !(env && env[0] == '0');
+  if (const char* e = getenv("AXSQ_XCD_SLICES")) h->xcd_slices = atoi(e) > 1 ? atoi(e) : 1;
+  if (const char* e = getenv("AXSQ_DISPERSE")) h->disperse = atoi(e) > 1 ? atoi(e) : 1;
+  if (const char* e = getenv("AXSQ_SDF_BUDGET")) h->budget = atoi(e) > 0 ? atoi(e) : 0x7fffffff;
+  if (const char* e = getenv("AXSQ_SDF_GROUP"))
+    h->group = (atoi(e) >= 1 && atoi(e) <= 64) ? atoi(e) : 1;
   int rc = AXSQ_OK;
   auto fail = [&](int code) {
     axsq_sdf_destroy(h);

@@ -277,7 +277,8 @@ __global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf
   }
   nd.lc = lc;
   nd.rc = rc;
-  for (int k = 0; k < 6; ++k) nd.pad[k] = 0;
+  nd.flags = (box_valid(nd.l) ? 1 : 0) | (box_valid(nd.r) ? 2 : 0);
+  for (int k = 0; k < 5; ++k) nd.pad[k] = 0;
   nodes[i] = nd;
   if (nodes32) {
     Node32 h;
@@ -289,7 +290,8 @@ __global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf
     }
     h.lc = lc;
     h.rc = rc;
-    h.pad[0] = h.pad[1] = 0;
+    h.flags = (box_valid_f(h.l) ? 1 : 0) | (box_valid_f(h.r) ? 2 : 0);
+    h.pad = 0;
     nodes32[i] = h;
   }
 }

@@ -22,11 +22,14 @@ struct Box {
 
 // FP64 inner node: both child boxes and both child links in one 128-byte
 // cache line. Link >= 0: inner node index; < 0: -(leaf position + 1).
+// flags bit 0 / bit 1: left / right box is valid (BoundingBox::isValid of the
+// stored box, precomputed so traversal does not re-test it).
 struct __attribute__((aligned(128))) Node64 {
   double l[6];  // left  min xyz, max xyz
   double r[6];  // right min xyz, max xyz
   int32_t lc, rc;
-  int32_t pad[6];
+  int32_t flags;
+  int32_t pad[5];
 };
 static_assert(sizeof(Node64) == 128, "Node64 must be one cache line");
 
@@ -35,7 +38,8 @@ struct __attribute__((aligned(64))) Node32 {
   float l[6];
   float r[6];
   int32_t lc, rc;
-  int32_t pad[2];
+  int32_t flags;  // as Node64::flags, for the FP32 boxes
+  int32_t pad;
 };
 static_assert(sizeof(Node32) == 64, "Node32 must be half a cache line");
 
@@ -178,12 +182,16 @@ __device__ __forceinline__ int32_t karras_delta(int32_t a, int32_t b, int32_t in
 
 // squared_distance(Point, Box), primal/operators/squared_distance.hpp:77-102.
 // For a point inside the box the clamped vector is exactly zero, so the
-// contains() shortcut of the reference yields the same 0.
+// contains() shortcut of the reference yields the same 0. The clamp
+// (clampVal, core/utilities/Utilities.hpp:147-150) is written as max/min:
+// on a valid box both give the same value except for the sign of a zero
+// coordinate, which the subtraction and squaring erase; a NaN query gives NaN
+// either way.
 __device__ __forceinline__ double sqdist_box(double q0, double q1, double q2, const double b[6]) {
-  const double c0 = (q0 < b[0]) ? b[0] : (q0 > b[3]) ? b[3] : q0;
-  const double c1 = (q1 < b[1]) ? b[1] : (q1 > b[4]) ? b[4] : q1;
-  const double c2 = (q2 < b[2]) ? b[2] : (q2 > b[5]) ? b[5] : q2;
-  return dot3(c0 - q0, c1 - q1, c2 - q2, c0 - q0, c1 - q1, c2 - q2);
+  const double e0 = fmin(fmax(q0, b[0]), b[3]) - q0;
+  const double e1 = fmin(fmax(q1, b[1]), b[4]) - q1;
+  const double e2 = fmin(fmax(q2, b[2]), b[5]) - q2;
+  return dot3(e0, e1, e2, e0, e1, e2);
 }
 
 // closest_point(Point, Triangle, loc, EPS), primal/operators/closest_point.hpp:162-290.
@@ -262,12 +270,19 @@ __device__ __forceinline__ void tri_normal(const double* T, double n[3]) {
   cross3(u, v, n);
 }
 
-// Triangle::angle, Triangle.hpp:385-400
+// Triangle::angle, Triangle.hpp:385-400. The three corners are picked with
+// selects, not by a runtime index, so the triangle stays in registers.
 __device__ __forceinline__ double tri_angle(const double* T, int idx) {
-  const int i1 = (idx + 1) % 3, i2 = (idx + 2) % 3;
-  const double* pt = T + 3 * idx;
-  const double a[3] = {T[3 * i1] - pt[0], T[3 * i1 + 1] - pt[1], T[3 * i1 + 2] - pt[2]};
-  const double b[3] = {T[3 * i2] - pt[0], T[3 * i2 + 1] - pt[1], T[3 * i2 + 2] - pt[2]};
+  double pt[3], p1[3], p2[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double A = T[d], B = T[3 + d], C = T[6 + d];
+    pt[d] = (idx == 0) ? A : (idx == 1) ? B : C;
+    p1[d] = (idx == 0) ? B : (idx == 1) ? C : A;
+    p2[d] = (idx == 0) ? C : (idx == 1) ? A : B;
+  }
+  const double a[3] = {p1[0] - pt[0], p1[1] - pt[1], p1[2] - pt[2]};
+  const double b[3] = {p2[0] - pt[0], p2[1] - pt[1], p2[2] - pt[2]};
   double ua[3], ub[3];
   unit3(a, ua);
   unit3(b, ub);

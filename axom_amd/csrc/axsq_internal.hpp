@@ -44,7 +44,7 @@ struct DevBuf {
 
 // Scratch owned by the calling host thread (per device), reused across calls.
 struct Workspace {
-  DevBuf k0, v0, k1, v1, hist, tiles, misc, flag;
+  DevBuf k0, v0, k1, v1, hist, tiles, misc, flag, susp, cnt;
   int device = -1;
   void release();
 };
@@ -123,6 +123,11 @@ struct axsq_sdf {
   int64_t* stats_dev = nullptr;     // [visits, leaves]
   int64_t last_visits = 0, last_leaves = 0;
   bool sort_queries = true;
+  int32_t xcd_slices = 1;  // workgroup slicing over XCDs (1 = off)
+  int32_t disperse = 1;    // wave-chunk dispersal groups (1 = off)
+  int32_t budget = 2048;   // first-pass step budget per query walk
+  int32_t group = 1;       // second pass: suspended walks per wave
+  int64_t last_suspended = 0;
 };
 
 namespace axsq {

@@ -66,8 +66,10 @@ __device__ __forceinline__ bool pred32(const double* prim, const float* lo, cons
 
 template <int KIND, bool FILL, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
+  __shared__ int32_t s_stack[stack_lds_words(kBlock)];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.nq) return;
+  lds_i32* stk = lane_stack_base(s_stack);
   double prim[6];
   if (KIND == AXSQ_QUERY_POINTS) {
     prim[0] = a.x[i * a.stride];
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
   int32_t v;
   if (!MIXED) {
     v = traverse<Node64, double>(
-        a.nodes, a.ninner, [&](const double* b) { return pred64<KIND>(prim, a.tol, b); }, no_pref, emit);
+        a.nodes, a.ninner, stk, [&](const double* b) { return pred64<KIND>(prim, a.tol, b); }, no_pref, emit);
   } else {
     float lo[3], hi[3];
     const int hoff = (KIND == AXSQ_QUERY_BOXES) ? 3 : 0;
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
       hi[d] = __double2float_ru(prim[hoff + d]);
     }
     v = traverse<Node32, float>(
-        a.nodes32, a.ninner, [&](const float* b) { return pred32<KIND>(prim, lo, hi, a.tol, b); }, no_pref,
+        a.nodes32, a.ninner, stk, [&](const float* b) { return pred32<KIND>(prim, lo, hi, a.tol, b); }, no_pref,
         [&](int32_t pos) {
           double lb[6];
           const double2* p = reinterpret_cast<const double2*>(a.leaf_boxes + 6 * (int64_t)pos);

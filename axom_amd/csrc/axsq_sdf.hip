@@ -19,12 +19,25 @@
 // closest point, the pseudo-normal and the sign are FP64.
 //
 // Adjoint (no reference counterpart): see axsq_sdf_backward in axsq.h.
+#include <algorithm>
+
 #include "axsq_internal.hpp"
 #include "axsq_traverse.hpp"
+
+#ifndef AXSQ_SDF_BLOCK
+#define AXSQ_SDF_BLOCK 256
+#endif
+#if defined(AXSQ_SDF_WAVES) && AXSQ_SDF_WAVES > 0
+#define AXSQ_SDF_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(AXSQ_SDF_WAVES)))
+#else
+#define AXSQ_SDF_WAVES_ATTR
+#endif
 
 namespace axsq {
 
 namespace {
+
+constexpr int kSdfBlock = AXSQ_SDF_BLOCK;
 
 struct SdfArgs {
   const Node64* nodes;
@@ -37,6 +50,13 @@ struct SdfArgs {
   int64_t nq;
   int32_t ninner;
   const uint32_t* order;
+  int32_t xcd_slices;  // > 1: XCD-contiguous workgroup slices
+  int32_t disperse;    // > 1: spread 64-query wave chunks over this many groups
+  int32_t budget;      // first-pass step budget per walk
+  int32_t group;       // second pass: saved walks per wave
+  int64_t cap;         // capacity of susp
+  struct SdfSuspended* susp;
+  uint32_t* nsusp;
   double dom[6];
   int watertight;
   int compute_sign;
@@ -116,79 +136,156 @@ __device__ __forceinline__ void check_candidate(MinCand& m, const double q[3], c
   }
 }
 
-template <bool MIXED, bool STATS>
-__global__ __launch_bounds__(kBlock) void k_sdf_eval(SdfArgs a) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.nq) return;
-  const int64_t i = a.order ? (int64_t)a.order[t] : t;
-  const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
+// A query whose walk ran out of its step budget in the first pass: the
+// whole walk state, so the second pass continues exactly where it stopped.
+struct SdfSuspended {
+  double sq, p[3], sn[3];
+  int32_t qidx, cur, sp, visits, steps, leaves, type, pos;
+  int32_t stack[kStack];  // entries 1..sp
+};
+
+// RESUME = false: every query, in Morton order, with a step budget; a walk
+// that runs out of budget is saved (while there is room) and left to the
+// second pass. RESUME = true: `group` saved walks per wave run to the end.
+// Splitting keeps the rare very long walks (e.g. queries near the centre of
+// a closed surface, where thousands of triangles are almost equidistant) out
+// of the coherent waves of the first pass, which would otherwise all wait
+// for them.
+template <bool MIXED, bool STATS, bool RESUME>
+__global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfArgs a) {
+  __shared__ int32_t s_stack[stack_lds_words(kSdfBlock)];
+  LaneStack todo;
+  todo.lds = lane_stack_base(s_stack);
   MinCand m;
-  m.sq = kDblMax;
-  m.p[0] = m.p[1] = m.p[2] = 0.0;
-  m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
-  m.type = -1;
-  m.pos = 0;
-  const bool cn = a.compute_sign != 0;
+  Walk w;
+  int64_t i;
   int32_t leaves = 0;
+  int32_t budget = a.budget;
+  if (!RESUME) {
+    const int64_t blk =
+        a.xcd_slices > 1 ? group_spread(blockIdx.x, gridDim.x, a.xcd_slices) : blockIdx.x;
+    int64_t t = blk * blockDim.x + threadIdx.x;
+    if (t >= a.nq) return;
+    // Each 64-query chunk (one wave) stays together; optionally consecutive
+    // chunks are spread over distant parts of space.
+    const int64_t nfull = a.nq / kWaveLanes;
+    if (a.disperse > 1 && t / kWaveLanes < nfull)
+      t = group_spread(t / kWaveLanes, nfull, a.disperse) * kWaveLanes + t % kWaveLanes;
+    i = a.order ? (int64_t)a.order[t] : t;
+    m.sq = kDblMax;
+    m.p[0] = m.p[1] = m.p[2] = 0.0;
+    m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
+    m.type = -1;
+    m.pos = 0;
+    w = walk_start(todo);
+  } else {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t lane = t % kWaveLanes;
+    if (lane >= a.group) return;
+    const int64_t slot = (t / kWaveLanes) * a.group + lane;
+    if (slot >= (int64_t)*a.nsusp || slot >= a.cap) return;
+    const SdfSuspended& r = a.susp[slot];
+    i = r.qidx;
+    m.sq = r.sq;
+    for (int k = 0; k < 3; ++k) {
+      m.p[k] = r.p[k];
+      m.sn[k] = r.sn[k];
+    }
+    m.type = r.type;
+    m.pos = r.pos;
+    w.cur = r.cur;
+    w.sp = r.sp;
+    w.visits = r.visits;
+    w.steps = r.steps;
+    leaves = r.leaves;
+    todo.set(0, kBarrier);
+    for (int32_t k = 1; k <= w.sp; ++k) todo.set(k, r.stack[k]);
+    budget = 0x7fffffff;
+  }
+  const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
+  const bool cn = a.compute_sign != 0;
   auto leaf = [&](int32_t pos) {
     double T[9];
     load_tri(a.tris, pos, T);
     check_candidate(m, q, T, pos, cn);
     if (STATS) ++leaves;
   };
-  int32_t v;
-  if (!MIXED) {
-    // LinearBVH.hpp:75-82: nearer child (by centroid distance) first
-    auto pref = [&](const double* l, const double* r) {
-      const double cl0 = 0.5 * (l[0] + l[3]), cl1 = 0.5 * (l[1] + l[4]), cl2 = 0.5 * (l[2] + l[5]);
-      const double cr0 = 0.5 * (r[0] + r[3]), cr1 = 0.5 * (r[1] + r[4]), cr2 = 0.5 * (r[2] + r[5]);
-      const double dl = dot3(cl0 - q[0], cl1 - q[1], cl2 - q[2], cl0 - q[0], cl1 - q[1], cl2 - q[2]);
-      const double dr = dot3(cr0 - q[0], cr1 - q[1], cr2 - q[2], cr0 - q[0], cr1 - q[1], cr2 - q[2]);
-      return dl > dr;
-    };
-    v = traverse<Node64, double>(
-        a.nodes, a.ninner, [&](const double* b) { return sqdist_box(q[0], q[1], q[2], b) <= m.sq; }, pref,
-        leaf);
-  } else {
-    float qlo[3], qhi[3], qn[3];
-    for (int d = 0; d < 3; ++d) {
-      qlo[d] = __double2float_rd(q[d]);
-      qhi[d] = __double2float_ru(q[d]);
-      qn[d] = (float)q[d];
+  int st;
+  for (;;) {
+    if (!MIXED) {
+      // LinearBVH.hpp:75-82: nearer child (by centroid distance) first
+      auto pref = [&](const double* l, const double* r) {
+        const double cl0 = 0.5 * (l[0] + l[3]), cl1 = 0.5 * (l[1] + l[4]), cl2 = 0.5 * (l[2] + l[5]);
+        const double cr0 = 0.5 * (r[0] + r[3]), cr1 = 0.5 * (r[1] + r[4]), cr2 = 0.5 * (r[2] + r[5]);
+        const double dl = dot3(cl0 - q[0], cl1 - q[1], cl2 - q[2], cl0 - q[0], cl1 - q[1], cl2 - q[2]);
+        const double dr = dot3(cr0 - q[0], cr1 - q[1], cr2 - q[2], cr0 - q[0], cr1 - q[1], cr2 - q[2]);
+        return dl > dr;
+      };
+      st = traverse<Node64, double>(
+          a.nodes, a.ninner, todo, w, budget,
+          [&](const double* b) { return sqdist_box(q[0], q[1], q[2], b) <= m.sq; }, pref, leaf);
+    } else {
+      float qlo[3], qhi[3], qn[3];
+      for (int d = 0; d < 3; ++d) {
+        qlo[d] = __double2float_rd(q[d]);
+        qhi[d] = __double2float_ru(q[d]);
+        qn[d] = (float)q[d];
+      }
+      double last_sq = -1.0;
+      float thr = 0.0f;
+      auto pred = [&](const float* b) {
+        if (m.sq != last_sq) {
+          // relative slack covers the FP32 round-to-nearest error; the
+          // absolute 1e-36 covers errors of subnormal intermediate values
+          last_sq = m.sq;
+          thr = __double2float_ru(m.sq * (1.0 + 1e-6) + 1e-36);
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float g = fmaxf(fmaxf(b[d] - qhi[d], qlo[d] - b[3 + d]), 0.0f);
+          acc += g * g;
+        }
+        return acc <= thr;
+      };
+      auto pref = [&](const float* l, const float* r) {
+        float dl = 0.0f, dr = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const float el = 0.5f * (l[d] + l[3 + d]) - qn[d];
+          const float er = 0.5f * (r[d] + r[3 + d]) - qn[d];
+          dl += el * el;
+          dr += er * er;
+        }
+        return dl > dr;
+      };
+      st = traverse<Node32, float>(a.nodes32, a.ninner, todo, w, budget, pred, pref, leaf);
     }
-    double last_sq = -1.0;
-    float thr = 0.0f;
-    auto pred = [&](const float* b) {
-      if (m.sq != last_sq) {
-        // relative slack covers the FP32 round-to-nearest error; the absolute
-        // 1e-36 covers errors of subnormal intermediate values
-        last_sq = m.sq;
-        thr = __double2float_ru(m.sq * (1.0 + 1e-6) + 1e-36);
+    if (RESUME || st != kWalkSuspended) break;
+    const uint32_t slot = atomicAdd(a.nsusp, 1u);
+    if (slot < (uint32_t)a.cap) {
+      SdfSuspended& r = a.susp[slot];
+      r.qidx = (int32_t)i;
+      r.sq = m.sq;
+      for (int k = 0; k < 3; ++k) {
+        r.p[k] = m.p[k];
+        r.sn[k] = m.sn[k];
       }
-      float acc = 0.0f;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const float g = fmaxf(fmaxf(b[d] - qhi[d], qlo[d] - b[3 + d]), 0.0f);
-        acc += g * g;
-      }
-      return acc <= thr;
-    };
-    auto pref = [&](const float* l, const float* r) {
-      float dl = 0.0f, dr = 0.0f;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const float el = 0.5f * (l[d] + l[3 + d]) - qn[d];
-        const float er = 0.5f * (r[d] + r[3 + d]) - qn[d];
-        dl += el * el;
-        dr += er * er;
-      }
-      return dl > dr;
-    };
-    v = traverse<Node32, float>(a.nodes32, a.ninner, pred, pref, leaf);
+      r.type = m.type;
+      r.pos = m.pos;
+      r.cur = w.cur;
+      r.sp = w.sp;
+      r.visits = w.visits;
+      r.steps = w.steps;
+      r.leaves = leaves;
+      for (int32_t k = 1; k <= w.sp; ++k) r.stack[k] = todo.get(k);
+      return;
+    }
+    budget = 0x7fffffff;  // no room left: finish here
   }
-  if (v < 0) atomicOr(a.status, 1);
+  if (st == kWalkError) atomicOr(a.status, 1);
   if (STATS) {
-    atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)(v < 0 ? 0 : v));
+    atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)(st == kWalkError ? 0 : w.visits));
     atomicAdd((unsigned long long*)&a.stats[1], (unsigned long long)leaves);
   }
   // surface normal (getSurfaceNormal :741-749) and sign (:580-592)
@@ -612,6 +709,8 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   a.elem = elem;
   a.status = b->status;
   a.stats = h->stats_dev;
+  a.xcd_slices = h->xcd_slices;
+  a.disperse = h->disperse;
   if (h->sort_queries && nq > 4096) {
     AXSQ_TRY(ws.k0.ensure(sizeof(uint32_t) * nq));
     AXSQ_TRY(ws.v0.ensure(sizeof(uint32_t) * nq));
@@ -635,16 +734,44 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     a.order = sv;
   }
   const bool mixed = (fp_mode == AXSQ_FP32_TRAVERSAL);
+  // room for suspended walks: every query up to 2^20, then nq / 8
+  const int64_t cap = nq <= (1 << 20) ? nq : std::max<int64_t>(1 << 20, nq / 8);
+  AXSQ_TRY(ws.susp.ensure(sizeof(SdfSuspended) * (size_t)cap));
+  AXSQ_TRY(ws.cnt.ensure(sizeof(uint32_t) * 2));
+  a.susp = ws.susp.as<SdfSuspended>();
+  a.nsusp = ws.cnt.as<uint32_t>();
+  a.cap = cap;
+  a.budget = h->budget;
+  a.group = h->group;
+  AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(a.nsusp), 0, 1, s));
   ProfRange pr(mixed ? "sdf_eval_fp32" : "sdf_eval_fp64", s);
+  const unsigned g = grid_for(nq, kSdfBlock);
   if (mixed) {
-    k_sdf_eval<true, false><<<grid_for(nq), kBlock, 0, s>>>(a);
+    k_sdf_eval<true, false, false><<<g, kSdfBlock, 0, s>>>(a);
   } else if (stats) {
-    k_sdf_eval<false, true><<<grid_for(nq), kBlock, 0, s>>>(a);
+    k_sdf_eval<false, true, false><<<g, kSdfBlock, 0, s>>>(a);
   } else {
-    k_sdf_eval<false, false><<<grid_for(nq), kBlock, 0, s>>>(a);
+    k_sdf_eval<false, false, false><<<g, kSdfBlock, 0, s>>>(a);
+  }
+  AXSQ_CHECK_HIP(hipGetLastError());
+  uint32_t nsusp = 0;
+  AXSQ_CHECK_HIP(hipMemcpyAsync(&nsusp, a.nsusp, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+  const int64_t nres = std::min<int64_t>(nsusp, cap);
+  const_cast<axsq_sdf*>(h)->last_suspended = nres;
+  if (nres > 0) {
+    const int64_t threads = ((nres + a.group - 1) / a.group) * kWaveLanes;
+    const unsigned g2 = grid_for(threads, kSdfBlock);
+    if (mixed) {
+      k_sdf_eval<true, false, true><<<g2, kSdfBlock, 0, s>>>(a);
+    } else if (stats) {
+      k_sdf_eval<false, true, true><<<g2, kSdfBlock, 0, s>>>(a);
+    } else {
+      k_sdf_eval<false, false, true><<<g2, kSdfBlock, 0, s>>>(a);
+    }
+    AXSQ_CHECK_HIP(hipGetLastError());
   }
   pr.stop();
-  AXSQ_CHECK_HIP(hipGetLastError());
   return AXSQ_OK;
 }
 

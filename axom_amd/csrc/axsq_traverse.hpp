@@ -20,24 +20,26 @@ namespace axsq {
 
 // Node access for the two node formats.
 __device__ __forceinline__ void load_node(const Node64* __restrict__ nodes, int32_t i, double l[6],
-                                          double r[6], int32_t& lc, int32_t& rc) {
+                                          double r[6], int32_t& lc, int32_t& rc, int32_t& flags) {
   const double2* p = reinterpret_cast<const double2*>(nodes + i);
   const double2 a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4], a5 = p[5];
-  const int2 c = *reinterpret_cast<const int2*>(p + 6);
+  const int4 c = *reinterpret_cast<const int4*>(p + 6);
   l[0] = a0.x; l[1] = a0.y; l[2] = a1.x; l[3] = a1.y; l[4] = a2.x; l[5] = a2.y;
   r[0] = a3.x; r[1] = a3.y; r[2] = a4.x; r[3] = a4.y; r[4] = a5.x; r[5] = a5.y;
   lc = c.x;
   rc = c.y;
+  flags = c.z;
 }
 
 __device__ __forceinline__ void load_node(const Node32* __restrict__ nodes, int32_t i, float l[6],
-                                          float r[6], int32_t& lc, int32_t& rc) {
+                                          float r[6], int32_t& lc, int32_t& rc, int32_t& flags) {
   const float4* p = reinterpret_cast<const float4*>(nodes + i);
   const float4 a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3];
   l[0] = a0.x; l[1] = a0.y; l[2] = a0.z; l[3] = a0.w; l[4] = a1.x; l[5] = a1.y;
   r[0] = a1.z; r[1] = a1.w; r[2] = a2.x; r[3] = a2.y; r[4] = a2.z; r[5] = a2.w;
   lc = __float_as_int(a3.x);
   rc = __float_as_int(a3.y);
+  flags = __float_as_int(a3.z);
 }
 
 template <class T>
@@ -45,32 +47,99 @@ __device__ __forceinline__ bool valid6(const T b[6]) {
   return !(b[0] > b[3]) && !(b[1] > b[4]) && !(b[2] > b[5]);
 }
 
+// Per-lane traversal stack. The first kLdsStack entries live in LDS laid out
+// [depth][lane] so that 64 lanes at 64 different depths still hit 64
+// different banks; deeper entries (trees deeper than kLdsStack levels, rare)
+// go to a private array. A private array indexed per lane would be placed in
+// scratch, where lanes at different depths touch different cache lines.
+constexpr int kLdsStack = 32;
+constexpr int kWaveLanes = 64;
+
+// LDS-qualified word: accesses compile to ds_read/ds_write, not flat ops.
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+
+struct LaneStack {
+  lds_i32* lds;  // &tile[0][lane] of this wave's [kLdsStack][64] tile
+  int32_t spill[kStack - kLdsStack];
+  __device__ __forceinline__ int32_t get(int32_t sp) const {
+    if (sp < kLdsStack) return lds[sp * kWaveLanes];
+    return spill[sp - kLdsStack];
+  }
+  __device__ __forceinline__ void set(int32_t sp, int32_t v) {
+    if (sp < kLdsStack)
+      lds[sp * kWaveLanes] = v;
+    else
+      spill[sp - kLdsStack] = v;
+  }
+};
+
+// LDS words a block of `threads` lanes needs for its stacks.
+constexpr int stack_lds_words(int threads) { return threads * kLdsStack; }
+
+// This lane's stack view into a block-wide __shared__ array of
+// stack_lds_words(blockDim.x) words.
+__device__ __forceinline__ lds_i32* lane_stack_base(int32_t* block_stack) {
+  const int wave = threadIdx.x / kWaveLanes;
+  const int lane = threadIdx.x % kWaveLanes;
+  return (lds_i32*)(block_stack + wave * (kLdsStack * kWaveLanes) + lane);
+}
+
+// Bijection on [0, n) that sends p to the (p % g)-th of g contiguous groups:
+// consecutive p land in groups far apart. With g = 8 over workgroups it puts
+// each of MI355X's 8 XCDs (workgroups are dispatched round-robin over them,
+// each with a private L2) on one contiguous slice of the grid.
+__device__ __forceinline__ int64_t group_spread(int64_t p, int64_t n, int64_t g) {
+  const int64_t x = p % g, k = p / g;
+  const int64_t q = n / g, r = n % g;
+  return x * q + (x < r ? x : r) + k;
+}
+
+// Resumable walk state: the stack lives in the LaneStack, the rest here.
+// The walk can only stop at the top of its outer loop, where no leaf is held.
+struct Walk {
+  int32_t cur;     // next node (kBarrier when done)
+  int32_t sp;      // top of stack
+  int32_t visits;  // inner nodes visited so far
+  int32_t steps;   // inner visits + leaf actions so far
+};
+
+__device__ __forceinline__ Walk walk_start(LaneStack& todo) {
+  todo.set(0, kBarrier);
+  return Walk{0, 0, 0, 0};
+}
+
+enum WalkStatus { kWalkDone = 0, kWalkSuspended = 1, kWalkError = -1 };
+
 // pred(box) -> bool; pref(lbox, rbox) -> true to visit the right child first;
-// leaf(pos) runs the leaf action. Returns the number of visited inner nodes,
-// or -1 after a stack overflow. `ninner` bounds node links and visits: a
-// well-formed tree visits each inner node at most once, so the guards only
-// turn a malformed tree into an error instead of a stray access or an
-// endless walk.
+// leaf(pos) runs the leaf action. Runs until the walk ends (kWalkDone), or
+// until `budget` steps have been taken at the top of the outer loop
+// (kWalkSuspended, resumable with the same state), or kWalkError after a
+// stack overflow. `ninner` bounds node links and visits: a well-formed tree
+// visits each inner node at most once, so the guards only turn a malformed
+// tree into an error instead of a stray access or an endless walk.
 template <class NodeT, class Scalar, class Pred, class Pref, class Leaf>
-__device__ __forceinline__ int32_t traverse(const NodeT* __restrict__ nodes, int32_t ninner,
-                                            Pred&& pred, Pref&& pref, Leaf&& leaf) {
-  int32_t todo[kStack];
-  int32_t sp = 0;
-  todo[0] = kBarrier;
+__device__ __forceinline__ int traverse(const NodeT* __restrict__ nodes, int32_t ninner,
+                                        LaneStack& todo, Walk& w, int32_t budget, Pred&& pred,
+                                        Pref&& pref, Leaf&& leaf) {
+  int32_t sp = w.sp, cur = w.cur, visits = w.visits, steps = w.steps;
   int32_t found = 0;
-  int32_t cur = 0;
-  int32_t visits = 0;
+  int status = kWalkDone;
   while (cur != kBarrier) {
+    if (steps >= budget) {
+      status = kWalkSuspended;
+      break;
+    }
     while (cur >= 0) {
-      if (cur >= ninner || visits >= ninner) return -1;
+      if (cur >= ninner || visits >= ninner) return kWalkError;
       Scalar lb[6], rb[6];
-      int32_t lc, rc;
-      load_node(nodes, cur, lb, rb, lc, rc);
+      int32_t lc, rc, flags;
+      load_node(nodes, cur, lb, rb, lc, rc, flags);
       ++visits;
-      const bool in_l = valid6(lb) ? pred(lb) : false;
-      const bool in_r = valid6(rb) ? pred(rb) : false;
+      ++steps;
+      const bool in_l = (flags & 1) ? pred(lb) : false;
+      const bool in_r = (flags & 2) ? pred(rb) : false;
       if (!in_l && !in_r) {
-        cur = todo[sp];
+        cur = todo.get(sp);
         sp--;
       } else {
         cur = in_l ? lc : rc;
@@ -81,31 +150,48 @@ __device__ __forceinline__ int32_t traverse(const NodeT* __restrict__ nodes, int
             rc = t;
           }
           sp++;
-          if (sp >= kStack) return -1;
-          todo[sp] = rc;
+          if (sp >= kStack) return kWalkError;
+          todo.set(sp, rc);
         }
       }
       if (cur < 0 && !(found < 0)) {
         found = cur;
         if (cur != kBarrier) {
-          cur = todo[sp];
+          cur = todo.get(sp);
           sp--;
         }
       }
     }
     while (found < 0 && found != kBarrier) {
-      if (-found - 1 > ninner) return -1;  // ninner + 1 leaves
+      if (-found - 1 > ninner) return kWalkError;  // ninner + 1 leaves
       leaf(-found - 1);
+      ++steps;
       found = cur;
       if (cur < 0 && cur != kBarrier) {
-        if (sp < 0) return -1;
-        cur = todo[sp];
+        if (sp < 0) return kWalkError;
+        cur = todo.get(sp);
         sp--;
       }
     }
     found = 0;
   }
-  return visits;
+  w.sp = sp;
+  w.cur = cur;
+  w.visits = visits;
+  w.steps = steps;
+  return status;
+}
+
+// One-shot walk to completion; returns visited inner nodes or -1.
+template <class NodeT, class Scalar, class Pred, class Pref, class Leaf>
+__device__ __forceinline__ int32_t traverse(const NodeT* __restrict__ nodes, int32_t ninner,
+                                            lds_i32* lds_stack, Pred&& pred, Pref&& pref,
+                                            Leaf&& leaf) {
+  LaneStack todo;
+  todo.lds = lds_stack;
+  Walk w = walk_start(todo);
+  const int st = traverse<NodeT, Scalar>(nodes, ninner, todo, w, 0x7fffffff, pred, pref, leaf);
+  return st == kWalkDone ? w.visits : -1;
 }
 
 }  // namespace axsq

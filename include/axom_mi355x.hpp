@@ -1,0 +1,344 @@
+// axom_mi355x.hpp -- C++ host interface of the MI355X spatial-query path.
+//
+// Mirrors the reference's user-facing C++ API for this path over the C-ABI in
+// axsq.h, so user code keeps its shape:
+//   axom::spin::BVH<3, axom::MI355X_EXEC>          (src/axom/spin/BVH.hpp:129-419)
+//   axom::quest::SignedDistance<3, axom::MI355X_EXEC>
+//                                                 (src/axom/quest/SignedDistance.hpp:147-397)
+//   axom::execution_space<axom::MI355X_EXEC>       (src/axom/core/execution/execution_space.hpp:77-103)
+// Bulk data are device pointers wrapped in axom::ArrayView-like views, as the
+// reference requires for GPU exec spaces (BVH.hpp:198-203). Candidate arrays
+// are allocated by the BVH (LinearBVH.hpp:338-341) in an axom::Array-like
+// owning device array. Misuse throws axom::mi355x_error where the reference
+// aborts via SLIC_ERROR.
+//
+// Header-only; link with -laxsq (axom_amd/lib/libaxsq.so) and the HIP runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "axsq.h"
+
+namespace axom {
+
+using IndexType = std::int32_t;  // core/Types.hpp:63-65 (default, 32-bit)
+
+class mi355x_error : public std::runtime_error {
+ public:
+  mi355x_error(int code, const std::string& what) : std::runtime_error(what), code_(code) {}
+  int code() const { return code_; }
+
+ private:
+  int code_;
+};
+
+namespace detail {
+inline void check(int rc) {
+  if (rc != AXSQ_OK) throw mi355x_error(rc, axsq_last_error());
+}
+inline void hip_check(hipError_t e) {
+  if (e != hipSuccess) throw mi355x_error(AXSQ_ERR_HIP, hipGetErrorString(e));
+}
+}  // namespace detail
+
+// The single MI355X execution space.
+struct MI355X_EXEC {};
+
+template <typename ExecSpace>
+struct execution_space;
+
+template <>
+struct execution_space<MI355X_EXEC> {
+  static constexpr bool valid() noexcept { return true; }
+  static constexpr bool async() noexcept { return false; }  // calls return when done
+  static constexpr bool onDevice() noexcept { return true; }
+  static constexpr char* name() noexcept { return (char*)"[MI355X_EXEC]"; }
+  static int allocatorID() noexcept { return 0; }  // hipMalloc device memory
+  static bool usesAllocId(int id) noexcept { return id == 0; }
+};
+
+// Non-owning view of device memory (axom::ArrayView on device memory).
+template <typename T>
+class ArrayView {
+ public:
+  ArrayView() = default;
+  ArrayView(T* data, IndexType size) : data_(data), size_(size) {}
+  T* data() const { return data_; }
+  IndexType size() const { return size_; }
+
+ private:
+  T* data_ = nullptr;
+  IndexType size_ = 0;
+};
+
+// Owning device array (axom::Array with a device allocator).
+template <typename T>
+class Array {
+ public:
+  Array() = default;
+  explicit Array(IndexType n) { resize(n); }
+  Array(const Array&) = delete;
+  Array& operator=(const Array&) = delete;
+  Array(Array&& o) noexcept { swap(o); }
+  Array& operator=(Array&& o) noexcept {
+    swap(o);
+    return *this;
+  }
+  ~Array() { reset(); }
+  void resize(IndexType n) {
+    reset();
+    if (n > 0) detail::hip_check(hipMalloc((void**)&data_, sizeof(T) * (size_t)n));
+    size_ = n;
+  }
+  // Takes ownership of library-allocated memory (released with axsq_free).
+  void adopt(T* p, IndexType n) {
+    reset();
+    data_ = p;
+    size_ = n;
+    lib_owned_ = true;
+  }
+  T* data() const { return data_; }
+  IndexType size() const { return size_; }
+  ArrayView<T> view() const { return ArrayView<T>(data_, size_); }
+
+ private:
+  void reset() {
+    if (data_) {
+      if (lib_owned_)
+        (void)axsq_free(data_);
+      else
+        (void)hipFree(data_);
+    }
+    data_ = nullptr;
+    size_ = 0;
+    lib_owned_ = false;
+  }
+  void swap(Array& o) {
+    std::swap(data_, o.data_);
+    std::swap(size_, o.size_);
+    std::swap(lib_owned_, o.lib_owned_);
+  }
+  T* data_ = nullptr;
+  IndexType size_ = 0;
+  bool lib_owned_ = false;
+};
+
+namespace primal {
+// Device layouts of primal::BoundingBox<double,3> (min, max) and
+// primal::Point<double,3>; the library reads them as plain doubles.
+struct BoundingBox3 {
+  double min[3];
+  double max[3];
+};
+struct Point3 {
+  double x[3];
+};
+// primal::Ray<double,3> stores a unit direction (Ray.hpp:122-128); make_ray
+// normalises the way Vector::unitVector does (Vector.hpp:477-490).
+struct Ray3 {
+  double origin[3];
+  double dir[3];
+};
+inline Ray3 make_ray(const double o[3], const double d[3]) {
+  Ray3 r{{o[0], o[1], o[2]}, {1.0, 0.0, 0.0}};
+  double len_sq = 0.0;
+  len_sq += d[0] * d[0];
+  len_sq += d[1] * d[1];
+  len_sq += d[2] * d[2];
+  if (len_sq >= 1e-50) {
+    const double inv = 1. / std::sqrt(len_sq);
+    for (int k = 0; k < 3; ++k) r.dir[k] = d[k] * inv;
+  }
+  return r;
+}
+}  // namespace primal
+
+namespace spin {
+
+enum BVHReturnCodes { BVH_BUILD_FAILED = -1, BVH_BUILD_OK = 0 };  // BVH.hpp:39-43
+
+template <int NDIMS, typename ExecSpace, typename FloatType = double>
+class BVH;
+
+// spin::BVH<3, MI355X_EXEC, double>: build over device boxes, find* over
+// device queries. The BVH copies the boxes; it keeps no pointer to them.
+template <>
+class BVH<3, MI355X_EXEC, double> {
+ public:
+  using BoxType = primal::BoundingBox3;
+  using PointType = primal::Point3;
+  using RayType = primal::Ray3;
+
+  static constexpr double DEFAULT_SCALE_FACTOR = 1.000123;  // BVH.hpp:410
+  static constexpr double DEFAULT_TOLERANCE = std::numeric_limits<double>::epsilon();
+
+  BVH() = default;
+  BVH(const BoxType* boxes, IndexType numItems, int allocatorID = 0,
+      double tolerance = DEFAULT_TOLERANCE, double scaleFactor = DEFAULT_SCALE_FACTOR)
+      : m_tolerance(tolerance), m_scaleFactor(scaleFactor) {
+    (void)allocatorID;
+    initialize(boxes, numItems);
+  }
+  BVH(const BVH&) = delete;
+  BVH& operator=(const BVH&) = delete;
+  ~BVH() { (void)axsq_bvh_destroy(m_bvh); }
+
+  // BVH.hpp:426-477
+  int initialize(const BoxType* boxes, IndexType numItems) {
+    (void)axsq_bvh_destroy(m_bvh);
+    m_bvh = nullptr;
+    detail::check(axsq_bvh_build(&m_bvh, reinterpret_cast<const double*>(boxes), numItems,
+                                 m_scaleFactor, m_fpMode, nullptr));
+    return BVH_BUILD_OK;
+  }
+  int initialize(ArrayView<const BoxType> boxes, IndexType numItems) {
+    return initialize(boxes.data(), numItems);
+  }
+  bool isInitialized() const { return m_bvh != nullptr; }
+
+  void setAllocatorID(int id) { (void)id; }
+  int getAllocatorID() const { return 0; }
+  void setScaleFactor(double s) { m_scaleFactor = s; }
+  double getScaleFactor() const { return m_scaleFactor; }
+  void setTolerance(double eps) { m_tolerance = eps; }
+  double getTolerance() const { return m_tolerance; }
+  // Precision of the traversal (MI355X addition): AXSQ_FP64 (default, the
+  // reference's form) or AXSQ_FP32_TRAVERSAL; takes effect at initialize().
+  void setTraversalPrecision(int fp_mode) { m_fpMode = fp_mode; }
+
+  // BVH.hpp:297-308; an invalid box before initialize().
+  BoxType getBounds() const {
+    BoxType b;
+    if (!m_bvh) {
+      for (int k = 0; k < 3; ++k) {
+        b.min[k] = std::numeric_limits<double>::max();
+        b.max[k] = std::numeric_limits<double>::lowest();
+      }
+      return b;
+    }
+    detail::check(axsq_bvh_bounds(m_bvh, b.min, b.max));
+    return b;
+  }
+
+  // BVH.hpp:482-505 (AoS points)
+  void findPoints(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
+                  Array<IndexType>& candidates, IndexType numPts, const PointType* points) const {
+    const double* p = reinterpret_cast<const double*>(points);
+    find(AXSQ_QUERY_POINTS, p, p + 1, p + 2, 3, offsets, counts, candidates, numPts);
+  }
+  // ZipPoint-style SoA points
+  void findPoints(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
+                  Array<IndexType>& candidates, IndexType numPts, const double* x,
+                  const double* y, const double* z) const {
+    find(AXSQ_QUERY_POINTS, x, y, z, 1, offsets, counts, candidates, numPts);
+  }
+  // BVH.hpp:510-536
+  void findRays(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
+                Array<IndexType>& candidates, IndexType numRays, const RayType* rays) const {
+    find(AXSQ_QUERY_RAYS, reinterpret_cast<const double*>(rays), nullptr, nullptr, 6, offsets,
+         counts, candidates, numRays);
+  }
+  // BVH.hpp:541-568
+  void findBoundingBoxes(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
+                         Array<IndexType>& candidates, IndexType numBoxes,
+                         const BoxType* boxes) const {
+    find(AXSQ_QUERY_BOXES, reinterpret_cast<const double*>(boxes), nullptr, nullptr, 6, offsets,
+         counts, candidates, numBoxes);
+  }
+
+  const axsq_bvh* handle() const { return m_bvh; }
+
+ private:
+  void find(int kind, const double* x, const double* y, const double* z, int64_t stride,
+            ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
+            Array<IndexType>& candidates, IndexType n) const {
+    // LinearBVH.hpp:284-285
+    if (offsets.size() != n) throw mi355x_error(AXSQ_ERR_INVALID_ARG, "offsets length != n");
+    if (counts.size() != n) throw mi355x_error(AXSQ_ERR_INVALID_ARG, "counts length != n");
+    if (!m_bvh) throw mi355x_error(AXSQ_ERR_STATE, "BVH is not initialized");
+    int32_t* cand = nullptr;
+    int64_t total = 0;
+    detail::check(axsq_bvh_find(m_bvh, kind, x, y, z, stride, n, m_tolerance, offsets.data(),
+                                counts.data(), &cand, &total, nullptr));
+    candidates.adopt(cand, (IndexType)total);
+  }
+
+  axsq_bvh* m_bvh = nullptr;
+  double m_tolerance = DEFAULT_TOLERANCE;
+  double m_scaleFactor = DEFAULT_SCALE_FACTOR;
+  int m_fpMode = AXSQ_FP64;
+};
+
+}  // namespace spin
+
+namespace quest {
+
+template <int NDIMS, typename ExecSpace>
+class SignedDistance;
+
+// quest::SignedDistance<3, MI355X_EXEC>. The mesh is given as device SoA
+// coordinates and int32 triangle connectivity (the arrays a
+// mint::UnstructuredMesh<SINGLE_SHAPE> of TRIANGLEs holds); it is copied.
+template <>
+class SignedDistance<3, MI355X_EXEC> {
+ public:
+  using PointType = primal::Point3;
+
+  SignedDistance(const double* x, const double* y, const double* z, IndexType nnodes,
+                 const IndexType* connectivity, IndexType ncells, bool isWatertight,
+                 bool computeSign = true, int fp_mode = AXSQ_FP64)
+      : m_fpMode(fp_mode) {
+    detail::check(axsq_sdf_create(&m_sdf, x, y, z, nnodes, connectivity, ncells,
+                                  isWatertight ? 1 : 0, computeSign ? 1 : 0, fp_mode, nullptr));
+  }
+  SignedDistance(const SignedDistance&) = delete;
+  SignedDistance& operator=(const SignedDistance&) = delete;
+  ~SignedDistance() { (void)axsq_sdf_destroy(m_sdf); }
+
+  // SignedDistance.hpp:529-605 with AoS points; outputs are device arrays
+  // (outClosestPts / outNormals may be null).
+  void computeDistances(int npts, const PointType* queryPts, double* outSgnDist,
+                        PointType* outClosestPts = nullptr,
+                        PointType* outNormals = nullptr) const {
+    const double* p = reinterpret_cast<const double*>(queryPts);
+    detail::check(axsq_sdf_eval(m_sdf, p, p + 1, p + 2, 3, npts, m_fpMode, outSgnDist,
+                                reinterpret_cast<double*>(outClosestPts),
+                                reinterpret_cast<double*>(outNormals), nullptr, nullptr));
+  }
+  // ZipPoint-style SoA points
+  void computeDistances(int npts, const double* x, const double* y, const double* z,
+                        double* outSgnDist) const {
+    detail::check(axsq_sdf_eval(m_sdf, x, y, z, 1, npts, m_fpMode, outSgnDist, nullptr, nullptr,
+                                nullptr, nullptr));
+  }
+  // SignedDistance.hpp:508-524: one host point, through the same batched kernel.
+  double computeDistance(const PointType& pt) const {
+    PointType* d_pt = nullptr;
+    double* d_out = nullptr;
+    detail::hip_check(hipMalloc((void**)&d_pt, sizeof(PointType)));
+    detail::hip_check(hipMalloc((void**)&d_out, sizeof(double)));
+    double out = 0.0;
+    detail::hip_check(hipMemcpy(d_pt, &pt, sizeof(PointType), hipMemcpyHostToDevice));
+    computeDistances(1, d_pt, d_out);
+    detail::hip_check(hipMemcpy(&out, d_out, sizeof(double), hipMemcpyDeviceToHost));
+    (void)hipFree(d_pt);
+    (void)hipFree(d_out);
+    return out;
+  }
+
+  const axsq_sdf* handle() const { return m_sdf; }
+
+ private:
+  axsq_sdf* m_sdf = nullptr;
+  int m_fpMode = AXSQ_FP64;
+};
+
+}  // namespace quest
+}  // namespace axom

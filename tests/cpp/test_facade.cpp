@@ -1,0 +1,197 @@
+// C++ tests of the axom_mi355x.hpp facade, written like the reference's own
+// tests for this path:
+//   spin/tests/spin_bvh.cpp:279-404 (findBoundingBoxes, 18 cells),
+//   :519-648 (findRays, 15 cells), :792-888 (findPoints, centroid -> cell)
+//   quest/tests/quest_signed_distance.cpp:88-170 (sphere l1/l2/linf)
+// Exit status 0 when every check passes.
+#include <cmath>
+#include <cstdio>
+#include <set>
+#include <vector>
+
+#include "axom_mi355x.hpp"
+
+static int g_fail = 0;
+#define EXPECT_TRUE(c)                                                    \
+  do {                                                                    \
+    if (!(c)) {                                                           \
+      std::printf("FAILED %s:%d: %s\n", __FILE__, __LINE__, #c);          \
+      ++g_fail;                                                           \
+    }                                                                     \
+  } while (0)
+#define EXPECT_EQ(a, b) EXPECT_TRUE((a) == (b))
+#define EXPECT_NEAR(a, b, t) EXPECT_TRUE(std::fabs((a) - (b)) <= (t))
+
+using axom::IndexType;
+using BVH = axom::spin::BVH<3, axom::MI355X_EXEC>;
+using Box = axom::primal::BoundingBox3;
+using Pt = axom::primal::Point3;
+
+template <typename T>
+static T* to_device(const std::vector<T>& h) {
+  T* d = nullptr;
+  axom::detail::hip_check(hipMalloc((void**)&d, sizeof(T) * (h.empty() ? 1 : h.size())));
+  if (!h.empty())
+    axom::detail::hip_check(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return d;
+}
+template <typename T>
+static std::vector<T> to_host(const T* d, size_t n) {
+  std::vector<T> h(n);
+  if (n) axom::detail::hip_check(hipMemcpy(h.data(), d, sizeof(T) * n, hipMemcpyDeviceToHost));
+  return h;
+}
+
+// 3 x 3 x 3 unit cells over [0,3]^3, i fastest (mint::UniformMesh order).
+static std::vector<Box> grid_boxes() {
+  std::vector<Box> b;
+  for (int k = 0; k < 3; ++k)
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) b.push_back(Box{{1.0 * i, 1.0 * j, 1.0 * k}, {i + 1.0, j + 1.0, k + 1.0}});
+  return b;
+}
+
+struct Found {
+  std::vector<IndexType> off, cnt, cand;
+};
+template <typename Fn>
+static Found run_find(IndexType n, Fn&& fn) {
+  axom::Array<IndexType> off(n), cnt(n), cand;
+  fn(off.view(), cnt.view(), cand);
+  return Found{to_host(off.data(), n), to_host(cnt.data(), n), to_host(cand.data(), cand.size())};
+}
+static std::set<IndexType> set_of(const Found& f, int i) {
+  return std::set<IndexType>(f.cand.begin() + f.off[i], f.cand.begin() + f.off[i] + f.cnt[i]);
+}
+
+static void test_bvh_grid() {
+  std::vector<Box> boxes = grid_boxes();
+  Box* d_boxes = to_device(boxes);
+  BVH bvh;
+  bvh.setScaleFactor(1.0);
+  EXPECT_EQ(bvh.initialize(d_boxes, (IndexType)boxes.size()), axom::spin::BVH_BUILD_OK);
+  Box b = bvh.getBounds();
+  for (int d = 0; d < 3; ++d) {
+    EXPECT_NEAR(b.min[d], 0.0, 1e-6);
+    EXPECT_NEAR(b.max[d], 3.0, 1e-6);
+  }
+  // findBoundingBoxes
+  std::vector<Box> q = {Box{{-1, -1, -1}, {2.5, 2.5, 1.5}}, Box{{-1, -1, -1}, {-0.5, -0.5, -0.5}}};
+  Box* d_q = to_device(q);
+  Found fb = run_find(2, [&](auto o, auto c, auto& cand) { bvh.findBoundingBoxes(o, c, cand, 2, d_q); });
+  EXPECT_EQ(fb.cnt[0], 18);
+  EXPECT_EQ(fb.cnt[1], 0);
+  std::set<IndexType> s = set_of(fb, 0);
+  for (IndexType i = 0; i < 18; ++i) EXPECT_TRUE(s.count(i) == 1);
+  // findRays
+  const double o[3] = {-1, -1, -1}, d1[3] = {1, 1, 1}, d2[3] = {-1, -1, -1};
+  std::vector<axom::primal::Ray3> rays = {axom::primal::make_ray(o, d1), axom::primal::make_ray(o, d2)};
+  auto* d_rays = to_device(rays);
+  Found fr = run_find(2, [&](auto oo, auto c, auto& cand) { bvh.findRays(oo, c, cand, 2, d_rays); });
+  EXPECT_EQ(fr.cnt[0], 15);
+  EXPECT_EQ(fr.cnt[1], 0);
+  EXPECT_TRUE(set_of(fr, 0) == std::set<IndexType>({0, 1, 3, 4, 9, 10, 12, 13, 14, 16, 17, 22, 23, 25, 26}));
+  // findPoints: centroids, then shifted by +10
+  std::vector<Pt> c, cs;
+  for (const Box& bx : boxes) {
+    Pt p{{0.5 * (bx.min[0] + bx.max[0]), 0.5 * (bx.min[1] + bx.max[1]), 0.5 * (bx.min[2] + bx.max[2])}};
+    c.push_back(p);
+    cs.push_back(Pt{{p.x[0] + 10, p.x[1] + 10, p.x[2] + 10}});
+  }
+  Pt* d_c = to_device(c);
+  Pt* d_cs = to_device(cs);
+  const IndexType n = (IndexType)c.size();
+  Found fp = run_find(n, [&](auto oo, auto cc, auto& cand) { bvh.findPoints(oo, cc, cand, n, d_c); });
+  for (IndexType i = 0; i < n; ++i) {
+    EXPECT_EQ(fp.cnt[i], 1);
+    EXPECT_EQ(fp.cand[fp.off[i]], i);
+  }
+  Found fs = run_find(n, [&](auto oo, auto cc, auto& cand) { bvh.findPoints(oo, cc, cand, n, d_cs); });
+  for (IndexType i = 0; i < n; ++i) EXPECT_EQ(fs.cnt[i], 0);
+  // offsets of the wrong length are rejected (LinearBVH.hpp:284-285)
+  bool threw = false;
+  try {
+    axom::Array<IndexType> off(3), cnt(n), cand;
+    bvh.findPoints(off.view(), cnt.view(), cand, n, d_c);
+  } catch (const axom::mi355x_error&) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);
+  for (void* p : {(void*)d_boxes, (void*)d_q, (void*)d_rays, (void*)d_c, (void*)d_cs}) (void)hipFree(p);
+}
+
+// quest/tests/quest_test_utilities.hpp:51-151 (sphere), :57-87 (16^3 grid)
+static void test_sdf_sphere() {
+  const double R = 0.5;
+  const int TR = 25, PR = 25;
+  const double deg = M_PI / 180.;
+  std::vector<double> x{0, 0}, y{0, 0}, z{R, -R};
+  const double dphi = 180 * deg / double(PR - 1), dth = 360 * deg / double(TR - 1);
+  for (int i = 0; i < TR; ++i)
+    for (int j = 0; j < PR - 2; ++j) {
+      const double th = i * dth, ph = j * dphi, r = R * std::sin(ph);
+      x.push_back(r * std::cos(th));
+      y.push_back(r * std::sin(th));
+      z.push_back(R * std::cos(ph));
+    }
+  const int pr = PR - 2, stride = pr * TR;
+  std::vector<IndexType> conn;
+  for (int i = 0; i < TR; ++i) conn.insert(conn.end(), {0, (pr * (i + 1) % stride) + 2, pr * i + 2});
+  for (int i = 0; i < TR; ++i) conn.insert(conn.end(), {1, (pr * (i + 1) % stride) + PR - 1, pr * i + PR - 1});
+  for (int i = 0; i < TR; ++i)
+    for (int j = 0; j < PR - 3; ++j) {
+      const int c0 = pr * i + j + 2, c2 = ((pr * (i + 1) + j) % stride) + 3;
+      conn.insert(conn.end(), {c0, c0 + 1, c2});
+      conn.insert(conn.end(), {c0, c2, c2 - 1});
+    }
+  double *dx = to_device(x), *dy = to_device(y), *dz = to_device(z);
+  IndexType* dc = to_device(conn);
+  axom::quest::SignedDistance<3, axom::MI355X_EXEC> sd(dx, dy, dz, (IndexType)x.size(), dc,
+                                                       (IndexType)conn.size() / 3, true, true);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (size_t v = 0; v < x.size(); ++v) {
+    const double p[3] = {x[v], y[v], z[v]};
+    for (int d = 0; d < 3; ++d) {
+      if (p[d] < lo[d]) lo[d] = p[d];
+      if (p[d] > hi[d]) hi[d] = p[d];
+    }
+  }
+  const int N = 16;
+  std::vector<Pt> q;
+  for (int d = 0; d < 3; ++d) {
+    lo[d] -= 2.0;
+    hi[d] += 2.0;
+  }
+  for (int k = 0; k < N; ++k)
+    for (int j = 0; j < N; ++j)
+      for (int i = 0; i < N; ++i)
+        q.push_back(Pt{{lo[0] + i * (hi[0] - lo[0]) / (N - 1), lo[1] + j * (hi[1] - lo[1]) / (N - 1),
+                        lo[2] + k * (hi[2] - lo[2]) / (N - 1)}});
+  Pt* dq = to_device(q);
+  double* dphi_out = nullptr;
+  axom::detail::hip_check(hipMalloc((void**)&dphi_out, sizeof(double) * q.size()));
+  sd.computeDistances((int)q.size(), dq, dphi_out);
+  std::vector<double> phi = to_host(dphi_out, q.size());
+  double l1 = 0, l2 = 0, linf = 0;
+  for (size_t i = 0; i < q.size(); ++i) {
+    const double e = std::sqrt(q[i].x[0] * q[i].x[0] + q[i].x[1] * q[i].x[1] + q[i].x[2] * q[i].x[2]) - R;
+    const double diff = phi[i] - e;
+    EXPECT_NEAR(phi[i], e, 1e-2);
+    l1 += std::fabs(diff);
+    l2 += diff;
+    linf = std::fabs(diff) > linf ? std::fabs(diff) : linf;
+  }
+  l2 = std::sqrt(l2);
+  EXPECT_NEAR(l1, 6.7051997372579715, 1e-3);
+  EXPECT_NEAR(l2, 2.5894400431865519, 1e-3);
+  EXPECT_NEAR(linf, 0.00532092, 1e-3);
+  EXPECT_NEAR(sd.computeDistance(q[0]), phi[0], 0.0);
+  for (void* p : {(void*)dx, (void*)dy, (void*)dz, (void*)dc, (void*)dq, (void*)dphi_out}) (void)hipFree(p);
+}
+
+int main() {
+  test_bvh_grid();
+  test_sdf_sphere();
+  std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "PASSED", g_fail);
+  return g_fail ? 1 : 0;
+}
