@@ -263,6 +263,84 @@ __device__ __forceinline__ int closest_point(const double P[3], const double* __
   return 3;
 }
 
+// The same closest point, written for a lane that runs alone in its wave:
+// every dot product and every candidate division is computed up front as
+// independent chains (instruction-level parallelism instead of a serial
+// chain of early exits), then the regions are tested in the reference's order
+// and the first match is returned. Each returned value is computed by the
+// same operations as in closest_point<>, so the results are bit-identical.
+__device__ __forceinline__ int closest_point_ilp(const double P[3], const double* __restrict__ T,
+                                                 double EPS, double cp[3]) {
+  const double* A = T;
+  const double* B = T + 3;
+  const double* C = T + 6;
+  const double ab0 = B[0] - A[0], ab1 = B[1] - A[1], ab2 = B[2] - A[2];
+  const double ac0 = C[0] - A[0], ac1 = C[1] - A[1], ac2 = C[2] - A[2];
+  const double ap0 = P[0] - A[0], ap1 = P[1] - A[1], ap2 = P[2] - A[2];
+  const double bp0 = P[0] - B[0], bp1 = P[1] - B[1], bp2 = P[2] - B[2];
+  const double cq0 = P[0] - C[0], cq1 = P[1] - C[1], cq2 = P[2] - C[2];
+  const double d1 = dot3(ab0, ab1, ab2, ap0, ap1, ap2);
+  const double d2 = dot3(ac0, ac1, ac2, ap0, ap1, ap2);
+  const double d3 = dot3(ab0, ab1, ab2, bp0, bp1, bp2);
+  const double d4 = dot3(ac0, ac1, ac2, bp0, bp1, bp2);
+  const double d5 = dot3(ab0, ab1, ab2, cq0, cq1, cq2);
+  const double d6 = dot3(ac0, ac1, ac2, cq0, cq1, cq2);
+  const double vc = d1 * d4 - d3 * d2;
+  const double vb = d5 * d2 - d1 * d6;
+  const double va = d3 * d6 - d5 * d4;
+  const double v_ab = d1 / (d1 - d3);
+  const double w_ac = d2 / (d2 - d6);
+  const double w_bc = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+  const double denom = 1.0 / (va + vb + vc);
+  int loc;
+  if (is_leq(d1, 0.0, EPS) && is_leq(d2, 0.0, EPS)) {
+    loc = 0;
+  } else if (is_geq(d3, 0.0, EPS) && is_leq(d4, d3, EPS)) {
+    loc = 1;
+  } else if (is_leq(vc, 0.0, EPS) && is_geq(d1, 0.0, EPS) && is_leq(d3, 0.0, EPS) &&
+             !near_eq(d1, d3, EPS)) {
+    loc = -1;
+  } else if (is_geq(d6, 0.0, EPS) && is_leq(d5, d6, EPS)) {
+    loc = 2;
+  } else if (is_leq(vb, 0.0, EPS) && is_geq(d2, 0.0, EPS) && is_leq(d6, 0.0, EPS)) {
+    loc = -3;
+  } else if (is_leq(va, 0.0, EPS) && is_geq(d4 - d3, 0.0, EPS) && is_geq(d5 - d6, 0.0, EPS)) {
+    loc = -2;
+  } else {
+    loc = 3;
+  }
+  switch (loc) {
+    case 0:
+      cp[0] = A[0]; cp[1] = A[1]; cp[2] = A[2];
+      break;
+    case 1:
+      cp[0] = B[0]; cp[1] = B[1]; cp[2] = B[2];
+      break;
+    case 2:
+      cp[0] = C[0]; cp[1] = C[1]; cp[2] = C[2];
+      break;
+    case -1:
+      cp[0] = A[0] + ab0 * v_ab; cp[1] = A[1] + ab1 * v_ab; cp[2] = A[2] + ab2 * v_ab;
+      break;
+    case -3:
+      cp[0] = A[0] + ac0 * w_ac; cp[1] = A[1] + ac1 * w_ac; cp[2] = A[2] + ac2 * w_ac;
+      break;
+    case -2:
+      cp[0] = B[0] + (C[0] - B[0]) * w_bc;
+      cp[1] = B[1] + (C[1] - B[1]) * w_bc;
+      cp[2] = B[2] + (C[2] - B[2]) * w_bc;
+      break;
+    default: {
+      const double v = vb * denom;
+      const double ww = vc * denom;
+      cp[0] = A[0] + (ab0 * v + ac0 * ww);
+      cp[1] = A[1] + (ab1 * v + ac1 * ww);
+      cp[2] = A[2] + (ab2 * v + ac2 * ww);
+    }
+  }
+  return loc;
+}
+
 // Triangle::normal, primal/geometry/Triangle.hpp:99-102
 __device__ __forceinline__ void tri_normal(const double* T, double n[3]) {
   const double u[3] = {T[3] - T[0], T[4] - T[1], T[5] - T[2]};

@@ -27,6 +27,9 @@
 #ifndef AXSQ_SDF_BLOCK
 #define AXSQ_SDF_BLOCK 256
 #endif
+#ifndef AXSQ_SDF_WARM
+#define AXSQ_SDF_WARM 0
+#endif
 #if defined(AXSQ_SDF_WAVES) && AXSQ_SDF_WAVES > 0
 #define AXSQ_SDF_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(AXSQ_SDF_WAVES)))
 #else
@@ -85,12 +88,14 @@ struct MinCand {
   int32_t pos;
 };
 
-// checkCandidate, SignedDistance.hpp:637-737 (triangle cells)
+// checkCandidate, SignedDistance.hpp:637-737 (triangle cells). ILP selects
+// the closest-point form written for lanes that run alone in their wave.
+template <bool ILP>
 __device__ __forceinline__ void check_candidate(MinCand& m, const double q[3], const double T[9],
                                                 int32_t pos, bool compute_normal) {
   constexpr double EPS = 1e-12;
   double cp[3], wdummy[3];
-  const int loc = closest_point<false>(q, T, EPS, cp, wdummy);
+  const int loc = ILP ? closest_point_ilp(q, T, EPS, cp) : closest_point<false>(q, T, EPS, cp, wdummy);
   const double sq = dot3(cp[0] - q[0], cp[1] - q[1], cp[2] - q[2], cp[0] - q[0], cp[1] - q[1],
                          cp[2] - q[2]);
   const int type = (loc < 0) ? 1 : (loc < 3) ? 0 : 2;
@@ -207,7 +212,7 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
   auto leaf = [&](int32_t pos) {
     double T[9];
     load_tri(a.tris, pos, T);
-    check_candidate(m, q, T, pos, cn);
+    check_candidate<RESUME>(m, q, T, pos, cn);
     if (STATS) ++leaves;
   };
   int st;
@@ -221,9 +226,19 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
         const double dr = dot3(cr0 - q[0], cr1 - q[1], cr2 - q[2], cr0 - q[0], cr1 - q[1], cr2 - q[2]);
         return dl > dr;
       };
-      st = traverse<Node64, double>(
-          a.nodes, a.ninner, todo, w, budget,
-          [&](const double* b) { return sqdist_box(q[0], q[1], q[2], b) <= m.sq; }, pref, leaf);
+      auto pred = [&](const double* b) { return sqdist_box(q[0], q[1], q[2], b) <= m.sq; };
+#if AXSQ_SDF_WARM
+      auto warm = [&](int32_t link) -> int32_t {
+        if (link >= a.ninner || -link - 1 > a.ninner) return 0;
+        const int32_t* p = link >= 0 ? reinterpret_cast<const int32_t*>(a.nodes + link)
+                                     : reinterpret_cast<const int32_t*>(a.tris + (-link - 1));
+        return *p;
+      };
+      st = traverse<Node64, double, RESUME>(a.nodes, a.ninner, todo, w, budget, pred, pref, leaf,
+                                            warm);
+#else
+      st = traverse<Node64, double, RESUME>(a.nodes, a.ninner, todo, w, budget, pred, pref, leaf);
+#endif
     } else {
       float qlo[3], qhi[3], qn[3];
       for (int d = 0; d < 3; ++d) {
@@ -259,7 +274,7 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
         }
         return dl > dr;
       };
-      st = traverse<Node32, float>(a.nodes32, a.ninner, todo, w, budget, pred, pref, leaf);
+      st = traverse<Node32, float, RESUME>(a.nodes32, a.ninner, todo, w, budget, pred, pref, leaf);
     }
     if (RESUME || st != kWalkSuspended) break;
     const uint32_t slot = atomicAdd(a.nsusp, 1u);
@@ -745,6 +760,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   a.group = h->group;
   AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(a.nsusp), 0, 1, s));
   ProfRange pr(mixed ? "sdf_eval_fp32" : "sdf_eval_fp64", s);
+  ProfRange p1(mixed ? "sdf_eval_fp32_pass1" : "sdf_eval_fp64_pass1", s);
   const unsigned g = grid_for(nq, kSdfBlock);
   if (mixed) {
     k_sdf_eval<true, false, false><<<g, kSdfBlock, 0, s>>>(a);
@@ -754,11 +770,13 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     k_sdf_eval<false, false, false><<<g, kSdfBlock, 0, s>>>(a);
   }
   AXSQ_CHECK_HIP(hipGetLastError());
+  p1.stop();
   uint32_t nsusp = 0;
   AXSQ_CHECK_HIP(hipMemcpyAsync(&nsusp, a.nsusp, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   AXSQ_CHECK_HIP(hipStreamSynchronize(s));
   const int64_t nres = std::min<int64_t>(nsusp, cap);
   const_cast<axsq_sdf*>(h)->last_suspended = nres;
+  ProfRange p2(mixed ? "sdf_eval_fp32_pass2" : "sdf_eval_fp64_pass2", s);
   if (nres > 0) {
     const int64_t threads = ((nres + a.group - 1) / a.group) * kWaveLanes;
     const unsigned g2 = grid_for(threads, kSdfBlock);
@@ -771,6 +789,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     }
     AXSQ_CHECK_HIP(hipGetLastError());
   }
+  p2.stop();
   pr.stop();
   return AXSQ_OK;
 }

@@ -117,13 +117,27 @@ enum WalkStatus { kWalkDone = 0, kWalkSuspended = 1, kWalkError = -1 };
 // stack overflow. `ninner` bounds node links and visits: a well-formed tree
 // visits each inner node at most once, so the guards only turn a malformed
 // tree into an error instead of a stray access or an endless walk.
-template <class NodeT, class Scalar, class Pred, class Pref, class Leaf>
+// Prefetch hook: given a child link, touch the cache line the walk will read
+// if it goes there (inner node record or leaf payload).
+struct NoWarm {
+  __device__ __forceinline__ int32_t operator()(int32_t) const { return 0; }
+};
+
+// EAGER_PREF evaluates the nearer-first test at every visit, alongside the box
+// tests (more instructions, shorter dependency chain: for lanes that run
+// alone); otherwise only when both children pass.
+template <class NodeT, class Scalar, bool EAGER_PREF, class Pred, class Pref, class Leaf,
+          class Warm = NoWarm>
 __device__ __forceinline__ int traverse(const NodeT* __restrict__ nodes, int32_t ninner,
                                         LaneStack& todo, Walk& w, int32_t budget, Pred&& pred,
-                                        Pref&& pref, Leaf&& leaf) {
+                                        Pref&& pref, Leaf&& leaf, Warm warm = Warm()) {
   int32_t sp = w.sp, cur = w.cur, visits = w.visits, steps = w.steps;
   int32_t found = 0;
   int status = kWalkDone;
+  // Warming loads of iteration k are consumed in iteration k+1, after that
+  // iteration's node loads have been issued; loads return in order, so the
+  // wait costs nothing beyond the node load itself.
+  int32_t warm_pending = 0, warm_sink = 0;
   while (cur != kBarrier) {
     if (steps >= budget) {
       status = kWalkSuspended;
@@ -134,8 +148,11 @@ __device__ __forceinline__ int traverse(const NodeT* __restrict__ nodes, int32_t
       Scalar lb[6], rb[6];
       int32_t lc, rc, flags;
       load_node(nodes, cur, lb, rb, lc, rc, flags);
+      warm_sink ^= warm_pending;
+      warm_pending = warm(lc) ^ warm(rc);
       ++visits;
       ++steps;
+      const bool swap_eager = EAGER_PREF ? pref(lb, rb) : false;
       const bool in_l = (flags & 1) ? pred(lb) : false;
       const bool in_r = (flags & 2) ? pred(rb) : false;
       if (!in_l && !in_r) {
@@ -144,7 +161,7 @@ __device__ __forceinline__ int traverse(const NodeT* __restrict__ nodes, int32_t
       } else {
         cur = in_l ? lc : rc;
         if (in_l && in_r) {
-          if (pref(lb, rb)) {
+          if (EAGER_PREF ? swap_eager : pref(lb, rb)) {
             const int32_t t = cur;
             cur = rc;
             rc = t;
@@ -175,6 +192,8 @@ __device__ __forceinline__ int traverse(const NodeT* __restrict__ nodes, int32_t
     }
     found = 0;
   }
+  warm_sink ^= warm_pending;
+  asm volatile("" ::"v"(warm_sink));
   w.sp = sp;
   w.cur = cur;
   w.visits = visits;
@@ -190,7 +209,8 @@ __device__ __forceinline__ int32_t traverse(const NodeT* __restrict__ nodes, int
   LaneStack todo;
   todo.lds = lds_stack;
   Walk w = walk_start(todo);
-  const int st = traverse<NodeT, Scalar>(nodes, ninner, todo, w, 0x7fffffff, pred, pref, leaf);
+  const int st =
+      traverse<NodeT, Scalar, false>(nodes, ninner, todo, w, 0x7fffffff, pred, pref, leaf);
   return st == kWalkDone ? w.visits : -1;
 }
 

@@ -119,6 +119,8 @@ def main():
     log("timing FP64 forward+backward")
     el64 = timed("fp64", args.steps, args.warmup)
     n_eval, ms_eval = prof("sdf_eval_fp64")
+    n_p1, ms_p1 = prof("sdf_eval_fp64_pass1")
+    n_p2, ms_p2 = prof("sdf_eval_fp64_pass2")
     n_bwd, ms_bwd = prof("sdf_backward")
     n_ord, ms_ord = prof("sdf_query_order")
     d64 = dist.clone()
@@ -133,6 +135,8 @@ def main():
                                "SignedDistance forward (dist, cp, elem) + adjoint" % nq,
                    "queries": nq, "triangles": int(t.shape[0]), "parallelism": "single GPU"},
         "kernels_ms_per_step": {"sdf_eval_fp64": ms_eval / max(n_eval, 1),
+                                "sdf_eval_fp64_pass1": ms_p1 / max(n_p1, 1),
+                                "sdf_eval_fp64_pass2": ms_p2 / max(n_p2, 1),
                                 "sdf_query_order": ms_ord / max(n_ord, 1),
                                 "sdf_backward": ms_bwd / max(n_bwd, 1)},
     }
