@@ -55,7 +55,82 @@ def parse():
     p.add_argument("--calib-copy", action="store_true",
                    help="also run one 1 GiB device copy (counter calibration)")
     p.add_argument("--cpu-sample", type=int, default=1_000_000)
+    p.add_argument("--workload", default="sdf_l6",
+                   choices=["sdf_l6", "find_points", "torus_mixed", "diff_l7", "shapefit_l8"],
+                   help="sdf_l6 is the headline; the others print their own JSON line")
     return p.parse_args()
+
+
+def other_workload(args, torch, dev):
+    """The remaining synthetic workloads (see axom_amd/synth.py for their inputs)."""
+    from axom_amd import autograd, quest, shapefit, spin, synth
+
+    def sync_time(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    if args.workload == "find_points":
+        boxes, pts = synth.workload1()
+        bx, p = torch.as_tensor(boxes, device=dev), torch.as_tensor(pts, device=dev)
+        bvh = spin.BVH(bx)
+        off = torch.empty(p.shape[0], dtype=torch.int32, device=dev)
+        cnt = torch.empty(p.shape[0], dtype=torch.int32, device=dev)
+        tb = sync_time(lambda: bvh.initialize(bx), args.steps)
+        tq = sync_time(lambda: bvh.findPoints(off, cnt, p.shape[0], p), args.steps)
+        return {"workload": "100,000 boxes U[0,1]^3 (half-extents U[0.001,0.01]), 1,000,000 "
+                            "points, FP64 build + findPoints",
+                "build_mprims_per_s": boxes.shape[0] / tb / 1e6, "build_ms": tb * 1e3,
+                "find_points_queries_per_s": p.shape[0] / tq, "find_ms": tq * 1e3}
+    if args.workload == "torus_mixed":
+        v, t, q = synth.workload3()
+        vd, td, qd = (torch.as_tensor(a, device=dev) for a in (v, t, q))
+        sdf = quest.SignedDistance(vd, td, fp_mode="fp32")
+        n = q.shape[0]
+        d32 = torch.empty(n, dtype=torch.float64, device=dev)
+        d64 = torch.empty(n, dtype=torch.float64, device=dev)
+        t32 = sync_time(lambda: sdf.computeDistances(n, qd, outSgnDist=d32, fp_mode="fp32"),
+                        args.steps)
+        t64 = sync_time(lambda: sdf.computeDistances(n, qd, outSgnDist=d64, fp_mode="fp64"),
+                        max(1, args.steps // 2))
+        rel = (d32 - d64).abs() / d64.abs().clamp_min(1e-300)
+        return {"workload": "torus R=1, r=0.3+0.02 sin(7u) cos(5v), 2,097,152 tris; 16M queries "
+                            "(half box, half |d|<0.01); SDF forward",
+                "fp32_traversal_queries_per_s": n / t32, "fp64_queries_per_s": n / t64,
+                "max_rel_err": float(rel.max()), "rms_rel_err": float(torch.sqrt((rel * rel).mean())),
+                "sign_mismatches": int((torch.signbit(d32) != torch.signbit(d64)).sum())}
+    if args.workload == "diff_l7":
+        v, t, q, g = synth.workload4()
+        vd = torch.as_tensor(v, device=dev).requires_grad_(True)
+        qd = torch.as_tensor(q, device=dev).requires_grad_(True)
+        td = torch.as_tensor(t, device=dev)
+        gd = torch.as_tensor(g, device=dev, dtype=torch.float64)
+        out = {"workload": "icosphere L7 (327,680 tris), radius 1+0.1 sin(4 theta) cos(3 phi); 1M "
+                           "queries at normal offsets U[-0.1,0.1]; SDF forward + adjoint"}
+        for mode in ("fp64", "fp32"):
+            sdf = quest.SignedDistance(vd.detach(), td, fp_mode="fp32")
+
+            def fb():
+                qd.grad = None
+                vd.grad = None
+                autograd.signed_distance(qd, vd, sdf, mode).backward(gd)
+            tt = sync_time(fb, args.steps)
+            out[f"{mode}_fwd_bwd_queries_per_s"] = q.shape[0] / tt
+        return out
+    v, t, q, target = synth.workload5()
+    vd, td, qd, tg = (torch.as_tensor(a, device=dev) for a in (v, t, q, target))
+    _, hist = shapefit.shape_fit(vd, td, qd, tg, iterations=20, lr=1e-3, fp_mode="fp32")
+    return {"workload": "icosphere L8 (1,310,720 tris) fitted to a rounded box (b=(0.8,0.6,0.5), "
+                        "r=0.1); 8M queries U[-1.2,1.2]^3; 20 iterations of rebuild + FP32-traversal "
+                        "forward + adjoint + step lr=1e-3",
+            "iterations": hist,
+            "mean_build_ms": sum(h["build_ms"] for h in hist[1:]) / 19,
+            "mean_forward_ms": sum(h["forward_ms"] for h in hist) / 20,
+            "mean_backward_ms": sum(h["backward_ms"] for h in hist) / 20}
 
 
 def log(msg):
@@ -77,6 +152,9 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     L = _lib.lib()
+    if args.workload != "sdf_l6":
+        print(json.dumps(other_workload(args, torch, dev)), flush=True)
+        return 0
 
     log("generating inputs")
     v, t, q = synth.workload2(nq=args.queries)
