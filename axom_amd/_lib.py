@@ -60,6 +60,7 @@ _SIGS = {
     "axsq_prof_get": (_i32, [ctypes.c_char_p, ctypes.POINTER(_i64), ctypes.POINTER(_dbl)]),
     "axsq_sdf_stats_enable": (_i32, [_vp, _i32]),
     "axsq_sdf_stats_get": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "axsq_sdf_last_suspended": (_i32, [_vp, ctypes.POINTER(_i64)]),
 }
 
 

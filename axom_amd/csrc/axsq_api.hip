@@ -70,6 +70,7 @@ void Workspace::release() {
   tiles.release();
   misc.release();
   flag.release();
+  redo.release();
   susp.release();
   cnt.release();
 }
@@ -381,8 +382,6 @@ int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const doub
   if (const char* e = getenv("AXSQ_XCD_SLICES")) h->xcd_slices = atoi(e) > 1 ? atoi(e) : 1;
   if (const char* e = getenv("AXSQ_DISPERSE")) h->disperse = atoi(e) > 1 ? atoi(e) : 1;
   if (const char* e = getenv("AXSQ_SDF_BUDGET")) h->budget = atoi(e) > 0 ? atoi(e) : 0x7fffffff;
-  if (const char* e = getenv("AXSQ_SDF_GROUP"))
-    h->group = (atoi(e) >= 1 && atoi(e) <= 64) ? atoi(e) : 1;
   int rc = AXSQ_OK;
   auto fail = [&](int code) {
     axsq_sdf_destroy(h);
@@ -563,6 +562,12 @@ int axsq_sdf_stats_get(const axsq_sdf* h, int64_t* visits, int64_t* leaves) {
   if (!h || !visits || !leaves) return AXSQ_ERR_INVALID_ARG;
   *visits = h->last_visits;
   *leaves = h->last_leaves;
+  return AXSQ_OK;
+}
+
+int axsq_sdf_last_suspended(const axsq_sdf* h, int64_t* count) {
+  if (!h || !count) return AXSQ_ERR_INVALID_ARG;
+  *count = h->last_suspended;
   return AXSQ_OK;
 }
 

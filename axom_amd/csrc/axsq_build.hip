@@ -165,7 +165,8 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const double* __restrict__ b
 __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ mc, int32_t inner,
                                                     int32_t* __restrict__ lchild,
                                                     int32_t* __restrict__ rchild,
-                                                    int32_t* __restrict__ parents) {
+                                                    int32_t* __restrict__ parents,
+                                                    int32_t* __restrict__ range) {
   const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= inner) return;
   const int32_t d =
@@ -190,6 +191,8 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
   const int32_t split = i + s * d + (d < 0 ? d : 0);
   const int32_t lo = (i < j) ? i : j;
   const int32_t hi = (i < j) ? j : i;
+  range[2 * i] = lo;
+  range[2 * i + 1] = hi;
   if (lo == split) {
     parents[split + inner] = i;
     lchild[i] = split + inner;
@@ -257,6 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf
                                                   int32_t inner,
                                                   const int32_t* __restrict__ lchild,
                                                   const int32_t* __restrict__ rchild,
+                                                  const int32_t* __restrict__ range,
                                                   Node64* __restrict__ nodes,
                                                   Node32* __restrict__ nodes32) {
   const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -278,7 +282,9 @@ __global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf
   nd.lc = lc;
   nd.rc = rc;
   nd.flags = (box_valid(nd.l) ? 1 : 0) | (box_valid(nd.r) ? 2 : 0);
-  for (int k = 0; k < 5; ++k) nd.pad[k] = 0;
+  nd.lo = range[2 * i];
+  nd.hi = range[2 * i + 1];
+  for (int k = 0; k < 3; ++k) nd.pad[k] = 0;
   nodes[i] = nd;
   if (nodes32) {
     Node32 h;
@@ -395,14 +401,15 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   AXSQ_TRY(ws.k1.ensure(sizeof(uint32_t) * n));
   AXSQ_TRY(ws.v1.ensure(sizeof(uint32_t) * n));
   // flag buffer holds: scaled boxes | partials | params | lchild | rchild |
-  // parents | counters | inner boxes
+  // parents | counters | leaf ranges | inner boxes
   const size_t off_part = sizeof(double) * 6 * n;
   const size_t off_prm = off_part + sizeof(double) * 6 * nparts;
   const size_t off_l = (off_prm + sizeof(BuildParams) + 255) & ~size_t(255);
   const size_t off_r = off_l + sizeof(int32_t) * inner;
   const size_t off_p = off_r + sizeof(int32_t) * inner;
   const size_t off_c = off_p + sizeof(int32_t) * (2 * n - 1);
-  const size_t off_ib = (off_c + sizeof(int32_t) * inner + 255) & ~size_t(255);
+  const size_t off_rg = off_c + sizeof(int32_t) * inner;
+  const size_t off_ib = (off_rg + sizeof(int32_t) * 2 * inner + 255) & ~size_t(255);
   const size_t total = off_ib + sizeof(double) * 6 * inner;
   AXSQ_TRY(ws.flag.ensure(total));
   char* base = ws.flag.as<char>();
@@ -413,6 +420,7 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   int32_t* rch = reinterpret_cast<int32_t*>(base + off_r);
   int32_t* par = reinterpret_cast<int32_t*>(base + off_p);
   int32_t* cnt = reinterpret_cast<int32_t*>(base + off_c);
+  int32_t* rng = reinterpret_cast<int32_t*>(base + off_rg);
   double* ib = reinterpret_cast<double*>(base + off_ib);
 
   ProfRange pr("bvh_build", s);
@@ -429,12 +437,12 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   k_reorder<<<grid_for(n), kBlock, 0, s>>>(scaled, sv, sk, n, b->leaf_boxes, b->leaf_ids,
                                            b->mcodes);
   AXSQ_CHECK_HIP(hipGetLastError());
-  k_karras<<<grid_for(inner), kBlock, 0, s>>>(b->mcodes, inner, lch, rch, par);
+  k_karras<<<grid_for(inner), kBlock, 0, s>>>(b->mcodes, inner, lch, rch, par, rng);
   AXSQ_CHECK_HIP(hipGetLastError());
   AXSQ_TRY(fill_i32(cnt, 0, inner, s));
   k_refit<<<grid_for(n), kBlock, 0, s>>>(b->leaf_boxes, inner, lch, rch, par, cnt, ib);
   AXSQ_CHECK_HIP(hipGetLastError());
-  k_emit<<<grid_for(inner), kBlock, 0, s>>>(b->leaf_boxes, ib, inner, lch, rch, b->nodes,
+  k_emit<<<grid_for(inner), kBlock, 0, s>>>(b->leaf_boxes, ib, inner, lch, rch, rng, b->nodes,
                                             b->nodes32);
   AXSQ_CHECK_HIP(hipGetLastError());
   pr.stop();

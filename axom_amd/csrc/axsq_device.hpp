@@ -23,13 +23,17 @@ struct Box {
 // FP64 inner node: both child boxes and both child links in one 128-byte
 // cache line. Link >= 0: inner node index; < 0: -(leaf position + 1).
 // flags bit 0 / bit 1: left / right box is valid (BoundingBox::isValid of the
-// stored box, precomputed so traversal does not re-test it).
+// stored box, precomputed so traversal does not re-test it). [lo, hi] is the
+// range of leaf positions under the node (Karras 2012: the node's index is lo
+// or hi, and the inner nodes of its subtree are the hi - lo indices starting
+// at lo, or at lo + 1 when the index is hi).
 struct __attribute__((aligned(128))) Node64 {
   double l[6];  // left  min xyz, max xyz
   double r[6];  // right min xyz, max xyz
   int32_t lc, rc;
   int32_t flags;
-  int32_t pad[5];
+  int32_t lo, hi;
+  int32_t pad[3];
 };
 static_assert(sizeof(Node64) == 128, "Node64 must be one cache line");
 

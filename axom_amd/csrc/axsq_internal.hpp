@@ -44,7 +44,7 @@ struct DevBuf {
 
 // Scratch owned by the calling host thread (per device), reused across calls.
 struct Workspace {
-  DevBuf k0, v0, k1, v1, hist, tiles, misc, flag, susp, cnt;
+  DevBuf k0, v0, k1, v1, hist, tiles, misc, flag, susp, cnt, redo;
   int device = -1;
   void release();
 };
@@ -126,8 +126,7 @@ struct axsq_sdf {
   int32_t xcd_slices = 1;  // workgroup slicing over XCDs (1 = off)
   int32_t disperse = 1;    // wave-chunk dispersal groups (1 = off)
   int32_t budget = 2048;   // first-pass step budget per query walk
-  int32_t group = 1;       // second pass: suspended walks per wave
-  int64_t last_suspended = 0;
+  int64_t last_suspended = 0;  // walks the last eval finished in its second pass
 };
 
 namespace axsq {

@@ -20,6 +20,7 @@
 //
 // Adjoint (no reference counterpart): see axsq_sdf_backward in axsq.h.
 #include <algorithm>
+#include <type_traits>
 
 #include "axsq_internal.hpp"
 #include "axsq_traverse.hpp"
@@ -27,8 +28,8 @@
 #ifndef AXSQ_SDF_BLOCK
 #define AXSQ_SDF_BLOCK 256
 #endif
-#ifndef AXSQ_SDF_WARM
-#define AXSQ_SDF_WARM 0
+#ifndef AXSQ_SDF_VOTE
+#define AXSQ_SDF_VOTE 1  // first pass: vote-scheduled walk (0: loop-by-loop)
 #endif
 #if defined(AXSQ_SDF_WAVES) && AXSQ_SDF_WAVES > 0
 #define AXSQ_SDF_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(AXSQ_SDF_WAVES)))
@@ -56,10 +57,11 @@ struct SdfArgs {
   int32_t xcd_slices;  // > 1: XCD-contiguous workgroup slices
   int32_t disperse;    // > 1: spread 64-query wave chunks over this many groups
   int32_t budget;      // first-pass step budget per walk
-  int32_t group;       // second pass: saved walks per wave
   int64_t cap;         // capacity of susp
   struct SdfSuspended* susp;
   uint32_t* nsusp;
+  int32_t* redo;       // queries to run again from the start (room for nq)
+  uint32_t* nredo;
   double dom[6];
   int watertight;
   int compute_sign;
@@ -88,222 +90,105 @@ struct MinCand {
   int32_t pos;
 };
 
-// checkCandidate, SignedDistance.hpp:637-737 (triangle cells). ILP selects
-// the closest-point form written for lanes that run alone in their wave.
-template <bool ILP>
-__device__ __forceinline__ void check_candidate(MinCand& m, const double q[3], const double T[9],
-                                                int32_t pos, bool compute_normal) {
+// The pseudo-normal term a triangle adds when its closest point is shared:
+// code 3 -- an edge: the unit face normal; code 0..2 -- that vertex: the unit
+// face normal times the triangle's angle there, or nothing for a degenerate
+// triangle (SignedDistance.hpp:700-734, Triangle::degenerate Triangle.hpp:327-330).
+// Zero is returned for nothing: adding +0.0 to the running sum changes nothing,
+// since that sum starts at +0.0 and a round-to-nearest sum is -0.0 only when
+// both terms are.
+__device__ __forceinline__ void normal_term(const double T[9], int code, double c[3]) {
+  double n[3], un[3];
+  tri_normal(T, n);
+  if (code == 3) {
+    unit3(n, un);
+    c[0] = un[0];
+    c[1] = un[1];
+    c[2] = un[2];
+    return;
+  }
+  c[0] = c[1] = c[2] = 0.0;
+  const double area = 0.5 * sqrt(dot3(n[0], n[1], n[2], n[0], n[1], n[2]));
+  if (!near_eq(area, 0.0, 1.0e-12)) {
+    const double alpha = tri_angle(T, code);
+    unit3(n, un);
+    c[0] = un[0] * alpha;
+    c[1] = un[1] * alpha;
+    c[2] = un[2] * alpha;
+  }
+}
+
+// checkCandidate, SignedDistance.hpp:637-737 (triangle cells), first-pass
+// form. Most pseudo-normal terms a walk computes are thrown away by a later
+// strictly closer candidate, and in a wave one lane's term costs every lane
+// the instructions (a square root, divisions, an arc cosine). So the walk
+// only records which triangles contribute since the last reset -- leaf
+// position and term code, packed as pos * 4 + code (positions below 2^30) --
+// and the terms are computed and added in the recorded order once the walk
+// has stopped: the same additions in the same order. The list holds
+// kNormList entries; one more sets kListOverflow, and such a walk is run
+// again from the start by the second pass, which adds terms as it goes (a
+// reset also clears the overflow: the terms before it no longer count).
+constexpr int kNormList = 8;
+constexpr int32_t kListOverflow = 1 << 16;
+
+__device__ __forceinline__ void check_candidate_deferred(MinCand& m, const double q[3],
+                                                         const double T[9], int32_t pos,
+                                                         bool compute_normal, lds_i32* list,
+                                                         int32_t& cnt) {
   constexpr double EPS = 1e-12;
   double cp[3], wdummy[3];
-  const int loc = ILP ? closest_point_ilp(q, T, EPS, cp) : closest_point<false>(q, T, EPS, cp, wdummy);
+  const int loc = closest_point<false>(q, T, EPS, cp, wdummy);
   const double sq = dot3(cp[0] - q[0], cp[1] - q[1], cp[2] - q[2], cp[0] - q[0], cp[1] - q[1],
                          cp[2] - q[2]);
   const int type = (loc < 0) ? 1 : (loc < 3) ? 0 : 2;
   const bool shared = (type != 2);
+  const double e0 = m.p[0] - cp[0], e1 = m.p[1] - cp[1], e2 = m.p[2] - cp[2];
+  const bool same = near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., EPS);
   bool upd;
   if (sq < m.sq) {
-    const double e0 = m.p[0] - cp[0], e1 = m.p[1] - cp[1], e2 = m.p[2] - cp[2];
-    const bool clear = !shared || (m.type != type) || !near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., EPS);
+    const bool clear = !shared || (m.type != type) || !same;
     m.sq = sq;
     m.p[0] = cp[0];
     m.p[1] = cp[1];
     m.p[2] = cp[2];
     m.type = type;
     m.pos = pos;
-    if (compute_normal && clear) {
-      m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
-    }
+    if (compute_normal && clear) cnt = 0;
     upd = compute_normal && shared;
   } else {
-    const double e0 = m.p[0] - cp[0], e1 = m.p[1] - cp[1], e2 = m.p[2] - cp[2];
-    upd = compute_normal && shared && (m.type == type) &&
-          near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., EPS);
+    upd = compute_normal && shared && (m.type == type) && same;
   }
   if (upd) {
-    double n[3], un[3];
-    tri_normal(T, n);
-    if (type == 1) {
-      unit3(n, un);
-      m.sn[0] += un[0];
-      m.sn[1] += un[1];
-      m.sn[2] += un[2];
+    if (cnt < kNormList) {
+      list[cnt * kWaveLanes] = (int32_t)((uint32_t)pos * 4u + (uint32_t)(type == 1 ? 3 : loc));
+      ++cnt;
     } else {
-      // Triangle::degenerate (Triangle.hpp:327-330): area = 0.5*|n|
-      const double area = 0.5 * sqrt(dot3(n[0], n[1], n[2], n[0], n[1], n[2]));
-      if (!near_eq(area, 0.0, 1.0e-12)) {
-        const double alpha = tri_angle(T, loc);
-        unit3(n, un);
-        m.sn[0] += un[0] * alpha;
-        m.sn[1] += un[1] * alpha;
-        m.sn[2] += un[2] * alpha;
-      }
+      cnt |= kListOverflow;
     }
   }
 }
 
-// A query whose walk ran out of its step budget in the first pass: the
-// whole walk state, so the second pass continues exactly where it stopped.
-struct SdfSuspended {
-  double sq, p[3], sn[3];
-  int32_t qidx, cur, sp, visits, steps, leaves, type, pos;
-  int32_t stack[kStack];  // entries 1..sp
-};
+// Adds the recorded terms, in order, to sn. `stride` is the distance between
+// consecutive entries (kWaveLanes for a lane's LDS column, 1 for a record).
+template <class P>
+__device__ __forceinline__ void add_terms(double sn[3], const TriRec* __restrict__ tris, P list,
+                                          int32_t stride, int32_t cnt) {
+  for (int32_t k = 0; k < cnt; ++k) {
+    const uint32_t e = (uint32_t)list[k * stride];
+    double T[9], c[3];
+    load_tri(tris, (int32_t)(e >> 2), T);
+    normal_term(T, (int)(e & 3u), c);
+    sn[0] += c[0];
+    sn[1] += c[1];
+    sn[2] += c[2];
+  }
+}
 
-// RESUME = false: every query, in Morton order, with a step budget; a walk
-// that runs out of budget is saved (while there is room) and left to the
-// second pass. RESUME = true: `group` saved walks per wave run to the end.
-// Splitting keeps the rare very long walks (e.g. queries near the centre of
-// a closed surface, where thousands of triangles are almost equidistant) out
-// of the coherent waves of the first pass, which would otherwise all wait
-// for them.
-template <bool MIXED, bool STATS, bool RESUME>
-__global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfArgs a) {
-  __shared__ int32_t s_stack[stack_lds_words(kSdfBlock)];
-  LaneStack todo;
-  todo.lds = lane_stack_base(s_stack);
-  MinCand m;
-  Walk w;
-  int64_t i;
-  int32_t leaves = 0;
-  int32_t budget = a.budget;
-  if (!RESUME) {
-    const int64_t blk =
-        a.xcd_slices > 1 ? group_spread(blockIdx.x, gridDim.x, a.xcd_slices) : blockIdx.x;
-    int64_t t = blk * blockDim.x + threadIdx.x;
-    if (t >= a.nq) return;
-    // Each 64-query chunk (one wave) stays together; optionally consecutive
-    // chunks are spread over distant parts of space.
-    const int64_t nfull = a.nq / kWaveLanes;
-    if (a.disperse > 1 && t / kWaveLanes < nfull)
-      t = group_spread(t / kWaveLanes, nfull, a.disperse) * kWaveLanes + t % kWaveLanes;
-    i = a.order ? (int64_t)a.order[t] : t;
-    m.sq = kDblMax;
-    m.p[0] = m.p[1] = m.p[2] = 0.0;
-    m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
-    m.type = -1;
-    m.pos = 0;
-    w = walk_start(todo);
-  } else {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t lane = t % kWaveLanes;
-    if (lane >= a.group) return;
-    const int64_t slot = (t / kWaveLanes) * a.group + lane;
-    if (slot >= (int64_t)*a.nsusp || slot >= a.cap) return;
-    const SdfSuspended& r = a.susp[slot];
-    i = r.qidx;
-    m.sq = r.sq;
-    for (int k = 0; k < 3; ++k) {
-      m.p[k] = r.p[k];
-      m.sn[k] = r.sn[k];
-    }
-    m.type = r.type;
-    m.pos = r.pos;
-    w.cur = r.cur;
-    w.sp = r.sp;
-    w.visits = r.visits;
-    w.steps = r.steps;
-    leaves = r.leaves;
-    todo.set(0, kBarrier);
-    for (int32_t k = 1; k <= w.sp; ++k) todo.set(k, r.stack[k]);
-    budget = 0x7fffffff;
-  }
-  const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
-  const bool cn = a.compute_sign != 0;
-  auto leaf = [&](int32_t pos) {
-    double T[9];
-    load_tri(a.tris, pos, T);
-    check_candidate<RESUME>(m, q, T, pos, cn);
-    if (STATS) ++leaves;
-  };
-  int st;
-  for (;;) {
-    if (!MIXED) {
-      // LinearBVH.hpp:75-82: nearer child (by centroid distance) first
-      auto pref = [&](const double* l, const double* r) {
-        const double cl0 = 0.5 * (l[0] + l[3]), cl1 = 0.5 * (l[1] + l[4]), cl2 = 0.5 * (l[2] + l[5]);
-        const double cr0 = 0.5 * (r[0] + r[3]), cr1 = 0.5 * (r[1] + r[4]), cr2 = 0.5 * (r[2] + r[5]);
-        const double dl = dot3(cl0 - q[0], cl1 - q[1], cl2 - q[2], cl0 - q[0], cl1 - q[1], cl2 - q[2]);
-        const double dr = dot3(cr0 - q[0], cr1 - q[1], cr2 - q[2], cr0 - q[0], cr1 - q[1], cr2 - q[2]);
-        return dl > dr;
-      };
-      auto pred = [&](const double* b) { return sqdist_box(q[0], q[1], q[2], b) <= m.sq; };
-#if AXSQ_SDF_WARM
-      auto warm = [&](int32_t link) -> int32_t {
-        if (link >= a.ninner || -link - 1 > a.ninner) return 0;
-        const int32_t* p = link >= 0 ? reinterpret_cast<const int32_t*>(a.nodes + link)
-                                     : reinterpret_cast<const int32_t*>(a.tris + (-link - 1));
-        return *p;
-      };
-      st = traverse<Node64, double, RESUME>(a.nodes, a.ninner, todo, w, budget, pred, pref, leaf,
-                                            warm);
-#else
-      st = traverse<Node64, double, RESUME>(a.nodes, a.ninner, todo, w, budget, pred, pref, leaf);
-#endif
-    } else {
-      float qlo[3], qhi[3], qn[3];
-      for (int d = 0; d < 3; ++d) {
-        qlo[d] = __double2float_rd(q[d]);
-        qhi[d] = __double2float_ru(q[d]);
-        qn[d] = (float)q[d];
-      }
-      double last_sq = -1.0;
-      float thr = 0.0f;
-      auto pred = [&](const float* b) {
-        if (m.sq != last_sq) {
-          // relative slack covers the FP32 round-to-nearest error; the
-          // absolute 1e-36 covers errors of subnormal intermediate values
-          last_sq = m.sq;
-          thr = __double2float_ru(m.sq * (1.0 + 1e-6) + 1e-36);
-        }
-        float acc = 0.0f;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const float g = fmaxf(fmaxf(b[d] - qhi[d], qlo[d] - b[3 + d]), 0.0f);
-          acc += g * g;
-        }
-        return acc <= thr;
-      };
-      auto pref = [&](const float* l, const float* r) {
-        float dl = 0.0f, dr = 0.0f;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const float el = 0.5f * (l[d] + l[3 + d]) - qn[d];
-          const float er = 0.5f * (r[d] + r[3 + d]) - qn[d];
-          dl += el * el;
-          dr += er * er;
-        }
-        return dl > dr;
-      };
-      st = traverse<Node32, float, RESUME>(a.nodes32, a.ninner, todo, w, budget, pred, pref, leaf);
-    }
-    if (RESUME || st != kWalkSuspended) break;
-    const uint32_t slot = atomicAdd(a.nsusp, 1u);
-    if (slot < (uint32_t)a.cap) {
-      SdfSuspended& r = a.susp[slot];
-      r.qidx = (int32_t)i;
-      r.sq = m.sq;
-      for (int k = 0; k < 3; ++k) {
-        r.p[k] = m.p[k];
-        r.sn[k] = m.sn[k];
-      }
-      r.type = m.type;
-      r.pos = m.pos;
-      r.cur = w.cur;
-      r.sp = w.sp;
-      r.visits = w.visits;
-      r.steps = w.steps;
-      r.leaves = leaves;
-      for (int32_t k = 1; k <= w.sp; ++k) r.stack[k] = todo.get(k);
-      return;
-    }
-    budget = 0x7fffffff;  // no room left: finish here
-  }
-  if (st == kWalkError) atomicOr(a.status, 1);
-  if (STATS) {
-    atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)(st == kWalkError ? 0 : w.visits));
-    atomicAdd((unsigned long long*)&a.stats[1], (unsigned long long)leaves);
-  }
-  // surface normal (getSurfaceNormal :741-749) and sign (:580-592)
+// Surface normal (getSurfaceNormal, SignedDistance.hpp:741-749), sign
+// (:580-592) and the outputs of query i.
+__device__ __forceinline__ void sdf_finish(const SdfArgs& a, int64_t i, const double q[3],
+                                           const MinCand& m, bool cn) {
   double n[3];
   double T[9];
   load_tri(a.tris, m.pos, T);
@@ -338,6 +223,482 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
   }
   if (a.elem) a.elem[i] = reinterpret_cast<const int32_t*>(a.tris + m.pos)[18];
 }
+
+// A walk the first pass stopped at its step budget: the whole walk state, so
+// the second pass continues exactly where it stopped. The pseudo-normal sum
+// is its list of terms (the sum itself is zero until the terms are added).
+struct SdfSuspended {
+  double sq, p[3];
+  int32_t qidx, cur, sp, visits, steps, leaves, type, pos;
+  int32_t nterms;
+  int32_t terms[kNormList];
+  int32_t stack[kStack];  // entries 1..sp
+};
+
+// box key, pruning test and nearer-first test of the two walks
+template <bool MIXED>
+struct BoxTests {
+  using Scalar = typename std::conditional<MIXED, float, double>::type;
+  double q[3];
+  float qlo[3], qhi[3], qn[3];
+  double last_sq;
+  float thr;
+  __device__ __forceinline__ explicit BoxTests(const double qq[3]) : last_sq(-1.0), thr(0.0f) {
+    for (int d = 0; d < 3; ++d) {
+      q[d] = qq[d];
+      qlo[d] = __double2float_rd(qq[d]);
+      qhi[d] = __double2float_ru(qq[d]);
+      qn[d] = (float)qq[d];
+    }
+  }
+  // squared distance to the box; for the FP32 boxes, the sum of squared gaps
+  // to the interval [rd(q), ru(q)] in round-to-nearest FP32
+  __device__ __forceinline__ Scalar key(const Scalar* b) const {
+    if constexpr (!MIXED) {
+      return sqdist_box(q[0], q[1], q[2], b);
+    } else {
+      float acc = 0.0f;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float g = fmaxf(fmaxf(b[d] - qhi[d], qlo[d] - b[3 + d]), 0.0f);
+        acc += g * g;
+      }
+      return acc;
+    }
+  }
+  // squared_distance(p, box) <= minSqDist; for the FP32 boxes the threshold
+  // ru(minSqDist * (1 + 1e-6) + 1e-36) covers the FP32 rounding errors
+  __device__ __forceinline__ bool keep(Scalar k, double min_sq) {
+    if constexpr (!MIXED) {
+      return k <= min_sq;
+    } else {
+      if (min_sq != last_sq) {
+        last_sq = min_sq;
+        thr = __double2float_ru(min_sq * (1.0 + 1e-6) + 1e-36);
+      }
+      return k <= thr;
+    }
+  }
+  // LinearBVH.hpp:75-82: true when the right child's centroid is nearer
+  __device__ __forceinline__ bool right_first(const Scalar* l, const Scalar* r) const {
+    if constexpr (!MIXED) {
+      const double cl0 = 0.5 * (l[0] + l[3]), cl1 = 0.5 * (l[1] + l[4]), cl2 = 0.5 * (l[2] + l[5]);
+      const double cr0 = 0.5 * (r[0] + r[3]), cr1 = 0.5 * (r[1] + r[4]), cr2 = 0.5 * (r[2] + r[5]);
+      const double dl = dot3(cl0 - q[0], cl1 - q[1], cl2 - q[2], cl0 - q[0], cl1 - q[1], cl2 - q[2]);
+      const double dr = dot3(cr0 - q[0], cr1 - q[1], cr2 - q[2], cr0 - q[0], cr1 - q[1], cr2 - q[2]);
+      return dl > dr;
+    } else {
+      float dl = 0.0f, dr = 0.0f;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float el = 0.5f * (l[d] + l[3 + d]) - qn[d];
+        const float er = 0.5f * (r[d] + r[3 + d]) - qn[d];
+        dl += el * el;
+        dr += er * er;
+      }
+      return dl > dr;
+    }
+  }
+};
+
+// First pass: every query, in Morton order, one lane each, with a step budget
+// and the vote-scheduled walk. A walk over its budget is saved (while there
+// is room) for the second pass; a walk whose term list overflowed is queued to
+// be run again from the start there. Splitting keeps the rare very long walks
+// (queries near the centre of a closed surface, where thousands of triangles
+// are almost equidistant) out of the coherent waves of this pass, which would
+// otherwise all wait for them.
+template <bool MIXED, bool STATS>
+__global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfArgs a) {
+  using NodeT = typename std::conditional<MIXED, Node32, Node64>::type;
+  using Scalar = typename std::conditional<MIXED, float, double>::type;
+  __shared__ int32_t s_stack[stack_lds_words(kSdfBlock)];
+  __shared__ int32_t s_terms[kNormList * kSdfBlock];
+  LaneStack todo;
+  todo.lds = lane_stack_base(s_stack);
+  lds_i32* terms =
+      (lds_i32*)(s_terms + (threadIdx.x / kWaveLanes) * (kNormList * kWaveLanes) +
+                 threadIdx.x % kWaveLanes);
+  const int64_t blk =
+      a.xcd_slices > 1 ? group_spread(blockIdx.x, gridDim.x, a.xcd_slices) : blockIdx.x;
+  int64_t t = blk * blockDim.x + threadIdx.x;
+  if (t >= a.nq) return;
+  // Each 64-query chunk (one wave) stays together; optionally consecutive
+  // chunks are spread over distant parts of space.
+  const int64_t nfull = a.nq / kWaveLanes;
+  if (a.disperse > 1 && t / kWaveLanes < nfull)
+    t = group_spread(t / kWaveLanes, nfull, a.disperse) * kWaveLanes + t % kWaveLanes;
+  const int64_t i = a.order ? (int64_t)a.order[t] : t;
+  MinCand m;
+  m.sq = kDblMax;
+  m.p[0] = m.p[1] = m.p[2] = 0.0;
+  m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
+  m.type = -1;
+  m.pos = 0;
+  Walk w = walk_start(todo);
+  int32_t leaves = 0, nterms = 0;
+  int32_t budget = a.budget;
+  const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
+  const bool cn = a.compute_sign != 0;
+  const NodeT* nodes;
+  if constexpr (MIXED)
+    nodes = a.nodes32;
+  else
+    nodes = a.nodes;
+  BoxTests<MIXED> bt(q);
+  auto pred = [&](const Scalar* b) { return bt.keep(bt.key(b), m.sq); };
+  auto pref = [&](const Scalar* l, const Scalar* r) { return bt.right_first(l, r); };
+  auto leaf = [&](int32_t pos) {
+    double T[9];
+    load_tri(a.tris, pos, T);
+    check_candidate_deferred(m, q, T, pos, cn, terms, nterms);
+    if (STATS) ++leaves;
+  };
+  int st;
+  for (;;) {
+#if AXSQ_SDF_VOTE
+    st = traverse_vote<NodeT, Scalar>(nodes, a.ninner, todo, w, budget, pred, pref, leaf);
+#else
+    st = traverse<NodeT, Scalar>(nodes, a.ninner, todo, w, budget, pred, pref, leaf);
+#endif
+    if (st != kWalkSuspended || (nterms & kListOverflow)) break;
+    const uint32_t slot = atomicAdd(a.nsusp, 1u);
+    if (slot < (uint32_t)a.cap) {
+      SdfSuspended& r = a.susp[slot];
+      r.qidx = (int32_t)i;
+      r.sq = m.sq;
+      for (int k = 0; k < 3; ++k) r.p[k] = m.p[k];
+      r.type = m.type;
+      r.pos = m.pos;
+      r.cur = w.cur;
+      r.sp = w.sp;
+      r.visits = w.visits;
+      r.steps = w.steps;
+      r.leaves = leaves;
+      r.nterms = nterms;
+      for (int32_t k = 0; k < nterms; ++k) r.terms[k] = terms[k * kWaveLanes];
+      for (int32_t k = 1; k <= w.sp; ++k) r.stack[k] = todo.get(k);
+      return;
+    }
+    budget = 0x7fffffff;  // no room left: finish here
+  }
+  if (st == kWalkError) {
+    atomicOr(a.status, 1);
+  } else if (nterms & kListOverflow) {
+    a.redo[atomicAdd(a.nredo, 1u)] = (int32_t)i;  // one slot per query
+    return;
+  }
+  if (STATS) {
+    atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)(st == kWalkError ? 0 : w.visits));
+    atomicAdd((unsigned long long*)&a.stats[1], (unsigned long long)leaves);
+  }
+  add_terms(m.sn, a.tris, (const lds_i32*)terms, kWaveLanes, nterms & (kListOverflow - 1));
+  sdf_finish(a, i, q, m, cn);
+}
+
+// ---------------------------------------------------------------------------
+// Second pass, wave-cooperative form: one walk per wave, either a saved walk
+// or (after a term-list overflow) a walk run again from the start. The walk
+// itself is sequential (its pruning depends on every earlier leaf), so all 64
+// lanes run it in lockstep on wave-uniform state, and the lanes are put to
+// work on what does not depend on that state: when the walk enters a subtree
+// of at most 64 leaves, lane k computes, for this query, the closest point and
+// normal term of leaf lo + k and the two child-box keys and the nearer-first
+// bit of inner node nb + k, in registers. The walk then reads them with
+// v_readlane instead of loading a node or a triangle and running the closest
+// point on the critical path. Each memo entry is computed by the same
+// operations as in the one-lane walk, and the walk visits nodes and leaves in
+// the same order, so the results are bit-identical. The stack is held one
+// entry per lane (entry k in lane k). Normal terms are added as they come.
+constexpr double kSdfEps = 1e-12;  // SignedDistance.hpp:647 (closest point, normal ties)
+
+// checkCandidate's per-triangle values (SignedDistance.hpp:637-737): squared
+// distance, closest point, location type and the pseudo-normal term the
+// triangle adds when it ties an edge or vertex closest point (zero for a face).
+__device__ __forceinline__ void leaf_record(const double q[3], const double T[9], bool cn,
+                                            double& sq, double cp[3], double c[3], int32_t& type) {
+  const int loc = closest_point_ilp(q, T, kSdfEps, cp);
+  sq = dot3(cp[0] - q[0], cp[1] - q[1], cp[2] - q[2], cp[0] - q[0], cp[1] - q[1], cp[2] - q[2]);
+  type = (loc < 0) ? 1 : (loc < 3) ? 0 : 2;
+  c[0] = c[1] = c[2] = 0.0;
+  if (cn && type != 2) normal_term(T, type == 1 ? 3 : loc, c);
+}
+
+// The state update of checkCandidate given leaf_record's values.
+__device__ __forceinline__ void apply_record(MinCand& m, int32_t pos, double sq, const double cp[3],
+                                             const double c[3], int32_t type, bool cn) {
+  const double e0 = m.p[0] - cp[0], e1 = m.p[1] - cp[1], e2 = m.p[2] - cp[2];
+  const bool same = near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., kSdfEps);
+  bool upd;
+  if (sq < m.sq) {
+    const bool clear = (type == 2) || (m.type != type) || !same;
+    m.sq = sq;
+    m.p[0] = cp[0];
+    m.p[1] = cp[1];
+    m.p[2] = cp[2];
+    m.type = type;
+    m.pos = pos;
+    if (cn && clear) m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
+    upd = cn && (type != 2);
+  } else {
+    upd = cn && (type != 2) && (m.type == type) && same;
+  }
+  if (upd) {
+    m.sn[0] += c[0];
+    m.sn[1] += c[1];
+    m.sn[2] += c[2];
+  }
+}
+
+__device__ __forceinline__ int32_t lane_get(int32_t v, int32_t l) {
+  return __builtin_amdgcn_readlane(v, l);
+}
+__device__ __forceinline__ float lane_get(float v, int32_t l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_get(double v, int32_t l) {
+  const int64_t b = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int32_t)b, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(b >> 32), l);
+  return __longlong_as_double((int64_t)(((uint64_t)hi << 32) | lo));
+}
+
+// One lane's share of a subtree memo: leaf lo + lane and inner node nb + lane.
+template <class Scalar>
+struct MemoLane {
+  double sq, p[3], c[3];
+  int32_t type;
+  Scalar lk, rk;  // child-box keys (BoxTests::key)
+  int32_t lc, rc;
+  int32_t bits;   // bit 0/1: child box valid, bit 2: right child first
+};
+
+template <bool MIXED, bool STATS>
+__global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64_t nres,
+                                                                 int64_t nredo) {
+  static_assert(kStack == kWaveLanes, "one stack entry per lane");
+  using NodeT = typename std::conditional<MIXED, Node32, Node64>::type;
+  using Scalar = typename std::conditional<MIXED, float, double>::type;
+  const int32_t lane = (int32_t)threadIdx.x;
+  const int64_t b = blockIdx.x;
+  if (b >= nres + nredo) return;
+  const bool cn = a.compute_sign != 0;
+  int64_t i;
+  MinCand m;
+  int32_t cur, sp, visits, leaves;
+  int32_t stk;
+  if (b < nres) {
+    const SdfSuspended& r = a.susp[b];
+    i = r.qidx;
+    m.sq = r.sq;
+    for (int k = 0; k < 3; ++k) m.p[k] = r.p[k];
+    m.type = r.type;
+    m.pos = r.pos;
+    cur = r.cur;
+    sp = r.sp;
+    visits = r.visits;
+    leaves = r.leaves;
+    if (sp < 0 || sp >= kStack || r.nterms < 0 || r.nterms > kNormList) {
+      if (lane == 0) atomicOr(a.status, 1);
+      return;
+    }
+    stk = (lane == 0) ? kBarrier : (lane <= sp ? r.stack[lane] : 0);
+    m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
+    add_terms(m.sn, a.tris, r.terms, 1, r.nterms);
+  } else {
+    i = a.redo[b - nres];
+    m.sq = kDblMax;
+    m.p[0] = m.p[1] = m.p[2] = 0.0;
+    m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
+    m.type = -1;
+    m.pos = 0;
+    cur = 0;
+    sp = 0;
+    visits = 0;
+    leaves = 0;
+    stk = kBarrier;
+  }
+  const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
+  const NodeT* nodes;
+  if constexpr (MIXED)
+    nodes = a.nodes32;
+  else
+    nodes = a.nodes;
+  BoxTests<MIXED> bt(q);
+
+  MemoLane<Scalar> mem;
+  int32_t clo = 1, chi = 0;  // memo leaves [clo, chi] (empty)
+  int32_t nb = 0, nn = 0;    // memo inner nodes [nb, nb + nn)
+  // a leaf held back by the walk when a new memo replaces the old one (a
+  // memo entry depends only on the query and the leaf, so it stays valid)
+  int32_t h_pos = -1, h_type = 0;
+  double h_sq = 0.0, h_p[3] = {0.0, 0.0, 0.0}, h_c[3] = {0.0, 0.0, 0.0};
+
+  auto fill = [&](int32_t c, int32_t lo, int32_t hi) {
+    const int32_t nb_new = (c == lo) ? lo : lo + 1;
+    const int32_t nn_new = hi - lo;
+    if (lane <= hi - lo) {
+      double T[9];
+      load_tri(a.tris, lo + lane, T);
+      leaf_record(q, T, cn, mem.sq, mem.p, mem.c, mem.type);
+    }
+    if (lane < nn_new) {
+      Scalar lb[6], rb[6];
+      int32_t flags;
+      load_node(nodes, nb_new + lane, lb, rb, mem.lc, mem.rc, flags);
+      mem.lk = bt.key(lb);
+      mem.rk = bt.key(rb);
+      mem.bits = (flags & 3) | (bt.right_first(lb, rb) ? 4 : 0);
+    }
+    clo = lo;
+    chi = hi;
+    nb = nb_new;
+    nn = nn_new;
+  };
+  auto leaf = [&](int32_t pos) {
+    double sq, cp[3], c[3];
+    int32_t type;
+    if (pos >= clo && pos <= chi) {
+      const int32_t k = pos - clo;
+      sq = lane_get(mem.sq, k);
+      for (int d = 0; d < 3; ++d) {
+        cp[d] = lane_get(mem.p[d], k);
+        c[d] = lane_get(mem.c[d], k);
+      }
+      type = lane_get(mem.type, k);
+    } else if (pos == h_pos) {
+      sq = h_sq;
+      for (int d = 0; d < 3; ++d) {
+        cp[d] = h_p[d];
+        c[d] = h_c[d];
+      }
+      type = h_type;
+    } else {
+      double T[9];
+      load_tri(a.tris, pos, T);
+      leaf_record(q, T, cn, sq, cp, c, type);
+    }
+    apply_record(m, pos, sq, cp, c, type, cn);
+  };
+
+  // the walk of traverse() (bvh_traverse.hpp:66-154), resumed
+  int32_t found = 0;
+  bool err = false;
+  while (cur != kBarrier && !err) {
+    if (cur < 0) {  // a walk stops only where it holds no leaf and cur is a node
+      err = true;
+      break;
+    }
+    while (cur >= 0) {
+      if (cur >= a.ninner || visits >= a.ninner) {
+        err = true;
+        break;
+      }
+      if (!(cur >= nb && cur < nb + nn)) {
+        const int2 rg = *reinterpret_cast<const int2*>(&a.nodes[cur].lo);
+        if (rg.x >= 0 && rg.y <= a.ninner && rg.x < rg.y && rg.y - rg.x < kWaveLanes &&
+            (cur == rg.x || cur == rg.y)) {
+          if (found < 0 && found != kBarrier) {
+            const int32_t hp = -found - 1;
+            if (hp >= clo && hp <= chi) {
+              const int32_t k = hp - clo;
+              h_pos = hp;
+              h_sq = lane_get(mem.sq, k);
+              for (int d = 0; d < 3; ++d) {
+                h_p[d] = lane_get(mem.p[d], k);
+                h_c[d] = lane_get(mem.c[d], k);
+              }
+              h_type = lane_get(mem.type, k);
+            }
+          }
+          fill(cur, rg.x, rg.y);
+        }
+      }
+      bool in_l, in_r, swap;
+      int32_t lc, rc;
+      if (cur >= nb && cur < nb + nn) {
+        const int32_t k = cur - nb;
+        const Scalar lk = lane_get(mem.lk, k), rk = lane_get(mem.rk, k);
+        lc = lane_get(mem.lc, k);
+        rc = lane_get(mem.rc, k);
+        const int32_t bits = lane_get(mem.bits, k);
+        in_l = (bits & 1) ? bt.keep(lk, m.sq) : false;
+        in_r = (bits & 2) ? bt.keep(rk, m.sq) : false;
+        swap = (bits & 4) != 0;
+      } else {
+        Scalar lb[6], rb[6];
+        int32_t flags;
+        load_node(nodes, cur, lb, rb, lc, rc, flags);
+        in_l = (flags & 1) ? bt.keep(bt.key(lb), m.sq) : false;
+        in_r = (flags & 2) ? bt.keep(bt.key(rb), m.sq) : false;
+        swap = bt.right_first(lb, rb);
+      }
+      ++visits;
+      if (!in_l && !in_r) {
+        cur = lane_get(stk, sp);
+        sp--;
+      } else {
+        cur = in_l ? lc : rc;
+        if (in_l && in_r) {
+          if (swap) {
+            const int32_t t = cur;
+            cur = rc;
+            rc = t;
+          }
+          sp++;
+          if (sp >= kStack) {
+            err = true;
+            break;
+          }
+          stk = (lane == sp) ? rc : stk;
+        }
+      }
+      if (cur < 0 && !(found < 0)) {
+        found = cur;
+        if (cur != kBarrier) {
+          if (sp < 0) {
+            err = true;
+            break;
+          }
+          cur = lane_get(stk, sp);
+          sp--;
+        }
+      }
+      if (sp < -1) {
+        err = true;
+        break;
+      }
+    }
+    if (err) break;
+    while (found < 0 && found != kBarrier) {
+      const int32_t pos = -found - 1;
+      if (pos > a.ninner) {
+        err = true;
+        break;
+      }
+      leaf(pos);
+      ++leaves;
+      found = cur;
+      if (cur < 0 && cur != kBarrier) {
+        if (sp < 0) {
+          err = true;
+          break;
+        }
+        cur = lane_get(stk, sp);
+        sp--;
+      }
+    }
+    found = 0;
+  }
+  if (lane != 0) return;
+  if (err) atomicOr(a.status, 1);
+  if (STATS) {
+    atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)(err ? 0 : visits));
+    atomicAdd((unsigned long long*)&a.stats[1], (unsigned long long)leaves);
+  }
+  sdf_finish(a, i, q, m, cn);
+}
+
 
 // Morton code of each query in the mesh's bounding box (ordering only).
 __global__ __launch_bounds__(kBlock) void k_query_codes(const double* __restrict__ qx,
@@ -749,43 +1110,46 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     a.order = sv;
   }
   const bool mixed = (fp_mode == AXSQ_FP32_TRAVERSAL);
-  // room for suspended walks: every query up to 2^20, then nq / 8
-  const int64_t cap = nq <= (1 << 20) ? nq : std::max<int64_t>(1 << 20, nq / 8);
+  // room for suspended walks: every query, up to 2^24 records (5.8 GB); a
+  // walk with no room left finishes in the first pass
+  const int64_t cap = std::min<int64_t>(nq, int64_t(1) << 24);
   AXSQ_TRY(ws.susp.ensure(sizeof(SdfSuspended) * (size_t)cap));
+  AXSQ_TRY(ws.redo.ensure(sizeof(int32_t) * (size_t)nq));
   AXSQ_TRY(ws.cnt.ensure(sizeof(uint32_t) * 2));
   a.susp = ws.susp.as<SdfSuspended>();
   a.nsusp = ws.cnt.as<uint32_t>();
+  a.redo = ws.redo.as<int32_t>();
+  a.nredo = ws.cnt.as<uint32_t>() + 1;
   a.cap = cap;
   a.budget = h->budget;
-  a.group = h->group;
-  AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(a.nsusp), 0, 1, s));
+  AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(a.nsusp), 0, 2, s));
   ProfRange pr(mixed ? "sdf_eval_fp32" : "sdf_eval_fp64", s);
   ProfRange p1(mixed ? "sdf_eval_fp32_pass1" : "sdf_eval_fp64_pass1", s);
   const unsigned g = grid_for(nq, kSdfBlock);
   if (mixed) {
-    k_sdf_eval<true, false, false><<<g, kSdfBlock, 0, s>>>(a);
+    k_sdf_eval<true, false><<<g, kSdfBlock, 0, s>>>(a);
   } else if (stats) {
-    k_sdf_eval<false, true, false><<<g, kSdfBlock, 0, s>>>(a);
+    k_sdf_eval<false, true><<<g, kSdfBlock, 0, s>>>(a);
   } else {
-    k_sdf_eval<false, false, false><<<g, kSdfBlock, 0, s>>>(a);
+    k_sdf_eval<false, false><<<g, kSdfBlock, 0, s>>>(a);
   }
   AXSQ_CHECK_HIP(hipGetLastError());
   p1.stop();
-  uint32_t nsusp = 0;
-  AXSQ_CHECK_HIP(hipMemcpyAsync(&nsusp, a.nsusp, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  uint32_t counts[2] = {0, 0};
+  AXSQ_CHECK_HIP(hipMemcpyAsync(counts, a.nsusp, sizeof(counts), hipMemcpyDeviceToHost, s));
   AXSQ_CHECK_HIP(hipStreamSynchronize(s));
-  const int64_t nres = std::min<int64_t>(nsusp, cap);
-  const_cast<axsq_sdf*>(h)->last_suspended = nres;
+  const int64_t nres = std::min<int64_t>(counts[0], cap);
+  const int64_t nredo = std::min<int64_t>(counts[1], nq);
+  const_cast<axsq_sdf*>(h)->last_suspended = nres + nredo;
   ProfRange p2(mixed ? "sdf_eval_fp32_pass2" : "sdf_eval_fp64_pass2", s);
-  if (nres > 0) {
-    const int64_t threads = ((nres + a.group - 1) / a.group) * kWaveLanes;
-    const unsigned g2 = grid_for(threads, kSdfBlock);
+  if (nres + nredo > 0) {
+    const unsigned g2 = (unsigned)(nres + nredo);
     if (mixed) {
-      k_sdf_eval<true, false, true><<<g2, kSdfBlock, 0, s>>>(a);
+      k_sdf_resume_wave<true, false><<<g2, kWaveLanes, 0, s>>>(a, nres, nredo);
     } else if (stats) {
-      k_sdf_eval<false, true, true><<<g2, kSdfBlock, 0, s>>>(a);
+      k_sdf_resume_wave<false, true><<<g2, kWaveLanes, 0, s>>>(a, nres, nredo);
     } else {
-      k_sdf_eval<false, false, true><<<g2, kSdfBlock, 0, s>>>(a);
+      k_sdf_resume_wave<false, false><<<g2, kWaveLanes, 0, s>>>(a, nres, nredo);
     }
     AXSQ_CHECK_HIP(hipGetLastError());
   }

@@ -117,27 +117,13 @@ enum WalkStatus { kWalkDone = 0, kWalkSuspended = 1, kWalkError = -1 };
 // stack overflow. `ninner` bounds node links and visits: a well-formed tree
 // visits each inner node at most once, so the guards only turn a malformed
 // tree into an error instead of a stray access or an endless walk.
-// Prefetch hook: given a child link, touch the cache line the walk will read
-// if it goes there (inner node record or leaf payload).
-struct NoWarm {
-  __device__ __forceinline__ int32_t operator()(int32_t) const { return 0; }
-};
-
-// EAGER_PREF evaluates the nearer-first test at every visit, alongside the box
-// tests (more instructions, shorter dependency chain: for lanes that run
-// alone); otherwise only when both children pass.
-template <class NodeT, class Scalar, bool EAGER_PREF, class Pred, class Pref, class Leaf,
-          class Warm = NoWarm>
+template <class NodeT, class Scalar, class Pred, class Pref, class Leaf>
 __device__ __forceinline__ int traverse(const NodeT* __restrict__ nodes, int32_t ninner,
                                         LaneStack& todo, Walk& w, int32_t budget, Pred&& pred,
-                                        Pref&& pref, Leaf&& leaf, Warm warm = Warm()) {
+                                        Pref&& pref, Leaf&& leaf) {
   int32_t sp = w.sp, cur = w.cur, visits = w.visits, steps = w.steps;
   int32_t found = 0;
   int status = kWalkDone;
-  // Warming loads of iteration k are consumed in iteration k+1, after that
-  // iteration's node loads have been issued; loads return in order, so the
-  // wait costs nothing beyond the node load itself.
-  int32_t warm_pending = 0, warm_sink = 0;
   while (cur != kBarrier) {
     if (steps >= budget) {
       status = kWalkSuspended;
@@ -148,11 +134,8 @@ __device__ __forceinline__ int traverse(const NodeT* __restrict__ nodes, int32_t
       Scalar lb[6], rb[6];
       int32_t lc, rc, flags;
       load_node(nodes, cur, lb, rb, lc, rc, flags);
-      warm_sink ^= warm_pending;
-      warm_pending = warm(lc) ^ warm(rc);
       ++visits;
       ++steps;
-      const bool swap_eager = EAGER_PREF ? pref(lb, rb) : false;
       const bool in_l = (flags & 1) ? pred(lb) : false;
       const bool in_r = (flags & 2) ? pred(rb) : false;
       if (!in_l && !in_r) {
@@ -161,7 +144,7 @@ __device__ __forceinline__ int traverse(const NodeT* __restrict__ nodes, int32_t
       } else {
         cur = in_l ? lc : rc;
         if (in_l && in_r) {
-          if (EAGER_PREF ? swap_eager : pref(lb, rb)) {
+          if (pref(lb, rb)) {
             const int32_t t = cur;
             cur = rc;
             rc = t;
@@ -192,8 +175,133 @@ __device__ __forceinline__ int traverse(const NodeT* __restrict__ nodes, int32_t
     }
     found = 0;
   }
-  warm_sink ^= warm_pending;
-  asm volatile("" ::"v"(warm_sink));
+  w.sp = sp;
+  w.cur = cur;
+  w.visits = visits;
+  w.steps = steps;
+  return status;
+}
+
+// The same walk, scheduled by vote across the wave. Each lane runs exactly the
+// sequence of node visits and leaf actions of traverse() above, but the wave
+// no longer runs the inner (node) loop until every lane has found its next
+// leaf and then the leaf loop until every lane is through its leaves: at each
+// step the lanes whose next action is a leaf action and those whose next
+// action is a node visit are counted, and the wave runs one leaf action (for
+// the first kind) if they are at least half of the lanes still walking, else
+// one node visit (for the second kind). A lane's own order of actions, and so
+// every pruning decision, is unchanged; only which lanes wait changes. With
+// walks of uneven shape in one wave this keeps about half the lanes busy in
+// each step where the loop-by-loop schedule keeps a fifth busy. All lanes of
+// the wave that are still running must call it together.
+template <class NodeT, class Scalar, class Pred, class Pref, class Leaf>
+__device__ __forceinline__ int traverse_vote(const NodeT* __restrict__ nodes, int32_t ninner,
+                                             LaneStack& todo, Walk& w, int32_t budget, Pred&& pred,
+                                             Pref&& pref, Leaf&& leaf) {
+  enum { kInner = 0, kLeaf = 1, kIdle = 2 };
+  int32_t sp = w.sp, cur = w.cur, visits = w.visits, steps = w.steps;
+  int32_t found = 0;
+  int status = kWalkDone;
+  int phase = kIdle;
+  // top of the outer loop of traverse(): done, out of budget, or a node visit next
+  auto top = [&]() {
+    if (cur == kBarrier) {
+      phase = kIdle;
+    } else if (steps >= budget) {
+      status = kWalkSuspended;
+      phase = kIdle;
+    } else if (cur < 0) {  // only reached from a malformed saved state
+      status = kWalkError;
+      phase = kIdle;
+    } else {
+      phase = kInner;
+    }
+  };
+  // end of a leaf-loop check: leave the leaf loop when no leaf is held
+  auto leaf_loop_test = [&]() {
+    if (!(found < 0 && found != kBarrier)) {
+      found = 0;
+      top();
+    }
+  };
+  top();
+  for (;;) {
+    const uint64_t want_node = __ballot(phase == kInner);
+    const uint64_t want_leaf = __ballot(phase == kLeaf);
+    if ((want_node | want_leaf) == 0) break;
+    const int n_node = __popcll(want_node), n_leaf = __popcll(want_leaf);
+    const bool leaf_step = n_leaf > 0 && 2 * n_leaf >= n_node + n_leaf;
+    if (leaf_step) {
+      if (phase == kLeaf) {
+        if (-found - 1 > ninner) {  // ninner + 1 leaves
+          status = kWalkError;
+          phase = kIdle;
+        } else {
+          leaf(-found - 1);
+          ++steps;
+          found = cur;
+          if (cur < 0 && cur != kBarrier) {
+            if (sp < 0) {
+              status = kWalkError;
+              phase = kIdle;
+            } else {
+              cur = todo.get(sp);
+              sp--;
+            }
+          }
+          if (phase == kLeaf) leaf_loop_test();
+        }
+      }
+    } else if (phase == kInner) {
+      if (cur >= ninner || visits >= ninner) {
+        status = kWalkError;
+        phase = kIdle;
+      } else {
+        Scalar lb[6], rb[6];
+        int32_t lc, rc, flags;
+        load_node(nodes, cur, lb, rb, lc, rc, flags);
+        ++visits;
+        ++steps;
+        const bool in_l = (flags & 1) ? pred(lb) : false;
+        const bool in_r = (flags & 2) ? pred(rb) : false;
+        bool overflow = false;
+        if (!in_l && !in_r) {
+          cur = todo.get(sp);
+          sp--;
+        } else {
+          cur = in_l ? lc : rc;
+          if (in_l && in_r) {
+            if (pref(lb, rb)) {
+              const int32_t t = cur;
+              cur = rc;
+              rc = t;
+            }
+            sp++;
+            if (sp >= kStack)
+              overflow = true;
+            else
+              todo.set(sp, rc);
+          }
+        }
+        if (overflow) {
+          status = kWalkError;
+          phase = kIdle;
+        } else {
+          if (cur < 0 && !(found < 0)) {
+            found = cur;
+            if (cur != kBarrier) {
+              cur = todo.get(sp);
+              sp--;
+            }
+          }
+          if (cur < 0) {  // the inner loop ends: on to the held leaves
+            phase = kLeaf;
+            leaf_loop_test();
+          }
+        }
+      }
+    }
+  }
   w.sp = sp;
   w.cur = cur;
   w.visits = visits;
@@ -209,8 +317,7 @@ __device__ __forceinline__ int32_t traverse(const NodeT* __restrict__ nodes, int
   LaneStack todo;
   todo.lds = lds_stack;
   Walk w = walk_start(todo);
-  const int st =
-      traverse<NodeT, Scalar, false>(nodes, ninner, todo, w, 0x7fffffff, pred, pref, leaf);
+  const int st = traverse<NodeT, Scalar>(nodes, ninner, todo, w, 0x7fffffff, pred, pref, leaf);
   return st == kWalkDone ? w.visits : -1;
 }
 

@@ -120,6 +120,12 @@ class SignedDistance:
         L.check(L.lib().axsq_sdf_stats_get(self._h, ctypes.byref(v), ctypes.byref(lf)))
         return v.value, lf.value
 
+    def lastSuspended(self):
+        """Walks of the last computeDistances finished by the second pass."""
+        n = ctypes.c_int64()
+        L.check(L.lib().axsq_sdf_last_suspended(self._h, ctypes.byref(n)))
+        return n.value
+
 
 # ---- quest C-style interface (quest/interface/signed_distance.{hpp,cpp}) ----
 class _Params:

@@ -126,8 +126,9 @@ def other_workload(args, torch, dev):
     _, hist = shapefit.shape_fit(vd, td, qd, tg, iterations=20, lr=1e-3, fp_mode="fp32")
     return {"workload": "icosphere L8 (1,310,720 tris) fitted to a rounded box (b=(0.8,0.6,0.5), "
                         "r=0.1); 8M queries U[-1.2,1.2]^3; 20 iterations of rebuild + FP32-traversal "
-                        "forward + adjoint + step lr=1e-3",
+                        "forward + loss ||d - target||^2 + adjoint + vertex step of length lr=1e-3",
             "iterations": hist,
+            "rms_residual_first_last": [hist[0]["rms_residual"], hist[-1]["rms_residual"]],
             "mean_build_ms": sum(h["build_ms"] for h in hist[1:]) / 19,
             "mean_forward_ms": sum(h["forward_ms"] for h in hist) / 20,
             "mean_backward_ms": sum(h["backward_ms"] for h in hist) / 20}
@@ -217,6 +218,7 @@ def main():
                                 "sdf_eval_fp64_pass2": ms_p2 / max(n_p2, 1),
                                 "sdf_query_order": ms_ord / max(n_ord, 1),
                                 "sdf_backward": ms_bwd / max(n_bwd, 1)},
+        "second_pass_walks": sdf.lastSuspended(),
     }
 
     # roofline of the dominant kernel: visits / leaf checks of the traversal

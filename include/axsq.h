@@ -169,6 +169,11 @@ int axsq_prof_get(const char* name, int64_t* launches, double* total_ms);
 int axsq_sdf_stats_enable(axsq_sdf* sdf, int on);
 int axsq_sdf_stats_get(const axsq_sdf* sdf, int64_t* visits, int64_t* leaves);
 
+/* Number of query walks of the last axsq_sdf_eval that ran past the first
+ * pass's step budget and were finished by the second pass (no reference
+ * counterpart: the reference runs every walk in one pass). */
+int axsq_sdf_last_suspended(const axsq_sdf* sdf, int64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -185,3 +185,78 @@ def test_traversal_stats_match_oracle():
     visits, leaves = sdf.lastStats()
     o = orc.OracleSDF(v, t).eval(q, want_stats=True)
     assert visits == int(o["visits"].sum()) and leaves == int(o["leaves"].sum())
+
+
+def _sdf_with(monkeypatch, v, t, budget, **kw):
+    # the first pass's step budget is read when the SignedDistance is created
+    monkeypatch.setenv("AXSQ_SDF_BUDGET", str(budget))
+    return quest.SignedDistance(_t(v), _t(t), **kw)
+
+
+@pytest.mark.parametrize("budget", [1, 37, 400])
+def test_suspended_walks_resume_bit_exact(monkeypatch, budget):
+    """Walks cut off by a small step budget and finished by the second pass
+    give the oracle's answers: budget 1 sends every walk to the second pass."""
+    v, t, q = synth.workload2(nq=20_000, level=4)
+    q = np.concatenate([q, 0.05 * synth.uniform(11, 0, 6000, -1, 1).reshape(-1, 3)])
+    sdf = _sdf_with(monkeypatch, v, t, budget)
+    g = _eval(sdf, q)
+    assert sdf.lastSuspended() > (q.shape[0] // 2 if budget == 1 else 0)
+    _compare(g, orc.OracleSDF(v, t).eval(q, nthreads=8))
+
+
+@pytest.mark.parametrize("watertight,compute_sign", [(False, True), (True, False)])
+def test_suspended_walks_open_surface_and_unsigned(monkeypatch, watertight, compute_sign):
+    v, t = synth.torus(96, 64)
+    q = np.concatenate([synth.surface_samples(v, t, 5, 0, 20_000, -0.01, 0.01),
+                        synth.uniform(5, 1, 60_000, -1.5, 1.5).reshape(-1, 3)])
+    sdf = _sdf_with(monkeypatch, v, t, 1, isWatertight=watertight, computeSign=compute_sign)
+    _compare(_eval(sdf, q), orc.OracleSDF(v, t, watertight=watertight,
+                                          compute_sign=compute_sign).eval(q, nthreads=8))
+
+
+def test_suspended_walks_fp32_traversal_unchanged(monkeypatch):
+    """The FP32-traversal form visits other nodes than the FP64 form, so its
+    check is against itself: cut into two passes it gives the one-pass answer
+    bit for bit (distances, points, normals, elements)."""
+    v, t, q = synth.workload2(nq=30_000, level=4)
+    one = _eval(_sdf_with(monkeypatch, v, t, 0, fp_mode="fp32"), q, "fp32")
+    two_sdf = _sdf_with(monkeypatch, v, t, 1, fp_mode="fp32")
+    two = _eval(two_sdf, q, "fp32")
+    assert two_sdf.lastSuspended() > q.shape[0] // 2
+    for k in ("dist", "cp", "normal", "elem"):
+        assert np.array_equal(one[k], two[k]), k
+
+
+def test_suspended_walks_stats_match_oracle(monkeypatch):
+    v, t, q = synth.workload2(nq=20_000, level=4)
+    sdf = _sdf_with(monkeypatch, v, t, 1)
+    sdf.enableStats(True)
+    sdf.computeDistances(q.shape[0], _t(q))
+    visits, leaves = sdf.lastStats()
+    o = orc.OracleSDF(v, t).eval(q, want_stats=True)
+    assert visits == int(o["visits"].sum()) and leaves == int(o["leaves"].sum())
+
+
+@pytest.mark.parametrize("budget", [0, 1])
+def test_many_tied_triangles_at_a_vertex(monkeypatch, budget):
+    """Queries whose closest point is a pole of a UV sphere, where 25 triangles
+    meet: more pseudo-normal terms than the first pass records, so those walks
+    are run again from the start by the second pass. Normals, signs and the
+    winning element must still be the oracle's."""
+    s = GOLD["sdf_sphere"]
+    v, t = refmeshes.sphere_uv_mesh(s["center"], s["radius"], s["theta_res"], s["phi_res"])
+    z = np.linspace(0.55, 2.0, 64)
+    e = 1e-9 * synth.uniform(4, 0, 128, -1, 1).reshape(-1, 2)
+    poles = np.concatenate([np.stack([0 * z, 0 * z, z], 1), np.stack([0 * z, 0 * z, -z], 1),
+                            np.stack([e[:64, 0], e[:64, 1], z], 1),
+                            np.stack([e[64:, 0], e[64:, 1], -z], 1)])
+    q = np.concatenate([poles, synth.uniform(4, 1, 30_000, -1.0, 1.0).reshape(-1, 3)])
+    sdf = _sdf_with(monkeypatch, v, t, budget)
+    g = _eval(sdf, q)
+    assert sdf.lastSuspended() >= 128  # the exact-axis pole queries at least
+    o = orc.OracleSDF(v, t).eval(q, nthreads=8)
+    _compare(g, o)
+    # the pole queries are outside, with the pole as closest point
+    assert np.all(g["dist"][:128] > 0)
+    assert np.array_equal(g["cp"][:64], np.tile(v[0], (64, 1)))

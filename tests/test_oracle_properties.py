@@ -71,6 +71,40 @@ def _check_tree(ex, n, scaled):
         for c in (ex["lchild"][i], ex["rchild"][i]):
             assert ex["parents"][c] == i
     assert ex["parents"][0] == -1
+    _check_subtree_ranges(ch, inner)
+
+
+def _check_subtree_ranges(ch, inner):
+    # Karras numbering: the leaves under inner node i are a contiguous range
+    # [lo, hi] with i == lo or i == hi, and its subtree's inner nodes are the
+    # hi - lo indices from lo (i == lo) or from lo + 1 (i == hi). The GPU's
+    # wave-cooperative second pass stores [lo, hi] per node and relies on this
+    # to memoise a subtree with contiguous index ranges.
+    lo, hi = [0] * inner, [0] * inner
+    ins = [None] * inner
+    order, stack = [], [0]
+    while stack:
+        i = stack.pop()
+        order.append(i)
+        for c in (ch[2 * i], ch[2 * i + 1]):
+            if c >= 0:
+                stack.append(c // 2)
+    for i in reversed(order):
+        l_set, i_set = [], [i]
+        for c in (ch[2 * i], ch[2 * i + 1]):
+            if c < 0:
+                l_set.append((-c - 1, -c - 1))
+            else:
+                j = c // 2
+                l_set.append((lo[j], hi[j]))
+                i_set.extend(ins[j])
+        (a0, a1), (b0, b1) = l_set
+        assert a1 + 1 == b0, "children's leaf ranges must be adjacent"
+        lo[i], hi[i] = a0, b1
+        assert i in (lo[i], hi[i])
+        first = lo[i] if i == lo[i] else lo[i] + 1
+        assert sorted(i_set) == list(range(first, first + hi[i] - lo[i]))
+        ins[i] = i_set
 
 
 @pytest.mark.parametrize("n", [2, 5, 64, 3000])
