@@ -28,6 +28,9 @@
 #ifndef AXSQ_SDF_BLOCK
 #define AXSQ_SDF_BLOCK 256
 #endif
+#ifndef AXSQ_SDF_P1_ILP
+#define AXSQ_SDF_P1_ILP 0  // first pass: branch-free closest point
+#endif
 #ifndef AXSQ_SDF_VOTE
 #define AXSQ_SDF_VOTE 1  // first pass: vote-scheduled walk (0: loop-by-loop)
 #endif
@@ -137,8 +140,13 @@ __device__ __forceinline__ void check_candidate_deferred(MinCand& m, const doubl
                                                          bool compute_normal, lds_i32* list,
                                                          int32_t& cnt) {
   constexpr double EPS = 1e-12;
-  double cp[3], wdummy[3];
+  double cp[3];
+#if AXSQ_SDF_P1_ILP
+  const int loc = closest_point_ilp(q, T, EPS, cp);
+#else
+  double wdummy[3];
   const int loc = closest_point<false>(q, T, EPS, cp, wdummy);
+#endif
   const double sq = dot3(cp[0] - q[0], cp[1] - q[1], cp[2] - q[2], cp[0] - q[0], cp[1] - q[1],
                          cp[2] - q[2]);
   const int type = (loc < 0) ? 1 : (loc < 3) ? 0 : 2;
@@ -234,6 +242,7 @@ struct SdfSuspended {
   int32_t terms[kNormList];
   int32_t stack[kStack];  // entries 1..sp
 };
+static_assert(sizeof(SdfSuspended) == 360, "record size quoted in the capacity comment");
 
 // box key, pruning test and nearer-first test of the two walks
 template <bool MIXED>
@@ -413,20 +422,21 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
 constexpr double kSdfEps = 1e-12;  // SignedDistance.hpp:647 (closest point, normal ties)
 
 // checkCandidate's per-triangle values (SignedDistance.hpp:637-737): squared
-// distance, closest point, location type and the pseudo-normal term the
-// triangle adds when it ties an edge or vertex closest point (zero for a face).
-__device__ __forceinline__ void leaf_record(const double q[3], const double T[9], bool cn,
-                                            double& sq, double cp[3], double c[3], int32_t& type) {
+// distance, closest point, location type and the code of the pseudo-normal
+// term the triangle adds if it ties an edge (3) or vertex (0..2) closest
+// point. The term itself is only computed when it is added (rarely).
+__device__ __forceinline__ void leaf_record(const double q[3], const double T[9], double& sq,
+                                            double cp[3], int32_t& type, int32_t& code) {
   const int loc = closest_point_ilp(q, T, kSdfEps, cp);
   sq = dot3(cp[0] - q[0], cp[1] - q[1], cp[2] - q[2], cp[0] - q[0], cp[1] - q[1], cp[2] - q[2]);
   type = (loc < 0) ? 1 : (loc < 3) ? 0 : 2;
-  c[0] = c[1] = c[2] = 0.0;
-  if (cn && type != 2) normal_term(T, type == 1 ? 3 : loc, c);
+  code = (loc < 0) ? 3 : loc;
 }
 
 // The state update of checkCandidate given leaf_record's values.
-__device__ __forceinline__ void apply_record(MinCand& m, int32_t pos, double sq, const double cp[3],
-                                             const double c[3], int32_t type, bool cn) {
+__device__ __forceinline__ void apply_record(MinCand& m, const TriRec* __restrict__ tris,
+                                             int32_t pos, double sq, const double cp[3],
+                                             int32_t type, int32_t code, bool cn) {
   const double e0 = m.p[0] - cp[0], e1 = m.p[1] - cp[1], e2 = m.p[2] - cp[2];
   const bool same = near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., kSdfEps);
   bool upd;
@@ -444,6 +454,9 @@ __device__ __forceinline__ void apply_record(MinCand& m, int32_t pos, double sq,
     upd = cn && (type != 2) && (m.type == type) && same;
   }
   if (upd) {
+    double T[9], c[3];
+    load_tri(tris, pos, T);
+    normal_term(T, code, c);
     m.sn[0] += c[0];
     m.sn[1] += c[1];
     m.sn[2] += c[2];
@@ -466,11 +479,13 @@ __device__ __forceinline__ double lane_get(double v, int32_t l) {
 // One lane's share of a subtree memo: leaf lo + lane and inner node nb + lane.
 template <class Scalar>
 struct MemoLane {
-  double sq, p[3], c[3];
-  int32_t type;
+  double sq, p[3];
+  int32_t type, code;
   Scalar lk, rk;  // child-box keys (BoxTests::key)
   int32_t lc, rc;
   int32_t bits;   // bit 0/1: child box valid, bit 2: right child first
+  int32_t pn;     // this node's parent in the memo, 2 * index + (1 if right child); -1: root
+  int32_t pl;     // this leaf's parent in the memo, 2 * index + (1 if right child)
 };
 
 template <bool MIXED, bool STATS>
@@ -479,6 +494,7 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
   static_assert(kStack == kWaveLanes, "one stack entry per lane");
   using NodeT = typename std::conditional<MIXED, Node32, Node64>::type;
   using Scalar = typename std::conditional<MIXED, float, double>::type;
+  __shared__ int32_t s_pn[kWaveLanes], s_pl[kWaveLanes], s_seq[2 * kWaveLanes];
   const int32_t lane = (int32_t)threadIdx.x;
   const int64_t b = blockIdx.x;
   if (b >= nres + nredo) return;
@@ -532,15 +548,36 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
   // a leaf held back by the walk when a new memo replaces the old one (a
   // memo entry depends only on the query and the leaf, so it stays valid)
   int32_t h_pos = -1, h_type = 0;
-  double h_sq = 0.0, h_p[3] = {0.0, 0.0, 0.0}, h_c[3] = {0.0, 0.0, 0.0};
+  int32_t h_code = 0;
+  double h_sq = 0.0, h_p[3] = {0.0, 0.0, 0.0};
 
+  // Wave masks over the memo, bit k for lane k: the node's left / right child
+  // passes the pruning test at the current minimum, the node visits its right
+  // child first, the leaf would change the running state (a strictly closer
+  // point, or a pseudo-normal term to add). The minimum only changes at a leaf
+  // of the last kind, after which the masks are recomputed, so a node visit is
+  // a few bit tests and a leaf without effect costs nothing.
+  uint64_t m_l = 0, m_r = 0, m_sw = 0, m_eff = 0;
+  auto remask = [&]() {
+    const bool node_ok = lane < nn;
+    m_l = __ballot(node_ok && (mem.bits & 1) && bt.keep(mem.lk, m.sq));
+    m_r = __ballot(node_ok && (mem.bits & 2) && bt.keep(mem.rk, m.sq));
+    bool eff = false;
+    if (lane <= chi - clo) {
+      const double e0 = m.p[0] - mem.p[0], e1 = m.p[1] - mem.p[1], e2 = m.p[2] - mem.p[2];
+      eff = (mem.sq < m.sq) ||
+            (cn && mem.type != 2 && m.type == mem.type &&
+             near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., kSdfEps));
+    }
+    m_eff = __ballot(eff);
+  };
   auto fill = [&](int32_t c, int32_t lo, int32_t hi) {
     const int32_t nb_new = (c == lo) ? lo : lo + 1;
     const int32_t nn_new = hi - lo;
     if (lane <= hi - lo) {
       double T[9];
       load_tri(a.tris, lo + lane, T);
-      leaf_record(q, T, cn, mem.sq, mem.p, mem.c, mem.type);
+      leaf_record(q, T, mem.sq, mem.p, mem.type, mem.code);
     }
     if (lane < nn_new) {
       Scalar lb[6], rb[6];
@@ -550,46 +587,302 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
       mem.rk = bt.key(rb);
       mem.bits = (flags & 3) | (bt.right_first(lb, rb) ? 4 : 0);
     }
+    // parent links inside the memo, by scatter through LDS
+    s_pn[lane] = -1;
+    s_pl[lane] = -1;
+    __syncthreads();
+    if (lane < nn_new) {
+      const int32_t ch[2] = {mem.lc, mem.rc};
+      for (int side = 0; side < 2; ++side) {
+        const int32_t c2 = ch[side];
+        if (c2 >= 0) {
+          if (c2 - nb_new >= 0 && c2 - nb_new < nn_new) s_pn[c2 - nb_new] = 2 * lane + side;
+        } else {
+          const int32_t j = (-c2 - 1) - lo;
+          if (j >= 0 && j <= hi - lo) s_pl[j] = 2 * lane + side;
+        }
+      }
+    }
+    __syncthreads();
+    mem.pn = s_pn[lane];
+    mem.pl = s_pl[lane];
     clo = lo;
     chi = hi;
     nb = nb_new;
     nn = nn_new;
+    m_sw = __ballot(lane < nn && (mem.bits & 4));
+    remask();
   };
   auto leaf = [&](int32_t pos) {
-    double sq, cp[3], c[3];
-    int32_t type;
+    double sq, cp[3];
+    int32_t type, code;
     if (pos >= clo && pos <= chi) {
       const int32_t k = pos - clo;
+      if (!((m_eff >> k) & 1)) return;  // no change to the running state
       sq = lane_get(mem.sq, k);
-      for (int d = 0; d < 3; ++d) {
-        cp[d] = lane_get(mem.p[d], k);
-        c[d] = lane_get(mem.c[d], k);
-      }
+      for (int d = 0; d < 3; ++d) cp[d] = lane_get(mem.p[d], k);
       type = lane_get(mem.type, k);
+      code = lane_get(mem.code, k);
     } else if (pos == h_pos) {
       sq = h_sq;
-      for (int d = 0; d < 3; ++d) {
-        cp[d] = h_p[d];
-        c[d] = h_c[d];
-      }
+      for (int d = 0; d < 3; ++d) cp[d] = h_p[d];
       type = h_type;
+      code = h_code;
     } else {
       double T[9];
       load_tri(a.tris, pos, T);
-      leaf_record(q, T, cn, sq, cp, c, type);
+      leaf_record(q, T, sq, cp, type, code);
     }
-    apply_record(m, pos, sq, cp, c, type, cn);
+    apply_record(m, a.tris, pos, sq, cp, type, code, cn);
+    remask();
   };
 
-  // the walk of traverse() (bvh_traverse.hpp:66-154), resumed
+  // The walk of traverse() (bvh_traverse.hpp:66-154), resumed, written as a
+  // state machine: kInner -- cur is the next inner node to visit and `found`
+  // the held leaf (or 0); kLeaf -- the leaf loop, `found` is the next leaf to
+  // check. A tight loop takes every step it can from the memo masks alone
+  // (scalar code, no memory access); anything else -- a node outside the
+  // memo, a leaf that changes the running state, a guard -- is one step of
+  // the general path, after which the tight loop resumes.
+  enum { kInner = 0, kLeaf = 1, kDone = 2 };
+  int phase = kInner;  // a saved walk stops at the top of its outer loop
   int32_t found = 0;
   bool err = false;
-  while (cur != kBarrier && !err) {
-    if (cur < 0) {  // a walk stops only where it holds no leaf and cur is a node
-      err = true;
-      break;
+  if (cur == kBarrier) phase = kDone;
+  else if (cur < 0) err = true;  // only from a malformed record
+  // after a node visit: hold a first leaf, leave the inner loop at a second
+  auto after_visit = [&]() {
+    if (cur < 0 && !(found < 0)) {
+      found = cur;
+      if (cur != kBarrier) {
+        if (sp < 0) {
+          err = true;
+          return;
+        }
+        cur = lane_get(stk, sp);
+        sp--;
+      }
     }
-    while (cur >= 0) {
+    if (cur < 0) {
+      if (found < 0 && found != kBarrier) {
+        phase = kLeaf;
+      } else {
+        found = 0;
+        phase = (cur == kBarrier) ? kDone : kInner;
+        if (phase == kInner) err = true;  // cur < 0 here only for a malformed tree
+      }
+    }
+  };
+  // after a leaf check: next leaf, or back to the top of the outer loop
+  auto after_leaf = [&]() {
+    found = cur;
+    if (cur < 0 && cur != kBarrier) {
+      if (sp < 0) {
+        err = true;
+        return;
+      }
+      cur = lane_get(stk, sp);
+      sp--;
+    }
+    if (!(found < 0 && found != kBarrier)) {
+      found = 0;
+      phase = (cur == kBarrier) ? kDone : kInner;
+    }
+  };
+  // A subtree whose walk changes nothing: the walk has just reached the
+  // memo's root (cur), and no leaf it would check changes the running state
+  // (nor the held leaf, if any, which it would check at the first leaf). The
+  // minimum then stays fixed through the subtree, so the nodes it visits and
+  // the leaves it reaches follow from the memo masks alone, and so does the
+  // order (pre-order, nearer child first): it is computed for all memo nodes
+  // at once, level by level, instead of step by step. What the walk state
+  // needs afterwards is the visit and check counts and which leaf, if any, is
+  // held when the walk leaves the subtree; the latter follows from the runs
+  // of consecutive leaves in that order (a lone leaf is held, or frees a held
+  // one; a run of two or more is checked in the leaf loop). Returns false,
+  // changing nothing, when the case does not apply.
+  auto skip_subtree = [&]() -> bool {
+    const int32_t r = cur - nb;
+    if (r < 0 || r >= nn || sp < 0) return false;
+    const bool held = (found < 0 && found != kBarrier);
+    bool held_eff = true;
+    if (held && -found - 1 == h_pos) {
+      const double e0 = m.p[0] - h_p[0], e1 = m.p[1] - h_p[1], e2 = m.p[2] - h_p[2];
+      held_eff = (h_sq < m.sq) || (cn && h_type != 2 && m.type == h_type &&
+                                   near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., kSdfEps));
+    }
+    const bool node_lane = lane < nn;
+    const bool leaf_lane = lane <= chi - clo;
+    // nodes the walk visits: the root, and each child its parent lets through
+    uint64_t vis = uint64_t(1) << r;
+    int32_t levels = 0;
+    for (;;) {
+      bool v = (lane == r);
+      if (node_lane && mem.pn >= 0) {
+        const int32_t p = mem.pn >> 1;
+        const uint64_t pass = (mem.pn & 1) ? m_r : m_l;
+        v = ((vis >> p) & 1u) && ((pass >> p) & 1u);
+      }
+      const uint64_t nv = __ballot(node_lane && v);
+      ++levels;
+      if (nv == vis) break;
+      vis = nv;
+      if (levels > kWaveLanes) return false;
+    }
+    bool lr = false;
+    if (leaf_lane && mem.pl >= 0) {
+      const int32_t p = mem.pl >> 1;
+      const uint64_t pass = (mem.pl & 1) ? m_r : m_l;
+      lr = ((vis >> p) & 1u) && ((pass >> p) & 1u);
+    }
+    const uint64_t reached = __ballot(leaf_lane && lr);
+    if (reached & m_eff) return false;
+    const int32_t n_nodes = __popcll(vis), n_leaves = __popcll(reached);
+    if (held && n_leaves > 0 && held_eff) return false;
+    // pushes inside the subtree stay below the depth (levels); the visit
+    // guard must not trip either: leave those cases to the step-by-step walk
+    if (sp + levels >= kStack || visits + n_nodes > a.ninner) return false;
+    // items (visited nodes and reached leaves) under each visited node
+    const bool mine = node_lane && ((vis >> lane) & 1u);
+    const bool l_pass = mine && ((m_l >> lane) & 1u), r_pass = mine && ((m_r >> lane) & 1u);
+    const bool l_inner = mem.lc >= 0, r_inner = mem.rc >= 0;
+    const int32_t li = (node_lane && l_inner) ? mem.lc - nb : lane;
+    const int32_t ri = (node_lane && r_inner) ? mem.rc - nb : lane;
+    int32_t size = mine ? 1 : 0, size_l = 0, size_r = 0;
+    for (int32_t it = 0;; ++it) {
+      const int32_t sl = __shfl(size, li), sr = __shfl(size, ri);
+      size_l = l_pass ? (l_inner ? sl : 1) : 0;
+      size_r = r_pass ? (r_inner ? sr : 1) : 0;
+      const int32_t ns = mine ? 1 + size_l + size_r : 0;
+      const bool changed = __ballot(ns != size) != 0;
+      size = ns;
+      if (!changed || it > kWaveLanes) break;
+    }
+    // pre-order positions: the first child follows its parent, the second
+    // follows the first child's items
+    const bool right_first = (l_pass && r_pass) ? (((m_sw >> lane) & 1u) != 0) : !l_pass;
+    const int32_t first_size = right_first ? size_r : size_l;
+    const int32_t both = (l_pass && r_pass) ? 1 : 0;
+    // each lane's parent (itself for the root and for lanes outside the memo)
+    // and the parent's choices; every lane takes part in every shuffle
+    const int32_t pn = node_lane ? mem.pn : -1;
+    const int32_t pnode = pn >= 0 ? (pn >> 1) : lane;
+    const int32_t pleaf = (leaf_lane && mem.pl >= 0) ? (mem.pl >> 1) : lane;
+    auto second_child = [&](int32_t link, int32_t p) -> int32_t {  // 0 / 1, and the skip
+      const int32_t p_rf = __shfl((int32_t)right_first, p);
+      const int32_t p_both = __shfl(both, p);
+      return (p_both && ((link & 1) == (p_rf ? 0 : 1))) ? 1 : 0;
+    };
+    const int32_t node_second = second_child(pn, pnode);
+    const int32_t node_skip = __shfl(first_size, pnode);
+    const int32_t leaf_second = second_child(leaf_lane ? mem.pl : 0, pleaf);
+    const int32_t leaf_skip = __shfl(first_size, pleaf);
+    int32_t pre = 0;
+    for (int32_t it = 0;; ++it) {
+      const int32_t pre_p = __shfl(pre, pnode);
+      const int32_t np = (mine && lane != r) ? pre_p + 1 + (node_second ? node_skip : 0) : 0;
+      const bool changed = __ballot(np != pre) != 0;
+      pre = np;
+      if (!changed || it > kWaveLanes) break;
+    }
+    const int32_t lpre_p = __shfl(pre, pleaf);
+    const int32_t lpos = lpre_p + 1 + (leaf_second ? leaf_skip : 0);
+    const int32_t total = lane_get(size, r);
+    if (total != n_nodes + n_leaves || total > 2 * kWaveLanes) return false;
+    // leaf at each position (memo leaf index), -1 for a node
+    s_seq[lane] = -1;
+    s_seq[kWaveLanes + lane] = -1;
+    __syncthreads();
+    if (leaf_lane && lr && lpos >= 0 && lpos < total) s_seq[lpos] = lane;
+    __syncthreads();
+    const int32_t q0 = s_seq[lane], q1 = s_seq[kWaveLanes + lane];
+    const uint64_t lm0 = __ballot(q0 >= 0), lm1 = __ballot(q1 >= 0);
+    if (__popcll(lm0) + __popcll(lm1) != n_leaves) return false;
+    // runs of consecutive leaves, in order: 0 = nothing held, 1 = one held, 2 = in the leaf loop
+    auto bit = [&](int32_t i) -> bool {
+      return i < kWaveLanes ? ((lm0 >> i) & 1u) : ((lm1 >> (i - kWaveLanes)) & 1u);
+    };
+    int32_t state = held ? 1 : 0;
+    int32_t held_pos = held ? (-found - 1) : -1;
+    int32_t checks = 0;
+    int32_t i = 1;  // position 0 is the root
+    while (i < total) {
+      if (!bit(i)) {
+        if (state == 2) state = 0;  // a node visit ends the leaf loop
+        ++i;
+        continue;
+      }
+      int32_t j = i;
+      while (j < total && bit(j)) ++j;
+      const int32_t run = j - i;
+      if (state == 0) {
+        if (run == 1) {
+          state = 1;
+          held_pos = clo + (i < kWaveLanes ? lane_get(q0, i) : lane_get(q1, i - kWaveLanes));
+        } else {
+          state = 2;
+          checks += run;
+        }
+      } else {
+        state = 2;
+        checks += run + 1;
+      }
+      i = j;
+    }
+    visits += n_nodes;
+    leaves += checks;
+    found = (state == 1) ? -(held_pos + 1) : 0;
+    cur = lane_get(stk, sp);
+    sp--;
+    if (state == 2) {
+      phase = kLeaf;
+      after_leaf();
+    } else {
+      phase = kInner;
+      after_visit();
+    }
+    return true;
+  };
+  while (!err && phase != kDone) {
+    // ---- tight loop over the memo
+    for (;;) {
+      if (phase == kInner) {
+        const int32_t k = cur - nb;
+        if ((uint32_t)k >= (uint32_t)nn || visits >= a.ninner) break;
+        const uint32_t bl = (uint32_t)(m_l >> k) & 1u;
+        const uint32_t br = (uint32_t)(m_r >> k) & 1u;
+        if ((bl & br) && sp + 1 >= kStack) break;  // the general path reports it
+        ++visits;
+        if ((bl | br) == 0) {
+          cur = lane_get(stk, sp);
+          sp--;
+        } else {
+          const int32_t lc = lane_get(mem.lc, k), rc = lane_get(mem.rc, k);
+          if (bl & br) {
+            const bool right_first = ((m_sw >> k) & 1u) != 0;
+            const int32_t near = right_first ? rc : lc, far = right_first ? lc : rc;
+            ++sp;
+            stk = (lane == sp) ? far : stk;
+            cur = near;
+          } else {
+            cur = bl ? lc : rc;
+          }
+        }
+        after_visit();
+        if (err) break;
+      } else if (phase == kLeaf) {
+        const int32_t k = -found - 1 - clo;
+        if ((uint32_t)k > (uint32_t)(chi - clo) || ((m_eff >> k) & 1u)) break;
+        ++leaves;  // a leaf that changes nothing
+        after_leaf();
+        if (err) break;
+      } else {
+        break;
+      }
+    }
+    if (err || phase == kDone) break;
+    // ---- one step of the general path
+    if (phase == kInner) {
       if (cur >= a.ninner || visits >= a.ninner) {
         err = true;
         break;
@@ -604,27 +897,25 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
               const int32_t k = hp - clo;
               h_pos = hp;
               h_sq = lane_get(mem.sq, k);
-              for (int d = 0; d < 3; ++d) {
-                h_p[d] = lane_get(mem.p[d], k);
-                h_c[d] = lane_get(mem.c[d], k);
-              }
+              for (int d = 0; d < 3; ++d) h_p[d] = lane_get(mem.p[d], k);
               h_type = lane_get(mem.type, k);
+              h_code = lane_get(mem.code, k);
             }
           }
           fill(cur, rg.x, rg.y);
+          skip_subtree();
+          continue;  // the tight loop takes it from here
         }
       }
       bool in_l, in_r, swap;
       int32_t lc, rc;
       if (cur >= nb && cur < nb + nn) {
         const int32_t k = cur - nb;
-        const Scalar lk = lane_get(mem.lk, k), rk = lane_get(mem.rk, k);
         lc = lane_get(mem.lc, k);
         rc = lane_get(mem.rc, k);
-        const int32_t bits = lane_get(mem.bits, k);
-        in_l = (bits & 1) ? bt.keep(lk, m.sq) : false;
-        in_r = (bits & 2) ? bt.keep(rk, m.sq) : false;
-        swap = (bits & 4) != 0;
+        in_l = (m_l >> k) & 1;
+        in_r = (m_r >> k) & 1;
+        swap = (m_sw >> k) & 1;
       } else {
         Scalar lb[6], rb[6];
         int32_t flags;
@@ -653,24 +944,9 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
           stk = (lane == sp) ? rc : stk;
         }
       }
-      if (cur < 0 && !(found < 0)) {
-        found = cur;
-        if (cur != kBarrier) {
-          if (sp < 0) {
-            err = true;
-            break;
-          }
-          cur = lane_get(stk, sp);
-          sp--;
-        }
-      }
-      if (sp < -1) {
-        err = true;
-        break;
-      }
-    }
-    if (err) break;
-    while (found < 0 && found != kBarrier) {
+      after_visit();
+      if (sp < -1) err = true;
+    } else {
       const int32_t pos = -found - 1;
       if (pos > a.ninner) {
         err = true;
@@ -678,17 +954,8 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
       }
       leaf(pos);
       ++leaves;
-      found = cur;
-      if (cur < 0 && cur != kBarrier) {
-        if (sp < 0) {
-          err = true;
-          break;
-        }
-        cur = lane_get(stk, sp);
-        sp--;
-      }
+      after_leaf();
     }
-    found = 0;
   }
   if (lane != 0) return;
   if (err) atomicOr(a.status, 1);
@@ -1110,7 +1377,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     a.order = sv;
   }
   const bool mixed = (fp_mode == AXSQ_FP32_TRAVERSAL);
-  // room for suspended walks: every query, up to 2^24 records (5.8 GB); a
+  // room for suspended walks: every query, up to 2^24 records (6.0 GB); a
   // walk with no room left finishes in the first pass
   const int64_t cap = std::min<int64_t>(nq, int64_t(1) << 24);
   AXSQ_TRY(ws.susp.ensure(sizeof(SdfSuspended) * (size_t)cap));

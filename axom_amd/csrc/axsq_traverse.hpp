@@ -52,7 +52,10 @@ __device__ __forceinline__ bool valid6(const T b[6]) {
 // different banks; deeper entries (trees deeper than kLdsStack levels, rare)
 // go to a private array. A private array indexed per lane would be placed in
 // scratch, where lanes at different depths touch different cache lines.
-constexpr int kLdsStack = 32;
+#ifndef AXSQ_LDS_STACK
+#define AXSQ_LDS_STACK 32
+#endif
+constexpr int kLdsStack = AXSQ_LDS_STACK;
 constexpr int kWaveLanes = 64;
 
 // LDS-qualified word: accesses compile to ds_read/ds_write, not flat ops.

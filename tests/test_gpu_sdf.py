@@ -247,7 +247,7 @@ def test_many_tied_triangles_at_a_vertex(monkeypatch, budget):
     s = GOLD["sdf_sphere"]
     v, t = refmeshes.sphere_uv_mesh(s["center"], s["radius"], s["theta_res"], s["phi_res"])
     z = np.linspace(0.55, 2.0, 64)
-    e = 1e-9 * synth.uniform(4, 0, 128, -1, 1).reshape(-1, 2)
+    e = 1e-9 * synth.uniform(4, 0, 256, -1, 1).reshape(-1, 2)
     poles = np.concatenate([np.stack([0 * z, 0 * z, z], 1), np.stack([0 * z, 0 * z, -z], 1),
                             np.stack([e[:64, 0], e[:64, 1], z], 1),
                             np.stack([e[64:, 0], e[64:, 1], -z], 1)])
