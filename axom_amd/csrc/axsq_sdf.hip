@@ -574,19 +574,18 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
   auto fill = [&](int32_t c, int32_t lo, int32_t hi) {
     const int32_t nb_new = (c == lo) ? lo : lo + 1;
     const int32_t nn_new = hi - lo;
-    if (lane <= hi - lo) {
-      double T[9];
-      load_tri(a.tris, lo + lane, T);
-      leaf_record(q, T, mem.sq, mem.p, mem.type, mem.code);
-    }
-    if (lane < nn_new) {
-      Scalar lb[6], rb[6];
-      int32_t flags;
-      load_node(nodes, nb_new + lane, lb, rb, mem.lc, mem.rc, flags);
-      mem.lk = bt.key(lb);
-      mem.rk = bt.key(rb);
-      mem.bits = (flags & 3) | (bt.right_first(lb, rb) ? 4 : 0);
-    }
+    // both loads first (indices clamped into the subtree, so every lane
+    // loads), so that their latencies overlap; lanes past the end keep
+    // values nothing reads
+    double T[9];
+    Scalar lb[6], rb[6];
+    int32_t flags;
+    load_tri(a.tris, lo + min(lane, hi - lo), T);
+    load_node(nodes, nb_new + min(lane, nn_new - 1), lb, rb, mem.lc, mem.rc, flags);
+    leaf_record(q, T, mem.sq, mem.p, mem.type, mem.code);
+    mem.lk = bt.key(lb);
+    mem.rk = bt.key(rb);
+    mem.bits = (flags & 3) | (bt.right_first(lb, rb) ? 4 : 0);
     // parent links inside the memo, by scatter through LDS
     s_pn[lane] = -1;
     s_pl[lane] = -1;
@@ -798,36 +797,49 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
     const int32_t q0 = s_seq[lane], q1 = s_seq[kWaveLanes + lane];
     const uint64_t lm0 = __ballot(q0 >= 0), lm1 = __ballot(q1 >= 0);
     if (__popcll(lm0) + __popcll(lm1) != n_leaves) return false;
-    // runs of consecutive leaves, in order: 0 = nothing held, 1 = one held, 2 = in the leaf loop
-    auto bit = [&](int32_t i) -> bool {
-      return i < kWaveLanes ? ((lm0 >> i) & 1u) : ((lm1 >> (i - kWaveLanes)) & 1u);
-    };
-    int32_t state = held ? 1 : 0;
-    int32_t held_pos = held ? (-found - 1) : -1;
-    int32_t checks = 0;
-    int32_t i = 1;  // position 0 is the root
-    while (i < total) {
-      if (!bit(i)) {
-        if (state == 2) state = 0;  // a node visit ends the leaf loop
-        ++i;
-        continue;
+    // Runs of consecutive leaves in that order. Walk states: 0 nothing held,
+    // 1 one leaf held, 2 in the leaf loop. A lone leaf met with nothing held
+    // is held; a lone leaf met while one is held checks both; a run of two or
+    // more checks all of them (and a held one); a node visit ends the leaf
+    // loop. So every reached leaf is checked except one held at the exit, and
+    // the exit state follows from the last run: after a long run (or a lone
+    // leaf that found one held) the walk is in the leaf loop, or at state 0
+    // if nodes follow; a lone last leaf is held when the lone runs since the
+    // last long run (or since the start, from the entry state) leave nothing
+    // held before it.
+    int32_t state, held_pos = held ? (-found - 1) : -1;
+    int32_t checks = n_leaves + ((held && n_leaves > 0) ? 1 : 0);
+    if (n_leaves == 0) {
+      state = held ? 1 : 0;
+    } else {
+      typedef unsigned __int128 u128;
+      const u128 lmask = ((u128)lm1 << 64) | (u128)lm0;
+      const u128 lone = lmask & ~(lmask << 1) & ~(lmask >> 1);
+      const u128 longb = lmask & ~lone;
+      auto top_bit = [](u128 x) -> int32_t {
+        const uint64_t hi = (uint64_t)(x >> 64), lo = (uint64_t)x;
+        return hi ? 127 - __clzll(hi) : (lo ? 63 - __clzll(lo) : -1);
+      };
+      auto count = [](u128 x) -> int32_t {
+        return __popcll((uint64_t)(x >> 64)) + __popcll((uint64_t)x);
+      };
+      const int32_t h = top_bit(lmask);
+      bool hold_at_exit = false;
+      if ((lone >> h) & 1) {
+        const int32_t e = top_bit(longb);  // end of the last long run, -1 if none
+        const u128 between = (h > 0 ? (((u128)1 << h) - 1) : 0) & ~((((u128)1 << (e + 1)) - 1));
+        const int32_t lone_before = count(lone & between);
+        const int32_t base = (e >= 0) ? 0 : (held ? 1 : 0);
+        const int32_t entry_h = base ^ (lone_before & 1);
+        hold_at_exit = (entry_h == 0);
       }
-      int32_t j = i;
-      while (j < total && bit(j)) ++j;
-      const int32_t run = j - i;
-      if (state == 0) {
-        if (run == 1) {
-          state = 1;
-          held_pos = clo + (i < kWaveLanes ? lane_get(q0, i) : lane_get(q1, i - kWaveLanes));
-        } else {
-          state = 2;
-          checks += run;
-        }
+      if (hold_at_exit) {
+        state = 1;
+        held_pos = clo + (h < kWaveLanes ? lane_get(q0, h) : lane_get(q1, h - kWaveLanes));
+        checks -= 1;
       } else {
-        state = 2;
-        checks += run + 1;
+        state = (h < total - 1) ? 0 : 2;
       }
-      i = j;
     }
     visits += n_nodes;
     leaves += checks;
