@@ -191,8 +191,9 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
   const int32_t split = i + s * d + (d < 0 ? d : 0);
   const int32_t lo = (i < j) ? i : j;
   const int32_t hi = (i < j) ? j : i;
-  range[2 * i] = lo;
-  range[2 * i + 1] = hi;
+  range[3 * i] = lo;
+  range[3 * i + 1] = hi;
+  range[3 * i + 2] = split;
   if (lo == split) {
     parents[split + inner] = i;
     lchild[i] = split + inner;
@@ -282,9 +283,10 @@ __global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf
   nd.lc = lc;
   nd.rc = rc;
   nd.flags = (box_valid(nd.l) ? 1 : 0) | (box_valid(nd.r) ? 2 : 0);
-  nd.lo = range[2 * i];
-  nd.hi = range[2 * i + 1];
-  for (int k = 0; k < 3; ++k) nd.pad[k] = 0;
+  nd.lo = range[3 * i];
+  nd.hi = range[3 * i + 1];
+  nd.mid = range[3 * i + 2];
+  for (int k = 0; k < 2; ++k) nd.pad[k] = 0;
   nodes[i] = nd;
   if (nodes32) {
     Node32 h;
@@ -409,7 +411,7 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   const size_t off_p = off_r + sizeof(int32_t) * inner;
   const size_t off_c = off_p + sizeof(int32_t) * (2 * n - 1);
   const size_t off_rg = off_c + sizeof(int32_t) * inner;
-  const size_t off_ib = (off_rg + sizeof(int32_t) * 2 * inner + 255) & ~size_t(255);
+  const size_t off_ib = (off_rg + sizeof(int32_t) * 3 * inner + 255) & ~size_t(255);
   const size_t total = off_ib + sizeof(double) * 6 * inner;
   AXSQ_TRY(ws.flag.ensure(total));
   char* base = ws.flag.as<char>();

@@ -26,14 +26,15 @@ struct Box {
 // stored box, precomputed so traversal does not re-test it). [lo, hi] is the
 // range of leaf positions under the node (Karras 2012: the node's index is lo
 // or hi, and the inner nodes of its subtree are the hi - lo indices starting
-// at lo, or at lo + 1 when the index is hi).
+// at lo, or at lo + 1 when the index is hi); the left child covers [lo, mid],
+// the right child [mid + 1, hi].
 struct __attribute__((aligned(128))) Node64 {
   double l[6];  // left  min xyz, max xyz
   double r[6];  // right min xyz, max xyz
   int32_t lc, rc;
   int32_t flags;
-  int32_t lo, hi;
-  int32_t pad[3];
+  int32_t lo, hi, mid;
+  int32_t pad[2];
 };
 static_assert(sizeof(Node64) == 128, "Node64 must be one cache line");
 

@@ -26,13 +26,21 @@
 #include "axsq_traverse.hpp"
 
 #ifndef AXSQ_SDF_BLOCK
-#define AXSQ_SDF_BLOCK 256
+#define AXSQ_SDF_BLOCK 64  // one wave per block: a finished wave frees its LDS at once
 #endif
 #ifndef AXSQ_SDF_P1_ILP
 #define AXSQ_SDF_P1_ILP 0  // first pass: branch-free closest point
 #endif
 #ifndef AXSQ_SDF_VOTE
 #define AXSQ_SDF_VOTE 1  // first pass: vote-scheduled walk (0: loop-by-loop)
+#endif
+#ifndef AXSQ_RESUME_WAVES
+#define AXSQ_RESUME_WAVES 3  // second pass: 3 waves per SIMD (<= 168 VGPRs)
+#endif
+#if AXSQ_RESUME_WAVES > 0
+#define AXSQ_RESUME_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(AXSQ_RESUME_WAVES)))
+#else
+#define AXSQ_RESUME_WAVES_ATTR
 #endif
 #if defined(AXSQ_SDF_WAVES) && AXSQ_SDF_WAVES > 0
 #define AXSQ_SDF_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(AXSQ_SDF_WAVES)))
@@ -483,13 +491,21 @@ struct MemoLane {
   int32_t type, code;
   Scalar lk, rk;  // child-box keys (BoxTests::key)
   int32_t lc, rc;
-  int32_t bits;   // bit 0/1: child box valid, bit 2: right child first
-  int32_t pn;     // this node's parent in the memo, 2 * index + (1 if right child); -1: root
-  int32_t pl;     // this leaf's parent in the memo, 2 * index + (1 if right child)
+  int32_t bits;     // bit 0/1: child box valid, bit 2: right child first
+  int32_t nlo, nhi; // the node's leaf range
+  int32_t pn;       // this node's parent in the memo, 2 * index + (1 if right child); -1: root
+  int32_t pl;       // this leaf's parent in the memo, 2 * index + (1 if right child)
 };
 
+// bits [lo, hi) of a 64-bit mask (0 <= lo <= hi <= 64)
+__device__ __forceinline__ uint64_t bit_range(int32_t lo, int32_t hi) {
+  const uint64_t below_hi = hi >= 64 ? ~uint64_t(0) : ((uint64_t(1) << hi) - 1);
+  const uint64_t below_lo = lo >= 64 ? ~uint64_t(0) : ((uint64_t(1) << lo) - 1);
+  return below_hi & ~below_lo;
+}
+
 template <bool MIXED, bool STATS>
-__global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64_t nres,
+__global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resume_wave(SdfArgs a, int64_t nres,
                                                                  int64_t nredo) {
   static_assert(kStack == kWaveLanes, "one stack entry per lane");
   using NodeT = typename std::conditional<MIXED, Node32, Node64>::type;
@@ -502,7 +518,10 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
   int64_t i;
   MinCand m;
   int32_t cur, sp, visits, leaves;
-  int32_t stk;
+  // the stack, entry k in lane k, with the leaf range of each node entry
+  // (lo < 0: not known, read from the node when needed)
+  int32_t stk, stk_lo = -1, stk_hi = -1;
+  int32_t cur_lo = -1, cur_hi = -1;  // leaf range of cur, if known
   if (b < nres) {
     const SdfSuspended& r = a.susp[b];
     i = r.qidx;
@@ -529,6 +548,8 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
     m.type = -1;
     m.pos = 0;
     cur = 0;
+    cur_lo = 0;
+    cur_hi = a.ninner;
     sp = 0;
     visits = 0;
     leaves = 0;
@@ -541,6 +562,18 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
   else
     nodes = a.nodes;
   BoxTests<MIXED> bt(q);
+
+  auto pop = [&]() {
+    cur = lane_get(stk, sp);
+    cur_lo = lane_get(stk_lo, sp);
+    cur_hi = lane_get(stk_hi, sp);
+    sp--;
+  };
+  auto push = [&](int32_t v, int32_t vlo, int32_t vhi) {  // sp already raised
+    stk = (lane == sp) ? v : stk;
+    stk_lo = (lane == sp) ? vlo : stk_lo;
+    stk_hi = (lane == sp) ? vhi : stk_hi;
+  };
 
   MemoLane<Scalar> mem;
   int32_t clo = 1, chi = 0;  // memo leaves [clo, chi] (empty)
@@ -580,12 +613,16 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
     double T[9];
     Scalar lb[6], rb[6];
     int32_t flags;
+    const int32_t my_node = nb_new + min(lane, nn_new - 1);
     load_tri(a.tris, lo + min(lane, hi - lo), T);
-    load_node(nodes, nb_new + min(lane, nn_new - 1), lb, rb, mem.lc, mem.rc, flags);
+    load_node(nodes, my_node, lb, rb, mem.lc, mem.rc, flags);
+    const int2 rg = *reinterpret_cast<const int2*>(&a.nodes[my_node].lo);
     leaf_record(q, T, mem.sq, mem.p, mem.type, mem.code);
     mem.lk = bt.key(lb);
     mem.rk = bt.key(rb);
     mem.bits = (flags & 3) | (bt.right_first(lb, rb) ? 4 : 0);
+    mem.nlo = rg.x;
+    mem.nhi = rg.y;
     // parent links inside the memo, by scatter through LDS
     s_pn[lane] = -1;
     s_pl[lane] = -1;
@@ -658,8 +695,7 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
           err = true;
           return;
         }
-        cur = lane_get(stk, sp);
-        sp--;
+        pop();
       }
     }
     if (cur < 0) {
@@ -680,8 +716,7 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
         err = true;
         return;
       }
-      cur = lane_get(stk, sp);
-      sp--;
+      pop();
     }
     if (!(found < 0 && found != kBarrier)) {
       found = 0;
@@ -692,9 +727,12 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
   // memo's root (cur), and no leaf it would check changes the running state
   // (nor the held leaf, if any, which it would check at the first leaf). The
   // minimum then stays fixed through the subtree, so the nodes it visits and
-  // the leaves it reaches follow from the memo masks alone, and so does the
-  // order (pre-order, nearer child first): it is computed for all memo nodes
-  // at once, level by level, instead of step by step. What the walk state
+  // the leaves it reaches follow from the memo masks alone, and so does their
+  // order (pre-order, nearer child first). They are found for all memo nodes
+  // at once: by pointer jumping along parent links (a node is visited when no
+  // link on its path is cut; its position is the sum along its path of its
+  // offsets after its parent), and the item count under a node is a
+  // popcount over its contiguous node and leaf ranges. What the walk state
   // needs afterwards is the visit and check counts and which leaf, if any, is
   // held when the walk leaves the subtree; the latter follows from the runs
   // of consecutive leaves in that order (a lone leaf is held, or frees a held
@@ -712,22 +750,24 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
     }
     const bool node_lane = lane < nn;
     const bool leaf_lane = lane <= chi - clo;
-    // nodes the walk visits: the root, and each child its parent lets through
-    uint64_t vis = uint64_t(1) << r;
-    int32_t levels = 0;
+    // parent lane (itself for the root and for lanes outside the memo) and
+    // whether the parent lets this node through
+    const int32_t pn = (node_lane && lane != r) ? mem.pn : -1;
+    const int32_t pnode = pn >= 0 ? (pn >> 1) : lane;
+    const bool cut = pn >= 0 && !((((pn & 1) ? m_r : m_l) >> pnode) & 1u);
+    // visited: no cut on the path to the root
+    int32_t anc = pnode, acc = cut ? 1 : 0;
+    int32_t rounds = 0;
     for (;;) {
-      bool v = (lane == r);
-      if (node_lane && mem.pn >= 0) {
-        const int32_t p = mem.pn >> 1;
-        const uint64_t pass = (mem.pn & 1) ? m_r : m_l;
-        v = ((vis >> p) & 1u) && ((pass >> p) & 1u);
-      }
-      const uint64_t nv = __ballot(node_lane && v);
-      ++levels;
-      if (nv == vis) break;
-      vis = nv;
-      if (levels > kWaveLanes) return false;
+      const int32_t acc_a = __shfl(acc, anc), anc_a = __shfl(anc, anc);
+      acc |= acc_a;
+      anc = anc_a;
+      ++rounds;
+      if (__ballot(node_lane && anc != r && anc != lane) == 0 || rounds > 7) break;
     }
+    if (rounds > 7) return false;
+    const uint64_t vis = __ballot(node_lane && acc == 0);
+    if (!((vis >> r) & 1u)) return false;
     bool lr = false;
     if (leaf_lane && mem.pl >= 0) {
       const int32_t p = mem.pl >> 1;
@@ -738,54 +778,52 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
     if (reached & m_eff) return false;
     const int32_t n_nodes = __popcll(vis), n_leaves = __popcll(reached);
     if (held && n_leaves > 0 && held_eff) return false;
-    // pushes inside the subtree stay below the depth (levels); the visit
-    // guard must not trip either: leave those cases to the step-by-step walk
-    if (sp + levels >= kStack || visits + n_nodes > a.ninner) return false;
-    // items (visited nodes and reached leaves) under each visited node
+    if (visits + n_nodes > a.ninner) return false;  // the visit guard would trip
+    // items under each visited node: popcounts over its node and leaf ranges
     const bool mine = node_lane && ((vis >> lane) & 1u);
+    int32_t size = 0;
+    if (mine) {
+      const int32_t first_node = ((nb + lane) == mem.nlo ? mem.nlo : mem.nlo + 1) - nb;
+      const int32_t n_in = mem.nhi - mem.nlo;
+      size = __popcll(vis & bit_range(first_node, first_node + n_in)) +
+             __popcll(reached & bit_range(mem.nlo - clo, mem.nhi - clo + 1));
+    }
     const bool l_pass = mine && ((m_l >> lane) & 1u), r_pass = mine && ((m_r >> lane) & 1u);
     const bool l_inner = mem.lc >= 0, r_inner = mem.rc >= 0;
     const int32_t li = (node_lane && l_inner) ? mem.lc - nb : lane;
     const int32_t ri = (node_lane && r_inner) ? mem.rc - nb : lane;
-    int32_t size = mine ? 1 : 0, size_l = 0, size_r = 0;
-    for (int32_t it = 0;; ++it) {
-      const int32_t sl = __shfl(size, li), sr = __shfl(size, ri);
-      size_l = l_pass ? (l_inner ? sl : 1) : 0;
-      size_r = r_pass ? (r_inner ? sr : 1) : 0;
-      const int32_t ns = mine ? 1 + size_l + size_r : 0;
-      const bool changed = __ballot(ns != size) != 0;
-      size = ns;
-      if (!changed || it > kWaveLanes) break;
-    }
-    // pre-order positions: the first child follows its parent, the second
-    // follows the first child's items
+    const int32_t sl = __shfl(size, li), sr = __shfl(size, ri);
+    const int32_t size_l = l_pass ? (l_inner ? sl : 1) : 0;
+    const int32_t size_r = r_pass ? (r_inner ? sr : 1) : 0;
+    // pre-order: the first child follows its parent, the second follows the
+    // first child's items
     const bool right_first = (l_pass && r_pass) ? (((m_sw >> lane) & 1u) != 0) : !l_pass;
     const int32_t first_size = right_first ? size_r : size_l;
     const int32_t both = (l_pass && r_pass) ? 1 : 0;
-    // each lane's parent (itself for the root and for lanes outside the memo)
-    // and the parent's choices; every lane takes part in every shuffle
-    const int32_t pn = node_lane ? mem.pn : -1;
-    const int32_t pnode = pn >= 0 ? (pn >> 1) : lane;
-    const int32_t pleaf = (leaf_lane && mem.pl >= 0) ? (mem.pl >> 1) : lane;
-    auto second_child = [&](int32_t link, int32_t p) -> int32_t {  // 0 / 1, and the skip
+    auto offset = [&](int32_t link, int32_t p) -> int32_t {
       const int32_t p_rf = __shfl((int32_t)right_first, p);
       const int32_t p_both = __shfl(both, p);
-      return (p_both && ((link & 1) == (p_rf ? 0 : 1))) ? 1 : 0;
+      const int32_t p_fs = __shfl(first_size, p);
+      const bool second = p_both && ((link & 1) == (p_rf ? 0 : 1));
+      return 1 + (second ? p_fs : 0);
     };
-    const int32_t node_second = second_child(pn, pnode);
-    const int32_t node_skip = __shfl(first_size, pnode);
-    const int32_t leaf_second = second_child(leaf_lane ? mem.pl : 0, pleaf);
-    const int32_t leaf_skip = __shfl(first_size, pleaf);
-    int32_t pre = 0;
-    for (int32_t it = 0;; ++it) {
-      const int32_t pre_p = __shfl(pre, pnode);
-      const int32_t np = (mine && lane != r) ? pre_p + 1 + (node_second ? node_skip : 0) : 0;
-      const bool changed = __ballot(np != pre) != 0;
-      pre = np;
-      if (!changed || it > kWaveLanes) break;
+    const int32_t node_off = offset(pn, pnode);
+    const int32_t pleaf = (leaf_lane && mem.pl >= 0) ? (mem.pl >> 1) : lane;
+    const int32_t leaf_off = offset(leaf_lane ? mem.pl : 0, pleaf);
+    // positions and depths: sums along the path, by pointer jumping
+    int32_t pre = (mine && lane != r) ? node_off : 0;
+    int32_t dep = (mine && lane != r) ? 1 : 0;
+    anc = (mine && lane != r) ? pnode : lane;
+    for (int32_t k = 0; k < rounds; ++k) {
+      const int32_t pre_a = __shfl(pre, anc), dep_a = __shfl(dep, anc), anc_a = __shfl(anc, anc);
+      pre += pre_a;
+      dep += dep_a;
+      anc = anc_a;
     }
-    const int32_t lpre_p = __shfl(pre, pleaf);
-    const int32_t lpos = lpre_p + 1 + (leaf_second ? leaf_skip : 0);
+    // a node whose children both pass pushes one entry above the at most
+    // `dep` its ancestors left: leave an overflow to the step-by-step walk
+    if (__ballot(mine && sp + dep + 1 >= kStack) != 0) return false;
+    const int32_t lpos = __shfl(pre, pleaf) + leaf_off;
     const int32_t total = lane_get(size, r);
     if (total != n_nodes + n_leaves || total > 2 * kWaveLanes) return false;
     // leaf at each position (memo leaf index), -1 for a node
@@ -844,8 +882,7 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
     visits += n_nodes;
     leaves += checks;
     found = (state == 1) ? -(held_pos + 1) : 0;
-    cur = lane_get(stk, sp);
-    sp--;
+    pop();
     if (state == 2) {
       phase = kLeaf;
       after_leaf();
@@ -866,19 +903,19 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
         if ((bl & br) && sp + 1 >= kStack) break;  // the general path reports it
         ++visits;
         if ((bl | br) == 0) {
-          cur = lane_get(stk, sp);
-          sp--;
+          pop();
         } else {
           const int32_t lc = lane_get(mem.lc, k), rc = lane_get(mem.rc, k);
           if (bl & br) {
             const bool right_first = ((m_sw >> k) & 1u) != 0;
             const int32_t near = right_first ? rc : lc, far = right_first ? lc : rc;
             ++sp;
-            stk = (lane == sp) ? far : stk;
+            push(far, -1, -1);  // inside the memo: no range needed
             cur = near;
           } else {
             cur = bl ? lc : rc;
           }
+          cur_lo = cur_hi = -1;
         }
         after_visit();
         if (err) break;
@@ -899,10 +936,15 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
         err = true;
         break;
       }
-      if (!(cur >= nb && cur < nb + nn)) {
-        const int2 rg = *reinterpret_cast<const int2*>(&a.nodes[cur].lo);
-        if (rg.x >= 0 && rg.y <= a.ninner && rg.x < rg.y && rg.y - rg.x < kWaveLanes &&
-            (cur == rg.x || cur == rg.y)) {
+      const bool in_memo = cur >= nb && cur < nb + nn;
+      if (!in_memo) {
+        if (cur_lo < 0) {
+          const int2 rg = *reinterpret_cast<const int2*>(&a.nodes[cur].lo);
+          cur_lo = rg.x;
+          cur_hi = rg.y;
+        }
+        if (cur_lo >= 0 && cur_hi <= a.ninner && cur_lo < cur_hi && cur_hi - cur_lo < kWaveLanes &&
+            (cur == cur_lo || cur == cur_hi)) {
           if (found < 0 && found != kBarrier) {
             const int32_t hp = -found - 1;
             if (hp >= clo && hp <= chi) {
@@ -914,14 +956,15 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
               h_code = lane_get(mem.code, k);
             }
           }
-          fill(cur, rg.x, rg.y);
+          fill(cur, cur_lo, cur_hi);
           skip_subtree();
           continue;  // the tight loop takes it from here
         }
       }
       bool in_l, in_r, swap;
       int32_t lc, rc;
-      if (cur >= nb && cur < nb + nn) {
+      int32_t l_lo = -1, l_hi = -1, r_lo = -1, r_hi = -1;  // children's leaf ranges
+      if (in_memo) {
         const int32_t k = cur - nb;
         lc = lane_get(mem.lc, k);
         rc = lane_get(mem.rc, k);
@@ -935,25 +978,35 @@ __global__ __launch_bounds__(kWaveLanes) void k_sdf_resume_wave(SdfArgs a, int64
         in_l = (flags & 1) ? bt.keep(bt.key(lb), m.sq) : false;
         in_r = (flags & 2) ? bt.keep(bt.key(rb), m.sq) : false;
         swap = bt.right_first(lb, rb);
+        const int32_t mid = a.nodes[cur].mid;
+        l_lo = cur_lo;
+        l_hi = mid;
+        r_lo = mid + 1;
+        r_hi = cur_hi;
       }
       ++visits;
       if (!in_l && !in_r) {
-        cur = lane_get(stk, sp);
-        sp--;
+        pop();
       } else {
         cur = in_l ? lc : rc;
+        cur_lo = in_l ? l_lo : r_lo;
+        cur_hi = in_l ? l_hi : r_hi;
         if (in_l && in_r) {
+          int32_t far = rc, far_lo = r_lo, far_hi = r_hi;
           if (swap) {
-            const int32_t t = cur;
+            far = cur;
+            far_lo = cur_lo;
+            far_hi = cur_hi;
             cur = rc;
-            rc = t;
+            cur_lo = r_lo;
+            cur_hi = r_hi;
           }
           sp++;
           if (sp >= kStack) {
             err = true;
             break;
           }
-          stk = (lane == sp) ? rc : stk;
+          push(far, far_lo, far_hi);
         }
       }
       after_visit();
