@@ -8,6 +8,9 @@
 #include <string>
 #include <vector>
 
+#include <algorithm>
+#include <cmath>
+
 #include "axsq_internal.hpp"
 
 namespace axsq {
@@ -381,6 +384,12 @@ int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const doub
   h->sort_queries = !(env && env[0] == '0');
   if (const char* e = getenv("AXSQ_XCD_SLICES")) h->xcd_slices = atoi(e) > 1 ? atoi(e) : 1;
   if (const char* e = getenv("AXSQ_DISPERSE")) h->disperse = atoi(e) > 1 ? atoi(e) : 1;
+  // First-pass step budget. The long walks (queries nearly equidistant from
+  // many triangles) grow with the triangle count, and the second pass only
+  // pays off for walks well above the typical length: 2048 steps at 81,920
+  // triangles, scaled by (triangles / 81,920)^(3/4), within [2048, 65536].
+  // It changes speed only, never results.
+  h->budget = (int32_t)std::min(65536.0, std::max(2048.0, 2048.0 * std::pow((double)ntris / 81920.0, 0.75)));
   if (const char* e = getenv("AXSQ_SDF_BUDGET")) h->budget = atoi(e) > 0 ? atoi(e) : 0x7fffffff;
   int rc = AXSQ_OK;
   auto fail = [&](int code) {

@@ -34,14 +34,16 @@
 #ifndef AXSQ_SDF_VOTE
 #define AXSQ_SDF_VOTE 1  // first pass: vote-scheduled walk (0: loop-by-loop)
 #endif
+// second pass: waves per SIMD, FP64 / FP32-traversal form (the FP32 form,
+// with its two box tests, would spill at 3)
 #ifndef AXSQ_RESUME_WAVES
-#define AXSQ_RESUME_WAVES 3  // second pass: 3 waves per SIMD (<= 168 VGPRs)
+#define AXSQ_RESUME_WAVES 3
 #endif
-#if AXSQ_RESUME_WAVES > 0
-#define AXSQ_RESUME_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(AXSQ_RESUME_WAVES)))
-#else
-#define AXSQ_RESUME_WAVES_ATTR
+#ifndef AXSQ_RESUME_WAVES_MIXED
+#define AXSQ_RESUME_WAVES_MIXED 2
 #endif
+#define AXSQ_RESUME_WAVES_ATTR \
+  __attribute__((amdgpu_waves_per_eu(MIXED ? AXSQ_RESUME_WAVES_MIXED : AXSQ_RESUME_WAVES)))
 #if defined(AXSQ_SDF_WAVES) && AXSQ_SDF_WAVES > 0
 #define AXSQ_SDF_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(AXSQ_SDF_WAVES)))
 #else
