@@ -382,8 +382,6 @@ int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const doub
   h->bvh->device = h->device;
   const char* env = getenv("AXSQ_SORT_QUERIES");
   h->sort_queries = !(env && env[0] == '0');
-  if (const char* e = getenv("AXSQ_XCD_SLICES")) h->xcd_slices = atoi(e) > 1 ? atoi(e) : 1;
-  if (const char* e = getenv("AXSQ_DISPERSE")) h->disperse = atoi(e) > 1 ? atoi(e) : 1;
   // First-pass step budget. The long walks (queries nearly equidistant from
   // many triangles) grow with the triangle count, and the second pass only
   // pays off for walks well above the typical length: 2048 steps at 81,920

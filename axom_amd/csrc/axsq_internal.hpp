@@ -123,8 +123,6 @@ struct axsq_sdf {
   int64_t* stats_dev = nullptr;     // [visits, leaves]
   int64_t last_visits = 0, last_leaves = 0;
   bool sort_queries = true;
-  int32_t xcd_slices = 1;  // workgroup slicing over XCDs (1 = off)
-  int32_t disperse = 1;    // wave-chunk dispersal groups (1 = off)
   int32_t budget = 2048;   // first-pass step budget per query walk
   int64_t last_suspended = 0;  // walks the last eval finished in its second pass
 };

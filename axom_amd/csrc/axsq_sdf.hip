@@ -67,8 +67,6 @@ struct SdfArgs {
   int64_t nq;
   int32_t ninner;
   const uint32_t* order;
-  int32_t xcd_slices;  // > 1: XCD-contiguous workgroup slices
-  int32_t disperse;    // > 1: spread 64-query wave chunks over this many groups
   int32_t budget;      // first-pass step budget per walk
   int64_t cap;         // capacity of susp
   struct SdfSuspended* susp;
@@ -338,15 +336,8 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
   lds_i32* terms =
       (lds_i32*)(s_terms + (threadIdx.x / kWaveLanes) * (kNormList * kWaveLanes) +
                  threadIdx.x % kWaveLanes);
-  const int64_t blk =
-      a.xcd_slices > 1 ? group_spread(blockIdx.x, gridDim.x, a.xcd_slices) : blockIdx.x;
-  int64_t t = blk * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nq) return;
-  // Each 64-query chunk (one wave) stays together; optionally consecutive
-  // chunks are spread over distant parts of space.
-  const int64_t nfull = a.nq / kWaveLanes;
-  if (a.disperse > 1 && t / kWaveLanes < nfull)
-    t = group_spread(t / kWaveLanes, nfull, a.disperse) * kWaveLanes + t % kWaveLanes;
   const int64_t i = a.order ? (int64_t)a.order[t] : t;
   MinCand m;
   m.sq = kDblMax;
@@ -1419,8 +1410,6 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   a.elem = elem;
   a.status = b->status;
   a.stats = h->stats_dev;
-  a.xcd_slices = h->xcd_slices;
-  a.disperse = h->disperse;
   if (h->sort_queries && nq > 4096) {
     AXSQ_TRY(ws.k0.ensure(sizeof(uint32_t) * nq));
     AXSQ_TRY(ws.v0.ensure(sizeof(uint32_t) * nq));

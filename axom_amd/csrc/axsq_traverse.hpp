@@ -87,16 +87,6 @@ __device__ __forceinline__ lds_i32* lane_stack_base(int32_t* block_stack) {
   return (lds_i32*)(block_stack + wave * (kLdsStack * kWaveLanes) + lane);
 }
 
-// Bijection on [0, n) that sends p to the (p % g)-th of g contiguous groups:
-// consecutive p land in groups far apart. With g = 8 over workgroups it puts
-// each of MI355X's 8 XCDs (workgroups are dispatched round-robin over them,
-// each with a private L2) on one contiguous slice of the grid.
-__device__ __forceinline__ int64_t group_spread(int64_t p, int64_t n, int64_t g) {
-  const int64_t x = p % g, k = p / g;
-  const int64_t q = n / g, r = n % g;
-  return x * q + (x < r ? x : r) + k;
-}
-
 // Resumable walk state: the stack lives in the LaneStack, the rest here.
 // The walk can only stop at the top of its outer loop, where no leaf is held.
 struct Walk {
