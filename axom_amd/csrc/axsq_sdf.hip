@@ -73,6 +73,7 @@ struct SdfArgs {
   uint32_t* nsusp;
   int32_t* redo;       // queries to run again from the start (room for nq)
   uint32_t* nredo;
+  const uint32_t* susp_order;  // second pass: saved walks in launch order (NULL: as saved)
   double dom[6];
   int watertight;
   int compute_sign;
@@ -516,7 +517,7 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
   int32_t stk, stk_lo = -1, stk_hi = -1;
   int32_t cur_lo = -1, cur_hi = -1;  // leaf range of cur, if known
   if (b < nres) {
-    const SdfSuspended& r = a.susp[b];
+    const SdfSuspended& r = a.susp[a.susp_order ? (int64_t)a.susp_order[b] : b];
     i = r.qidx;
     m.sq = r.sq;
     for (int k = 0; k < 3; ++k) m.p[k] = r.p[k];
@@ -1025,6 +1026,20 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
 }
 
 
+// Launch order of the saved walks: the farthest from the surface first (the
+// running minimum at suspension, descending). Those are the walks that
+// still have most of the tree to cover -- queries deep inside a closed
+// surface -- so the second pass starts them first and fills the rest around
+// them. The top 32 bits of a non-negative double order like the double.
+__global__ __launch_bounds__(kBlock) void k_susp_keys(const SdfSuspended* __restrict__ susp,
+                                                      int64_t n, uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  keys[j] = ~(uint32_t)((uint64_t)__double_as_longlong(susp[j].sq) >> 32);
+  vals[j] = (uint32_t)j;
+}
+
 // Morton code of each query in the mesh's bounding box (ordering only).
 __global__ __launch_bounds__(kBlock) void k_query_codes(const double* __restrict__ qx,
                                                          const double* __restrict__ qy,
@@ -1401,6 +1416,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   a.nq = nq;
   a.ninner = (int32_t)(b->n - 1);
   a.order = nullptr;
+  a.susp_order = nullptr;
   for (int k = 0; k < 6; ++k) a.dom[k] = h->domain[k];
   a.watertight = h->watertight;
   a.compute_sign = h->compute_sign;
@@ -1465,6 +1481,20 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   const int64_t nredo = std::min<int64_t>(counts[1], nq);
   const_cast<axsq_sdf*>(h)->last_suspended = nres + nredo;
   ProfRange p2(mixed ? "sdf_eval_fp32_pass2" : "sdf_eval_fp64_pass2", s);
+  a.susp_order = nullptr;
+  if (nres > 64) {  // the query order is no longer needed: reuse its buffers
+    AXSQ_TRY(ws.k0.ensure(sizeof(uint32_t) * nres));
+    AXSQ_TRY(ws.v0.ensure(sizeof(uint32_t) * nres));
+    AXSQ_TRY(ws.k1.ensure(sizeof(uint32_t) * nres));
+    AXSQ_TRY(ws.v1.ensure(sizeof(uint32_t) * nres));
+    k_susp_keys<<<grid_for(nres), kBlock, 0, s>>>(a.susp, nres, ws.k0.as<uint32_t>(),
+                                                  ws.v0.as<uint32_t>());
+    AXSQ_CHECK_HIP(hipGetLastError());
+    uint32_t *sk, *sv;
+    AXSQ_TRY(radix_sort_pairs(ws.k0.as<uint32_t>(), ws.v0.as<uint32_t>(), ws.k1.as<uint32_t>(),
+                              ws.v1.as<uint32_t>(), nres, 32, ws, s, &sk, &sv));
+    a.susp_order = sv;
+  }
   if (nres + nredo > 0) {
     const unsigned g2 = (unsigned)(nres + nredo);
     if (mixed) {
