@@ -52,6 +52,11 @@ __device__ __forceinline__ bool valid6(const T b[6]) {
 // different banks; deeper entries (trees deeper than kLdsStack levels, rare)
 // go to a private array. A private array indexed per lane would be placed in
 // scratch, where lanes at different depths touch different cache lines.
+// traverse_vote: a leaf-check step runs when at least this percentage of the
+// walking lanes want one
+#ifndef AXSQ_VOTE_LEAF_PCT
+#define AXSQ_VOTE_LEAF_PCT 50
+#endif
 #ifndef AXSQ_LDS_STACK
 #define AXSQ_LDS_STACK 32
 #endif
@@ -223,7 +228,7 @@ __device__ __forceinline__ int traverse_vote(const NodeT* __restrict__ nodes, in
     const uint64_t want_leaf = __ballot(phase == kLeaf);
     if ((want_node | want_leaf) == 0) break;
     const int n_node = __popcll(want_node), n_leaf = __popcll(want_leaf);
-    const bool leaf_step = n_leaf > 0 && 2 * n_leaf >= n_node + n_leaf;
+    const bool leaf_step = n_leaf > 0 && 100 * n_leaf >= AXSQ_VOTE_LEAF_PCT * (n_node + n_leaf);
     if (leaf_step) {
       if (phase == kLeaf) {
         if (-found - 1 > ninner) {  // ninner + 1 leaves
