@@ -14,7 +14,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libaxsq.so")
-SOURCES = ["axsq_prims.hip", "axsq_build.hip", "axsq_query.hip", "axsq_sdf.hip", "axsq_api.hip"]
+SOURCES = ["axsq_prims.hip", "axsq_build.hip", "axsq_query.hip", "axsq_sdf.hip", "axsq_mesh.hip",
+           "axsq_api.hip"]
 ARCH = os.environ.get("AXSQ_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -224,3 +224,50 @@ def signed_distance_get_mesh_bounds():
 def signed_distance_finalize():
     global _query
     _query = None
+
+
+# ---- mesh tester ---------------------------------------------------------------
+class _DeviceArray:
+    """A library-owned device int32 array seen through __cuda_array_interface__."""
+
+    def __init__(self, ptr, n):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<i4",
+                                         "data": (ptr, False), "version": 2}
+
+
+def _take_int32(ptr, n, device):
+    """Copies a library-allocated device int32 array into a torch tensor and
+    releases the library's copy."""
+    try:
+        if n == 0:
+            return torch.empty(0, dtype=torch.int32, device=device)
+        view = torch.as_tensor(_DeviceArray(ptr, n), device=device)
+        out = view.clone()
+        torch.cuda.current_stream(device).synchronize()
+        return out
+    finally:
+        if ptr:
+            L.lib().axsq_free(ctypes.c_void_p(ptr))
+
+
+def findTriMeshIntersectionsBVH(vertices, triangles, intersectionThreshold=1e-8):
+    """quest::findTriMeshIntersectionsBVH<MI355X_EXEC, double>
+    (src/axom/quest/MeshTester.hpp:66-87).
+
+    vertices: (V,3) float64 CUDA tensor or (x, y, z); triangles: (T,3) int32
+    CUDA tensor. Returns (intersections, degenerateIndices): a (K,2) int32 CUDA
+    tensor of intersecting triangle pairs (first < second) in the reference's
+    sequential order, and the degenerate triangles' indices, ascending."""
+    x, y, z = _soa(vertices)
+    t = _dev(triangles, torch.int32, "triangles", 3)
+    pf, ps, pd = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    npairs, ndeg = ctypes.c_int64(), ctypes.c_int64()
+    L.check(L.lib().axsq_mesh_self_intersections(
+        _ptr(x), _ptr(y), _ptr(z), x.shape[0], _ptr(t), t.shape[0], float(intersectionThreshold),
+        ctypes.byref(pf), ctypes.byref(ps), ctypes.byref(npairs), ctypes.byref(pd),
+        ctypes.byref(ndeg), stream_ptr()))
+    first = _take_int32(pf.value, npairs.value, x.device)
+    second = _take_int32(ps.value, npairs.value, x.device)
+    degen = _take_int32(pd.value, ndeg.value, x.device)
+    return torch.stack([first, second], dim=1), degen
+

@@ -174,6 +174,31 @@ int axsq_sdf_stats_get(const axsq_sdf* sdf, int64_t* visits, int64_t* leaves);
  * counterpart: the reference runs every walk in one pass). */
 int axsq_sdf_last_suspended(const axsq_sdf* sdf, int64_t* count);
 
+/* ---- Triangle intersection and mesh self-intersection ----------------------
+ * primal::intersect(Triangle3 t1, Triangle3 t2, includeBoundary, EPS)
+ * (src/axom/primal/operators/intersect.hpp:65-71 ->
+ * primal/operators/detail/intersect_impl.hpp:155-382) over n pairs: t1, t2
+ * device AoS corners (9 doubles per triangle: A, B, C); out device uint8
+ * (1: the pair intersects). */
+int axsq_tri_tri_intersect(const double* t1, const double* t2, int64_t n, int include_boundary,
+                           double eps, uint8_t* out, void* stream);
+
+/* quest::findTriMeshIntersectionsBVH<Exec, double>(mesh, intersections,
+ * degenerateIndices, intersectionThreshold) (src/axom/quest/MeshTester.hpp:
+ * 66-87 -> quest/detail/MeshTester_detail.hpp:159-341). x, y, z: device SoA
+ * vertices; tris: device int32 connectivity (3*ntris). The library allocates
+ * *first and *second (*npairs entries each) and *degenerate (*ndegen
+ * entries), all device int32; release each with axsq_free. Pairs (first[k],
+ * second[k]) have first < second and come in the reference's sequential
+ * order: by first, then by second's place in first's BVH candidate list.
+ * Degenerate triangles (area within 1e-12 of zero) are listed ascending and,
+ * as in the reference's BVH tester, still take part in the pair tests. */
+int axsq_mesh_self_intersections(const double* x, const double* y, const double* z,
+                                 int64_t nverts, const int32_t* tris, int64_t ntris,
+                                 double threshold, int32_t** first, int32_t** second,
+                                 int64_t* npairs, int32_t** degenerate, int64_t* ndegen,
+                                 void* stream);
+
 #ifdef __cplusplus
 }
 #endif

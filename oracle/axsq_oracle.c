@@ -1315,3 +1315,327 @@ int orc_sdf_backward(const orc_sdf* s, const double* q, int64_t nq, const double
   free(wts);
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Triangle-triangle intersection (Devillers & Guigue 2002, as the reference */
+/* states it) and the BVH mesh self-intersection test.                       */
+/* ------------------------------------------------------------------------ */
+
+/* twoDcross, primal/operators/detail/intersect_impl.hpp:538-544 */
+static inline double cross2(const double a[2], const double b[2], const double c[2])
+{
+  return ((a[0] - c[0]) * (b[1] - c[1]) - (a[1] - c[1]) * (b[0] - c[0]));
+}
+/* isGpeq / isLpeq against 0, fuzzy_comparators.hpp:55-84 */
+static inline int gpeq0(double x, int inc, double eps)
+{
+  return (inc && near_eq(x, 0.0, eps)) || is_gt(x, 0.0, eps);
+}
+static inline int lpeq0(double x, int inc, double eps)
+{
+  return (inc && near_eq(x, 0.0, eps)) || is_lt(x, 0.0, eps);
+}
+
+/* checkEdge, intersect_impl.hpp:1344-1392 */
+static int tt_check_edge(const double* p1, const double* q1, const double* r1, const double* p2,
+                         const double* r2, int inc, double eps)
+{
+  if(gpeq0(cross2(r2, p2, q1), inc, eps))
+  {
+    if(!gpeq0(cross2(r2, p1, q1), inc, eps)) return 0;
+    if(gpeq0(cross2(p1, p2, q1), inc, eps)) return 1;
+    return gpeq0(cross2(p1, p2, r1), inc, eps) && gpeq0(cross2(q1, r1, p2), inc, eps);
+  }
+  return gpeq0(cross2(r2, p2, r1), inc, eps) && gpeq0(cross2(q1, r1, r2), inc, eps) &&
+    gpeq0(cross2(p1, p2, r1), inc, eps);
+}
+
+/* checkVertex, intersect_impl.hpp:1394-1484 */
+static int tt_check_vertex(const double* p1, const double* q1, const double* r1, const double* p2,
+                           const double* q2, const double* r2, int inc, double eps)
+{
+  if(gpeq0(cross2(r2, p2, q1), inc, eps))
+  {
+    if(gpeq0(cross2(q2, r2, q1), inc, eps))
+    {
+      if(gpeq0(cross2(p1, p2, q1), inc, eps)) return lpeq0(cross2(p1, q2, q1), inc, eps);
+      return gpeq0(cross2(p1, p2, r1), inc, eps) && lpeq0(cross2(q1, p2, r1), inc, eps);
+    }
+    return lpeq0(cross2(p1, q2, q1), inc, eps) && gpeq0(cross2(q2, r2, r1), inc, eps) &&
+      gpeq0(cross2(q1, r1, q2), inc, eps);
+  }
+  if(!gpeq0(cross2(r2, p2, r1), inc, eps)) return 0;
+  if(gpeq0(cross2(q1, r1, r2), inc, eps)) return gpeq0(cross2(r1, p1, p2), inc, eps);
+  return gpeq0(cross2(q1, r1, q2), inc, eps) && gpeq0(cross2(q2, r2, r1), inc, eps);
+}
+
+/* intersectPermuted2DTriangles, intersect_impl.hpp:1281-1342: where p1 lies
+ * against the three edges of triangle 2 picks the edge or vertex test. */
+static int tt_permuted_2d(const double* p1, const double* q1, const double* r1, const double* p2,
+                          const double* q2, const double* r2, int inc, double eps)
+{
+  const int a = gpeq0(cross2(p2, q2, p1), inc, eps);
+  const int b = gpeq0(cross2(q2, r2, p1), inc, eps);
+  const int c = gpeq0(cross2(r2, p2, p1), inc, eps);
+  if(a && b) return c ? 1 : tt_check_edge(p1, q1, r1, p2, r2, inc, eps);
+  if(a) return c ? tt_check_edge(p1, q1, r1, r2, q2, inc, eps)
+                 : tt_check_vertex(p1, q1, r1, p2, q2, r2, inc, eps);
+  if(b) return c ? tt_check_edge(p1, q1, r1, q2, p2, inc, eps)
+                 : tt_check_vertex(p1, q1, r1, q2, r2, p2, inc, eps);
+  return tt_check_vertex(p1, q1, r1, r2, p2, q2, inc, eps);
+}
+
+/* TriangleIntersection2D, intersect_impl.hpp:1251-1279: both triangles made
+ * counter-clockwise by swapping their last two corners. */
+static int tt_tri_2d(const double t1[6], const double t2[6], int inc, double eps)
+{
+  const int f1 = is_lt(cross2(t1, t1 + 2, t1 + 4), 0.0, eps);
+  const int f2 = is_lt(cross2(t2, t2 + 2, t2 + 4), 0.0, eps);
+  return tt_permuted_2d(t1, f1 ? t1 + 4 : t1 + 2, f1 ? t1 + 2 : t1 + 4, t2, f2 ? t2 + 4 : t2 + 2,
+                        f2 ? t2 + 2 : t2 + 4, inc, eps);
+}
+
+/* intersectCoplanar3DTriangles, intersect_impl.hpp:1190-1249: project on the
+ * coordinate plane where triangle 1's normal is largest. */
+static int tt_coplanar(const double* p1, const double* q1, const double* r1, const double* p2,
+                       const double* q2, const double* r2, const double n_in[3], int inc,
+                       double eps)
+{
+  const double n[3] = {fabs(n_in[0]), fabs(n_in[1]), fabs(n_in[2])};
+  int u, v, qpr; /* coordinates kept; qpr: corner order q,p,r instead of p,q,r */
+  if(is_gt(n[0], n[2], eps) && is_geq(n[0], n[1], eps))
+  {
+    u = 2;
+    v = 1;
+    qpr = 1;
+  }
+  else if(is_gt(n[1], n[2], eps) && is_geq(n[1], n[0], eps))
+  {
+    u = 0;
+    v = 2;
+    qpr = 1;
+  }
+  else
+  {
+    u = 0;
+    v = 1;
+    qpr = 0;
+  }
+  const double* c1[3] = {qpr ? q1 : p1, qpr ? p1 : q1, r1};
+  const double* c2[3] = {qpr ? q2 : p2, qpr ? p2 : q2, r2};
+  double a[6], b[6];
+  for(int k = 0; k < 3; ++k)
+  {
+    a[2 * k] = c1[k][u];
+    a[2 * k + 1] = c1[k][v];
+    b[2 * k] = c2[k][u];
+    b[2 * k + 1] = c2[k][v];
+  }
+  return tt_tri_2d(a, b, inc, eps);
+}
+
+/* intersectTwoPermutedTriangles, intersect_impl.hpp:456-471 (Devillers &
+ * Guigue's interval test): Vector3(a,b) = b - a, Triangle3(a,b,c).normal() =
+ * (b - a) x (c - a). */
+static int tt_two_permuted(const double* p1, const double* q1, const double* r1,
+                           const double* p2, const double* q2, const double* r2, int inc,
+                           double eps)
+{
+  double u[3], v[3], w[3], n[3];
+  for(int d = 0; d < 3; ++d)
+  {
+    u[d] = q2[d] - q1[d];
+    v[d] = p2[d] - q1[d];
+    w[d] = p1[d] - q1[d];
+  }
+  cross3(v, w, n);
+  if(!lpeq0(dot3(u, n), inc, eps)) return 0;
+  for(int d = 0; d < 3; ++d)
+  {
+    u[d] = r2[d] - p1[d];
+    v[d] = p2[d] - p1[d];
+    w[d] = r1[d] - p1[d];
+  }
+  cross3(v, w, n);
+  return lpeq0(dot3(u, n), inc, eps);
+}
+
+/* The permutation step of intersect_tri3D_tri3D (intersect_impl.hpp:213-381
+ * for triangle 1) and of intersectOnePermutedTriangle (:1116-1187 for
+ * triangle 2), which share one table: from the signed distances d of one
+ * triangle's corners to the other's plane, the rotation that brings the lone
+ * corner first (rot: corner rot becomes p) and whether the OTHER triangle's
+ * last two corners swap. Returns -1 when all three are near zero. */
+static int tt_orient(const double d[3], double eps, int* rot, int* swap)
+{
+  const double p = d[0], q = d[1], r = d[2];
+  if(is_gt(p, 0.0, eps))
+  {
+    if(is_gt(q, 0.0, eps)) { *rot = 2; *swap = 1; }
+    else if(is_gt(r, 0.0, eps)) { *rot = 1; *swap = 1; }
+    else { *rot = 0; *swap = 0; }
+  }
+  else if(is_lt(p, 0.0, eps))
+  {
+    if(is_lt(q, 0.0, eps)) { *rot = 2; *swap = 0; }
+    else if(is_lt(r, 0.0, eps)) { *rot = 1; *swap = 0; }
+    else { *rot = 0; *swap = 1; }
+  }
+  else if(is_lt(q, 0.0, eps))
+  {
+    if(is_geq(r, 0.0, eps)) { *rot = 1; *swap = 1; }
+    else { *rot = 0; *swap = 0; }
+  }
+  else if(is_gt(q, 0.0, eps))
+  {
+    if(is_gt(r, 0.0, eps)) { *rot = 0; *swap = 1; }
+    else { *rot = 1; *swap = 0; }
+  }
+  else if(is_gt(r, 0.0, eps)) { *rot = 2; *swap = 0; }
+  else if(is_lt(r, 0.0, eps)) { *rot = 2; *swap = 1; }
+  else return -1;
+  return 0;
+}
+
+/* nonzeroSignMatch / countZeros / twoZeros / oneZeroOthersMatch,
+ * intersect_impl.hpp:547-592 */
+static inline int sgn(double x) { return (0 < x) - (x < 0); }
+static int tt_all_one_side(const double d[3], double eps)
+{
+  return !near_eq(d[0], 0., eps) && !near_eq(d[1], 0., eps) && !near_eq(d[2], 0., eps) &&
+    sgn(d[0]) == sgn(d[1]) && sgn(d[0]) == sgn(d[2]);
+}
+static int tt_touch_only(const double d[3], double eps)
+{
+  const int z0 = near_eq(d[0], 0.0, eps), z1 = near_eq(d[1], 0.0, eps), z2 = near_eq(d[2], 0.0, eps);
+  const int nz = z0 + z1 + z2;
+  if(nz == 2) return 1;
+  return nz == 1 &&
+    ((z0 && is_gt(d[1] * d[2], 0.0, eps)) || (z1 && is_gt(d[2] * d[0], 0.0, eps)) ||
+     (z2 && is_gt(d[0] * d[1], 0.0, eps)));
+}
+
+/* Triangle::normal().unitVector() (Triangle.hpp:99-102, Vector.hpp:477-490)
+ * and the signed distances (t[k] - o[2]) . n of triangle t's corners. */
+static void tt_plane_dists(const double* t, const double* o, double n[3], double d[3])
+{
+  double u[3], v[3], c[3];
+  for(int k = 0; k < 3; ++k)
+  {
+    u[k] = o[3 + k] - o[k];
+    v[k] = o[6 + k] - o[k];
+  }
+  cross3(u, v, c);
+  orc_unit_vector(c, n);
+  for(int k = 0; k < 3; ++k)
+  {
+    const double e[3] = {t[3 * k] - o[6], t[3 * k + 1] - o[7], t[3 * k + 2] - o[8]};
+    d[k] = dot3(e, n);
+  }
+}
+
+/* primal::intersect(Triangle3, Triangle3, includeBoundary, EPS)
+ * (primal/operators/intersect.hpp:65-71 -> intersect_impl.hpp:155-382). */
+int orc_tri_tri(const double t1[9], const double t2[9], int include_boundary, double eps)
+{
+  const int inc = include_boundary != 0;
+  double n1[3], n2[3], d1[3], d2[3];
+  tt_plane_dists(t1, t2, n2, d1);
+  if(tt_all_one_side(d1, eps)) return 0;
+  if(!inc && tt_touch_only(d1, eps)) return 0;
+  tt_plane_dists(t2, t1, n1, d2);
+  if(tt_all_one_side(d2, eps)) return 0;
+  if(!inc && tt_touch_only(d2, eps)) return 0;
+  int rot1, swap2;
+  if(tt_orient(d1, eps, &rot1, &swap2) < 0)
+    return tt_coplanar(t1, t1 + 3, t1 + 6, t2, t2 + 3, t2 + 6, n1, inc, eps);
+  const double* P1 = t1 + 3 * rot1;
+  const double* Q1 = t1 + 3 * ((rot1 + 1) % 3);
+  const double* R1 = t1 + 3 * ((rot1 + 2) % 3);
+  const double* c2[3] = {t2, swap2 ? t2 + 6 : t2 + 3, swap2 ? t2 + 3 : t2 + 6};
+  const double e2[3] = {d2[0], swap2 ? d2[2] : d2[1], swap2 ? d2[1] : d2[2]};
+  int rot2, swap1;
+  if(tt_orient(e2, eps, &rot2, &swap1) < 0)
+    return tt_coplanar(P1, Q1, R1, c2[0], c2[1], c2[2], n1, inc, eps);
+  return tt_two_permuted(P1, swap1 ? R1 : Q1, swap1 ? Q1 : R1, c2[rot2], c2[(rot2 + 1) % 3],
+                         c2[(rot2 + 2) % 3], inc, eps);
+}
+
+/* quest::findTriMeshIntersectionsBVH (quest/MeshTester.hpp:66-87) through
+ * CandidateFinderBase::initialize / findTriMeshIntersections and the BVH
+ * CandidateFinder (quest/detail/MeshTester_detail.hpp:159-341): per triangle
+ * its corner box and the degenerate flag (Triangle::degenerate, Triangle.hpp:
+ * 104-109,327-330: |0.5 |n|| <= 1e-12); a BVH (default scale 1.000123) over
+ * the boxes; findBoundingBoxes of every triangle's box; the pairs (i, j) with
+ * i < j in candidate order, kept when the triangles intersect (boundary
+ * excluded, tolerance thresh). Writes up to cap pairs in the sequential
+ * order and degen[ntris] flags; returns the number of pairs, or -1. */
+int64_t orc_mesh_self_isect(const double* x, const double* y, const double* z, int32_t nverts,
+                            const int32_t* conn, int32_t ntris, double thresh, int32_t* first,
+                            int32_t* second, int64_t cap, uint8_t* degen, int nthreads)
+{
+  (void)nverts;
+  double* tri = (double*)malloc(sizeof(double) * 9 * (size_t)(ntris > 0 ? ntris : 1));
+  double* boxes = (double*)calloc(6 * (size_t)(ntris > 0 ? ntris : 1), sizeof(double));
+  for(int32_t c = 0; c < ntris; ++c)
+  {
+    double* t = tri + 9 * (size_t)c;
+    for(int k = 0; k < 3; ++k)
+    {
+      const int32_t v = conn[3 * (size_t)c + k];
+      t[3 * k] = x[v];
+      t[3 * k + 1] = y[v];
+      t[3 * k + 2] = z[v];
+    }
+    double u[3], w[3], n[3];
+    for(int k = 0; k < 3; ++k)
+    {
+      u[k] = t[3 + k] - t[k];
+      w[k] = t[6 + k] - t[k];
+    }
+    cross3(u, w, n);
+    degen[c] = (uint8_t)near_eq(0.5 * sqrt(dot3(n, n)), 0.0, 1.0e-12);
+    /* BoundingBox{A, B, C} (compute_bounding_box.hpp:110-114) */
+    double* b = boxes + 6 * (size_t)c;
+    for(int k = 0; k < 3; ++k) b[k] = b[3 + k] = t[k];
+    for(int corner = 1; corner < 3; ++corner)
+      for(int k = 0; k < 3; ++k)
+      {
+        const double p = t[3 * corner + k];
+        if(p < b[k]) b[k] = p;
+        if(p > b[3 + k]) b[3 + k] = p;
+      }
+  }
+  orc_bvh* bvh = orc_bvh_new(boxes, ntris, 1.000123);
+  int32_t* counts = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ntris > 0 ? ntris : 1));
+  int32_t* offsets = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ntris > 0 ? ntris : 1));
+  int64_t total = orc_bvh_find(bvh, 1, boxes, ntris, 0.0, counts, offsets, NULL, nthreads);
+  int64_t npairs = -1;
+  if(total >= 0)
+  {
+    int32_t* cands = (int32_t*)malloc(sizeof(int32_t) * (size_t)(total > 0 ? total : 1));
+    orc_bvh_find(bvh, 1, boxes, ntris, 0.0, counts, offsets, cands, nthreads);
+    npairs = 0;
+    for(int32_t i = 0; i < ntris; ++i)
+      for(int32_t k = 0; k < counts[i]; ++k)
+      {
+        const int32_t j = cands[offsets[i] + k];
+        if(i < j && orc_tri_tri(tri + 9 * (size_t)i, tri + 9 * (size_t)j, 0, thresh))
+        {
+          if(npairs < cap)
+          {
+            first[npairs] = i;
+            second[npairs] = j;
+          }
+          ++npairs;
+        }
+      }
+    free(cands);
+  }
+  free(offsets);
+  free(counts);
+  orc_bvh_free(bvh);
+  free(boxes);
+  free(tri);
+  return npairs;
+}

@@ -103,6 +103,15 @@ int orc_sdf_backward(const orc_sdf* s, const double* q, int64_t nq, const double
                      const int32_t* elem, const float* gout, double* grad_q,
                      double* grad_v);
 
+/* primal::intersect(Triangle3, Triangle3, includeBoundary, EPS):
+ * t1, t2 = corners A, B, C (9 doubles each); 1 if they intersect. */
+int orc_tri_tri(const double t1[9], const double t2[9], int include_boundary, double eps);
+/* quest::findTriMeshIntersectionsBVH: pairs in sequential order (up to cap),
+ * degenerate flags; returns the number of pairs or -1. */
+int64_t orc_mesh_self_isect(const double* x, const double* y, const double* z, int32_t nverts,
+                            const int32_t* conn, int32_t ntris, double thresh, int32_t* first,
+                            int32_t* second, int64_t cap, uint8_t* degen, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

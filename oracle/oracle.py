@@ -73,6 +73,12 @@ def lib():
         L.orc_ray_box.argtypes = [_dp, _dp, _dp, ctypes.c_double]
         L.orc_box_scale.argtypes = [_dp, ctypes.c_double]
         L.orc_unit_vector.argtypes = [_dp, _dp]
+        L.orc_tri_tri.restype = ctypes.c_int
+        L.orc_tri_tri.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_double]
+        L.orc_mesh_self_isect.restype = ctypes.c_int64
+        L.orc_mesh_self_isect.argtypes = [_dp, _dp, _dp, ctypes.c_int32, _i32p, ctypes.c_int32,
+                                          ctypes.c_double, _i32p, _i32p, ctypes.c_int64,
+                                          ctypes.POINTER(ctypes.c_uint8), ctypes.c_int]
         _lib = L
     return _lib
 
@@ -122,6 +128,39 @@ def unit_vector(v):
     out = np.zeros(3)
     lib().orc_unit_vector(_p(f64(v), _dp), _p(out, _dp))
     return out
+
+
+def tri_tri(t1, t2, include_boundary=False, eps=1e-8):
+    """primal::intersect(Triangle3, Triangle3, includeBoundary, EPS); t1, t2 are
+    (3, 3) corner arrays."""
+    return bool(lib().orc_tri_tri(_p(f64(t1).reshape(9), _dp), _p(f64(t2).reshape(9), _dp),
+                                  int(bool(include_boundary)), float(eps)))
+
+
+def mesh_self_intersections(verts, tris, threshold=1e-8, nthreads=1):
+    """quest::findTriMeshIntersectionsBVH: returns (pairs (k, 2) int32 in the
+    sequential order, degenerate triangle indices)."""
+    v = f64(verts)
+    t = i32(tris)
+    x, y, z = (np.ascontiguousarray(v[:, k]) for k in range(3))
+    nt = t.shape[0]
+    degen = np.zeros(max(nt, 1), dtype=np.uint8)
+    cap = 0
+    while True:
+        first = np.zeros(max(cap, 1), dtype=np.int32)
+        second = np.zeros(max(cap, 1), dtype=np.int32)
+        n = lib().orc_mesh_self_isect(_p(x, _dp), _p(y, _dp), _p(z, _dp), v.shape[0],
+                                      _p(t.reshape(-1), _i32p), nt, float(threshold),
+                                      _p(first, _i32p), _p(second, _i32p), cap,
+                                      degen.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                      nthreads)
+        if n < 0:
+            raise RuntimeError("oracle mesh self-intersection failed")
+        if n <= cap:
+            break
+        cap = n
+    pairs = np.stack([first[:n], second[:n]], axis=1) if n else np.zeros((0, 2), np.int32)
+    return pairs, np.nonzero(degen[:nt])[0].astype(np.int32)
 
 
 # ------------------------------------------------------------------- BVH ----
