@@ -76,6 +76,14 @@ void Workspace::release() {
   redo.release();
   susp.release();
   cnt.release();
+  mesh_tri.release();
+  mesh_box.release();
+  mesh_int.release();
+  mesh_slots.release();
+  if (mesh_bvh) {
+    axsq_bvh_destroy(mesh_bvh);
+    mesh_bvh = nullptr;
+  }
 }
 
 namespace {

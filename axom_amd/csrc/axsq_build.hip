@@ -9,8 +9,8 @@
 //   radix sort    stable LSD sort of (code, id)
 //   k_reorder     scaled leaf boxes in leaf (sorted) order
 //   k_karras      Karras 2012 hierarchy over the sorted codes
-//   k_refit       bottom-up boxes; the second thread to reach a node goes on
-//                 (agent-scope acquire/release counters)
+//   k_refit_*     bottom-up boxes; the second thread to reach a node goes on
+//                 (hand-offs kept inside workgroups by tiers of leaf spans)
 //   k_emit        128-byte FP64 nodes (+ 64-byte FP32 nodes, rounded outward)
 #include "axsq_internal.hpp"
 
@@ -211,42 +211,114 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
   if (i == 0) parents[0] = -1;
 }
 
-// propagate_aabbs, build_radix_tree.hpp:505-576. The counter increment is an
-// agent-scope acq_rel RMW: the first thread's box stores are released before
-// its increment, and the second thread acquires them with its increment, so
-// no volatile polling is needed. Every thread finishes after at most
-// depth-many steps.
-__global__ __launch_bounds__(kBlock) void k_refit(const double* __restrict__ leaf_boxes,
-                                                   int32_t inner,
-                                                   const int32_t* __restrict__ lchild,
-                                                   const int32_t* __restrict__ rchild,
-                                                   const int32_t* __restrict__ parents,
-                                                   int32_t* counters, double* inner_boxes) {
-  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+// propagate_aabbs, build_radix_tree.hpp:505-576: every leaf thread climbs
+// towards the root; at each node the first thread to arrive stops and the
+// second unions the two child boxes and goes on. Here the climb is split by
+// leaf spans so that every hand-off stays inside one workgroup (on gfx950 an
+// agent-scope release/acquire writes back and invalidates the XCD's L2, which
+// per node would cost far more than the refit itself). Inner node i of a
+// Karras tree covers a contiguous leaf range and its index is that range's
+// low or high end, so the nodes whose range lies inside a span of leaves form
+// subtrees of that span.
+//   k_refit_chunks  spans of kBlock leaves: counters and boxes in LDS
+//   k_refit_span    spans of kRefitSpan leaves, one workgroup each, then one
+//                   workgroup for the rest of the tree: global counters and
+//                   boxes, workgroup-scope hand-off
+// A thread that reaches a node whose range leaves its span records the child
+// it comes from (an exit) for the next tier and stops. No thread waits on
+// another, and every thread stops after at most depth-many steps.
+constexpr int kRefitThreads = 1024;
+constexpr int64_t kRefitSpan = 16384;  // a multiple of kBlock
+
+__global__ __launch_bounds__(kBlock) void k_refit_chunks(const double* __restrict__ leaf_boxes,
+                                                          int32_t inner,
+                                                          const int32_t* __restrict__ lchild,
+                                                          const int32_t* __restrict__ rchild,
+                                                          const int32_t* __restrict__ parents,
+                                                          const int32_t* __restrict__ range,
+                                                          double* __restrict__ inner_boxes,
+                                                          int32_t* __restrict__ exits,
+                                                          int32_t* __restrict__ n_exits) {
+  __shared__ int32_t s_cnt[kBlock];
+  __shared__ double s_box[6][kBlock];
+  const int32_t base = (int32_t)(blockIdx.x * kBlock);
+  const int32_t top = base + kBlock - 1;  // the chunk is leaves [base, top]
+  s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int32_t i = base + (int32_t)threadIdx.x;
   if (i > inner) return;
   double acc[6];
   load_box(leaf_boxes + 6 * (int64_t)i, acc);
   int32_t last = inner + i;
   int32_t cur = parents[inner + i];
   while (cur != -1) {
-    const int32_t old = __hip_atomic_fetch_add(&counters[cur], 1, __ATOMIC_ACQ_REL,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) break;
+    if (range[3 * cur] < base || range[3 * cur + 1] > top) {
+      const int64_t g = i / kRefitSpan;  // the next tier's span (kRefitSpan slots each)
+      exits[g * kRefitSpan + atomicAdd(&n_exits[g], 1)] = last;
+      break;
+    }
+    if (__hip_atomic_fetch_add(&s_cnt[cur - base], 1, __ATOMIC_ACQ_REL,
+                               __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+      break;
     const int32_t lc = lchild[cur], rc = rchild[cur];
-    const int32_t other = (lc == last) ? rc : lc;
+    const int32_t other = (lc == last) ? rc : lc;  // a chunk node or a leaf
     double ob[6];
     if (other >= inner)
       load_box(leaf_boxes + 6 * (int64_t)(other - inner), ob);
     else
-      for (int k = 0; k < 6; ++k)
-        ob[k] = __hip_atomic_load(&inner_boxes[6 * (int64_t)other + k], __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
+      for (int k = 0; k < 6; ++k) ob[k] = s_box[k][other - base];
     box_add_box(acc, ob);
-    for (int k = 0; k < 6; ++k)
-      __hip_atomic_store(&inner_boxes[6 * (int64_t)cur + k], acc[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = 0; k < 6; ++k) s_box[k][cur - base] = acc[k];
+    store_box(inner_boxes + 6 * (int64_t)cur, acc);
     last = cur;
     cur = parents[cur];
+  }
+}
+
+// Workgroup g climbs from the exits in exits_in[g * cap_in ...] through the
+// nodes whose range lies in leaves [g * span, (g + 1) * span - 1]; the rest go
+// to exits_out (one list). With span covering every leaf there are none.
+// Boxes below the exits were stored by the previous tier (visible across the
+// kernel boundary); within the tier the hand-off is between waves of one
+// workgroup.
+__global__ __launch_bounds__(kRefitThreads) void k_refit_span(
+    const double* __restrict__ leaf_boxes, int32_t inner, const int32_t* __restrict__ lchild,
+    const int32_t* __restrict__ rchild, const int32_t* __restrict__ parents,
+    const int32_t* __restrict__ range, int64_t span, const int32_t* __restrict__ exits_in,
+    const int32_t* __restrict__ n_in, int64_t cap_in, int32_t* __restrict__ exits_out,
+    int32_t* __restrict__ n_out, int32_t* counters, double* inner_boxes) {
+  const int64_t g = blockIdx.x;
+  const int64_t lo = g * span, hi = lo + span - 1;
+  const int32_t ne = n_in[g];
+  const int32_t* ex = exits_in + g * cap_in;
+  for (int32_t e = (int32_t)threadIdx.x; e < ne; e += kRefitThreads) {
+    int32_t last = ex[e];
+    double acc[6];
+    if (last >= inner)
+      load_box(leaf_boxes + 6 * (int64_t)(last - inner), acc);
+    else
+      load_box(inner_boxes + 6 * (int64_t)last, acc);
+    int32_t cur = parents[last];
+    while (cur != -1) {
+      if (range[3 * cur] < lo || range[3 * cur + 1] > hi) {
+        if (exits_out) exits_out[atomicAdd(n_out, 1)] = last;  // none in the last tier
+        break;
+      }
+      if (__hip_atomic_fetch_add(&counters[cur], 1, __ATOMIC_ACQ_REL,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+        break;
+      const int32_t lc = lchild[cur], rc = rchild[cur];
+      const int32_t other = (lc == last) ? rc : lc;
+      double ob[6];
+      if (other >= inner)
+        load_box(leaf_boxes + 6 * (int64_t)(other - inner), ob);
+      else
+        load_box(inner_boxes + 6 * (int64_t)other, ob);
+      box_add_box(acc, ob);
+      store_box(inner_boxes + 6 * (int64_t)cur, acc);
+      last = cur;
+      cur = parents[cur];
+    }
   }
 }
 
@@ -412,7 +484,11 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   const size_t off_c = off_p + sizeof(int32_t) * (2 * n - 1);
   const size_t off_rg = off_c + sizeof(int32_t) * inner;
   const size_t off_ib = (off_rg + sizeof(int32_t) * 3 * inner + 255) & ~size_t(255);
-  const size_t total = off_ib + sizeof(double) * 6 * inner;
+  // refit exits: per-span counts of the first tier, one count for the second,
+  // then the two exit lists
+  const int64_t nspans = (n + kRefitSpan - 1) / kRefitSpan;
+  const size_t off_ex = off_ib + sizeof(double) * 6 * inner;
+  const size_t total = off_ex + sizeof(int32_t) * (nspans + 1 + nspans * kRefitSpan + n);
   AXSQ_TRY(ws.flag.ensure(total));
   char* base = ws.flag.as<char>();
   double* scaled = reinterpret_cast<double*>(base);
@@ -424,6 +500,10 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   int32_t* cnt = reinterpret_cast<int32_t*>(base + off_c);
   int32_t* rng = reinterpret_cast<int32_t*>(base + off_rg);
   double* ib = reinterpret_cast<double*>(base + off_ib);
+  int32_t* n_ex0 = reinterpret_cast<int32_t*>(base + off_ex);
+  int32_t* n_ex1 = n_ex0 + nspans;
+  int32_t* ex0 = n_ex1 + 1;
+  int32_t* ex1 = ex0 + nspans * kRefitSpan;
 
   ProfRange pr("bvh_build", s);
   k_transform<<<(unsigned)nparts, kBlock, 0, s>>>(boxes, n_in, n, scale, scaled, partials);
@@ -442,7 +522,17 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   k_karras<<<grid_for(inner), kBlock, 0, s>>>(b->mcodes, inner, lch, rch, par, rng);
   AXSQ_CHECK_HIP(hipGetLastError());
   AXSQ_TRY(fill_i32(cnt, 0, inner, s));
-  k_refit<<<grid_for(n), kBlock, 0, s>>>(b->leaf_boxes, inner, lch, rch, par, cnt, ib);
+  AXSQ_TRY(fill_i32(n_ex0, 0, nspans + 1, s));
+  k_refit_chunks<<<grid_for(n), kBlock, 0, s>>>(b->leaf_boxes, inner, lch, rch, par, rng, ib,
+                                                ex0, n_ex0);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  k_refit_span<<<(unsigned)nspans, kRefitThreads, 0, s>>>(b->leaf_boxes, inner, lch, rch, par, rng,
+                                                          kRefitSpan, ex0, n_ex0, kRefitSpan, ex1,
+                                                          n_ex1, cnt, ib);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  k_refit_span<<<1, kRefitThreads, 0, s>>>(b->leaf_boxes, inner, lch, rch, par, rng,
+                                           nspans * kRefitSpan, ex1, n_ex1, 0, nullptr, nullptr,
+                                           cnt, ib);
   AXSQ_CHECK_HIP(hipGetLastError());
   k_emit<<<grid_for(inner), kBlock, 0, s>>>(b->leaf_boxes, ib, inner, lch, rch, rng, b->nodes,
                                             b->nodes32);

@@ -45,6 +45,8 @@ struct DevBuf {
 // Scratch owned by the calling host thread (per device), reused across calls.
 struct Workspace {
   DevBuf k0, v0, k1, v1, hist, tiles, misc, flag, susp, cnt, redo;
+  DevBuf mesh_tri, mesh_box, mesh_int, mesh_slots;  // mesh self-intersection
+  axsq_bvh* mesh_bvh = nullptr;                      // its BVH, rebuilt per call
   int device = -1;
   void release();
 };
@@ -57,7 +59,10 @@ struct ProfRange {
   const char* name = nullptr;
   hipStream_t s = nullptr;
   ProfRange(const char* n, hipStream_t st);
-  void stop();
+  ProfRange(const ProfRange&) = delete;
+  ProfRange& operator=(const ProfRange&) = delete;
+  ~ProfRange() { stop(); }
+  void stop();  // records the end event once; later calls do nothing
 };
 void prof_flush();  // after a stream sync: folds finished ranges into totals
 

@@ -1,17 +1,21 @@
 // axsq_mesh.hip -- triangle-triangle intersection and the BVH mesh
 // self-intersection test (quest::findTriMeshIntersectionsBVH) for gfx950.
 //
-// The pipeline of quest/detail/MeshTester_detail.hpp:159-341 -- per-triangle
-// corner boxes and degenerate flags, a BVH over the boxes, findBoundingBoxes
-// of every box, the narrow phase on the pairs i < j -- with the sequential
-// path's output order: pairs grouped by the first triangle, the second in
-// the first's candidate (BVH traversal) order; degenerate indices ascending.
-// The narrow phase runs one triangle per lane over its candidate list, in two
-// passes (flags + counts, then emit after a scan), so the order needs no
-// atomics.
+// The reference (quest/detail/MeshTester_detail.hpp:159-341) builds corner
+// boxes and degenerate flags, a BVH over the boxes, runs findBoundingBoxes
+// for every box (count, scan, fill of all candidates), keeps the pairs i < j
+// and tests them. Here the triangle test is the traversal's leaf action
+// (the custom leaf action of quest/examples/quest_bvh_two_pass.cpp:144-232):
+// one lane per triangle walks the tree with its box and tests each reached
+// triangle j > i on the spot, keeping the first kPairSlots hits. A scan of the
+// hit counts places every triangle's pairs; triangles with more hits walk
+// again to write them all. The candidate lists are never stored. The output
+// order is the sequential path's: pairs grouped by the first triangle, the
+// second in the first's BVH traversal order; degenerate indices ascending.
 #include <hip/hip_runtime.h>
 
 #include "axsq_internal.hpp"
+#include "axsq_traverse.hpp"
 #include "axsq_tritri.hpp"
 
 namespace axsq {
@@ -60,52 +64,89 @@ __global__ __launch_bounds__(kBlock) void k_tri_setup(const double* __restrict__
   for (int k = 0; k < 6; ++k) boxes[6 * c + k] = b[k];
 }
 
-// Narrow phase, pass 1: for triangle i, flag each candidate j > i that
-// intersects it (boundary excluded, tolerance eps) and count the flags.
-__global__ __launch_bounds__(kBlock) void k_tt_flags(const double* __restrict__ tri,
-                                                     const int32_t* __restrict__ offsets,
-                                                     const int32_t* __restrict__ counts,
-                                                     const int32_t* __restrict__ cands, int64_t n,
-                                                     double eps, uint8_t* __restrict__ flag,
-                                                     int32_t* __restrict__ npair) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double* ti = tri + 9 * i;
+constexpr int kPairSlots = 4;  // hits kept per triangle by the first walk
+
+struct IsectArgs {
+  const Node64* nodes;
+  int32_t ninner;
+  const int32_t* leaf_ids;
+  const double* tri;    // 9 doubles per triangle
+  const double* boxes;  // 6 doubles per triangle (the unscaled corner box)
+  int64_t n;
+  double eps;
+  int32_t* npair;  // hits per triangle
+  int32_t* slots;  // kPairSlots per triangle
+  const int32_t* poff;
+  int32_t* first;
+  int32_t* second;
+  int32_t* status;
+};
+
+// One walk of triangle i's box through the tree (BoundingBox::intersectsWith,
+// BVH.hpp:558-560, as findBoundingBoxes tests it) that calls hit(j) for each
+// reached triangle j > i that intersects triangle i (boundary excluded).
+template <class Hit>
+__device__ __forceinline__ void isect_walk(const IsectArgs& a, int64_t i, lds_i32* stk, Hit&& hit) {
+  double prim[6];
+  for (int k = 0; k < 6; ++k) prim[k] = a.boxes[6 * i + k];
+  const double* ti = a.tri + 9 * i;
   const V3 a0 = load_v3(ti), a1 = load_v3(ti + 3), a2 = load_v3(ti + 6);
-  const int32_t off = offsets[i], cnt = counts[i];
-  int32_t c = 0;
-  for (int32_t k = 0; k < cnt; ++k) {
-    const int32_t j = cands[off + k];
-    bool f = false;
-    if ((int64_t)j > i && j < n) {
-      const double* tj = tri + 9 * (int64_t)j;
-      f = tri_tri_intersect(a0, a1, a2, load_v3(tj), load_v3(tj + 3), load_v3(tj + 6), false, eps);
-    }
-    flag[off + k] = f ? 1 : 0;
-    c += f ? 1 : 0;
-  }
-  npair[i] = c;
+  auto no_pref = [](const auto*, const auto*) { return false; };
+  const int32_t v = traverse<Node64, double>(
+      a.nodes, a.ninner, stk,
+      [&](const double* b) {
+        return (prim[3] >= b[0] && prim[0] <= b[3]) && (prim[4] >= b[1] && prim[1] <= b[4]) &&
+               (prim[5] >= b[2] && prim[2] <= b[5]);
+      },
+      no_pref,
+      [&](int32_t pos) {
+        const int32_t j = a.leaf_ids[pos];
+        if ((int64_t)j <= i || (int64_t)j >= a.n) return;
+        const double* tj = a.tri + 9 * (int64_t)j;
+        if (tri_tri_intersect(a0, a1, a2, load_v3(tj), load_v3(tj + 3), load_v3(tj + 6), false,
+                              a.eps))
+          hit(j);
+      });
+  if (v < 0) atomicOr(a.status, 1);
 }
 
-// Narrow phase, pass 2: write the flagged pairs at the scanned offsets.
-__global__ __launch_bounds__(kBlock) void k_tt_emit(const int32_t* __restrict__ offsets,
-                                                    const int32_t* __restrict__ counts,
-                                                    const int32_t* __restrict__ cands,
-                                                    const uint8_t* __restrict__ flag,
-                                                    const int32_t* __restrict__ poff, int64_t n,
-                                                    int32_t* __restrict__ first,
-                                                    int32_t* __restrict__ second) {
+// Walk 1: count every triangle's hits, keep the first kPairSlots.
+__global__ __launch_bounds__(kBlock) void k_isect_count(IsectArgs a) {
+  __shared__ int32_t s_stack[stack_lds_words(kBlock)];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int32_t off = offsets[i], cnt = counts[i];
-  int32_t o = poff[i];
-  for (int32_t k = 0; k < cnt; ++k) {
-    if (flag[off + k]) {
-      first[o] = (int32_t)i;
-      second[o] = cands[off + k];
-      ++o;
+  if (i >= a.n) return;
+  int32_t c = 0;
+  isect_walk(a, i, lane_stack_base(s_stack), [&](int32_t j) {
+    if (c < kPairSlots) a.slots[kPairSlots * i + c] = j;
+    ++c;
+  });
+  a.npair[i] = c;
+}
+
+// Write the pairs at the scanned offsets; triangles with more hits than
+// slots walk again (the walk is deterministic, so the same hits come in the
+// same order).
+__global__ __launch_bounds__(kBlock) void k_isect_emit(IsectArgs a) {
+  __shared__ int32_t s_stack[stack_lds_words(kBlock)];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t c = (i < a.n) ? a.npair[i] : 0;
+  if (c == 0) return;  // (no block-wide barrier follows)
+  const int32_t o = a.poff[i];
+  if (c <= kPairSlots) {
+    for (int32_t k = 0; k < c; ++k) {
+      a.first[o + k] = (int32_t)i;
+      a.second[o + k] = a.slots[kPairSlots * i + k];
     }
+    return;
   }
+  int32_t m = 0;
+  isect_walk(a, i, lane_stack_base(s_stack), [&](int32_t j) {
+    if (m < c) {
+      a.first[o + m] = (int32_t)i;
+      a.second[o + m] = j;
+    }
+    ++m;
+  });
 }
 
 __global__ __launch_bounds__(kBlock) void k_flag_emit(const int32_t* __restrict__ flag,
@@ -155,19 +196,6 @@ struct Scratch {
   }
 };
 
-struct BvhGuard {
-  axsq_bvh* b = nullptr;
-  ~BvhGuard() {
-    if (b) axsq_bvh_destroy(b);
-  }
-};
-
-int read_total(const int64_t* dev, int64_t* host, hipStream_t s) {
-  AXSQ_CHECK_HIP(hipMemcpyAsync(host, dev, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  AXSQ_CHECK_HIP(hipStreamSynchronize(s));
-  return AXSQ_OK;
-}
-
 }  // namespace
 }  // namespace axsq
 
@@ -207,95 +235,106 @@ extern "C" int axsq_mesh_self_intersections(const double* x, const double* y, co
     return AXSQ_ERR_INVALID_ARG;
   }
   hipStream_t s = (hipStream_t)stream;
-  Scratch sc;
+  Scratch out;  // the caller's arrays until they are handed over
   int32_t *pf = nullptr, *ps = nullptr, *pd = nullptr;
   if (ntris == 0) {
-    AXSQ_TRY(sc.alloc(&pf, 1));
-    AXSQ_TRY(sc.alloc(&ps, 1));
-    AXSQ_TRY(sc.alloc(&pd, 1));
-    sc.keep(pf);
-    sc.keep(ps);
-    sc.keep(pd);
-    *first = pf;
-    *second = ps;
-    *degenerate = pd;
-    return AXSQ_OK;
+    AXSQ_TRY(out.alloc(&pf, 1));
+    AXSQ_TRY(out.alloc(&ps, 1));
+    AXSQ_TRY(out.alloc(&pd, 1));
+  } else {
+    const int64_t n = ntris;
+    Workspace& ws = workspace();
+    AXSQ_TRY(ws.mesh_tri.ensure(sizeof(double) * 9 * n));
+    AXSQ_TRY(ws.mesh_box.ensure(sizeof(double) * 6 * n));
+    AXSQ_TRY(ws.mesh_int.ensure(sizeof(int32_t) * (4 * n + 2) + 2 * sizeof(int64_t) + 64));
+    AXSQ_TRY(ws.mesh_slots.ensure(sizeof(int32_t) * kPairSlots * n));
+    double* tri = ws.mesh_tri.as<double>();
+    double* boxes = ws.mesh_box.as<double>();
+    int64_t* tot = reinterpret_cast<int64_t*>(ws.mesh_int.as<char>());
+    int32_t* degf = reinterpret_cast<int32_t*>(tot + 2);
+    int32_t* degoff = degf + n;
+    int32_t* npair = degoff + n;
+    int32_t* poff = npair + n;
+    int32_t* status = poff + n;
+    AXSQ_TRY(fill_i32(status, 0, 1, s));
+    {
+      ProfRange pr("mesh_isect_setup", s);
+      k_tri_setup<<<grid_for(n), kBlock, 0, s>>>(x, y, z, tris, n, nverts, tri, boxes, degf,
+                                                 status);
+      AXSQ_CHECK_HIP(hipGetLastError());
+    }
+    int32_t st = 0;
+    AXSQ_CHECK_HIP(hipMemcpyAsync(&st, status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+    if (st) {
+      set_error("axsq_mesh_self_intersections: connectivity refers to a vertex outside [0, %lld)",
+                (long long)nverts);
+      return AXSQ_ERR_INVALID_ARG;
+    }
+    // the BVH over the corner boxes, default scale (MeshTester_detail.hpp:322-338)
+    if (!ws.mesh_bvh) {
+      ws.mesh_bvh = new axsq_bvh();
+      hipGetDevice(&ws.mesh_bvh->device);
+    }
+    const int rc = bvh_build_into(ws.mesh_bvh, boxes, n, 1.000123, AXSQ_FP64, s);
+    if (rc != AXSQ_OK) {
+      axsq_bvh_destroy(ws.mesh_bvh);
+      ws.mesh_bvh = nullptr;
+      return rc;
+    }
+    const axsq_bvh* b = ws.mesh_bvh;
+    IsectArgs a;
+    a.nodes = b->nodes;
+    a.ninner = (int32_t)(b->n - 1);
+    a.leaf_ids = b->leaf_ids;
+    a.tri = tri;
+    a.boxes = boxes;
+    a.n = n;
+    a.eps = threshold;
+    a.npair = npair;
+    a.slots = ws.mesh_slots.as<int32_t>();
+    a.poff = poff;
+    a.status = b->status;
+    {
+      ProfRange pr("mesh_isect_walk", s);
+      k_isect_count<<<grid_for(n), kBlock, 0, s>>>(a);
+      AXSQ_CHECK_HIP(hipGetLastError());
+    }
+    AXSQ_TRY(scan_exclusive_i32(npair, poff, n, tot, ws, s));
+    AXSQ_TRY(scan_exclusive_i32(degf, degoff, n, tot + 1, ws, s));
+    int64_t totals[2] = {0, 0};
+    int32_t walk_status = 0;
+    AXSQ_CHECK_HIP(hipMemcpyAsync(totals, tot, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    AXSQ_CHECK_HIP(
+        hipMemcpyAsync(&walk_status, b->status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+    if (walk_status) {
+      AXSQ_TRY(fill_i32(b->status, 0, 1, s));
+      set_error("traversal stopped: stack deeper than %d entries or malformed tree", kStack);
+      return AXSQ_ERR_STACK_OVERFLOW;
+    }
+    AXSQ_TRY(out.alloc(&pf, totals[0]));
+    AXSQ_TRY(out.alloc(&ps, totals[0]));
+    AXSQ_TRY(out.alloc(&pd, totals[1]));
+    a.first = pf;
+    a.second = ps;
+    {
+      ProfRange pr("mesh_isect_emit", s);
+      k_isect_emit<<<grid_for(n), kBlock, 0, s>>>(a);
+      AXSQ_CHECK_HIP(hipGetLastError());
+      k_flag_emit<<<grid_for(n), kBlock, 0, s>>>(degf, degoff, n, pd);
+      AXSQ_CHECK_HIP(hipGetLastError());
+    }
+    AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+    prof_flush();
+    *npairs = totals[0];
+    *ndegen = totals[1];
   }
-  const int64_t n = ntris;
-  double *tri = nullptr, *boxes = nullptr;
-  int32_t *degf = nullptr, *degoff = nullptr, *counts = nullptr, *offsets = nullptr;
-  int32_t *npair = nullptr, *poff = nullptr, *status = nullptr;
-  int64_t* tot = nullptr;
-  AXSQ_TRY(sc.alloc(&tri, 9 * n));
-  AXSQ_TRY(sc.alloc(&boxes, 6 * n));
-  AXSQ_TRY(sc.alloc(&degf, n));
-  AXSQ_TRY(sc.alloc(&degoff, n));
-  AXSQ_TRY(sc.alloc(&counts, n));
-  AXSQ_TRY(sc.alloc(&offsets, n));
-  AXSQ_TRY(sc.alloc(&npair, n));
-  AXSQ_TRY(sc.alloc(&poff, n));
-  AXSQ_TRY(sc.alloc(&status, 1));
-  AXSQ_TRY(sc.alloc(&tot, 2));
-  AXSQ_TRY(fill_i32(status, 0, 1, s));
-  {
-    ProfRange pr("mesh_isect_setup", s);
-    k_tri_setup<<<grid_for(n), kBlock, 0, s>>>(x, y, z, tris, n, nverts, tri, boxes, degf, status);
-    AXSQ_CHECK_HIP(hipGetLastError());
-  }
-  int32_t st = 0;
-  AXSQ_CHECK_HIP(hipMemcpyAsync(&st, status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  AXSQ_CHECK_HIP(hipStreamSynchronize(s));
-  if (st) {
-    set_error("axsq_mesh_self_intersections: connectivity refers to a vertex outside [0, %lld)",
-              (long long)nverts);
-    return AXSQ_ERR_INVALID_ARG;
-  }
-  // broad phase: findBoundingBoxes of every triangle box against the BVH of
-  // all of them (MeshTester_detail.hpp:322-338)
-  BvhGuard bg;
-  AXSQ_TRY(axsq_bvh_build(&bg.b, boxes, n, 1.000123, AXSQ_FP64, stream));
-  int64_t total = 0;
-  AXSQ_TRY(axsq_bvh_find_count(bg.b, AXSQ_QUERY_BOXES, boxes, nullptr, nullptr, 6, n, 0.0, offsets,
-                               counts, &total, stream));
-  int32_t* cands = nullptr;
-  uint8_t* flag = nullptr;
-  AXSQ_TRY(sc.alloc(&cands, total));
-  AXSQ_TRY(sc.alloc(&flag, total));
-  if (total > 0)
-    AXSQ_TRY(axsq_bvh_find_fill(bg.b, AXSQ_QUERY_BOXES, boxes, nullptr, nullptr, 6, n, 0.0,
-                                offsets, cands, stream));
-  // narrow phase (MeshTester_detail.hpp:235-289)
-  Workspace& ws = workspace();
-  {
-    ProfRange pr("mesh_isect_narrow", s);
-    k_tt_flags<<<grid_for(n), kBlock, 0, s>>>(tri, offsets, counts, cands, n, threshold, flag,
-                                              npair);
-    AXSQ_CHECK_HIP(hipGetLastError());
-  }
-  AXSQ_TRY(scan_exclusive_i32(npair, poff, n, tot, ws, s));
-  AXSQ_TRY(scan_exclusive_i32(degf, degoff, n, tot + 1, ws, s));
-  int64_t totals[2] = {0, 0};
-  AXSQ_CHECK_HIP(hipMemcpyAsync(totals, tot, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  AXSQ_CHECK_HIP(hipStreamSynchronize(s));
-  AXSQ_TRY(sc.alloc(&pf, totals[0]));
-  AXSQ_TRY(sc.alloc(&ps, totals[0]));
-  AXSQ_TRY(sc.alloc(&pd, totals[1]));
-  {
-    ProfRange pr("mesh_isect_emit", s);
-    k_tt_emit<<<grid_for(n), kBlock, 0, s>>>(offsets, counts, cands, flag, poff, n, pf, ps);
-    AXSQ_CHECK_HIP(hipGetLastError());
-    k_flag_emit<<<grid_for(n), kBlock, 0, s>>>(degf, degoff, n, pd);
-    AXSQ_CHECK_HIP(hipGetLastError());
-  }
-  AXSQ_CHECK_HIP(hipStreamSynchronize(s));
-  prof_flush();
-  sc.keep(pf);
-  sc.keep(ps);
-  sc.keep(pd);
+  out.keep(pf);
+  out.keep(ps);
+  out.keep(pd);
   *first = pf;
   *second = ps;
   *degenerate = pd;
-  *npairs = totals[0];
-  *ndegen = totals[1];
   return AXSQ_OK;
 }

@@ -190,3 +190,16 @@ def workload5(nq=8_000_000, level=8):
     v, t = icosphere(level)
     q = uniform(s, 10, 3 * nq, -1.2, 1.2).reshape(nq, 3)
     return v, t, np.ascontiguousarray(q), rounded_box_sdf(q)
+
+
+def workload6(level=8):
+    """Triangle mesh for the self-intersection workload: an icosphere whose
+    vertices carry N(0, (0.25 * mean edge)^2) noise per coordinate, plus a
+    second copy shifted by (0.5, 0.1, 0) so that the two surfaces cross."""
+    s = SEED0 + 6
+    v, t = icosphere(level)
+    e = float(np.mean(np.linalg.norm(v[t[:, 0]] - v[t[:, 1]], axis=1)))
+    v = v + 0.25 * e * normal_f32(s, 0, v.size).astype(np.float64).reshape(-1, 3)
+    v2 = np.concatenate([v, v + np.array([0.5, 0.1, 0.0])])
+    t2 = np.concatenate([t, t + v.shape[0]]).astype(np.int32)
+    return np.ascontiguousarray(v2), np.ascontiguousarray(t2)

@@ -56,7 +56,8 @@ def parse():
                    help="also run one 1 GiB device copy (counter calibration)")
     p.add_argument("--cpu-sample", type=int, default=1_000_000)
     p.add_argument("--workload", default="sdf_l6",
-                   choices=["sdf_l6", "find_points", "torus_mixed", "diff_l7", "shapefit_l8"],
+                   choices=["sdf_l6", "find_points", "torus_mixed", "diff_l7", "shapefit_l8",
+                            "self_isect_l8"],
                    help="sdf_l6 is the headline; the others print their own JSON line")
     return p.parse_args()
 
@@ -121,6 +122,8 @@ def other_workload(args, torch, dev):
             tt = sync_time(fb, args.steps)
             out[f"{mode}_fwd_bwd_queries_per_s"] = q.shape[0] / tt
         return out
+    if args.workload == "self_isect_l8":
+        return self_isect_workload(args, torch, dev, sync_time)
     v, t, q, target = synth.workload5()
     vd, td, qd, tg = (torch.as_tensor(a, device=dev) for a in (v, t, q, target))
     _, hist = shapefit.shape_fit(vd, td, qd, tg, iterations=20, lr=1e-3, fp_mode="fp32")
@@ -132,6 +135,53 @@ def other_workload(args, torch, dev):
             "mean_build_ms": sum(h["build_ms"] for h in hist[1:]) / 19,
             "mean_forward_ms": sum(h["forward_ms"] for h in hist) / 20,
             "mean_backward_ms": sum(h["backward_ms"] for h in hist) / 20}
+
+
+def self_isect_workload(args, torch, dev, sync_time):
+    """quest::findTriMeshIntersectionsBVH on two crossing noisy L8 icospheres
+    (2,621,440 triangles): corner boxes, BVH build, findBoundingBoxes of every
+    box, triangle-triangle tests on the pairs i < j, ordered output. The
+    oracle runs the same test on this host for the CPU figure and the
+    pair-by-pair comparison."""
+    from axom_amd import _lib, quest, synth
+    v, t = synth.workload6()
+    vd, td = torch.as_tensor(v, device=dev), torch.as_tensor(t, device=dev)
+    L = _lib.lib()
+    res = {}
+
+    def run():
+        res["out"] = quest.findTriMeshIntersectionsBVH(vd, td)
+    L.axsq_prof_reset()
+    L.axsq_prof_enable(1)
+    reps = max(1, args.steps)
+    ts = sync_time(run, reps)
+    L.axsq_prof_enable(0)
+
+    def prof(name):
+        n, ms = ctypes.c_int64(), ctypes.c_double()
+        _lib.check(L.axsq_prof_get(name.encode(), ctypes.byref(n), ctypes.byref(ms)))
+        return ms.value / max(n.value, 1)
+    pairs, degen = res["out"]
+    out = {"workload": "two noisy icospheres L8 (2 x 1,310,720 tris, vertex noise N(0, (0.25 "
+                       "edge)^2), second copy shifted by (0.5, 0.1, 0)); "
+                       "findTriMeshIntersectionsBVH, threshold 1e-8",
+           "triangles": int(t.shape[0]), "pairs": int(pairs.shape[0]),
+           "degenerate": int(degen.shape[0]), "ms": ts * 1e3,
+           "triangles_per_s": t.shape[0] / ts,
+           "kernels_ms": {k: prof(k) for k in ("mesh_isect_setup", "bvh_build", "mesh_isect_walk",
+                                                "mesh_isect_emit")}}
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        nthreads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        want_p, want_d = orc.mesh_self_intersections(v, t, 1e-8, nthreads=nthreads)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"seconds": tc, "triangles_per_s": t.shape[0] / tc,
+                               "cores": nthreads, "kind": "port",
+                               "matches_gpu_pairwise": bool(
+                                   np.array_equal(pairs.cpu().numpy(), want_p) and
+                                   np.array_equal(degen.cpu().numpy(), want_d))}
+    return out
 
 
 def log(msg):

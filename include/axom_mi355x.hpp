@@ -22,7 +22,9 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "axsq.h"
 
@@ -339,6 +341,40 @@ class SignedDistance<3, MI355X_EXEC> {
   axsq_sdf* m_sdf = nullptr;
   int m_fpMode = AXSQ_FP64;
 };
+
+// quest::findTriMeshIntersectionsBVH<MI355X_EXEC, double> (MeshTester.hpp:
+// 66-87): the mesh as device SoA coordinates and int32 connectivity; results
+// on the host, as the reference returns them (std::vector of index pairs,
+// std::vector of degenerate indices), in the reference's sequential order.
+template <typename ExecSpace, typename FloatType>
+void findTriMeshIntersectionsBVH(const double* x, const double* y, const double* z,
+                                 IndexType nnodes, const IndexType* connectivity,
+                                 IndexType ncells,
+                                 std::vector<std::pair<int, int>>& intersections,
+                                 std::vector<int>& degenerateIndices,
+                                 double intersectionThreshold = 1E-8) {
+  static_assert(std::is_same<ExecSpace, MI355X_EXEC>::value && std::is_same<FloatType, double>::value,
+                "this build provides findTriMeshIntersectionsBVH<MI355X_EXEC, double>");
+  int32_t *first = nullptr, *second = nullptr, *degen = nullptr;
+  int64_t npairs = 0, ndegen = 0;
+  detail::check(axsq_mesh_self_intersections(x, y, z, nnodes, connectivity, ncells,
+                                             intersectionThreshold, &first, &second, &npairs,
+                                             &degen, &ndegen, nullptr));
+  Array<int32_t> f, g, d;  // released on every exit path
+  f.adopt(first, npairs);
+  g.adopt(second, npairs);
+  d.adopt(degen, ndegen);
+  std::vector<int32_t> hf((size_t)npairs), hg((size_t)npairs), hd((size_t)ndegen);
+  if (npairs) {
+    detail::hip_check(hipMemcpy(hf.data(), first, sizeof(int32_t) * npairs, hipMemcpyDeviceToHost));
+    detail::hip_check(hipMemcpy(hg.data(), second, sizeof(int32_t) * npairs, hipMemcpyDeviceToHost));
+  }
+  if (ndegen)
+    detail::hip_check(hipMemcpy(hd.data(), degen, sizeof(int32_t) * ndegen, hipMemcpyDeviceToHost));
+  intersections.resize((size_t)npairs);
+  for (int64_t k = 0; k < npairs; ++k) intersections[(size_t)k] = {hf[(size_t)k], hg[(size_t)k]};
+  degenerateIndices.assign(hd.begin(), hd.end());
+}
 
 }  // namespace quest
 }  // namespace axom

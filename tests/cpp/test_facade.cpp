@@ -3,6 +3,8 @@
 //   spin/tests/spin_bvh.cpp:279-404 (findBoundingBoxes, 18 cells),
 //   :519-648 (findRays, 15 cells), :792-888 (findPoints, centroid -> cell)
 //   quest/tests/quest_signed_distance.cpp:88-170 (sphere l1/l2/linf)
+//   quest/tests/quest_meshtester.cpp:268-305 (caved-in tetrahedron, with and
+//   without degenerate triangles; meshes of quest_test_utilities.hpp:353-442)
 // Exit status 0 when every check passes.
 #include <cmath>
 #include <cstdio>
@@ -189,9 +191,36 @@ static void test_sdf_sphere() {
   for (void* p : {(void*)dx, (void*)dy, (void*)dz, (void*)dc, (void*)dq, (void*)dphi_out}) (void)hipFree(p);
 }
 
+// The caved-in tetrahedron: triangle 0 pokes through triangles 1 and 2. With
+// the two degenerate triangles appended, they are reported as degenerate; the
+// BVH tester still pairs them (quest/detail/MeshTester_detail.hpp:235-289 does
+// not skip them), so only the pairs among the first four are compared.
+static void test_mesh_tester_caved_tet() {
+  const std::vector<double> x = {2.00003, -18.213671, 4.880339, -0.000003, 13.333334};
+  const std::vector<double> y = {1.00003, 4.880339, -18.213671, -0.000003, 13.333334};
+  const std::vector<double> z = {18.999999, -6.666668, -6.666668, 19.999999, -6.666663};
+  for (int with_degen = 0; with_degen < 2; ++with_degen) {
+    std::vector<IndexType> conn = {0, 1, 2, 3, 4, 1, 3, 2, 4, 1, 4, 2};
+    if (with_degen) conn.insert(conn.end(), {0, 0, 0, 3, 4, 3});
+    double *dx = to_device(x), *dy = to_device(y), *dz = to_device(z);
+    IndexType* dc = to_device(conn);
+    std::vector<std::pair<int, int>> isect;
+    std::vector<int> degen;
+    axom::quest::findTriMeshIntersectionsBVH<axom::MI355X_EXEC, double>(
+        dx, dy, dz, 5, dc, (IndexType)(conn.size() / 3), isect, degen);
+    std::set<std::pair<int, int>> got;
+    for (auto& p : isect)
+      if (p.first < 4 && p.second < 4) got.insert(p);
+    EXPECT_TRUE((got == std::set<std::pair<int, int>>{{0, 1}, {0, 2}}));
+    EXPECT_TRUE((degen == (with_degen ? std::vector<int>{4, 5} : std::vector<int>{})));
+    for (void* p : {(void*)dx, (void*)dy, (void*)dz, (void*)dc}) (void)hipFree(p);
+  }
+}
+
 int main() {
   test_bvh_grid();
   test_sdf_sphere();
+  test_mesh_tester_caved_tet();
   std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "PASSED", g_fail);
   return g_fail ? 1 : 0;
 }
