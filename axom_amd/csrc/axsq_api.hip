@@ -199,16 +199,16 @@ static int check_find_args(const axsq_bvh* b, int kind, const double* x, const d
     return AXSQ_ERR_INVALID_ARG;
   }
   if (kind == AXSQ_QUERY_POINTS) {
-    if (nq > 0 && (!y || !z)) {
-      set_error("point queries need x, y and z pointers");
+    if (nq > 0 && (!y || (!z && b->dims == 3))) {
+      set_error("point queries need x, y (and z for a 3-D tree) pointers");
       return AXSQ_ERR_INVALID_ARG;
     }
     if (stride < 1) {
       set_error("point stride must be >= 1");
       return AXSQ_ERR_INVALID_ARG;
     }
-  } else if (stride < 6) {
-    set_error("box/ray queries need a stride of at least 6 doubles");
+  } else if (stride < 2 * b->dims) {
+    set_error("box/ray queries need a stride of at least %d doubles", 2 * b->dims);
     return AXSQ_ERR_INVALID_ARG;
   }
   return AXSQ_OK;
@@ -253,6 +253,40 @@ int axsq_bvh_build(axsq_bvh** out, const double* boxes, int64_t n, double scale,
   *out = b;
   return AXSQ_OK;
 }
+
+int axsq_bvh_build2d(axsq_bvh** out, const double* boxes, int64_t n, double scale, int fp_mode,
+                     void* stream) {
+  clear_error();
+  if (!out) {
+    set_error("axsq_bvh_build2d: out is NULL");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  if (fp_mode != AXSQ_FP64 && fp_mode != AXSQ_FP32_TRAVERSAL) {
+    set_error("axsq_bvh_build2d: unknown fp_mode %d", fp_mode);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (n < 0 || (n > 0 && !boxes)) {
+    set_error("axsq_bvh_build2d: need n >= 0 boxes");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Workspace& ws = workspace();
+  AXSQ_TRY(ws.box2d.ensure(sizeof(double) * 6 * (size_t)(n > 0 ? n : 1)));
+  AXSQ_TRY(widen_boxes2d(boxes, n, ws.box2d.as<double>(), s));
+  axsq_bvh* b = new axsq_bvh();
+  hipGetDevice(&b->device);
+  const int rc = bvh_build_into(b, ws.box2d.as<double>(), n, scale, fp_mode, s, 2);
+  if (rc != AXSQ_OK) {
+    bvh_release(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return AXSQ_OK;
+}
+
+int axsq_bvh_dimension(const axsq_bvh* b) { return b ? b->dims : 0; }
 
 int axsq_bvh_destroy(axsq_bvh* b) {
   if (!b) return AXSQ_OK;

@@ -128,7 +128,7 @@ __global__ __launch_bounds__(1024) void k_bounds(const double* __restrict__ part
 }
 
 __global__ __launch_bounds__(kBlock) void k_mcodes(const double* __restrict__ boxes, int64_t n,
-                                                    const BuildParams* __restrict__ prm,
+                                                    const BuildParams* __restrict__ prm, int dims,
                                                     uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ vals) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_mcodes(const double* __restrict__ bo
     minc[d] = prm->minc[d];
     inv[d] = prm->inv[d];
   }
-  keys[i] = morton_of(c, minc, inv);
+  keys[i] = (dims == 2) ? morton2_of(c, minc, inv) : morton_of(c, minc, inv);
   vals[i] = (uint32_t)i;
 }
 
@@ -379,12 +379,18 @@ __global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf
 // Reference layout (LinearBVH.hpp:170-187,257-262): child boxes of inner
 // node i at 2i and 2i+1, child links doubled for inner children.
 __global__ __launch_bounds__(kBlock) void k_export(const Node64* __restrict__ nodes, int32_t inner,
-                                                    double* __restrict__ boxes,
+                                                    int dims, double* __restrict__ boxes,
                                                     int32_t* __restrict__ children) {
   const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i >= inner) return;
   const Node64 nd = nodes[i];
-  if (boxes) {
+  if (boxes && dims == 2) {  // BoundingBox<double, 2>: min x y, max x y
+    const int src[4] = {0, 1, 3, 4};
+    for (int k = 0; k < 4; ++k) {
+      boxes[8 * (int64_t)i + k] = nd.l[src[k]];
+      boxes[8 * (int64_t)i + 4 + k] = nd.r[src[k]];
+    }
+  } else if (boxes) {
     for (int k = 0; k < 6; ++k) {
       boxes[12 * (int64_t)i + k] = nd.l[k];
       boxes[12 * (int64_t)i + 6 + k] = nd.r[k];
@@ -396,13 +402,37 @@ __global__ __launch_bounds__(kBlock) void k_export(const Node64* __restrict__ no
   }
 }
 
+// 2-D boxes (min x y, max x y) as 3-D boxes with z range [0, 0]: every test,
+// union, scaling and bound of the build and the searches then gives the 2-D
+// result, since the z terms are exact zeros.
+__global__ __launch_bounds__(kBlock) void k_widen_boxes2d(const double* __restrict__ in,
+                                                          int64_t n, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* b = in + 4 * i;
+  double* o = out + 6 * i;
+  o[0] = b[0];
+  o[1] = b[1];
+  o[2] = 0.0;
+  o[3] = b[2];
+  o[4] = b[3];
+  o[5] = 0.0;
+}
+
 }  // namespace
+
+int widen_boxes2d(const double* boxes2d, int64_t n, double* out, hipStream_t s) {
+  if (n <= 0) return AXSQ_OK;
+  k_widen_boxes2d<<<grid_for(n), kBlock, 0, s>>>(boxes2d, n, out);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  return AXSQ_OK;
+}
 
 int bvh_export_launch(const axsq_bvh* b, double* inner_nodes, int32_t* children,
                       int32_t* leaf_nodes, uint32_t* mcodes, hipStream_t s) {
   const int32_t inner = (int32_t)(b->n - 1);
   if (inner_nodes || children) {
-    k_export<<<grid_for(inner), kBlock, 0, s>>>(b->nodes, inner, inner_nodes, children);
+    k_export<<<grid_for(inner), kBlock, 0, s>>>(b->nodes, inner, b->dims, inner_nodes, children);
     AXSQ_CHECK_HIP(hipGetLastError());
   }
   if (leaf_nodes)
@@ -441,7 +471,7 @@ static int dmalloc(T** p, size_t count) {
 }
 
 int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale, int fp_mode,
-                   hipStream_t s) {
+                   hipStream_t s, int dims) {
   if (n_in < 0 || n_in > 0x3fffffff) {
     set_error("axsq_bvh_build: box count %lld out of range", (long long)n_in);
     return AXSQ_ERR_INVALID_ARG;
@@ -467,6 +497,7 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   b->n = n;
   b->fp_mode = fp_mode;
   b->scale = scale;
+  b->dims = dims;
 
   // scratch: scaled boxes (6n doubles), keys/vals x2, tree arrays, partials
   const int64_t nparts = grid_for(n);
@@ -511,11 +542,11 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   k_bounds<<<1, 1024, 0, s>>>(partials, nparts, scaled, n, prm);
   AXSQ_CHECK_HIP(hipGetLastError());
   uint32_t *keys = ws.k0.as<uint32_t>(), *vals = ws.v0.as<uint32_t>();
-  k_mcodes<<<grid_for(n), kBlock, 0, s>>>(scaled, n, prm, keys, vals);
+  k_mcodes<<<grid_for(n), kBlock, 0, s>>>(scaled, n, prm, dims, keys, vals);
   AXSQ_CHECK_HIP(hipGetLastError());
   uint32_t *sk, *sv;
-  AXSQ_TRY(radix_sort_pairs(keys, vals, ws.k1.as<uint32_t>(), ws.v1.as<uint32_t>(), n, 30, ws, s,
-                            &sk, &sv));
+  AXSQ_TRY(radix_sort_pairs(keys, vals, ws.k1.as<uint32_t>(), ws.v1.as<uint32_t>(), n,
+                            dims == 2 ? 32 : 30, ws, s, &sk, &sv));
   k_reorder<<<grid_for(n), kBlock, 0, s>>>(scaled, sv, sk, n, b->leaf_boxes, b->leaf_ids,
                                            b->mcodes);
   AXSQ_CHECK_HIP(hipGetLastError());

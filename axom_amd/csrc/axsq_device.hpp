@@ -172,6 +172,29 @@ __device__ __forceinline__ uint32_t morton_of(const double c[3], const double mi
   return code;
 }
 
+// 16-bit spread and the 32-bit 2-D Morton code (morton32_encode with Dims = 2:
+// 16 bits per axis, x in the even bits; build_radix_tree.hpp:48-65 and the
+// 2-D Mortonizer, spin/MortonIndex.hpp:226-318, whose int32 expansion runs
+// the shifts 16, 8, 4, 2, 1).
+__device__ __forceinline__ uint32_t spread16(uint32_t v) {
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+__device__ __forceinline__ uint32_t morton2_of(const double c[3], const double minc[3],
+                                               const double inv[3]) {
+  uint32_t code = 0;
+  for (int d = 0; d < 2; ++d) {
+    const double u = (c[d] - minc[d]) * inv[d];
+    const int32_t q = (int32_t)fmin(fmax(u * 65536.0, 0.0), 65535.0);
+    code |= spread16((uint32_t)q) << d;
+  }
+  return code;
+}
+
 // Karras delta with index tie-break, build_radix_tree.hpp:265-287
 __device__ __forceinline__ int32_t karras_delta(int32_t a, int32_t b, int32_t inner,
                                                 const uint32_t* __restrict__ mc) {

@@ -46,6 +46,7 @@ struct DevBuf {
 struct Workspace {
   DevBuf k0, v0, k1, v1, hist, tiles, misc, flag, susp, cnt, redo;
   DevBuf mesh_tri, mesh_box, mesh_int, mesh_slots;  // mesh self-intersection
+  DevBuf box2d;                                      // 2-D boxes widened to 3-D
   axsq_bvh* mesh_bvh = nullptr;                      // its BVH, rebuilt per call
   int device = -1;
   void release();
@@ -98,6 +99,7 @@ inline int bits_for(uint64_t max_value) {
 // ---- handles -------------------------------------------------------------
 struct axsq_bvh {
   int64_t n = 0;        // leaves after padding
+  int dims = 3;         // 2: a 2-D tree (boxes held with z = [0, 0]), 16-bit-per-axis codes
   int fp_mode = AXSQ_FP64;
   double scale = 1.000123;
   double bounds[6];     // host copy of the root box
@@ -133,7 +135,11 @@ struct axsq_sdf {
 };
 
 namespace axsq {
+// boxes: AoS of 6 doubles (min xyz, max xyz); dims 2 builds the 2-D tree of
+// boxes whose z range is [0, 0].
 int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n, double scale, int fp_mode,
-                   hipStream_t s);
+                   hipStream_t s, int dims = 3);
 void bvh_release(axsq_bvh* b);
+// n 2-D boxes (4 doubles) -> 3-D boxes with z range [0, 0] (6 doubles)
+int widen_boxes2d(const double* boxes2d, int64_t n, double* out, hipStream_t s);
 }  // namespace axsq

@@ -27,6 +27,7 @@ struct FindArgs {
   const double* z;
   int64_t stride;
   int64_t nq;
+  int dims;  // 2: queries are 2-D (z = 0)
   int32_t ninner;
   double tol;
   int32_t* counts;
@@ -74,8 +75,16 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
   if (KIND == AXSQ_QUERY_POINTS) {
     prim[0] = a.x[i * a.stride];
     prim[1] = a.y[i * a.stride];
-    prim[2] = a.z[i * a.stride];
+    prim[2] = (a.dims == 2) ? 0.0 : a.z[i * a.stride];
     prim[3] = prim[4] = prim[5] = 0.0;
+  } else if (a.dims == 2) {  // box: min x y, max x y; ray: origin x y, direction x y
+    const double* p = a.x + i * a.stride;
+    prim[0] = p[0];
+    prim[1] = p[1];
+    prim[2] = 0.0;
+    prim[3] = p[2];
+    prim[4] = p[3];
+    prim[5] = 0.0;
   } else {
     for (int k = 0; k < 6; ++k) prim[k] = a.x[i * a.stride + k];
   }
@@ -143,6 +152,7 @@ int find_launch(const axsq_bvh* b, int kind, bool fill, const double* x, const d
   a.z = z;
   a.stride = stride;
   a.nq = nq;
+  a.dims = b->dims;
   a.ninner = (int32_t)(b->n - 1);
   a.tol = tol;
   a.counts = counts;

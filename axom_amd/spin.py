@@ -1,4 +1,4 @@
-"""Host mirror of axom::spin::BVH<3, MI355X_EXEC, double> over libaxsq.
+"""Host mirror of axom::spin::BVH<NDIMS, MI355X_EXEC, double> (NDIMS 3 or 2) over libaxsq.
 
 Method names, argument meaning and defaults follow src/axom/spin/BVH.hpp:129-419
 so that code and tests written against the reference read the same. Data
@@ -8,6 +8,8 @@ Layouts: boxes are (n, 6) float64 {min xyz, max xyz} (BoundingBox<double,3>);
 points are (n, 3) float64 (Point<double,3>) or an (x, y, z) tuple of (n,)
 float64 tensors (ZipPoint); rays are (n, 6) float64 {origin, unit direction}
 as built by make_rays (Ray's constructor normalises, Ray.hpp:122-128).
+With NDIMS=2: boxes (n, 4) {min xy, max xy}, points (n, 2) or (x, y), rays
+(n, 4) {origin xy, unit direction xy} (make_rays2d).
 """
 import ctypes
 
@@ -54,6 +56,32 @@ def point_args(points):
             p.shape[0], (p,))
 
 
+def point_args2d(points):
+    """(x, y, None, stride, n, keepalive) for AoS (n,2) or SoA (x,y) 2-D points."""
+    if isinstance(points, (tuple, list)):
+        x, y = (_dev(c, torch.float64, "points") for c in points)
+        if x.shape != y.shape or x.dim() != 1:
+            raise ValueError("SoA 2-D points need two (n,) tensors of equal length")
+        return _ptr(x), _ptr(y), None, 1, x.shape[0], (x, y)
+    p = _dev(points, torch.float64, "points", 2)
+    base = p.data_ptr()
+    return ctypes.c_void_p(base), ctypes.c_void_p(base + 8), None, 2, p.shape[0], (p,)
+
+
+def make_rays2d(origins, directions):
+    """2-D rays {origin xy, unit direction xy}, the direction normalised as
+    primal::Ray<double, 2> does (1/sqrt of (0 + x*x) + y*y)."""
+    o = _dev(origins, torch.float64, "origins", 2)
+    d = _dev(directions, torch.float64, "directions", 2)
+    len_sq = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]
+    inv = 1.0 / torch.sqrt(len_sq)
+    u = d * inv[:, None]
+    tiny = len_sq < 1e-50
+    if bool(tiny.any()):
+        u[tiny] = torch.tensor([1.0, 0.0], dtype=torch.float64, device=u.device)
+    return torch.cat([o, u], 1).contiguous()
+
+
 def make_rays(origins, directions):
     """Rays with directions normalised as primal::Ray does (Vector::unitVector,
     Vector.hpp:477-490: multiply by 1/sqrt of ((0 + x*x) + y*y) + z*z)."""
@@ -69,13 +97,17 @@ def make_rays(origins, directions):
 
 
 class BVH:
-    """axom::spin::BVH<3, MI355X_EXEC, double> (spin/BVH.hpp:129-419)."""
+    """axom::spin::BVH<NDIMS, MI355X_EXEC, double> (spin/BVH.hpp:129-419), NDIMS 3
+    (default) or 2."""
 
     DEFAULT_SCALE_FACTOR = 1.000123                         # BVH.hpp:410
     DEFAULT_TOLERANCE = float(np.finfo(np.float64).eps)     # BVH.hpp:411-412
 
     def __init__(self, boxes=None, numItems=None, tolerance=DEFAULT_TOLERANCE,
-                 scaleFactor=DEFAULT_SCALE_FACTOR, fp_mode="fp64"):
+                 scaleFactor=DEFAULT_SCALE_FACTOR, fp_mode="fp64", NDIMS=3):
+        if NDIMS not in (2, 3):
+            raise ValueError("NDIMS must be 2 or 3")
+        self.NDIMS = NDIMS
         self._h = None
         self._tol = float(tolerance)
         self._scale = float(scaleFactor)
@@ -90,7 +122,7 @@ class BVH:
 
     # -- BVH.hpp:246-247, 426-477
     def initialize(self, boxes, numItems=None):
-        b = _dev(boxes, torch.float64, "boxes", 6)
+        b = _dev(boxes, torch.float64, "boxes", 2 * self.NDIMS)
         n = b.shape[0] if numItems is None else int(numItems)
         if n > b.shape[0]:
             raise ValueError("numItems exceeds the number of boxes")
@@ -98,8 +130,8 @@ class BVH:
             L.lib().axsq_bvh_destroy(self._h)
             self._h = None
         h = ctypes.c_void_p()
-        L.check(L.lib().axsq_bvh_build(ctypes.byref(h), _ptr(b), n, self._scale, self._fp,
-                                       stream_ptr()))
+        build = L.lib().axsq_bvh_build2d if self.NDIMS == 2 else L.lib().axsq_bvh_build
+        L.check(build(ctypes.byref(h), _ptr(b), n, self._scale, self._fp, stream_ptr()))
         self._h = h
         return BVH_BUILD_OK
 
@@ -120,11 +152,12 @@ class BVH:
 
     def getBounds(self):
         """(lo, hi) of the root box; an invalid box before initialize()."""
+        d = self.NDIMS
         if not self._h:
-            return (np.full(3, np.finfo(np.float64).max), np.full(3, -np.finfo(np.float64).max))
+            return (np.full(d, np.finfo(np.float64).max), np.full(d, -np.finfo(np.float64).max))
         lo, hi = L._dbl3(), L._dbl3()
         L.check(L.lib().axsq_bvh_bounds(self._h, lo, hi))
-        return np.array(lo[:]), np.array(hi[:])
+        return np.array(lo[:d]), np.array(hi[:d])
 
     def numLeaves(self):
         return int(L.lib().axsq_bvh_num_leaves(self._h)) if self._h else 0
@@ -153,24 +186,26 @@ class BVH:
 
     # -- BVH.hpp:341-346, 482-505
     def findPoints(self, offsets, counts, numPts, points):
-        x, y, z, stride, n, keep = point_args(points)
+        x, y, z, stride, n, keep = (point_args2d if self.NDIMS == 2 else point_args)(points)
         return self._find(L.QUERY_POINTS, offsets, counts, int(numPts), x, y, z, stride)
 
     # -- BVH.hpp:367-372, 510-536 (rays from make_rays)
     def findRays(self, offsets, counts, numRays, rays):
-        r = _dev(rays, torch.float64, "rays", 6)
-        return self._find(L.QUERY_RAYS, offsets, counts, int(numRays), _ptr(r), None, None, 6)
+        w = 2 * self.NDIMS
+        r = _dev(rays, torch.float64, "rays", w)
+        return self._find(L.QUERY_RAYS, offsets, counts, int(numRays), _ptr(r), None, None, w)
 
     # -- BVH.hpp:393-398, 541-568
     def findBoundingBoxes(self, offsets, counts, numBoxes, boxes):
-        b = _dev(boxes, torch.float64, "boxes", 6)
-        return self._find(L.QUERY_BOXES, offsets, counts, int(numBoxes), _ptr(b), None, None, 6)
+        w = 2 * self.NDIMS
+        b = _dev(boxes, torch.float64, "boxes", w)
+        return self._find(L.QUERY_BOXES, offsets, counts, int(numBoxes), _ptr(b), None, None, w)
 
     def export(self):
         """Tree arrays in the reference's LinearBVH layout (device tensors)."""
         n = self.numLeaves()
         dev = torch.device("cuda", torch.cuda.current_device())
-        inner = torch.empty((2 * (n - 1), 6), dtype=torch.float64, device=dev)
+        inner = torch.empty((2 * (n - 1), 2 * self.NDIMS), dtype=torch.float64, device=dev)
         children = torch.empty(2 * (n - 1), dtype=torch.int32, device=dev)
         leafs = torch.empty(n, dtype=torch.int32, device=dev)
         mcodes = torch.empty(n, dtype=torch.int32, device=dev)

@@ -73,6 +73,19 @@ int axsq_bvh_build(axsq_bvh** out, const double* boxes, int64_t n, double scale,
                    int fp_mode, void* stream);
 int axsq_bvh_destroy(axsq_bvh* bvh);
 
+/* Builds a 2-D linear BVH over n boxes, AoS of 4 doubles (min x y, max x y).
+ * Replaces spin::BVH<2,Exec,double>::initialize (BVH.hpp:426-477 with
+ * NDIMS = 2): the same pipeline with 16-bit-per-axis Morton codes
+ * (build_radix_tree.hpp:48-65 with Dims = 2). The handle then serves
+ * axsq_bvh_find* with 2-D queries -- points x, y (z ignored, may be NULL),
+ * boxes AoS of 4 doubles, rays {origin x y, unit direction x y}, strides in
+ * doubles -- and axsq_bvh_export writes 2-D boxes (4 doubles each);
+ * axsq_bvh_bounds reports z = 0. */
+int axsq_bvh_build2d(axsq_bvh** out, const double* boxes, int64_t n, double scale,
+                     int fp_mode, void* stream);
+/* 2 or 3: the dimension the handle was built in (0 for NULL). */
+int axsq_bvh_dimension(const axsq_bvh* bvh);
+
 /* Root box of the BVH: BVH::getBounds (BVH.hpp:297-308). Host outputs. */
 int axsq_bvh_bounds(const axsq_bvh* bvh, double lo[3], double hi[3]);
 /* Number of leaves after padding. */

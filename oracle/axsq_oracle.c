@@ -81,13 +81,16 @@ void orc_unit_vector(const double v[3], double out[3])
 /* primal/geometry/Ray.hpp:122-128: the Ray stores direction.unitVector(). */
 void orc_ray_normalize(const double d[3], double out[3]) { orc_unit_vector(d, out); }
 
-/* primal/geometry/BoundingBox.hpp:451-462 */
-int orc_box_valid(const orc_box* b)
+/* primal/geometry/BoundingBox.hpp:451-462. The box operations take the
+ * dimension nd (2 or 3) of BoundingBox<double, NDIMS>; a 2-D box is held in
+ * orc_box with its z entries unused. */
+static inline int box_valid_d(const orc_box* b, int nd)
 {
-  for(int d = 0; d < 3; ++d)
+  for(int d = 0; d < nd; ++d)
     if(b->min[d] > b->max[d]) return 0;
   return 1;
 }
+int orc_box_valid(const orc_box* b) { return box_valid_d(b, 3); }
 
 /* BoundingBox.hpp:72-73,588-592 */
 static inline void box_clear(orc_box* b)
@@ -100,9 +103,9 @@ static inline void box_clear(orc_box* b)
 }
 
 /* BoundingBox.hpp:466-484 */
-static inline void box_add_point(orc_box* b, const double p[3])
+static inline void box_add_point_d(orc_box* b, const double p[3], int nd)
 {
-  for(int d = 0; d < 3; ++d)
+  for(int d = 0; d < nd; ++d)
   {
     if(p[d] < b->min[d]) b->min[d] = p[d];
     if(p[d] > b->max[d]) b->max[d] = p[d];
@@ -111,14 +114,14 @@ static inline void box_add_point(orc_box* b, const double p[3])
 
 /* BoundingBox.hpp:486-508: an invalid receiver is overwritten, an invalid
  * argument is ignored by a valid receiver. */
-void orc_box_add_box(orc_box* b, const orc_box* o)
+static inline void box_add_box_d(orc_box* b, const orc_box* o, int nd)
 {
-  if(orc_box_valid(b))
+  if(box_valid_d(b, nd))
   {
-    if(orc_box_valid(o))
+    if(box_valid_d(o, nd))
     {
-      box_add_point(b, o->min);
-      box_add_point(b, o->max);
+      box_add_point_d(b, o->min, nd);
+      box_add_point_d(b, o->max, nd);
     }
   }
   else
@@ -126,20 +129,21 @@ void orc_box_add_box(orc_box* b, const orc_box* o)
     *b = *o;
   }
 }
+void orc_box_add_box(orc_box* b, const orc_box* o) { box_add_box_d(b, o, 3); }
 
 /* BoundingBox.hpp:547-561 with Point::midpoint (Point.hpp:279-290) and
  * range() = max - min; fixBounds at :575-585. */
-void orc_box_scale(orc_box* b, double sf)
+static inline void box_scale_d(orc_box* b, double sf, int nd)
 {
-  if(!orc_box_valid(b)) return;
+  if(!box_valid_d(b, nd)) return;
   const double s = sf * 0.5;
   double mid[3], r[3];
-  for(int d = 0; d < 3; ++d)
+  for(int d = 0; d < nd; ++d)
   {
     mid[d] = 0.5 * (b->min[d] + b->max[d]);
     r[d] = (b->max[d] - b->min[d]) * s;
   }
-  for(int d = 0; d < 3; ++d)
+  for(int d = 0; d < nd; ++d)
   {
     b->min[d] = mid[d] - r[d];
     b->max[d] = mid[d] + r[d];
@@ -151,20 +155,22 @@ void orc_box_scale(orc_box* b, double sf)
     }
   }
 }
+void orc_box_scale(orc_box* b, double sf) { box_scale_d(b, sf, 3); }
 
 /* BoundingBox.hpp:389-401 */
-static inline int box_contains(const orc_box* b, const double p[3])
+static inline int box_contains_d(const orc_box* b, const double p[3], int nd)
 {
-  for(int d = 0; d < 3; ++d)
+  for(int d = 0; d < nd; ++d)
     if(p[d] < b->min[d] || p[d] > b->max[d]) return 0;
   return 1;
 }
+static inline int box_contains(const orc_box* b, const double p[3]) { return box_contains_d(b, p, 3); }
 
 /* BoundingBox.hpp:433-447 + intersect_bounding_box_impl.hpp:34-41;
  * q is the query box (this), b the tree box (otherBB). */
-static inline int box_intersects(const orc_box* q, const orc_box* b)
+static inline int box_intersects_d(const orc_box* q, const orc_box* b, int nd)
 {
-  for(int d = 0; d < 3; ++d)
+  for(int d = 0; d < nd; ++d)
     if(!(q->max[d] >= b->min[d] && q->min[d] <= b->max[d])) return 0;
   return 1;
 }
@@ -215,13 +221,18 @@ static inline int ray_slab(double x0, double n, double bmin, double bmax, double
 
 /* intersect_ray_impl.hpp:321-351: tmin starts at DBL_MIN (a ray, not a line),
  * the && short-circuits the remaining axes. */
-int orc_ray_box(const double o[3], const double dir[3], const orc_box* b, double eps)
+static inline int ray_box_d(const double o[3], const double dir[3], const orc_box* b, double eps,
+                            int nd)
 {
   double tmin = DBL_MIN, tmax = DBL_MAX;
   int hit = 1;
-  for(int d = 0; d < 3; ++d)
+  for(int d = 0; d < nd; ++d)
     hit = hit && ray_slab(o[d], dir[d], b->min[d], b->max[d], &tmin, &tmax, eps);
   return hit;
+}
+int orc_ray_box(const double o[3], const double dir[3], const orc_box* b, double eps)
+{
+  return ray_box_d(o, dir, b, eps, 3);
 }
 
 /* primal/operators/closest_point.hpp:162-290 (Ericson RTCD 5.1.5) with the
@@ -314,18 +325,34 @@ uint32_t orc_morton3(int32_t x, int32_t y, int32_t z)
   return spread10((uint32_t)x) | (spread10((uint32_t)y) << 1) | (spread10((uint32_t)z) << 2);
 }
 
-/* build_radix_tree.hpp:48-65 (morton32_encode) on the normalised centroid
- * computed at :154-175. */
-static inline uint32_t box_mcode(const orc_box* b, const double minc[3], const double inv[3])
+/* The 2-D Mortonizer (spin/MortonIndex.hpp:226-318): expandBits of an int32
+ * runs the shift/mask pairs 16/0x0000ffff, 8/0x00ff00ff, 4/0x0f0f0f0f,
+ * 2/0x33333333, 1/0x55555555; x in the even bits, y in the odd ones. */
+static inline uint32_t spread16(uint32_t v)
 {
+  v = (v | (v << 16)) & 0x0000ffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+uint32_t orc_morton2(int32_t x, int32_t y) { return spread16((uint32_t)x) | (spread16((uint32_t)y) << 1); }
+
+/* build_radix_tree.hpp:48-65 (morton32_encode: 32 / Dims bits per axis, so
+ * 10 in 3-D and 16 in 2-D) on the normalised centroid computed at :154-175. */
+static inline uint32_t box_mcode(const orc_box* b, const double minc[3], const double inv[3], int nd)
+{
+  const double scale = (nd == 2) ? 65536.0 : 1024.0;
   int32_t q[3];
-  for(int d = 0; d < 3; ++d)
+  for(int d = 0; d < nd; ++d)
   {
     const double c = 0.5 * (b->min[d] + b->max[d]);
     const double u = (c - minc[d]) * inv[d];
-    q[d] = (int32_t)fmin(fmax(u * 1024.0, 0.0), 1023.0);
+    q[d] = (int32_t)fmin(fmax(u * scale, 0.0), scale - 1.0);
   }
-  return orc_morton3(q[0], q[1], q[2]);
+  return (nd == 2) ? orc_morton2(q[0], q[1]) : orc_morton3(q[0], q[1], q[2]);
 }
 
 /* build_radix_tree.hpp:265-287 */
@@ -367,9 +394,11 @@ static void stable_sort_codes(uint32_t* keys, int32_t* idx, int32_t n)
   free(i2);
 }
 
-orc_bvh* orc_bvh_new(const double* boxes_in, int32_t n_in, double scale)
+/* boxes_in: AoS of 2 * nd doubles (min, max) per box. */
+static orc_bvh* bvh_new_d(const double* boxes_in, int32_t n_in, double scale, int nd)
 {
   orc_bvh* t = (orc_bvh*)calloc(1, sizeof(orc_bvh));
+  t->dims = nd;
   /* spin/BVH.hpp:441-464: pad 0 or 1 boxes to 2 with an invalid box. */
   const int32_t n = (n_in <= 1) ? 2 : n_in;
   const int32_t inner = n - 1;
@@ -388,16 +417,22 @@ orc_bvh* orc_bvh_new(const double* boxes_in, int32_t n_in, double scale)
   orc_box* aabb = (orc_box*)malloc(sizeof(orc_box) * n);
   for(int32_t i = 0; i < n; ++i)
   {
+    box_clear(&aabb[i]);
     if(n_in > 1 || (n_in == 1 && i == 0))
-      memcpy(&aabb[i], boxes_in + 6 * (size_t)i, sizeof(orc_box));
-    else
-      box_clear(&aabb[i]);
-    orc_box_scale(&aabb[i], scale);
+    {
+      const double* b = boxes_in + 2 * nd * (size_t)i;
+      for(int d = 0; d < 3; ++d)
+      {
+        aabb[i].min[d] = (d < nd) ? b[d] : 0.0;
+        aabb[i].max[d] = (d < nd) ? b[nd + d] : 0.0;
+      }
+    }
+    box_scale_d(&aabb[i], scale, nd);
   }
 
   /* reduce :134-142 (SEQ path: addBox from an invalid box) */
   box_clear(&t->bounds);
-  for(int32_t i = 0; i < n; ++i) orc_box_add_box(&t->bounds, &aabb[i]);
+  for(int32_t i = 0; i < n; ++i) box_add_box_d(&t->bounds, &aabb[i], nd);
 
   /* get_mcodes :146-176; 1.f/extent promotes to double, thresh 1e-8. */
   double minc[3], inv[3];
@@ -409,7 +444,7 @@ orc_bvh* orc_bvh_new(const double* boxes_in, int32_t n_in, double scale)
   }
   for(int32_t i = 0; i < n; ++i)
   {
-    t->mcodes[i] = box_mcode(&aabb[i], minc, inv);
+    t->mcodes[i] = box_mcode(&aabb[i], minc, inv, nd);
     t->leafs[i] = i;
   }
 
@@ -483,7 +518,7 @@ orc_bvh* orc_bvh_new(const double* boxes_in, int32_t n_in, double scale)
       const int32_t lc = t->lchild[cur], rc = t->rchild[cur];
       const int32_t other = (lc == last) ? rc : lc;
       const orc_box ob = (other >= inner) ? t->leaf_aabbs[other - inner] : t->inner_aabbs[other];
-      orc_box_add_box(&acc, &ob);
+      box_add_box_d(&acc, &ob, nd);
       t->inner_aabbs[cur] = acc;
       last = cur;
       cur = t->parents[cur];
@@ -522,6 +557,16 @@ orc_bvh* orc_bvh_new(const double* boxes_in, int32_t n_in, double scale)
     t->children[2 * i + 1] = rc;
   }
   return t;
+}
+
+orc_bvh* orc_bvh_new(const double* boxes_in, int32_t n_in, double scale)
+{
+  return bvh_new_d(boxes_in, n_in, scale, 3);
+}
+
+orc_bvh* orc_bvh_new2d(const double* boxes_in, int32_t n_in, double scale)
+{
+  return bvh_new_d(boxes_in, n_in, scale, 2);
 }
 
 void orc_bvh_free(orc_bvh* t)
@@ -662,22 +707,44 @@ static int point_pref(const orc_box* l, const orc_box* r, const void* prim)
 typedef struct
 {
   double tol;
+  int nd;
 } find_ctx;
 
+/* prim: a point (3 doubles), a box (orc_box) or a ray (origin, direction),
+ * with the unused z entries of a 2-D query set to 0 */
 static int pred_point(const void* prim, const orc_box* b, void* ctx)
 {
-  (void)ctx;
-  return box_contains(b, (const double*)prim);
+  return box_contains_d(b, (const double*)prim, ((find_ctx*)ctx)->nd);
 }
 static int pred_box(const void* prim, const orc_box* b, void* ctx)
 {
-  (void)ctx;
-  return box_intersects((const orc_box*)prim, b);
+  return box_intersects_d((const orc_box*)prim, b, ((find_ctx*)ctx)->nd);
 }
 static int pred_ray(const void* prim, const orc_box* b, void* ctx)
 {
   const double* r = (const double*)prim;
-  return orc_ray_box(r, r + 3, b, ((find_ctx*)ctx)->tol);
+  return ray_box_d(r, r + 3, b, ((find_ctx*)ctx)->tol, ((find_ctx*)ctx)->nd);
+}
+
+/* query i of an AoS array of nd-dimensional points (nd doubles), boxes or
+ * rays (2 * nd doubles) as the 3-D layout the predicates read */
+static void load_query(const double* prims, int kind, int nd, int64_t i, double q[6])
+{
+  const int w = (kind == 0) ? nd : 2 * nd;
+  const double* p = prims + (size_t)w * (size_t)i;
+  for(int k = 0; k < 6; ++k) q[k] = 0.0;
+  if(kind == 0)
+  {
+    for(int d = 0; d < nd; ++d) q[d] = p[d];
+  }
+  else
+  {
+    for(int d = 0; d < nd; ++d)
+    {
+      q[d] = p[d];
+      q[3 + d] = p[nd + d];
+    }
+  }
 }
 
 typedef struct
@@ -709,13 +776,14 @@ typedef struct
 static void* find_worker(void* arg)
 {
   find_job* j = (find_job*)arg;
-  const int stride = (j->kind == 0) ? 3 : 6;
   pred_fn B = (j->kind == 0) ? pred_point : (j->kind == 1) ? pred_box : pred_ray;
-  find_ctx fc = {j->tol};
+  find_ctx fc = {j->tol, j->t->dims};
   for(int64_t i = j->begin; i < j->end; ++i)
   {
     collect_ctx cc = {0, j->cands ? j->cands + j->offsets[i] : NULL};
-    if(traverse(j->t, j->prims + stride * i, B, &fc, leaf_count, &cc, no_pref) < 0)
+    double q[6];
+    load_query(j->prims, j->kind, j->t->dims, i, q);
+    if(traverse(j->t, q, B, &fc, leaf_count, &cc, no_pref) < 0)
       j->overflow = 1;
     if(!j->cands) j->counts[i] = cc.count;
   }
@@ -783,18 +851,18 @@ int64_t orc_bvh_brute(const orc_bvh* t, int kind, const double* prims, int64_t n
   /* scaled boxes in original-id order */
   orc_box* orig = (orc_box*)malloc(sizeof(orc_box) * n);
   for(int32_t p = 0; p < n; ++p) orig[t->leafs[p]] = t->leaf_aabbs[p];
-  const int stride = (kind == 0) ? 3 : 6;
-  find_ctx fc = {tol};
+  find_ctx fc = {tol, t->dims};
   int64_t total = 0;
   for(int64_t i = 0; i < nq; ++i)
   {
-    const double* pr = prims + stride * i;
+    double pr[6];
+    load_query(prims, kind, t->dims, i, pr);
     int32_t c = 0;
     for(int32_t b = 0; b < n; ++b)
     {
-      if(!orc_box_valid(&orig[b])) continue;
-      const int hit = (kind == 0) ? pred_point(pr, &orig[b], NULL)
-        : (kind == 1)             ? pred_box(pr, &orig[b], NULL)
+      if(!box_valid_d(&orig[b], t->dims)) continue;
+      const int hit = (kind == 0) ? pred_point(pr, &orig[b], &fc)
+        : (kind == 1)             ? pred_box(pr, &orig[b], &fc)
                                   : pred_ray(pr, &orig[b], &fc);
       if(hit)
       {
@@ -844,7 +912,7 @@ orc_sdf* orc_sdf_new(const double* x, const double* y, const double* z, int32_t 
   for(int32_t i = 0; i < nverts; ++i)
   {
     const double p[3] = {x[i], y[i], z[i]};
-    box_add_point(&s->domain, p);
+    box_add_point_d(&s->domain, p, 3);
   }
   /* getCellBoundingBox :609-633 */
   double* boxes = (double*)malloc(sizeof(orc_box) * (size_t)ntris);
@@ -856,7 +924,7 @@ orc_sdf* orc_sdf_new(const double* x, const double* y, const double* z, int32_t 
     {
       const int32_t v = conn[3 * (size_t)c + k];
       const double p[3] = {x[v], y[v], z[v]};
-      box_add_point(&b, p);
+      box_add_point_d(&b, p, 3);
     }
     memcpy(boxes + 6 * (size_t)c, &b, sizeof(orc_box));
   }

@@ -47,10 +47,12 @@ typedef struct orc_bvh
   orc_box* leaf_aabbs;  /* n scaled leaf boxes in sorted order */
   orc_box* nodes;       /* 2(n-1): child boxes of inner node i at 2i, 2i+1 */
   int32_t* children;    /* 2(n-1): 2*child for inner, -(pos+1) for leaf */
+  int32_t dims;         /* 3, or 2 for BVH<2>: boxes use only their x, y entries */
 } orc_bvh;
 
 /* ---- primal restatements, exported for unit tests ------------------------ */
 uint32_t orc_morton3(int32_t x, int32_t y, int32_t z);
+uint32_t orc_morton2(int32_t x, int32_t y);
 void orc_box_scale(orc_box* b, double sf);
 int orc_box_valid(const orc_box* b);
 void orc_box_add_box(orc_box* b, const orc_box* o);
@@ -63,6 +65,9 @@ void orc_ray_normalize(const double d[3], double out[3]);
 
 /* ---- BVH ---------------------------------------------------------------- */
 orc_bvh* orc_bvh_new(const double* boxes /* AoS n x 6 */, int32_t n, double scale);
+/* spin::BVH<2>: boxes AoS n x 4 (min x y, max x y); queries then are 2-D
+ * (points 2 doubles, boxes and rays 4) */
+orc_bvh* orc_bvh_new2d(const double* boxes /* AoS n x 4 */, int32_t n, double scale);
 void orc_bvh_free(orc_bvh* t);
 int32_t orc_bvh_size(const orc_bvh* t);
 void orc_bvh_export(const orc_bvh* t, double* bounds6, uint32_t* mcodes,
@@ -72,7 +77,8 @@ void orc_bvh_export(const orc_bvh* t, double* bounds6, uint32_t* mcodes,
 
 /* kind: 0 = points (box contains point), 1 = boxes (intersectsWith),
  *       2 = rays (slab test with tolerance tol; direction already unit).
- * prims is AoS: 3 doubles per point, 6 per box (min,max) or ray (origin,dir).
+ * prims is AoS: 3 doubles per point, 6 per box (min,max) or ray (origin,dir);
+ * for a 2-D tree 2 per point and 4 per box or ray.
  * With cands == NULL only counts/offsets are written; returns the total.   */
 int64_t orc_bvh_find(const orc_bvh* t, int kind, const double* prims, int64_t nq,
                      double tol, int32_t* counts, int32_t* offsets, int32_t* cands,

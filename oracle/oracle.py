@@ -36,6 +36,10 @@ def lib():
         L = ctypes.CDLL(_LIB_PATH)
         L.orc_bvh_new.restype = ctypes.c_void_p
         L.orc_bvh_new.argtypes = [_dp, ctypes.c_int32, ctypes.c_double]
+        L.orc_bvh_new2d.restype = ctypes.c_void_p
+        L.orc_bvh_new2d.argtypes = [_dp, ctypes.c_int32, ctypes.c_double]
+        L.orc_morton2.restype = ctypes.c_uint32
+        L.orc_morton2.argtypes = [ctypes.c_int32] * 2
         L.orc_bvh_free.argtypes = [ctypes.c_void_p]
         L.orc_bvh_size.restype = ctypes.c_int32
         L.orc_bvh_size.argtypes = [ctypes.c_void_p]
@@ -100,6 +104,10 @@ def i32(a):
 # ---------------------------------------------------------------- primal ----
 def morton3(x, y, z):
     return int(lib().orc_morton3(int(x), int(y), int(z)))
+
+
+def morton2(x, y):
+    return int(lib().orc_morton2(int(x), int(y)))
 
 
 def closest_point(p, a, b, c, eps=1e-12):
@@ -168,16 +176,20 @@ KIND = {"points": 0, "boxes": 1, "rays": 2}
 
 
 class OracleBVH:
-    """spin::BVH<3, SEQ_EXEC, double> restated (build + find*)."""
+    """spin::BVH<NDIMS, SEQ_EXEC, double> restated (build + find*); ndims 3 or 2.
+    Boxes are AoS (min, max) of 2 * ndims doubles; 2-D queries are points of 2
+    doubles and boxes / rays of 4."""
 
-    def __init__(self, boxes, scale=1.000123, _handle=None, _owner=None):
+    def __init__(self, boxes, scale=1.000123, _handle=None, _owner=None, ndims=3):
+        self.ndims = ndims
         if _handle is not None:
             self.h = _handle
             self._owner = _owner
         else:
-            b = f64(boxes).reshape(-1, 6)
+            b = f64(boxes).reshape(-1, 2 * ndims)
             self._boxes = b
-            self.h = lib().orc_bvh_new(_p(b, _dp), b.shape[0], scale)
+            new = lib().orc_bvh_new2d if ndims == 2 else lib().orc_bvh_new
+            self.h = new(_p(b, _dp), b.shape[0], scale)
             self._owner = None
 
     def __del__(self):
@@ -202,13 +214,18 @@ class OracleBVH:
                              _p(out["rchild"], _i32p), _p(out["parents"], _i32p),
                              _p(out["inner_aabbs"], _dp), _p(out["nodes"], _dp),
                              _p(out["children"], _i32p))
+        if self.ndims == 2:  # BoundingBox<double, 2>: min x y, max x y
+            for k in ("bounds", "inner_aabbs", "nodes"):
+                out[k] = np.ascontiguousarray(out[k][..., [0, 1, 3, 4]])
         return out
+
+    def _stride(self, k):
+        return self.ndims if k == 0 else 2 * self.ndims
 
     def find(self, kind, prims, tol=np.finfo(np.float64).eps, nthreads=1):
         """Returns (offsets, counts, candidates) like LinearBVH::findCandidatesImpl."""
         k = KIND[kind]
-        stride = 3 if k == 0 else 6
-        pr = f64(prims).reshape(-1, stride)
+        pr = f64(prims).reshape(-1, self._stride(k))
         nq = pr.shape[0]
         counts = np.zeros(nq, np.int32)
         offsets = np.zeros(nq, np.int32)
@@ -223,8 +240,7 @@ class OracleBVH:
 
     def brute(self, kind, prims, tol=np.finfo(np.float64).eps):
         k = KIND[kind]
-        stride = 3 if k == 0 else 6
-        pr = f64(prims).reshape(-1, stride)
+        pr = f64(prims).reshape(-1, self._stride(k))
         nq = pr.shape[0]
         counts = np.zeros(nq, np.int32)
         total = lib().orc_bvh_brute(self.h, k, _p(pr, _dp), nq, tol, _p(counts, _i32p), None, 0)
