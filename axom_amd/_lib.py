@@ -52,6 +52,8 @@ _SIGS = {
                              ctypes.POINTER(_vp), ctypes.POINTER(_i64), _vp]),
     "axsq_sdf_create": (_i32, [ctypes.POINTER(_vp), _vp, _vp, _vp, _i64, _vp, _i64, _i32, _i32,
                                _i32, _vp]),
+    "axsq_sdf_create_cells": (_i32, [ctypes.POINTER(_vp), _vp, _vp, _vp, _i64, _vp, _vp, _i64,
+                                     _i32, _i32, _i32, _vp]),
     "axsq_sdf_destroy": (_i32, [_vp]),
     "axsq_sdf_mesh_bounds": (_i32, [_vp, _dbl3, _dbl3]),
     "axsq_sdf_update_vertices": (_i32, [_vp, _vp, _vp, _vp, _vp]),

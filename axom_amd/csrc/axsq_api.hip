@@ -417,6 +417,7 @@ int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const doub
   hipGetDevice(&h->device);
   h->nverts = nverts;
   h->ntris = ntris;
+  h->nconn = 3 * ntris;
   h->watertight = watertight ? 1 : 0;
   h->compute_sign = compute_sign ? 1 : 0;
   h->fp_mode = fp_mode;
@@ -462,12 +463,98 @@ int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const doub
   return AXSQ_OK;
 }
 
+int axsq_sdf_create_cells(axsq_sdf** out, const double* x, const double* y, const double* z,
+                          int64_t nverts, const int32_t* conn, const int32_t* cell_offsets,
+                          int64_t ncells, int watertight, int compute_sign, int fp_mode,
+                          void* stream) {
+  clear_error();
+  if (!out) {
+    set_error("axsq_sdf_create_cells: out is NULL");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  if (!x || !y || !z || !conn || !cell_offsets || nverts < 3 || ncells < 1 ||
+      nverts > 0x7fffffffll || ncells > 0x3fffffffll) {
+    set_error("axsq_sdf_create_cells: need a mesh with >= 3 vertices and >= 1 cell");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (fp_mode != AXSQ_FP64 && fp_mode != AXSQ_FP32_TRAVERSAL) {
+    set_error("axsq_sdf_create_cells: unknown fp_mode %d", fp_mode);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<int32_t> off((size_t)ncells + 1);
+  if (hipMemcpyAsync(off.data(), cell_offsets, sizeof(int32_t) * off.size(), hipMemcpyDefault,
+                     s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("axsq_sdf_create_cells: cannot read the cell offsets");
+    return AXSQ_ERR_HIP;
+  }
+  bool all_tris = off[0] == 0;
+  for (int64_t c = 0; c < ncells; ++c) {
+    const int32_t n = off[c + 1] - off[c];
+    if (off[c] < 0 || (n != 3 && n != 4)) {
+      set_error("axsq_sdf_create_cells: cell %lld has %d nodes (triangles and quads only)",
+                (long long)c, n);
+      return AXSQ_ERR_INVALID_ARG;
+    }
+    all_tris = all_tris && (off[c] == 3 * c) && n == 3;
+  }
+  if (all_tris)  // a triangle mesh: the two-pass path
+    return axsq_sdf_create(out, x, y, z, nverts, conn + 0, ncells, watertight, compute_sign,
+                           fp_mode, stream);
+  axsq_sdf* h = new axsq_sdf();
+  hipGetDevice(&h->device);
+  h->nverts = nverts;
+  h->ntris = ncells;
+  h->cells = true;
+  h->nconn = off[(size_t)ncells];
+  h->watertight = watertight ? 1 : 0;
+  h->compute_sign = compute_sign ? 1 : 0;
+  h->fp_mode = AXSQ_FP64;  // the cell walk runs on FP64 nodes
+  h->bvh = new axsq_bvh();
+  h->bvh->device = h->device;
+  const char* env = getenv("AXSQ_SORT_QUERIES");
+  h->sort_queries = !(env && env[0] == '0');
+  auto fail = [&](int code) {
+    axsq_sdf_destroy(h);
+    return code;
+  };
+  if (hipMalloc((void**)&h->vx, sizeof(double) * nverts) != hipSuccess ||
+      hipMalloc((void**)&h->vy, sizeof(double) * nverts) != hipSuccess ||
+      hipMalloc((void**)&h->vz, sizeof(double) * nverts) != hipSuccess ||
+      hipMalloc((void**)&h->conn, sizeof(int32_t) * h->nconn) != hipSuccess ||
+      hipMalloc((void**)&h->cell_off, sizeof(int32_t) * off.size()) != hipSuccess ||
+      hipMalloc((void**)&h->stats_dev, sizeof(int64_t) * 2) != hipSuccess) {
+    set_error("axsq_sdf_create_cells: out of device memory");
+    return fail(AXSQ_ERR_NOMEM);
+  }
+  if (hipMemcpyAsync(h->vx, x, sizeof(double) * nverts, hipMemcpyDefault, s) != hipSuccess ||
+      hipMemcpyAsync(h->vy, y, sizeof(double) * nverts, hipMemcpyDefault, s) != hipSuccess ||
+      hipMemcpyAsync(h->vz, z, sizeof(double) * nverts, hipMemcpyDefault, s) != hipSuccess ||
+      hipMemcpyAsync(h->conn, conn, sizeof(int32_t) * h->nconn, hipMemcpyDefault, s) !=
+          hipSuccess ||
+      hipMemcpyAsync(h->cell_off, off.data(), sizeof(int32_t) * off.size(),
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("axsq_sdf_create_cells: mesh copy failed");
+    return fail(AXSQ_ERR_HIP);
+  }
+  int rc = sdf_check_conn(h, s);
+  if (rc != AXSQ_OK) return fail(rc);
+  rc = sdf_setup(h, s);
+  if (rc != AXSQ_OK) return fail(rc);
+  *out = h;
+  return AXSQ_OK;
+}
+
 int axsq_sdf_destroy(axsq_sdf* h) {
   if (!h) return AXSQ_OK;
   hipFree(h->vx);
   hipFree(h->vy);
   hipFree(h->vz);
   hipFree(h->conn);
+  hipFree(h->cell_off);
   hipFree(h->tris);
   hipFree(h->v2t_start);
   hipFree(h->v2t_end);
@@ -568,6 +655,10 @@ int axsq_sdf_backward(const axsq_sdf* h, const double* x, const double* y, const
   }
   if (!x || !y || !z || !dist || !elem || !gout) {
     set_error("axsq_sdf_backward: NULL input");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (h->cells) {
+    set_error("axsq_sdf_backward: the adjoint is for triangle meshes (this one has quad cells)");
     return AXSQ_ERR_INVALID_ARG;
   }
   AXSQ_TRY(sdf_backward_launch(h, x, y, z, stride, nq, dist, elem, gout, grad_q, grad_v, s));

@@ -113,7 +113,10 @@ struct axsq_bvh {
 };
 
 struct axsq_sdf {
-  int64_t nverts = 0, ntris = 0;
+  int64_t nverts = 0, ntris = 0;  // ntris: cells (triangles or quads) in cell mode
+  bool cells = false;             // some cell is a quad: two triangle records per leaf
+  int64_t nconn = 0;              // connectivity length
+  int32_t* cell_off = nullptr;    // cell mode: ncells + 1 offsets into conn
   int watertight = 1, compute_sign = 1, fp_mode = AXSQ_FP64;
   int device = 0;
   double* vx = nullptr;

@@ -1026,6 +1026,53 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
 }
 
 
+// Meshes with quad cells (SignedDistance.hpp:637-737 with the two triangles
+// of a quad, :649-658): one lane per query walks the whole tree in one pass
+// (traverse(), the reference's while-while order; no step budget) and checks
+// each reached cell's one or two triangles in order, adding pseudo-normal
+// terms as they come. The leaf's records are tris[2 * pos + e]; the winner's
+// record index is MinCand::pos, so the finish is the triangle path's.
+template <bool STATS>
+__global__ __launch_bounds__(kSdfBlock) void k_sdf_eval_cells(SdfArgs a) {
+  __shared__ int32_t s_stack[stack_lds_words(kSdfBlock)];
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.nq) return;
+  const int64_t i = a.order ? (int64_t)a.order[j] : j;
+  const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
+  const bool cn = a.compute_sign != 0;
+  MinCand m;
+  m.sq = kDblMax;
+  m.p[0] = m.p[1] = m.p[2] = 0.0;
+  m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
+  m.type = -1;
+  m.pos = 0;
+  BoxTests<false> bt(q);
+  int32_t leaves = 0;
+  const int32_t v = traverse<Node64, double>(
+      a.nodes, a.ninner, lane_stack_base(s_stack),
+      [&](const double* b) { return bt.keep(bt.key(b), m.sq); },
+      [&](const double* l, const double* r) { return bt.right_first(l, r); },
+      [&](int32_t pos) {
+        const int32_t ntri = reinterpret_cast<const int32_t*>(a.tris + 2 * (int64_t)pos)[19];
+        for (int e = 0; e < 2; ++e) {
+          if (e < ntri) {
+            double T[9], sq, cp[3];
+            int32_t type, code;
+            load_tri(a.tris, 2 * pos + e, T);
+            leaf_record(q, T, sq, cp, type, code);
+            apply_record(m, a.tris, 2 * pos + e, sq, cp, type, code, cn);
+          }
+        }
+        ++leaves;
+      });
+  if (v < 0) atomicOr(a.status, 1);
+  if (STATS) {
+    atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)(v < 0 ? 0 : v));
+    atomicAdd((unsigned long long*)&a.stats[1], (unsigned long long)leaves);
+  }
+  sdf_finish(a, i, q, m, cn);
+}
+
 // Launch order of the saved walks: the farthest from the surface first (the
 // running minimum at suspension, descending). Those are the walks that
 // still have most of the tree to cover -- queries deep inside a closed
@@ -1168,6 +1215,62 @@ __global__ __launch_bounds__(kBlock) void k_gather_tris(const double* __restrict
 
 // boundaries of sorted keys: start[key] = first index, end[key] = one past
 // last. A key >= nseg is not written and raises bit 2 of *status.
+// Box of a cell's three or four nodes: addPoint from an invalid box
+// (getCellBoundingBox, SignedDistance.hpp:609-633).
+__global__ __launch_bounds__(kBlock) void k_cell_boxes(const double* __restrict__ x,
+                                                        const double* __restrict__ y,
+                                                        const double* __restrict__ z,
+                                                        const int32_t* __restrict__ conn,
+                                                        const int32_t* __restrict__ off,
+                                                        int64_t nc, double* __restrict__ boxes) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double b[6] = {kDblMax, kDblMax, kDblMax, -kDblMax, -kDblMax, -kDblMax};
+  for (int32_t k = off[c]; k < off[c + 1]; ++k) {
+    const int32_t v = conn[k];
+    const double p[3] = {x[v], y[v], z[v]};
+    for (int d = 0; d < 3; ++d) {
+      if (p[d] < b[d]) b[d] = p[d];
+      if (p[d] > b[3 + d]) b[3 + d] = p[d];
+    }
+  }
+  for (int k = 0; k < 6; ++k) boxes[6 * c + k] = b[k];
+}
+
+// Two records per BVH leaf: the cell's triangles (n0, n1, n2) and, for a
+// quad, (n0, n2, n3), which checkCandidate tests in that order
+// (SignedDistance.hpp:649-658). TriRec::id is the cell id; the first
+// record's pad is the cell's triangle count (0 for a padding leaf).
+__global__ __launch_bounds__(kBlock) void k_gather_cells(const double* __restrict__ x,
+                                                          const double* __restrict__ y,
+                                                          const double* __restrict__ z,
+                                                          const int32_t* __restrict__ conn,
+                                                          const int32_t* __restrict__ off,
+                                                          const int32_t* __restrict__ leaf_ids,
+                                                          int64_t nc, int64_t nleaves,
+                                                          TriRec* __restrict__ tris) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nleaves) return;
+  const int32_t c = leaf_ids[p];
+  const bool real = c >= 0 && c < nc;
+  const int32_t b = real ? off[c] : 0;
+  const int32_t nn = real ? off[c + 1] - b : 0;
+  const int32_t corner[2][3] = {{0, 1, 2}, {0, 2, 3}};
+  for (int e = 0; e < 2; ++e) {
+    TriRec r;
+    const bool have = real && (e == 0 || nn == 4);
+    for (int k = 0; k < 3; ++k) {
+      const int32_t v = have ? conn[b + corner[e][k]] : 0;
+      r.v[3 * k] = have ? x[v] : 0.0;
+      r.v[3 * k + 1] = have ? y[v] : 0.0;
+      r.v[3 * k + 2] = have ? z[v] : 0.0;
+    }
+    r.id = real ? c : -1;
+    r.pad = (e == 0) ? (real ? (nn == 4 ? 2 : 1) : 0) : 0;
+    tris[2 * p + e] = r;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_segments(const uint32_t* __restrict__ keys, int64_t n,
                                                       int64_t nseg, int32_t* __restrict__ start,
                                                       int32_t* __restrict__ end,
@@ -1318,7 +1421,7 @@ int sdf_check_conn(const axsq_sdf* h, hipStream_t s) {
   AXSQ_TRY(dmalloc(&flag, 1));
   int rc = fill_i32(flag, 0, 1, s);
   if (rc == AXSQ_OK) {
-    k_check_conn<<<grid_for(3 * h->ntris), kBlock, 0, s>>>(h->conn, 3 * h->ntris, h->nverts, flag);
+    k_check_conn<<<grid_for(h->nconn), kBlock, 0, s>>>(h->conn, h->nconn, h->nverts, flag);
     int32_t hf = 0;
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(&hf, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -1326,7 +1429,7 @@ int sdf_check_conn(const axsq_sdf* h, hipStream_t s) {
       set_error("connectivity check failed to run");
       rc = AXSQ_ERR_HIP;
     } else if (hf) {
-      set_error("triangle connectivity names a vertex outside [0, %lld)", (long long)h->nverts);
+      set_error("cell connectivity names a vertex outside [0, %lld)", (long long)h->nverts);
       rc = AXSQ_ERR_INVALID_ARG;
     }
   }
@@ -1352,7 +1455,11 @@ int sdf_setup(axsq_sdf* h, hipStream_t s) {
   // whole workspace for its own scratch
   double* boxes = nullptr;
   AXSQ_TRY(dmalloc(&boxes, 6 * (size_t)nt));
-  k_tri_boxes<<<grid_for(nt), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, nt, boxes);
+  if (h->cells)
+    k_cell_boxes<<<grid_for(nt), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, h->cell_off, nt,
+                                                 boxes);
+  else
+    k_tri_boxes<<<grid_for(nt), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, nt, boxes);
   hipError_t e = hipGetLastError();
   int rc = (e == hipSuccess) ? AXSQ_OK : AXSQ_ERR_HIP;
   if (rc == AXSQ_OK) rc = bvh_build_into(h->bvh, boxes, nt, 1.000123, h->fp_mode, s);
@@ -1360,9 +1467,13 @@ int sdf_setup(axsq_sdf* h, hipStream_t s) {
   if (e != hipSuccess) set_error("k_tri_boxes: %s", hipGetErrorString(e));
   AXSQ_TRY(rc);
   const int64_t nl = h->bvh->n;
-  if (!h->tris) AXSQ_TRY(dmalloc(&h->tris, (size_t)nl));
-  k_gather_tris<<<grid_for(nl), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, h->bvh->leaf_ids,
-                                                nt, nl, h->tris);
+  if (!h->tris) AXSQ_TRY(dmalloc(&h->tris, (size_t)(h->cells ? 2 * nl : nl)));
+  if (h->cells)
+    k_gather_cells<<<grid_for(nl), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, h->cell_off,
+                                                   h->bvh->leaf_ids, nt, nl, h->tris);
+  else
+    k_gather_tris<<<grid_for(nl), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, h->bvh->leaf_ids,
+                                                  nt, nl, h->tris);
   AXSQ_CHECK_HIP(hipGetLastError());
   AXSQ_CHECK_HIP(hipStreamSynchronize(s));
   return AXSQ_OK;
@@ -1447,6 +1558,16 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
                               ws.v1.as<uint32_t>(), nq, 30, ws, s, &sk, &sv));
     pq.stop();
     a.order = sv;
+  }
+  if (h->cells) {  // quad cells: the one-pass walk over FP64 nodes
+    ProfRange pr("sdf_eval_cells", s);
+    if (stats)
+      k_sdf_eval_cells<true><<<grid_for(nq, kSdfBlock), kSdfBlock, 0, s>>>(a);
+    else
+      k_sdf_eval_cells<false><<<grid_for(nq, kSdfBlock), kSdfBlock, 0, s>>>(a);
+    AXSQ_CHECK_HIP(hipGetLastError());
+    const_cast<axsq_sdf*>(h)->last_suspended = 0;
+    return AXSQ_OK;
   }
   const bool mixed = (fp_mode == AXSQ_FP32_TRAVERSAL);
   // room for suspended walks: every query, up to 2^24 records (6.0 GB); a

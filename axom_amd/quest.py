@@ -33,17 +33,31 @@ class SignedDistance:
     """
 
     def __init__(self, vertices, triangles, isWatertight=True, computeSign=True,
-                 fp_mode="fp64"):
+                 fp_mode="fp64", cell_offsets=None):
+        """cell_offsets: for a mesh of triangle and quad cells, the (ncells + 1,)
+        int32 offsets into the flat int32 connectivity `triangles` (a mint
+        MIXED_SHAPE layout; a QUAD mesh has offsets 0, 4, 8, ...)."""
         self._h = None
         x, y, z = _soa(vertices)
-        t = _dev(triangles, torch.int32, "triangles", 3)
         self.nverts = x.shape[0]
-        self.ntris = t.shape[0]
         self._fp = FP_MODES[fp_mode]
         h = ctypes.c_void_p()
-        L.check(L.lib().axsq_sdf_create(ctypes.byref(h), _ptr(x), _ptr(y), _ptr(z), self.nverts,
-                                        _ptr(t), self.ntris, int(bool(isWatertight)),
-                                        int(bool(computeSign)), self._fp, stream_ptr()))
+        if cell_offsets is None:
+            t = _dev(triangles, torch.int32, "triangles", 3)
+            self.ntris = t.shape[0]
+            L.check(L.lib().axsq_sdf_create(ctypes.byref(h), _ptr(x), _ptr(y), _ptr(z),
+                                            self.nverts, _ptr(t), self.ntris,
+                                            int(bool(isWatertight)), int(bool(computeSign)),
+                                            self._fp, stream_ptr()))
+        else:
+            t = _dev(triangles, torch.int32, "connectivity").reshape(-1)
+            off = _dev(cell_offsets, torch.int32, "cell_offsets").reshape(-1)
+            self.ntris = off.shape[0] - 1
+            L.check(L.lib().axsq_sdf_create_cells(ctypes.byref(h), _ptr(x), _ptr(y), _ptr(z),
+                                                  self.nverts, _ptr(t), _ptr(off), self.ntris,
+                                                  int(bool(isWatertight)),
+                                                  int(bool(computeSign)), self._fp,
+                                                  stream_ptr()))
         self._h = h
 
     def __del__(self):

@@ -203,3 +203,45 @@ def workload6(level=8):
     v2 = np.concatenate([v, v + np.array([0.5, 0.1, 0.0])])
     t2 = np.concatenate([t, t + v.shape[0]]).astype(np.int32)
     return np.ascontiguousarray(v2), np.ascontiguousarray(t2)
+
+
+def cube_sphere(n, radius=1.0, project=True):
+    """A closed quad mesh: the surface of the cube [-1,1]^3 as 6 n x n grids of
+    quads (outward, counter-clockwise seen from outside), its vertices
+    projected to the sphere of the given radius when `project`. Returns
+    (verts (V,3), quads (6 n^2, 4) int32)."""
+    ids = {}
+    verts = []
+    quads = []
+
+    def vid(p):
+        key = tuple(np.round(p, 12))
+        if key not in ids:
+            ids[key] = len(verts)
+            verts.append(p)
+        return ids[key]
+
+    g = np.linspace(-1.0, 1.0, n + 1)
+    for axis in range(3):
+        for side in (-1.0, 1.0):
+            u_ax, v_ax = [a for a in range(3) if a != axis]
+            for i in range(n):
+                for j in range(n):
+                    corners = []
+                    for (a, b) in ((i, j), (i + 1, j), (i + 1, j + 1), (i, j + 1)):
+                        p = np.zeros(3)
+                        p[axis] = side
+                        p[u_ax] = g[a]
+                        p[v_ax] = g[b]
+                        corners.append(vid(p))
+                    # outward orientation: (u x v) points along +axis for this order
+                    nrm = np.zeros(3)
+                    nrm[axis] = 1.0
+                    uv = np.cross(np.eye(3)[u_ax], np.eye(3)[v_ax])
+                    if np.dot(uv, nrm) * side < 0:
+                        corners = corners[::-1]
+                    quads.append(corners)
+    v = np.array(verts, dtype=np.float64)
+    if project:
+        v = radius * v / np.linalg.norm(v, axis=1, keepdims=True)
+    return np.ascontiguousarray(v), np.array(quads, dtype=np.int32)

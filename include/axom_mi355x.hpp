@@ -133,8 +133,9 @@ class Array {
 };
 
 namespace primal {
-// Device layouts of primal::BoundingBox<double,3> (min, max) and
-// primal::Point<double,3>; the library reads them as plain doubles.
+// Device layouts of primal::BoundingBox<double,N> (min, max),
+// primal::Point<double,N> and primal::Ray<double,N> (origin, unit direction);
+// the library reads them as plain doubles.
 struct BoundingBox3 {
   double min[3];
   double max[3];
@@ -142,24 +143,40 @@ struct BoundingBox3 {
 struct Point3 {
   double x[3];
 };
-// primal::Ray<double,3> stores a unit direction (Ray.hpp:122-128); make_ray
-// normalises the way Vector::unitVector does (Vector.hpp:477-490).
 struct Ray3 {
   double origin[3];
   double dir[3];
 };
-inline Ray3 make_ray(const double o[3], const double d[3]) {
-  Ray3 r{{o[0], o[1], o[2]}, {1.0, 0.0, 0.0}};
+struct BoundingBox2 {
+  double min[2];
+  double max[2];
+};
+struct Point2 {
+  double x[2];
+};
+struct Ray2 {
+  double origin[2];
+  double dir[2];
+};
+// primal::Ray stores a unit direction (Ray.hpp:122-128); make_ray normalises
+// the way Vector::unitVector does (Vector.hpp:477-490).
+template <int N, class RayT>
+inline RayT make_ray_n(const double* o, const double* d) {
+  RayT r;
   double len_sq = 0.0;
-  len_sq += d[0] * d[0];
-  len_sq += d[1] * d[1];
-  len_sq += d[2] * d[2];
+  for (int k = 0; k < N; ++k) {
+    r.origin[k] = o[k];
+    len_sq += d[k] * d[k];
+    r.dir[k] = (k == 0) ? 1.0 : 0.0;
+  }
   if (len_sq >= 1e-50) {
     const double inv = 1. / std::sqrt(len_sq);
-    for (int k = 0; k < 3; ++k) r.dir[k] = d[k] * inv;
+    for (int k = 0; k < N; ++k) r.dir[k] = d[k] * inv;
   }
   return r;
 }
+inline Ray3 make_ray(const double o[3], const double d[3]) { return make_ray_n<3, Ray3>(o, d); }
+inline Ray2 make_ray2(const double o[2], const double d[2]) { return make_ray_n<2, Ray2>(o, d); }
 }  // namespace primal
 
 namespace spin {
@@ -169,35 +186,49 @@ enum BVHReturnCodes { BVH_BUILD_FAILED = -1, BVH_BUILD_OK = 0 };  // BVH.hpp:39-
 template <int NDIMS, typename ExecSpace, typename FloatType = double>
 class BVH;
 
-// spin::BVH<3, MI355X_EXEC, double>: build over device boxes, find* over
-// device queries. The BVH copies the boxes; it keeps no pointer to them.
+namespace detail {
+template <int NDIMS>
+struct BVHTypes;
 template <>
-class BVH<3, MI355X_EXEC, double> {
- public:
+struct BVHTypes<3> {
   using BoxType = primal::BoundingBox3;
   using PointType = primal::Point3;
   using RayType = primal::Ray3;
+};
+template <>
+struct BVHTypes<2> {
+  using BoxType = primal::BoundingBox2;
+  using PointType = primal::Point2;
+  using RayType = primal::Ray2;
+};
+
+// The body of spin::BVH<NDIMS, MI355X_EXEC, double> for NDIMS 3 and 2: build
+// over device boxes, find* over device queries. The BVH copies the boxes; it
+// keeps no pointer to them.
+template <int NDIMS>
+class MI355XLinearBVH {
+ public:
+  using BoxType = typename BVHTypes<NDIMS>::BoxType;
+  using PointType = typename BVHTypes<NDIMS>::PointType;
+  using RayType = typename BVHTypes<NDIMS>::RayType;
 
   static constexpr double DEFAULT_SCALE_FACTOR = 1.000123;  // BVH.hpp:410
   static constexpr double DEFAULT_TOLERANCE = std::numeric_limits<double>::epsilon();
 
-  BVH() = default;
-  BVH(const BoxType* boxes, IndexType numItems, int allocatorID = 0,
-      double tolerance = DEFAULT_TOLERANCE, double scaleFactor = DEFAULT_SCALE_FACTOR)
-      : m_tolerance(tolerance), m_scaleFactor(scaleFactor) {
-    (void)allocatorID;
-    initialize(boxes, numItems);
-  }
-  BVH(const BVH&) = delete;
-  BVH& operator=(const BVH&) = delete;
-  ~BVH() { (void)axsq_bvh_destroy(m_bvh); }
+  MI355XLinearBVH() = default;
+  MI355XLinearBVH(const MI355XLinearBVH&) = delete;
+  MI355XLinearBVH& operator=(const MI355XLinearBVH&) = delete;
+  ~MI355XLinearBVH() { (void)axsq_bvh_destroy(m_bvh); }
 
   // BVH.hpp:426-477
   int initialize(const BoxType* boxes, IndexType numItems) {
     (void)axsq_bvh_destroy(m_bvh);
     m_bvh = nullptr;
-    detail::check(axsq_bvh_build(&m_bvh, reinterpret_cast<const double*>(boxes), numItems,
-                                 m_scaleFactor, m_fpMode, nullptr));
+    const double* b = reinterpret_cast<const double*>(boxes);
+    if (NDIMS == 2)
+      ::axom::detail::check(axsq_bvh_build2d(&m_bvh, b, numItems, m_scaleFactor, m_fpMode, nullptr));
+    else
+      ::axom::detail::check(axsq_bvh_build(&m_bvh, b, numItems, m_scaleFactor, m_fpMode, nullptr));
     return BVH_BUILD_OK;
   }
   int initialize(ArrayView<const BoxType> boxes, IndexType numItems) {
@@ -218,14 +249,19 @@ class BVH<3, MI355X_EXEC, double> {
   // BVH.hpp:297-308; an invalid box before initialize().
   BoxType getBounds() const {
     BoxType b;
+    double lo[3], hi[3];
     if (!m_bvh) {
       for (int k = 0; k < 3; ++k) {
-        b.min[k] = std::numeric_limits<double>::max();
-        b.max[k] = std::numeric_limits<double>::lowest();
+        lo[k] = std::numeric_limits<double>::max();
+        hi[k] = std::numeric_limits<double>::lowest();
       }
-      return b;
+    } else {
+      ::axom::detail::check(axsq_bvh_bounds(m_bvh, lo, hi));
     }
-    detail::check(axsq_bvh_bounds(m_bvh, b.min, b.max));
+    for (int k = 0; k < NDIMS; ++k) {
+      b.min[k] = lo[k];
+      b.max[k] = hi[k];
+    }
     return b;
   }
 
@@ -233,26 +269,27 @@ class BVH<3, MI355X_EXEC, double> {
   void findPoints(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
                   Array<IndexType>& candidates, IndexType numPts, const PointType* points) const {
     const double* p = reinterpret_cast<const double*>(points);
-    find(AXSQ_QUERY_POINTS, p, p + 1, p + 2, 3, offsets, counts, candidates, numPts);
+    find(AXSQ_QUERY_POINTS, p, p + 1, NDIMS == 3 ? p + 2 : nullptr, NDIMS, offsets, counts,
+         candidates, numPts);
   }
-  // ZipPoint-style SoA points
+  // ZipPoint-style SoA points (z is ignored, and may be null, in 2-D)
   void findPoints(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
                   Array<IndexType>& candidates, IndexType numPts, const double* x,
-                  const double* y, const double* z) const {
+                  const double* y, const double* z = nullptr) const {
     find(AXSQ_QUERY_POINTS, x, y, z, 1, offsets, counts, candidates, numPts);
   }
   // BVH.hpp:510-536
   void findRays(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
                 Array<IndexType>& candidates, IndexType numRays, const RayType* rays) const {
-    find(AXSQ_QUERY_RAYS, reinterpret_cast<const double*>(rays), nullptr, nullptr, 6, offsets,
-         counts, candidates, numRays);
+    find(AXSQ_QUERY_RAYS, reinterpret_cast<const double*>(rays), nullptr, nullptr, 2 * NDIMS,
+         offsets, counts, candidates, numRays);
   }
   // BVH.hpp:541-568
   void findBoundingBoxes(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
                          Array<IndexType>& candidates, IndexType numBoxes,
                          const BoxType* boxes) const {
-    find(AXSQ_QUERY_BOXES, reinterpret_cast<const double*>(boxes), nullptr, nullptr, 6, offsets,
-         counts, candidates, numBoxes);
+    find(AXSQ_QUERY_BOXES, reinterpret_cast<const double*>(boxes), nullptr, nullptr, 2 * NDIMS,
+         offsets, counts, candidates, numBoxes);
   }
 
   const axsq_bvh* handle() const { return m_bvh; }
@@ -267,15 +304,35 @@ class BVH<3, MI355X_EXEC, double> {
     if (!m_bvh) throw mi355x_error(AXSQ_ERR_STATE, "BVH is not initialized");
     int32_t* cand = nullptr;
     int64_t total = 0;
-    detail::check(axsq_bvh_find(m_bvh, kind, x, y, z, stride, n, m_tolerance, offsets.data(),
-                                counts.data(), &cand, &total, nullptr));
+    ::axom::detail::check(axsq_bvh_find(m_bvh, kind, x, y, z, stride, n, m_tolerance,
+                                        offsets.data(), counts.data(), &cand, &total, nullptr));
     candidates.adopt(cand, (IndexType)total);
   }
 
+ protected:
   axsq_bvh* m_bvh = nullptr;
   double m_tolerance = DEFAULT_TOLERANCE;
   double m_scaleFactor = DEFAULT_SCALE_FACTOR;
   int m_fpMode = AXSQ_FP64;
+};
+}  // namespace detail
+
+// spin::BVH<3, MI355X_EXEC, double> and spin::BVH<2, MI355X_EXEC, double>
+template <int NDIMS>
+class BVH<NDIMS, MI355X_EXEC, double> : public detail::MI355XLinearBVH<NDIMS> {
+  static_assert(NDIMS == 2 || NDIMS == 3, "spin::BVH is 2-D or 3-D");
+  using Base = detail::MI355XLinearBVH<NDIMS>;
+
+ public:
+  using typename Base::BoxType;
+  BVH() = default;
+  BVH(const BoxType* boxes, IndexType numItems, int allocatorID = 0,
+      double tolerance = Base::DEFAULT_TOLERANCE, double scaleFactor = Base::DEFAULT_SCALE_FACTOR) {
+    (void)allocatorID;
+    this->m_tolerance = tolerance;
+    this->m_scaleFactor = scaleFactor;
+    this->initialize(boxes, numItems);
+  }
 };
 
 }  // namespace spin

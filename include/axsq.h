@@ -134,6 +134,19 @@ int axsq_bvh_find(const axsq_bvh* bvh, int kind, const double* x, const double* 
 int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const double* z,
                     int64_t nverts, const int32_t* tris, int64_t ntris, int watertight,
                     int compute_sign, int fp_mode, void* stream);
+/* The same for a surface mesh of linear triangle and quad cells (a
+ * mint::UnstructuredMesh of TRIANGLE or QUAD, SINGLE_SHAPE or MIXED_SHAPE;
+ * SignedDistance.hpp:609-660, SignedDistance.cpp:16-45). conn: device int32
+ * node ids; cell_offsets: ncells + 1 int32 offsets into conn (host or device),
+ * each cell 3 or 4 nodes. A quad (n0, n1, n2, n3) is checked as the
+ * triangles (n0, n1, n2) then (n0, n2, n3). A mesh of triangles only takes
+ * the axsq_sdf_create path; one with quads walks the tree in one pass on FP64
+ * nodes (fp_mode is then ignored) and has no adjoint (axsq_sdf_backward
+ * returns AXSQ_ERR_INVALID_ARG). */
+int axsq_sdf_create_cells(axsq_sdf** out, const double* x, const double* y, const double* z,
+                          int64_t nverts, const int32_t* conn, const int32_t* cell_offsets,
+                          int64_t ncells, int watertight, int compute_sign, int fp_mode,
+                          void* stream);
 int axsq_sdf_destroy(axsq_sdf* sdf);
 /* Mesh bounding box (signed_distance_get_mesh_bounds,
  * quest/interface/signed_distance.cpp:468-488). Host outputs. */

@@ -883,28 +883,38 @@ int64_t orc_bvh_brute(const orc_bvh* t, int kind, const double* prims, int64_t n
 
 struct orc_sdf
 {
-  int32_t nverts, ntris;
+  int32_t nverts, ntris; /* ntris: cells */
   double *x, *y, *z;
   int32_t* conn;
+  int32_t* off; /* cell offsets into conn (NULL: triangles, 3 nodes each) */
   int watertight, compute_sign;
   orc_box domain; /* m_boxDomain */
   orc_bvh* bvh;
 };
 
-orc_sdf* orc_sdf_new(const double* x, const double* y, const double* z, int32_t nverts,
-                     const int32_t* conn, int32_t ntris, int watertight, int compute_sign)
+/* off: ncells + 1 offsets into conn (cells of 3 or 4 nodes), or NULL for
+ * triangles (SignedDistance.cpp:16-45 reads either mint mesh layout). */
+static orc_sdf* sdf_new(const double* x, const double* y, const double* z, int32_t nverts,
+                        const int32_t* conn, const int32_t* off, int32_t ntris, int watertight,
+                        int compute_sign)
 {
   orc_sdf* s = (orc_sdf*)calloc(1, sizeof(orc_sdf));
   s->nverts = nverts;
   s->ntris = ntris;
+  const size_t nconn = off ? (size_t)off[ntris] : 3 * (size_t)ntris;
   s->x = (double*)malloc(sizeof(double) * nverts);
   s->y = (double*)malloc(sizeof(double) * nverts);
   s->z = (double*)malloc(sizeof(double) * nverts);
-  s->conn = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)ntris);
+  s->conn = (int32_t*)malloc(sizeof(int32_t) * (nconn ? nconn : 1));
   memcpy(s->x, x, sizeof(double) * nverts);
   memcpy(s->y, y, sizeof(double) * nverts);
   memcpy(s->z, z, sizeof(double) * nverts);
-  memcpy(s->conn, conn, sizeof(int32_t) * 3 * (size_t)ntris);
+  memcpy(s->conn, conn, sizeof(int32_t) * nconn);
+  if(off)
+  {
+    s->off = (int32_t*)malloc(sizeof(int32_t) * ((size_t)ntris + 1));
+    memcpy(s->off, off, sizeof(int32_t) * ((size_t)ntris + 1));
+  }
   s->watertight = watertight;
   s->compute_sign = compute_sign;
   /* setMesh :470-475 (SEQ: addPoint over nodes) */
@@ -920,9 +930,10 @@ orc_sdf* orc_sdf_new(const double* x, const double* y, const double* z, int32_t 
   {
     orc_box b;
     box_clear(&b);
-    for(int k = 0; k < 3; ++k)
+    const int32_t b0 = s->off ? s->off[c] : 3 * c, b1 = s->off ? s->off[c + 1] : 3 * c + 3;
+    for(int32_t k = b0; k < b1; ++k)
     {
-      const int32_t v = conn[3 * (size_t)c + k];
+      const int32_t v = s->conn[k];
       const double p[3] = {x[v], y[v], z[v]};
       box_add_point_d(&b, p, 3);
     }
@@ -935,6 +946,19 @@ orc_sdf* orc_sdf_new(const double* x, const double* y, const double* z, int32_t 
   return s;
 }
 
+orc_sdf* orc_sdf_new(const double* x, const double* y, const double* z, int32_t nverts,
+                     const int32_t* conn, int32_t ntris, int watertight, int compute_sign)
+{
+  return sdf_new(x, y, z, nverts, conn, NULL, ntris, watertight, compute_sign);
+}
+
+orc_sdf* orc_sdf_new_cells(const double* x, const double* y, const double* z, int32_t nverts,
+                           const int32_t* conn, const int32_t* off, int32_t ncells,
+                           int watertight, int compute_sign)
+{
+  return sdf_new(x, y, z, nverts, conn, off, ncells, watertight, compute_sign);
+}
+
 void orc_sdf_free(orc_sdf* s)
 {
   if(!s) return;
@@ -942,6 +966,7 @@ void orc_sdf_free(orc_sdf* s)
   free(s->y);
   free(s->z);
   free(s->conn);
+  free(s->off);
   orc_bvh_free(s->bvh);
   free(s);
 }
@@ -963,6 +988,7 @@ typedef struct
   int minType;
   int minLoc;
   int32_t minElem;
+  int minSub; /* which triangle of the winning cell (minTri) */
   double sumN[3];
   int minCount;
   const orc_sdf* s;
@@ -972,9 +998,17 @@ typedef struct
 
 static inline int loc_type(int loc) { return (loc < 0) ? 1 : (loc < 3) ? 0 : 2; }
 
-static inline void tri_points(const orc_sdf* s, int32_t c, double A[3], double B[3], double C[3])
+/* Triangle sub of cell c: (n0, n1, n2), or (n0, n2, n3) for sub 1 of a quad
+ * (SignedDistance.hpp:649-658). */
+static inline int cell_tris(const orc_sdf* s, int32_t c)
 {
-  const int32_t* v = s->conn + 3 * (size_t)c;
+  return (s->off && s->off[c + 1] - s->off[c] == 4) ? 2 : 1;
+}
+static inline void tri_points(const orc_sdf* s, int32_t c, int sub, double A[3], double B[3],
+                              double C[3])
+{
+  const int32_t* n = s->conn + (s->off ? (size_t)s->off[c] : 3 * (size_t)c);
+  const int32_t v[3] = {n[0], n[sub ? 2 : 1], n[sub ? 3 : 2]};
   A[0] = s->x[v[0]];
   A[1] = s->y[v[0]];
   A[2] = s->z[v[0]];
@@ -1010,12 +1044,12 @@ static inline double tri_angle(const double* P[3], int idx)
   return acos(dp);
 }
 
-/* checkCandidate, SignedDistance.hpp:637-737 (triangles only). */
-static void check_candidate(min_cand* m, int32_t cell)
+/* checkCandidate, SignedDistance.hpp:637-737, for one triangle of the cell */
+static void check_triangle(min_cand* m, int32_t cell, int sub)
 {
   const double EPS = 1e-12;
   double A[3], B[3], C[3], cp[3];
-  tri_points(m->s, cell, A, B, C);
+  tri_points(m->s, cell, sub, A, B, C);
   const int loc = orc_closest_point(m->q, A, B, C, EPS, cp);
   const double sq = sqdist_pp(m->q, cp);
   const int type = loc_type(loc);
@@ -1030,6 +1064,7 @@ static void check_candidate(min_cand* m, int32_t cell)
     m->minType = type;
     m->minLoc = loc;
     m->minElem = cell;
+    m->minSub = sub;
     if(cn && clear)
     {
       m->sumN[0] = m->sumN[1] = m->sumN[2] = 0.0;
@@ -1066,6 +1101,13 @@ static void check_candidate(min_cand* m, int32_t cell)
   }
 }
 
+/* the cell's one or two triangles, in order */
+static void check_candidate(min_cand* m, int32_t cell)
+{
+  const int nt = cell_tris(m->s, cell);
+  for(int sub = 0; sub < nt; ++sub) check_triangle(m, cell, sub);
+}
+
 static void sdf_leaf(int32_t pos, const int32_t* leafs, void* ctx)
 {
   min_cand* m = (min_cand*)ctx;
@@ -1086,7 +1128,7 @@ static inline void surface_normal(const min_cand* m, double n[3])
   if(m->minType == 2)
   {
     double A[3], B[3], C[3];
-    tri_points(m->s, m->minElem, A, B, C);
+    tri_points(m->s, m->minElem, m->minSub, A, B, C);
     tri_normal(A, B, C, n);
   }
   else
@@ -1197,9 +1239,10 @@ static void* brute_worker(void* arg)
     double best = DBL_MAX;
     int32_t be = -1;
     for(int32_t c = 0; c < s->ntris; ++c)
+      for(int sub = 0; sub < cell_tris(s, c); ++sub)
     {
       double A[3], B[3], C[3], cp[3];
-      tri_points(s, c, A, B, C);
+      tri_points(s, c, sub, A, B, C);
       orc_closest_point(q, A, B, C, 1e-12, cp);
       const double d = sqdist_pp(q, cp);
       if(d < best)
@@ -1315,6 +1358,7 @@ static int cmp_pair(const void* a, const void* b)
 int orc_sdf_backward(const orc_sdf* s, const double* q, int64_t nq, const double* dist,
                      const int32_t* elem, const float* gout, double* grad_q, double* grad_v)
 {
+  if(s->off) return -1; /* the adjoint is for triangle meshes */
   const int32_t nt = s->ntris, nv = s->nverts;
   /* per-query direction u = sgn*(q-cp)/|q-cp| scaled by g */
   double* dir = (double*)malloc(sizeof(double) * 3 * (size_t)nq);
@@ -1325,7 +1369,7 @@ int orc_sdf_backward(const orc_sdf* s, const double* q, int64_t nq, const double
     const int32_t c = elem[i];
     double A[3], B[3], C[3], cp[3], w[3];
     int loc;
-    tri_points(s, c, A, B, C);
+    tri_points(s, c, 0, A, B, C);  /* the adjoint is for triangle meshes */
     cp_weights(P, A, B, C, cp, w, &loc);
     const double sgn = signbit(dist[i]) ? -1.0 : 1.0;
     double r[3] = {P[0] - cp[0], P[1] - cp[1], P[2] - cp[2]};

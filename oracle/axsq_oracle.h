@@ -91,6 +91,11 @@ int64_t orc_bvh_brute(const orc_bvh* t, int kind, const double* prims, int64_t n
 typedef struct orc_sdf orc_sdf;
 orc_sdf* orc_sdf_new(const double* x, const double* y, const double* z, int32_t nverts,
                      const int32_t* conn, int32_t ntris, int watertight, int compute_sign);
+/* A mesh of triangle and quad cells: off = ncells + 1 offsets into conn
+ * (3 or 4 nodes per cell); a quad is checked as (n0,n1,n2) then (n0,n2,n3). */
+orc_sdf* orc_sdf_new_cells(const double* x, const double* y, const double* z, int32_t nverts,
+                           const int32_t* conn, const int32_t* off, int32_t ncells,
+                           int watertight, int compute_sign);
 void orc_sdf_free(orc_sdf* s);
 const orc_bvh* orc_sdf_bvh(const orc_sdf* s);
 void orc_sdf_bounds(const orc_sdf* s, double lo[3], double hi[3]);

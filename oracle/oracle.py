@@ -55,6 +55,9 @@ def lib():
         L.orc_sdf_new.argtypes = [_dp, _dp, _dp, ctypes.c_int32, _i32p, ctypes.c_int32,
                                   ctypes.c_int, ctypes.c_int]
         L.orc_sdf_free.argtypes = [ctypes.c_void_p]
+        L.orc_sdf_new_cells.restype = ctypes.c_void_p
+        L.orc_sdf_new_cells.argtypes = [_dp, _dp, _dp, ctypes.c_int32, _i32p, _i32p,
+                                        ctypes.c_int32, ctypes.c_int, ctypes.c_int]
         L.orc_sdf_bvh.restype = ctypes.c_void_p
         L.orc_sdf_bvh.argtypes = [ctypes.c_void_p]
         L.orc_sdf_bounds.argtypes = [ctypes.c_void_p, _dp, _dp]
@@ -254,14 +257,26 @@ class OracleBVH:
 class OracleSDF:
     """quest::SignedDistance<3, SEQ_EXEC> restated, plus the adjoint."""
 
-    def __init__(self, verts, tris, watertight=True, compute_sign=True):
+    def __init__(self, verts, tris, watertight=True, compute_sign=True, cell_offsets=None):
+        """tris: (T, 3) triangles; or, with cell_offsets (ncells + 1), the flat
+        connectivity of cells of 3 or 4 nodes (triangles and quads)."""
         v = f64(verts).reshape(-1, 3)
-        t = i32(tris).reshape(-1, 3)
         self.nverts = v.shape[0]
-        self.ntris = t.shape[0]
         x, y, z = f64(v[:, 0]), f64(v[:, 1]), f64(v[:, 2])
-        self.h = lib().orc_sdf_new(_p(x, _dp), _p(y, _dp), _p(z, _dp), self.nverts,
-                                   _p(t, _i32p), self.ntris, int(watertight), int(compute_sign))
+        self._xyz = (x, y, z)
+        if cell_offsets is None:
+            t = i32(tris).reshape(-1, 3)
+            self.ntris = t.shape[0]
+            self.h = lib().orc_sdf_new(_p(x, _dp), _p(y, _dp), _p(z, _dp), self.nverts,
+                                       _p(t, _i32p), self.ntris, int(watertight),
+                                       int(compute_sign))
+        else:
+            t = i32(tris).reshape(-1)
+            off = i32(cell_offsets)
+            self.ntris = off.shape[0] - 1
+            self.h = lib().orc_sdf_new_cells(_p(x, _dp), _p(y, _dp), _p(z, _dp), self.nverts,
+                                             _p(t, _i32p), _p(off, _i32p), self.ntris,
+                                             int(watertight), int(compute_sign))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -3,6 +3,7 @@
 //   spin/tests/spin_bvh.cpp:279-404 (findBoundingBoxes, 18 cells),
 //   :519-648 (findRays, 15 cells), :792-888 (findPoints, centroid -> cell)
 //   quest/tests/quest_signed_distance.cpp:88-170 (sphere l1/l2/linf)
+//   spin/tests/spin_bvh.cpp:409-517, :651-790, :906-1006 (the 2-D versions)
 //   quest/tests/quest_meshtester.cpp:268-305 (caved-in tetrahedron, with and
 //   without degenerate triangles; meshes of quest_test_utilities.hpp:353-442)
 // Exit status 0 when every check passes.
@@ -217,8 +218,63 @@ static void test_mesh_tester_caved_tet() {
   }
 }
 
+// spin::BVH<2>: 3 x 3 unit cells over [0,3]^2 (i fastest). The box
+// [-1,2.5] x [-1,1.5] meets cells 0-5, the ray from (-1,-1) along (1,1) meets
+// all but cells 2 and 6, and each cell centroid finds its own cell.
+static void test_bvh2d_grid() {
+  using BVH2 = axom::spin::BVH<2, axom::MI355X_EXEC>;
+  using Box2 = axom::primal::BoundingBox2;
+  std::vector<Box2> cells;
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) cells.push_back(Box2{{1.0 * i, 1.0 * j}, {i + 1.0, j + 1.0}});
+  Box2* dcells = to_device(cells);
+  BVH2 bvh;
+  bvh.setScaleFactor(1.0);
+  bvh.initialize(dcells, 9);
+  const Box2 bb = bvh.getBounds();
+  EXPECT_TRUE(bb.min[0] == 0.0 && bb.min[1] == 0.0 && bb.max[0] == 3.0 && bb.max[1] == 3.0);
+  std::vector<Box2> q = {Box2{{-1.0, -1.0}, {2.5, 1.5}}, Box2{{-1.0, -1.0}, {-0.1, -0.1}}};
+  Box2* dq = to_device(q);
+  {
+    axom::Array<IndexType> off(2), cnt(2), cand;
+    bvh.findBoundingBoxes(off.view(), cnt.view(), cand, 2, dq);
+    auto c = to_host(cnt.data(), 2);
+    auto ids = to_host(cand.data(), (size_t)cand.size());
+    EXPECT_TRUE(c[0] == 6 && c[1] == 0);
+    EXPECT_TRUE((std::set<IndexType>(ids.begin(), ids.end()) == std::set<IndexType>{0, 1, 2, 3, 4, 5}));
+  }
+  {
+    const double o[2] = {-1.0, -1.0}, d0[2] = {1.0, 1.0}, d1[2] = {-1.0, -1.0};
+    std::vector<axom::primal::Ray2> rays = {axom::primal::make_ray2(o, d0),
+                                            axom::primal::make_ray2(o, d1)};
+    auto* dr = to_device(rays);
+    axom::Array<IndexType> off(2), cnt(2), cand;
+    bvh.findRays(off.view(), cnt.view(), cand, 2, dr);
+    auto c = to_host(cnt.data(), 2);
+    auto ids = to_host(cand.data(), (size_t)cand.size());
+    EXPECT_TRUE(c[0] == 7 && c[1] == 0);
+    EXPECT_TRUE((std::set<IndexType>(ids.begin(), ids.end()) == std::set<IndexType>{0, 1, 3, 4, 5, 7, 8}));
+    (void)hipFree(dr);
+  }
+  {
+    std::vector<axom::primal::Point2> pts;
+    for (auto& c : cells) pts.push_back({{0.5 * (c.min[0] + c.max[0]), 0.5 * (c.min[1] + c.max[1])}});
+    auto* dp = to_device(pts);
+    axom::Array<IndexType> off(9), cnt(9), cand;
+    bvh.findPoints(off.view(), cnt.view(), cand, 9, dp);
+    auto c = to_host(cnt.data(), 9);
+    auto ids = to_host(cand.data(), (size_t)cand.size());
+    for (int k = 0; k < 9; ++k) EXPECT_EQ(c[k], 1);
+    for (int k = 0; k < 9 && k < (int)ids.size(); ++k) EXPECT_EQ(ids[k], k);
+    (void)hipFree(dp);
+  }
+  (void)hipFree(dcells);
+  (void)hipFree(dq);
+}
+
 int main() {
   test_bvh_grid();
+  test_bvh2d_grid();
   test_sdf_sphere();
   test_mesh_tester_caved_tet();
   std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "PASSED", g_fail);
