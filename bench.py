@@ -57,7 +57,7 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1_000_000)
     p.add_argument("--workload", default="sdf_l6",
                    choices=["sdf_l6", "find_points", "torus_mixed", "diff_l7", "shapefit_l8",
-                            "self_isect_l8"],
+                            "self_isect_l8", "find_points_2d", "sdf_quads"],
                    help="sdf_l6 is the headline; the others print their own JSON line")
     return p.parse_args()
 
@@ -124,6 +124,49 @@ def other_workload(args, torch, dev):
         return out
     if args.workload == "self_isect_l8":
         return self_isect_workload(args, torch, dev, sync_time)
+    if args.workload == "find_points_2d":
+        rng = np.random.default_rng(20)
+        c = rng.random((100_000, 2))
+        h = rng.uniform(0.001, 0.01, (100_000, 2))
+        boxes = np.concatenate([c - h, c + h], 1)
+        pts = rng.random((1_000_000, 2))
+        bx, p = torch.as_tensor(boxes, device=dev), torch.as_tensor(pts, device=dev)
+        bvh = spin.BVH(bx, NDIMS=2)
+        off = torch.empty(p.shape[0], dtype=torch.int32, device=dev)
+        cnt = torch.empty(p.shape[0], dtype=torch.int32, device=dev)
+        tb = sync_time(lambda: bvh.initialize(bx), args.steps)
+        tq = sync_time(lambda: bvh.findPoints(off, cnt, p.shape[0], p), args.steps)
+        return {"workload": "2-D: 100,000 boxes U[0,1]^2 (half-extents U[0.001,0.01]), "
+                            "1,000,000 points, FP64 build + findPoints (spin::BVH<2>)",
+                "build_mprims_per_s": boxes.shape[0] / tb / 1e6, "build_ms": tb * 1e3,
+                "find_points_queries_per_s": p.shape[0] / tq, "find_ms": tq * 1e3}
+    if args.workload == "sdf_quads":
+        v, quads = synth.cube_sphere(128)
+        conn = quads.reshape(-1)
+        off = np.arange(0, conn.size + 1, 4, dtype=np.int32)
+        q = synth.uniform(synth.SEED0 + 7, 0, 3 * args.queries, -1.5, 1.5).reshape(-1, 3)
+        vd, cd, od, qd = (torch.as_tensor(a, device=dev) for a in (v, conn, off, q))
+        sdf = quest.SignedDistance(vd, cd, cell_offsets=od)
+        n = q.shape[0]
+        d = torch.empty(n, dtype=torch.float64, device=dev)
+        tq = sync_time(lambda: sdf.computeDistances(n, qd, outSgnDist=d), max(1, args.steps // 2))
+        out = {"workload": f"quad sphere (cube subdivided 128 x 128 per face, projected: "
+                           f"{quads.shape[0]:,} quads), {n:,} queries U[-1.5,1.5]^3; "
+                           "SignedDistance forward over quad cells",
+               "quads": int(quads.shape[0]), "queries_per_s": n / tq, "ms": tq * 1e3}
+        if not args.no_cpu:
+            from oracle import oracle as orc
+            m = min(n, 200_000)
+            nthreads = min(16, os.cpu_count() or 1)
+            o = orc.OracleSDF(v, conn, cell_offsets=off)
+            t0 = time.perf_counter()
+            r = o.eval(q[:m], nthreads=nthreads)
+            tc = time.perf_counter() - t0
+            out["cpu_baseline"] = {"queries_per_s": m / tc, "cores": nthreads, "kind": "port",
+                                   "sample": f"first {m:,} queries",
+                                   "matches_gpu_bitwise": bool(np.array_equal(
+                                       r["dist"], d[:m].cpu().numpy()))}
+        return out
     v, t, q, target = synth.workload5()
     vd, td, qd, tg = (torch.as_tensor(a, device=dev) for a in (v, t, q, target))
     _, hist = shapefit.shape_fit(vd, td, qd, tg, iterations=20, lr=1e-3, fp_mode="fp32")
