@@ -42,6 +42,7 @@ _SIGS = {
     "axsq_bvh_destroy": (_i32, [_vp]),
     "axsq_bvh_build2d": (_i32, [ctypes.POINTER(_vp), _vp, _i64, _dbl, _i32, _vp]),
     "axsq_bvh_dimension": (_i32, [_vp]),
+    "axsq_bvh_rebuild": (_i32, [_vp, _vp, _i64, _dbl, _i32, _vp]),
     "axsq_bvh_bounds": (_i32, [_vp, _dbl3, _dbl3]),
     "axsq_bvh_num_leaves": (_i64, [_vp]),
     "axsq_bvh_export": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),

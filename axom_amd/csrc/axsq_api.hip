@@ -286,6 +286,38 @@ int axsq_bvh_build2d(axsq_bvh** out, const double* boxes, int64_t n, double scal
   return AXSQ_OK;
 }
 
+int axsq_bvh_rebuild(axsq_bvh* b, const double* boxes, int64_t n, double scale, int fp_mode,
+                     void* stream) {
+  clear_error();
+  if (!b) {
+    set_error("axsq_bvh_rebuild: bvh is NULL");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (fp_mode != AXSQ_FP64 && fp_mode != AXSQ_FP32_TRAVERSAL) {
+    set_error("axsq_bvh_rebuild: unknown fp_mode %d", fp_mode);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (n < 0 || (n > 0 && !boxes)) {
+    set_error("axsq_bvh_rebuild: need n >= 0 boxes");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  int dev = -1;
+  AXSQ_CHECK_HIP(hipGetDevice(&dev));
+  if (dev != b->device) {
+    set_error("axsq_bvh_rebuild: the tree lives on device %d, the current device is %d",
+              b->device, dev);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (b->dims == 2) {
+    Workspace& ws = workspace();
+    AXSQ_TRY(ws.box2d.ensure(sizeof(double) * 6 * (size_t)(n > 0 ? n : 1)));
+    AXSQ_TRY(widen_boxes2d(boxes, n, ws.box2d.as<double>(), s));
+    return bvh_build_into(b, ws.box2d.as<double>(), n, scale, fp_mode, s, 2);
+  }
+  return bvh_build_into(b, boxes, n, scale, fp_mode, s, 3);
+}
+
 int axsq_bvh_dimension(const axsq_bvh* b) { return b ? b->dims : 0; }
 
 int axsq_bvh_destroy(axsq_bvh* b) {

@@ -162,12 +162,17 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const double* __restrict__ b
 }
 
 // build_tree, build_radix_tree.hpp:290-384
+// The same launch clears the refit's counters (zero_a[0, n_a), zero_b[0, n_b)).
 __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ mc, int32_t inner,
                                                     int32_t* __restrict__ lchild,
                                                     int32_t* __restrict__ rchild,
                                                     int32_t* __restrict__ parents,
-                                                    int32_t* __restrict__ range) {
+                                                    int32_t* __restrict__ range,
+                                                    int32_t* __restrict__ zero_a, int32_t n_a,
+                                                    int32_t* __restrict__ zero_b, int32_t n_b) {
   const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n_a) zero_a[i] = 0;
+  if (i < n_b) zero_b[i] = 0;
   if (i >= inner) return;
   const int32_t d =
       (karras_delta(i, i + 1, inner, mc) - karras_delta(i, i - 1, inner, mc)) < 0 ? -1 : 1;
@@ -220,16 +225,25 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
 // Karras tree covers a contiguous leaf range and its index is that range's
 // low or high end, so the nodes whose range lies inside a span of leaves form
 // subtrees of that span.
-//   k_refit_chunks  spans of kBlock leaves: counters and boxes in LDS
-//   k_refit_span    spans of kRefitSpan leaves, one workgroup each, then one
-//                   workgroup for the rest of the tree: global counters and
-//                   boxes, workgroup-scope hand-off
+//   k_refit_chunks  spans of kBlock leaves: links, counters and boxes in LDS
+//   k_refit_span    spans of kRefitSpan leaves, one workgroup each (counters
+//                   in LDS), then one workgroup for the rest of the tree
+//                   (global counters); boxes in global memory, workgroup-scope
+//                   hand-off
 // A thread that reaches a node whose range leaves its span records the child
 // it comes from (an exit) for the next tier and stops. No thread waits on
 // another, and every thread stops after at most depth-many steps.
 constexpr int kRefitThreads = 1024;
 constexpr int64_t kRefitSpan = 16384;  // a multiple of kBlock
 
+constexpr int kChunksPerSpan = (int)(kRefitSpan / kBlock);
+static_assert(kChunksPerSpan == 64, "one wave scans the exit counts of a span's chunks");
+
+// The chunk's own inner nodes (indices [base, base + kBlock)) and leaves are
+// staged in LDS, and the climb runs in LDS alone: the boxes it makes are
+// written out afterwards in one coalesced pass, and the chunk's exits go to
+// its own kBlock slots of `exits`, with their number in n_exits[chunk] (a
+// chunk's exits root disjoint sets of its leaves, so there are at most kBlock).
 __global__ __launch_bounds__(kBlock) void k_refit_chunks(const double* __restrict__ leaf_boxes,
                                                           int32_t inner,
                                                           const int32_t* __restrict__ lchild,
@@ -240,75 +254,159 @@ __global__ __launch_bounds__(kBlock) void k_refit_chunks(const double* __restric
                                                           int32_t* __restrict__ exits,
                                                           int32_t* __restrict__ n_exits) {
   __shared__ int32_t s_cnt[kBlock];
+  __shared__ int32_t s_lo[kBlock], s_hi[kBlock], s_lc[kBlock], s_rc[kBlock], s_par[kBlock];
+  __shared__ int32_t s_ex[kBlock];
+  __shared__ int32_t s_nex;
   __shared__ double s_box[6][kBlock];
+  __shared__ double s_leaf[6][kBlock];
+  const int32_t t = (int32_t)threadIdx.x;
   const int32_t base = (int32_t)(blockIdx.x * kBlock);
   const int32_t top = base + kBlock - 1;  // the chunk is leaves [base, top]
-  s_cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const int32_t i = base + (int32_t)threadIdx.x;
-  if (i > inner) return;
-  double acc[6];
-  load_box(leaf_boxes + 6 * (int64_t)i, acc);
-  int32_t last = inner + i;
-  int32_t cur = parents[inner + i];
-  while (cur != -1) {
-    if (range[3 * cur] < base || range[3 * cur + 1] > top) {
-      const int64_t g = i / kRefitSpan;  // the next tier's span (kRefitSpan slots each)
-      exits[g * kRefitSpan + atomicAdd(&n_exits[g], 1)] = last;
-      break;
-    }
-    if (__hip_atomic_fetch_add(&s_cnt[cur - base], 1, __ATOMIC_ACQ_REL,
-                               __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
-      break;
-    const int32_t lc = lchild[cur], rc = rchild[cur];
-    const int32_t other = (lc == last) ? rc : lc;  // a chunk node or a leaf
-    double ob[6];
-    if (other >= inner)
-      load_box(leaf_boxes + 6 * (int64_t)(other - inner), ob);
-    else
-      for (int k = 0; k < 6; ++k) ob[k] = s_box[k][other - base];
-    box_add_box(acc, ob);
-    for (int k = 0; k < 6; ++k) s_box[k][cur - base] = acc[k];
-    store_box(inner_boxes + 6 * (int64_t)cur, acc);
-    last = cur;
-    cur = parents[cur];
+  const int32_t i = base + t;             // leaf i and inner node i
+  s_cnt[t] = 0;
+  if (t == 0) s_nex = 0;
+  if (i < inner) {
+    s_lo[t] = range[3 * i];
+    s_hi[t] = range[3 * i + 1];
+    s_lc[t] = lchild[i];
+    s_rc[t] = rchild[i];
+    s_par[t] = parents[i];
   }
+  double acc[6];
+  int32_t cur = -1;
+  if (i <= inner) {
+    load_box(leaf_boxes + 6 * (int64_t)i, acc);
+    for (int k = 0; k < 6; ++k) s_leaf[k][t] = acc[k];
+    cur = parents[inner + i];
+  }
+  __syncthreads();
+  if (i <= inner) {
+    int32_t last = inner + i;
+    while (cur != -1) {
+      // a node outside [base, top] has its range's low or high end there
+      const int32_t c = cur - base;
+      if (c < 0 || c >= kBlock || s_lo[c] < base || s_hi[c] > top) {
+        s_ex[atomicAdd(&s_nex, 1)] = last;
+        break;
+      }
+      if (__hip_atomic_fetch_add(&s_cnt[c], 1, __ATOMIC_ACQ_REL,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+        break;
+      const int32_t lc = s_lc[c], rc = s_rc[c];
+      const int32_t other = (lc == last) ? rc : lc;  // a chunk node or a chunk leaf
+      double ob[6];
+      if (other >= inner)
+        for (int k = 0; k < 6; ++k) ob[k] = s_leaf[k][other - inner - base];
+      else
+        for (int k = 0; k < 6; ++k) ob[k] = s_box[k][other - base];
+      box_add_box(acc, ob);
+      for (int k = 0; k < 6; ++k) s_box[k][c] = acc[k];
+      last = cur;
+      cur = s_par[c];
+    }
+  }
+  __syncthreads();
+  if (i < inner && s_lo[t] >= base && s_hi[t] <= top) {  // one of the chunk's own nodes
+    double b[6];
+    for (int k = 0; k < 6; ++k) b[k] = s_box[k][t];
+    store_box(inner_boxes + 6 * (int64_t)i, b);
+  }
+  const int32_t ne = s_nex;
+  for (int32_t j = t; j < ne; j += kBlock) exits[(int64_t)blockIdx.x * kBlock + j] = s_ex[j];
+  if (t == 0) n_exits[blockIdx.x] = ne;
 }
 
-// Workgroup g climbs from the exits in exits_in[g * cap_in ...] through the
-// nodes whose range lies in leaves [g * span, (g + 1) * span - 1]; the rest go
-// to exits_out (one list). With span covering every leaf there are none.
-// Boxes below the exits were stored by the previous tier (visible across the
-// kernel boundary); within the tier the hand-off is between waves of one
-// workgroup.
+// Above the chunks. FROM_CHUNKS: workgroup g climbs from the exits of chunks
+// [g * kChunksPerSpan, (g + 1) * kChunksPerSpan) (kBlock slots per chunk,
+// counts per chunk, numbered across the span by a scan of the counts) through the nodes whose
+// range lies in leaves [g * kRefitSpan, (g + 1) * kRefitSpan - 1], with the
+// arrival counters in LDS (a node of the span has its index in the span), and
+// records where it leaves the span in exits_out (one list, count *n_out).
+// Otherwise one workgroup climbs from that list to the root with global
+// counters[node]. Boxes below the exits were stored by the previous kernel;
+// within a kernel the hand-off is between waves of one workgroup. A node's
+// links and range are loaded one level ahead, while the hand-off at the node
+// below is in flight.
+struct RefitTopo {
+  int32_t lo, hi, lc, rc, par;
+};
+
+__device__ __forceinline__ RefitTopo refit_topo(const int32_t* __restrict__ lchild,
+                                                const int32_t* __restrict__ rchild,
+                                                const int32_t* __restrict__ parents,
+                                                const int32_t* __restrict__ range, int32_t c) {
+  const int32_t k = c < 0 ? 0 : c;  // the root's parent (-1): a harmless read
+  return RefitTopo{range[3 * k], range[3 * k + 1], lchild[k], rchild[k], parents[k]};
+}
+
+template <bool FROM_CHUNKS>
 __global__ __launch_bounds__(kRefitThreads) void k_refit_span(
     const double* __restrict__ leaf_boxes, int32_t inner, const int32_t* __restrict__ lchild,
     const int32_t* __restrict__ rchild, const int32_t* __restrict__ parents,
-    const int32_t* __restrict__ range, int64_t span, const int32_t* __restrict__ exits_in,
-    const int32_t* __restrict__ n_in, int64_t cap_in, int32_t* __restrict__ exits_out,
+    const int32_t* __restrict__ range, const int32_t* __restrict__ exits_in,
+    const int32_t* __restrict__ n_in, int64_t nchunks, int32_t* __restrict__ exits_out,
     int32_t* __restrict__ n_out, int32_t* counters, double* inner_boxes) {
-  const int64_t g = blockIdx.x;
-  const int64_t lo = g * span, hi = lo + span - 1;
-  const int32_t ne = n_in[g];
-  const int32_t* ex = exits_in + g * cap_in;
-  for (int32_t e = (int32_t)threadIdx.x; e < ne; e += kRefitThreads) {
-    int32_t last = ex[e];
+  __shared__ int32_t s_cnt[FROM_CHUNKS ? kRefitSpan : 1];
+  __shared__ int32_t s_off[kChunksPerSpan + 1];  // exits of the span's chunks, scanned
+  const int32_t t = (int32_t)threadIdx.x;
+  int64_t lo = 0, hi = (int64_t)inner;  // leaves of the span
+  int32_t ne;
+  const int64_t c0 = (int64_t)blockIdx.x * kChunksPerSpan;
+  if constexpr (FROM_CHUNKS) {
+    lo = (int64_t)blockIdx.x * kRefitSpan;
+    hi = lo + kRefitSpan - 1;
+    for (int32_t j = t; j < (int32_t)kRefitSpan; j += kRefitThreads) s_cnt[j] = 0;
+    if (t < kChunksPerSpan) {  // one wave: inclusive scan of the chunks' exit counts
+      int32_t v = (c0 + t < nchunks) ? n_in[c0 + t] : 0;
+      for (int off = 1; off < kChunksPerSpan; off <<= 1) {
+        const int32_t y = __shfl_up(v, off, kChunksPerSpan);
+        if (t >= off) v += y;
+      }
+      s_off[t + 1] = v;
+      if (t == 0) s_off[0] = 0;
+    }
+    __syncthreads();
+    ne = s_off[kChunksPerSpan];
+  } else {
+    ne = *n_in;
+  }
+  for (int32_t e = t; e < ne; e += kRefitThreads) {
+    int32_t last;
+    if constexpr (FROM_CHUNKS) {  // the chunk k with s_off[k] <= e < s_off[k + 1]
+      int32_t a = 0, b = kChunksPerSpan;
+      while (b - a > 1) {
+        const int32_t m = (a + b) >> 1;
+        if (s_off[m] <= e)
+          a = m;
+        else
+          b = m;
+      }
+      last = exits_in[(c0 + a) * kBlock + (e - s_off[a])];
+    } else {
+      last = exits_in[e];
+    }
     double acc[6];
     if (last >= inner)
       load_box(leaf_boxes + 6 * (int64_t)(last - inner), acc);
     else
       load_box(inner_boxes + 6 * (int64_t)last, acc);
     int32_t cur = parents[last];
+    RefitTopo tp = refit_topo(lchild, rchild, parents, range, cur);
     while (cur != -1) {
-      if (range[3 * cur] < lo || range[3 * cur + 1] > hi) {
-        if (exits_out) exits_out[atomicAdd(n_out, 1)] = last;  // none in the last tier
+      if (tp.lo < lo || tp.hi > hi) {
+        if (exits_out) exits_out[atomicAdd(n_out, 1)] = last;  // none above the spans
         break;
       }
-      if (__hip_atomic_fetch_add(&counters[cur], 1, __ATOMIC_ACQ_REL,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+      const int32_t nxt = tp.par;
+      const RefitTopo tn = refit_topo(lchild, rchild, parents, range, nxt);
+      int32_t* ctr;
+      if constexpr (FROM_CHUNKS)
+        ctr = &s_cnt[cur - lo];
+      else
+        ctr = &counters[cur];
+      if (__hip_atomic_fetch_add(ctr, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
         break;
-      const int32_t lc = lchild[cur], rc = rchild[cur];
-      const int32_t other = (lc == last) ? rc : lc;
+      const int32_t other = (tp.lc == last) ? tp.rc : tp.lc;
       double ob[6];
       if (other >= inner)
         load_box(leaf_boxes + 6 * (int64_t)(other - inner), ob);
@@ -317,7 +415,8 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_span(
       box_add_box(acc, ob);
       store_box(inner_boxes + 6 * (int64_t)cur, acc);
       last = cur;
-      cur = parents[cur];
+      cur = nxt;
+      tp = tn;
     }
   }
 }
@@ -335,8 +434,10 @@ __global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf
                                                   const int32_t* __restrict__ rchild,
                                                   const int32_t* __restrict__ range,
                                                   Node64* __restrict__ nodes,
-                                                  Node32* __restrict__ nodes32) {
+                                                  Node32* __restrict__ nodes32,
+                                                  int32_t* __restrict__ status) {
   const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i == 0) *status = 0;  // the tree's error flag starts clear
   if (i >= inner) return;
   Node64 nd;
   int32_t lc = lchild[i], rc = rchild[i];
@@ -515,11 +616,12 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   const size_t off_c = off_p + sizeof(int32_t) * (2 * n - 1);
   const size_t off_rg = off_c + sizeof(int32_t) * inner;
   const size_t off_ib = (off_rg + sizeof(int32_t) * 3 * inner + 255) & ~size_t(255);
-  // refit exits: per-span counts of the first tier, one count for the second,
-  // then the two exit lists
+  // refit exits: per-chunk counts of the first tier, one count for the
+  // second, then the exit lists (kBlock slots per chunk; one list)
+  const int64_t nchunks = (n + kBlock - 1) / kBlock;
   const int64_t nspans = (n + kRefitSpan - 1) / kRefitSpan;
   const size_t off_ex = off_ib + sizeof(double) * 6 * inner;
-  const size_t total = off_ex + sizeof(int32_t) * (nspans + 1 + nspans * kRefitSpan + n);
+  const size_t total = off_ex + sizeof(int32_t) * (nchunks + 1 + nchunks * kBlock + n);
   AXSQ_TRY(ws.flag.ensure(total));
   char* base = ws.flag.as<char>();
   double* scaled = reinterpret_cast<double*>(base);
@@ -532,9 +634,9 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   int32_t* rng = reinterpret_cast<int32_t*>(base + off_rg);
   double* ib = reinterpret_cast<double*>(base + off_ib);
   int32_t* n_ex0 = reinterpret_cast<int32_t*>(base + off_ex);
-  int32_t* n_ex1 = n_ex0 + nspans;
+  int32_t* n_ex1 = n_ex0 + nchunks;
   int32_t* ex0 = n_ex1 + 1;
-  int32_t* ex1 = ex0 + nspans * kRefitSpan;
+  int32_t* ex1 = ex0 + nchunks * kBlock;
 
   ProfRange pr("bvh_build", s);
   k_transform<<<(unsigned)nparts, kBlock, 0, s>>>(boxes, n_in, n, scale, scaled, partials);
@@ -550,28 +652,28 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   k_reorder<<<grid_for(n), kBlock, 0, s>>>(scaled, sv, sk, n, b->leaf_boxes, b->leaf_ids,
                                            b->mcodes);
   AXSQ_CHECK_HIP(hipGetLastError());
-  k_karras<<<grid_for(inner), kBlock, 0, s>>>(b->mcodes, inner, lch, rch, par, rng);
+  k_karras<<<grid_for(inner), kBlock, 0, s>>>(b->mcodes, inner, lch, rch, par, rng, cnt, inner,
+                                              n_ex1, 1);
   AXSQ_CHECK_HIP(hipGetLastError());
-  AXSQ_TRY(fill_i32(cnt, 0, inner, s));
-  AXSQ_TRY(fill_i32(n_ex0, 0, nspans + 1, s));
   k_refit_chunks<<<grid_for(n), kBlock, 0, s>>>(b->leaf_boxes, inner, lch, rch, par, rng, ib,
                                                 ex0, n_ex0);
   AXSQ_CHECK_HIP(hipGetLastError());
-  k_refit_span<<<(unsigned)nspans, kRefitThreads, 0, s>>>(b->leaf_boxes, inner, lch, rch, par, rng,
-                                                          kRefitSpan, ex0, n_ex0, kRefitSpan, ex1,
-                                                          n_ex1, cnt, ib);
+  k_refit_span<true><<<(unsigned)nspans, kRefitThreads, 0, s>>>(
+      b->leaf_boxes, inner, lch, rch, par, rng, ex0, n_ex0, nchunks, ex1, n_ex1, cnt, ib);
   AXSQ_CHECK_HIP(hipGetLastError());
-  k_refit_span<<<1, kRefitThreads, 0, s>>>(b->leaf_boxes, inner, lch, rch, par, rng,
-                                           nspans * kRefitSpan, ex1, n_ex1, 0, nullptr, nullptr,
-                                           cnt, ib);
+  if (nspans > 1) {  // one span holds the whole tree: no exits left
+    k_refit_span<false><<<1, kRefitThreads, 0, s>>>(b->leaf_boxes, inner, lch, rch, par, rng,
+                                                    ex1, n_ex1, 0, nullptr, nullptr, cnt, ib);
+    AXSQ_CHECK_HIP(hipGetLastError());
+  }
   AXSQ_CHECK_HIP(hipGetLastError());
   k_emit<<<grid_for(inner), kBlock, 0, s>>>(b->leaf_boxes, ib, inner, lch, rch, rng, b->nodes,
-                                            b->nodes32);
+                                            fp_mode == AXSQ_FP32_TRAVERSAL ? b->nodes32 : nullptr,
+                                            b->status);
   AXSQ_CHECK_HIP(hipGetLastError());
   pr.stop();
   AXSQ_CHECK_HIP(hipMemcpyAsync(b->bounds, prm->bounds, sizeof(double) * 6,
                                 hipMemcpyDeviceToHost, s));
-  AXSQ_TRY(fill_i32(b->status, 0, 1, s));
   AXSQ_CHECK_HIP(hipStreamSynchronize(s));
   prof_flush();
   return AXSQ_OK;

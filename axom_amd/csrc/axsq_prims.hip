@@ -1,11 +1,14 @@
 // axsq_prims.hip -- scan and stable radix sort for gfx950 (wave64).
 //
-// Radix sort: least-significant-digit first, 8-bit digits, three kernels per
-// pass (tile histogram, exclusive scan of the digit-major histogram, stable
-// scatter). Inside a tile of 256 threads x 16 keys, keys are ranked in global
-// order: sub-tile k covers [base + 256k, base + 256k + 256) and each 64-lane
-// wave finds the lanes that share its digit with 8 ballots, so equal keys
-// keep their input order and the result equals a stable sort.
+// Radix sort: least-significant-digit first, 8-bit digits; per pass a tile
+// histogram, an exclusive scan of the digit-major histogram and a stable
+// scatter. With at most 256 tiles the scatter does the scan itself (two
+// kernels per pass instead of five), and inputs of at most 262,144 keys use
+// tiles of 1,024 keys instead of 4,096. Inside a tile of 256 threads x ITEMS
+// keys, keys are ranked in global order: sub-tile k covers
+// [base + 256k, base + 256k + 256) and each 64-lane wave finds the lanes that
+// share its digit with 8 ballots, so equal keys keep their input order and the
+// result equals a stable sort.
 #include <stdarg.h>
 #include <stdio.h>
 
@@ -143,6 +146,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(const T* __restrict__ in,
 }
 
 // ---- radix sort -------------------------------------------------------------
+// ITEMS keys per thread make a tile of kBlock * ITEMS keys per workgroup. The
+// histogram is digit-major (hist[digit * nblocks + tile], the order the scan
+// needs) or, for a scatter that sums it itself, tile-major
+// (hist[tile * 256 + digit], so that those sums read it coalesced).
+template <int ITEMS, bool TILE_MAJOR>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys,
                                                         int64_t n, int shift, int nbits,
                                                         uint32_t* __restrict__ hist,
@@ -151,40 +159,78 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restric
   s_h[threadIdx.x] = 0;
   __syncthreads();
   const uint32_t mask = (1u << nbits) - 1u;
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  const int64_t base = (int64_t)blockIdx.x * (kBlock * ITEMS);
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
+  for (int k = 0; k < ITEMS; ++k) {
     const int64_t i = base + k * kBlock + threadIdx.x;
     if (i < n) atomicAdd(&s_h[(keys[i] >> shift) & mask], 1u);
   }
   __syncthreads();
-  hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
+  if (TILE_MAJOR)
+    hist[(int64_t)blockIdx.x * 256 + threadIdx.x] = s_h[threadIdx.x];
+  else
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = s_h[threadIdx.x];
 }
 
+// Stable scatter of one digit. The output position of digit d's first key in
+// this tile is either read from the exclusively scanned digit-major
+// histogram (SCANNED) or, when there are few tiles, found here from the raw
+// tile-major histogram: the keys of smaller digits everywhere plus the keys of
+// digit d in earlier tiles (16 loads in flight per thread). All of the tile's keys are loaded before the ranking loop so
+// that their latencies overlap.
+template <int ITEMS, bool SCANNED>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n, int shift,
-    int nbits, const uint32_t* __restrict__ hist_scanned, int64_t nblocks,
+    int nbits, const uint32_t* __restrict__ hist, int64_t nblocks,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
   __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_wcnt[kWavesPerBlock][256];
+  __shared__ uint32_t s_w[kWavesPerBlock];
   const int t = threadIdx.x;
   const int wid = t / kWave;
   const uint64_t lt = lanemask_lt();
   const uint32_t mask = (1u << nbits) - 1u;
-  s_base[t] = hist_scanned[(int64_t)t * nblocks + blockIdx.x];
+  const int64_t base = (int64_t)blockIdx.x * (kBlock * ITEMS);
+  uint32_t kk[ITEMS], vv[ITEMS];
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const int64_t i = base + k * kBlock + t;
+    kk[k] = (i < n) ? keys[i] : 0u;
+    vv[k] = (i < n) ? vals[i] : 0u;
+  }
+  if constexpr (SCANNED) {
+    s_base[t] = hist[(int64_t)t * nblocks + blockIdx.x];
+  } else {
+    const uint32_t* h = hist + t;
+    const int64_t me = blockIdx.x;
+    uint32_t pre = 0, tot = 0;
+    int64_t b = 0;
+    for (; b + 16 <= nblocks; b += 16) {
+      uint32_t x[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) x[u] = h[(b + u) * 256];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        tot += x[u];
+        pre += (b + u < me) ? x[u] : 0u;
+      }
+    }
+    for (; b < nblocks; ++b) {
+      const uint32_t x = h[b * 256];
+      tot += x;
+      pre += (b < me) ? x : 0u;
+    }
+    s_base[t] = block_exclusive<uint32_t>(tot, s_w, nullptr) + pre;
+  }
 #pragma unroll
   for (int w = 0; w < kWavesPerBlock; ++w) s_wcnt[w][t] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kTile;
-  for (int k = 0; k < kItems; ++k) {
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
     const int64_t i = base + k * kBlock + t;
     const bool valid = i < n;
-    uint32_t key = 0, val = 0, d = 0;
-    if (valid) {
-      key = keys[i];
-      val = vals[i];
-      d = (key >> shift) & mask;
-    }
+    const uint32_t key = kk[k], val = vv[k];
+    const uint32_t d = valid ? (key >> shift) & mask : 0u;
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < nbits; ++b) {
       const bool bit = (d >> b) & 1u;
@@ -250,7 +296,13 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_
     set_error("radix sort: %lld keys exceed the int32 position range", (long long)n);
     return AXSQ_ERR_INVALID_ARG;
   }
-  const int64_t nb = (n + kTile - 1) / kTile;
+  // Small inputs take 1,024-key tiles, so that more workgroups share the work;
+  // with at most kFewTiles tiles the scatter scans the histogram itself.
+  constexpr int64_t kFewTiles = 256;
+  const bool small = n <= kFewTiles * kBlock * 4;
+  const int64_t tile = small ? kBlock * 4 : kTile;
+  const int64_t nb = (n + tile - 1) / tile;
+  const bool few = nb <= kFewTiles;
   const int64_t nh = 256 * nb;
   AXSQ_TRY(ws.hist.ensure(sizeof(uint32_t) * (size_t)nh));
   uint32_t* hist = ws.hist.as<uint32_t>();
@@ -258,11 +310,27 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_
   uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
   for (int shift = 0; shift < bits; shift += 8) {
     const int nbits = (bits - shift < 8) ? bits - shift : 8;
-    k_radix_hist<<<(unsigned)nb, kBlock, 0, s>>>(ki, n, shift, nbits, hist, nb);
-    AXSQ_CHECK_HIP(hipGetLastError());
-    AXSQ_TRY(scan_exclusive_i32((const int32_t*)hist, (int32_t*)hist, nh,
-                                ws.misc.as<int64_t>(), ws, s));
-    k_radix_scatter<<<(unsigned)nb, kBlock, 0, s>>>(ki, vi, n, shift, nbits, hist, nb, ko, vo);
+    if (small) {
+      k_radix_hist<4, true><<<(unsigned)nb, kBlock, 0, s>>>(ki, n, shift, nbits, hist, nb);
+      AXSQ_CHECK_HIP(hipGetLastError());
+      k_radix_scatter<4, false><<<(unsigned)nb, kBlock, 0, s>>>(ki, vi, n, shift, nbits, hist,
+                                                                  nb, ko, vo);
+    } else {
+      if (few)
+        k_radix_hist<kItems, true><<<(unsigned)nb, kBlock, 0, s>>>(ki, n, shift, nbits, hist, nb);
+      else
+        k_radix_hist<kItems, false><<<(unsigned)nb, kBlock, 0, s>>>(ki, n, shift, nbits, hist, nb);
+      AXSQ_CHECK_HIP(hipGetLastError());
+      if (few) {
+        k_radix_scatter<kItems, false><<<(unsigned)nb, kBlock, 0, s>>>(ki, vi, n, shift, nbits,
+                                                                       hist, nb, ko, vo);
+      } else {
+        AXSQ_TRY(scan_exclusive_i32((const int32_t*)hist, (int32_t*)hist, nh,
+                                    ws.misc.as<int64_t>(), ws, s));
+        k_radix_scatter<kItems, true><<<(unsigned)nb, kBlock, 0, s>>>(ki, vi, n, shift, nbits,
+                                                                      hist, nb, ko, vo);
+      }
+    }
     AXSQ_CHECK_HIP(hipGetLastError());
     uint32_t* t = ki;
     ki = ko;

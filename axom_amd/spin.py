@@ -126,9 +126,14 @@ class BVH:
         n = b.shape[0] if numItems is None else int(numItems)
         if n > b.shape[0]:
             raise ValueError("numItems exceeds the number of boxes")
-        if self._h:
-            L.lib().axsq_bvh_destroy(self._h)
-            self._h = None
+        if self._h:  # build again in place, keeping the device arrays
+            rc = L.lib().axsq_bvh_rebuild(self._h, _ptr(b), n, self._scale, self._fp,
+                                          stream_ptr())
+            if rc != 0:
+                L.lib().axsq_bvh_destroy(self._h)
+                self._h = None
+                L.check(rc)
+            return BVH_BUILD_OK
         h = ctypes.c_void_p()
         build = L.lib().axsq_bvh_build2d if self.NDIMS == 2 else L.lib().axsq_bvh_build
         L.check(build(ctypes.byref(h), _ptr(b), n, self._scale, self._fp, stream_ptr()))

@@ -380,6 +380,29 @@ def main():
         out["find_points"] = {"queries": int(p1.shape[0]), "boxes": int(bx1.shape[0]),
                               "ms": el * 1e3, "queries_per_s": p1.shape[0] / el,
                               "candidates": int(cand.numel())}
+        if not args.no_cpu:
+            # the oracle's build, one thread (the reference's SEQ build), and
+            # its findPoints over all host threads, on the same boxes and points
+            from oracle import oracle as orc
+            ncpu = min(os.cpu_count() or 1, 16)
+            t0 = time.perf_counter()
+            ob = orc.OracleBVH(boxes1)
+            el_b = time.perf_counter() - t0
+            ex = bvh.export()
+            same_tree = bool(np.array_equal(ob.export()["nodes"], ex["nodes"].cpu().numpy()))
+            t0 = time.perf_counter()
+            ooff, ocnt, ocand = ob.find("points", pts1, nthreads=ncpu)
+            el_f = time.perf_counter() - t0
+            same_find = bool(np.array_equal(ocnt, cnt.cpu().numpy()) and
+                             np.array_equal(ocand, cand.cpu().numpy()))
+            out["bvh_build"]["boxes_100k"]["cpu_baseline"] = {
+                "ms": el_b * 1e3, "mprims_per_s": boxes1.shape[0] / el_b / 1e6, "cores": 1,
+                "kind": "port", "sample": "the same 100,000 boxes",
+                "matches_gpu_bitwise": same_tree}
+            out["find_points"]["cpu_baseline"] = {
+                "ms": el_f * 1e3, "queries_per_s": pts1.shape[0] / el_f, "cores": ncpu,
+                "kind": "port", "sample": "the same 1,000,000 points",
+                "matches_gpu_bitwise": same_find}
 
     if args.calib_copy:
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)

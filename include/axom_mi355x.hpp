@@ -222,9 +222,16 @@ class MI355XLinearBVH {
 
   // BVH.hpp:426-477
   int initialize(const BoxType* boxes, IndexType numItems) {
-    (void)axsq_bvh_destroy(m_bvh);
-    m_bvh = nullptr;
     const double* b = reinterpret_cast<const double*>(boxes);
+    if (m_bvh) {  // build again in place, keeping the device arrays
+      const int rc = axsq_bvh_rebuild(m_bvh, b, numItems, m_scaleFactor, m_fpMode, nullptr);
+      if (rc != AXSQ_OK) {
+        (void)axsq_bvh_destroy(m_bvh);
+        m_bvh = nullptr;
+        ::axom::detail::check(rc);
+      }
+      return BVH_BUILD_OK;
+    }
     if (NDIMS == 2)
       ::axom::detail::check(axsq_bvh_build2d(&m_bvh, b, numItems, m_scaleFactor, m_fpMode, nullptr));
     else

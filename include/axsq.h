@@ -85,6 +85,13 @@ int axsq_bvh_build2d(axsq_bvh** out, const double* boxes, int64_t n, double scal
                      int fp_mode, void* stream);
 /* 2 or 3: the dimension the handle was built in (0 for NULL). */
 int axsq_bvh_dimension(const axsq_bvh* bvh);
+/* Builds again over new boxes into an existing tree (3-D or 2-D, as it was
+ * built), keeping its device arrays when the box count is unchanged: calling
+ * spin::BVH::initialize again on an initialized BVH (src/axom/spin/BVH.hpp:
+ * 426-477 replaces the tree in place). Must be called on the device the tree
+ * was built on. After an error the tree may only be destroyed. */
+int axsq_bvh_rebuild(axsq_bvh* bvh, const double* boxes, int64_t n, double scale, int fp_mode,
+                     void* stream);
 
 /* Root box of the BVH: BVH::getBounds (BVH.hpp:297-308). Host outputs. */
 int axsq_bvh_bounds(const axsq_bvh* bvh, double lo[3], double hi[3]);

@@ -111,6 +111,21 @@ static void test_bvh_grid() {
   }
   Found fs = run_find(n, [&](auto oo, auto cc, auto& cand) { bvh.findPoints(oo, cc, cand, n, d_cs); });
   for (IndexType i = 0; i < n; ++i) EXPECT_EQ(fs.cnt[i], 0);
+  // initialize again on the same object: the tree is rebuilt in place over
+  // boxes shifted by +10, which the shifted centroids now find
+  std::vector<Box> shifted = boxes;
+  for (Box& bx : shifted)
+    for (int d = 0; d < 3; ++d) {
+      bx.min[d] += 10;
+      bx.max[d] += 10;
+    }
+  Box* d_shifted = to_device(shifted);
+  EXPECT_EQ(bvh.initialize(d_shifted, n), axom::spin::BVH_BUILD_OK);
+  Found fa = run_find(n, [&](auto oo, auto cc, auto& cand) { bvh.findPoints(oo, cc, cand, n, d_cs); });
+  for (IndexType i = 0; i < n; ++i) {
+    EXPECT_EQ(fa.cnt[i], 1);
+    EXPECT_EQ(fa.cand[fa.off[i]], i);
+  }
   // offsets of the wrong length are rejected (LinearBVH.hpp:284-285)
   bool threw = false;
   try {
@@ -120,7 +135,8 @@ static void test_bvh_grid() {
     threw = true;
   }
   EXPECT_TRUE(threw);
-  for (void* p : {(void*)d_boxes, (void*)d_q, (void*)d_rays, (void*)d_c, (void*)d_cs}) (void)hipFree(p);
+  for (void* p : {(void*)d_boxes, (void*)d_q, (void*)d_rays, (void*)d_c, (void*)d_cs, (void*)d_shifted})
+    (void)hipFree(p);
 }
 
 // quest/tests/quest_test_utilities.hpp:51-151 (sphere), :57-87 (16^3 grid)

@@ -41,8 +41,13 @@ def _build_pair(boxes, scale, fp_mode="fp64"):
     return g, orc.OracleBVH(boxes, scale)
 
 
+# sizes either side of the build's internal limits: one refit span (16,384
+# leaves), 1,024-key sort tiles (262,144 keys), a sort that scans its own
+# histogram (at most 256 tiles of 4,096 keys) and one that does not
 @pytest.mark.parametrize("n,scale", [(0, 1.0), (1, 1.0), (2, 1.000123), (3, 1.0), (257, 1.000123),
-                                     (5000, 1.000123), (100_000, 1.000123)])
+                                     (5000, 1.000123), (16_384, 1.0), (16_385, 1.000123),
+                                     (100_000, 1.000123), (262_144, 1.000123), (262_145, 1.0),
+                                     (1_048_577, 1.000123)])
 def test_build_bit_exact(n, scale):
     boxes = _boxes(n, n + 1) if n < 100_000 else synth.workload1()[0]
     g, o = _build_pair(boxes, scale)
@@ -64,6 +69,30 @@ def test_build_duplicate_codes_and_invalid_boxes():
     assert np.array_equal(ex["children"].cpu().numpy(), oe["children"])
     assert np.array_equal(ex["nodes"].cpu().numpy(), oe["nodes"])
     assert np.array_equal(ex["leafs"].cpu().numpy(), oe["leafs"])
+
+
+def test_initialize_again_rebuilds_in_place():
+    """BVH::initialize on an initialized tree replaces it (BVH.hpp:426-477): the
+    same count (device arrays kept), another count, the FP32 nodes added, and
+    an empty input, each bit-exact against a fresh oracle build."""
+    g = spin.BVH(scaleFactor=1.000123)
+    for n, seed, mode in ((20_000, 1, "fp64"), (20_000, 2, "fp64"), (70_001, 3, "fp64"),
+                          (5000, 4, "fp32"), (0, 5, "fp64"), (300, 6, "fp64")):
+        g._fp = spin.FP_MODES[mode]  # the traversal precision the next build emits
+        boxes = _boxes(n, seed)
+        g.initialize(_t(boxes) if n else torch.zeros((0, 6), dtype=torch.float64, device=DEV))
+        ex, oe = g.export(), orc.OracleBVH(boxes.reshape(-1, 6), 1.000123).export()
+        for k in ("children", "nodes", "leafs"):
+            assert np.array_equal(ex[k].cpu().numpy(), oe[k]), (n, k)
+        lo, hi = g.getBounds()
+        assert np.array_equal(np.concatenate([lo, hi]), oe["bounds"]), n
+        if n:
+            p = np.random.default_rng(seed).random((2000, 3))
+            off = torch.empty(2000, dtype=torch.int32, device=DEV)
+            cnt = torch.empty_like(off)
+            cand = g.findPoints(off, cnt, 2000, _t(p)).cpu().numpy()
+            ooff, ocnt, ocand = orc.OracleBVH(boxes, 1.000123).find("points", p)
+            assert np.array_equal(cnt.cpu().numpy(), ocnt) and np.array_equal(cand, ocand)
 
 
 def test_build_is_deterministic():
