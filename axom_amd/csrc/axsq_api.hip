@@ -15,9 +15,12 @@
 
 namespace axsq {
 
-int find_launch(const axsq_bvh* b, int kind, bool fill, const double* x, const double* y,
+int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const double* y,
                 const double* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
-                const int32_t* offsets, int32_t* cands, hipStream_t s);
+                const int32_t* offsets, int32_t* cands, int32_t* stash, int32_t* stash_cnt,
+                int32_t* over, uint32_t* nover, hipStream_t s);
+int stash_copy_launch(const int32_t* stash, const int32_t* stash_cnt, const int32_t* offsets,
+                      int64_t nq, int32_t* cands, hipStream_t s);
 int bvh_export_launch(const axsq_bvh* b, double* inner_nodes, int32_t* children,
                       int32_t* leaf_nodes, uint32_t* mcodes, hipStream_t s);
 int sdf_check_conn(const axsq_sdf* h, hipStream_t s);
@@ -80,6 +83,11 @@ void Workspace::release() {
   mesh_box.release();
   mesh_int.release();
   mesh_slots.release();
+  find_stash.release();
+  find_cnt.release();
+  find_over.release();
+  find_n.release();
+  find_key.valid = false;
   if (mesh_bvh) {
     axsq_bvh_destroy(mesh_bvh);
     mesh_bvh = nullptr;
@@ -367,13 +375,35 @@ int axsq_bvh_find_count(const axsq_bvh* b, int kind, const double* x, const doub
   *total = 0;
   if (nq == 0) return AXSQ_OK;
   Workspace& ws = workspace();
-  AXSQ_TRY(find_launch(b, kind, false, x, y, z, stride, nq, tol, counts, nullptr, nullptr, s));
-  AXSQ_TRY(ws.misc.ensure(sizeof(int64_t) * 2));
-  int64_t* tot_dev = ws.misc.as<int64_t>() + 1;
+  ws.find_key.valid = false;
+  AXSQ_TRY(ws.find_stash.ensure(sizeof(int32_t) * kStash * (size_t)nq));
+  AXSQ_TRY(ws.find_cnt.ensure(sizeof(int32_t) * (size_t)nq));
+  AXSQ_TRY(ws.find_over.ensure(sizeof(int32_t) * (size_t)nq));
+  AXSQ_TRY(ws.find_n.ensure(sizeof(int64_t) * 2));
+  // find_n: [0] the scanned total (int64), [1] the listed queries (uint32)
+  int64_t* tot_dev = ws.find_n.as<int64_t>();
+  uint32_t* nover = reinterpret_cast<uint32_t*>(tot_dev + 1);
+  AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(nover), 0, 1, s));
+  AXSQ_TRY(find_launch(b, kind, 0, x, y, z, stride, nq, tol, counts, nullptr, nullptr,
+                       ws.find_stash.as<int32_t>(), ws.find_cnt.as<int32_t>(),
+                       ws.find_over.as<int32_t>(), nover, s));
   AXSQ_TRY(scan_exclusive_i32(counts, offsets, nq, tot_dev, ws, s));
-  int64_t t = 0;
-  AXSQ_CHECK_HIP(hipMemcpyAsync(&t, tot_dev, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  int64_t tn[2] = {0, 0};
+  AXSQ_CHECK_HIP(hipMemcpyAsync(tn, tot_dev, sizeof(tn), hipMemcpyDeviceToHost, s));
   AXSQ_TRY(finish(b->status, s));
+  const int64_t t = tn[0];
+  Workspace::FindKey& key = ws.find_key;
+  key.bvh = b;
+  key.generation = b->generation;
+  key.kind = kind;
+  key.x = x;
+  key.y = y;
+  key.z = z;
+  key.stride = stride;
+  key.nq = nq;
+  key.tol = tol;
+  key.nover = (int64_t)(uint32_t)tn[1];
+  key.valid = true;
   if (t > 0x7fffffffll) {
     set_error("candidate total %lld exceeds the int32 index range", (long long)t);
     return AXSQ_ERR_INVALID_ARG;
@@ -393,7 +423,23 @@ int axsq_bvh_find_fill(const axsq_bvh* b, int kind, const double* x, const doubl
   }
   if (nq == 0) return AXSQ_OK;
   hipStream_t s = (hipStream_t)stream;
-  AXSQ_TRY(find_launch(b, kind, true, x, y, z, stride, nq, tol, nullptr, offsets, candidates, s));
+  Workspace& ws = workspace();
+  Workspace::FindKey& key = ws.find_key;
+  const bool kept = key.valid && key.bvh == b && key.generation == b->generation &&
+                    key.kind == kind && key.x == x && key.y == y && key.z == z &&
+                    key.stride == stride && key.nq == nq && key.tol == tol;
+  key.valid = false;  // the kept ids serve one fill
+  if (kept) {  // copy the kept ids; search again only for the listed queries
+    AXSQ_TRY(stash_copy_launch(ws.find_stash.as<int32_t>(), ws.find_cnt.as<int32_t>(), offsets,
+                               nq, candidates, s));
+    if (key.nover > 0)
+      AXSQ_TRY(find_launch(b, kind, 2, x, y, z, stride, key.nover, tol, nullptr, offsets,
+                           candidates, nullptr, nullptr, ws.find_over.as<int32_t>(), nullptr,
+                           s));
+  } else {
+    AXSQ_TRY(find_launch(b, kind, 1, x, y, z, stride, nq, tol, nullptr, offsets, candidates,
+                         nullptr, nullptr, nullptr, nullptr, s));
+  }
   return finish(b->status, s);
 }
 

@@ -48,6 +48,21 @@ struct Workspace {
   DevBuf mesh_tri, mesh_box, mesh_int, mesh_slots;  // mesh self-intersection
   DevBuf box2d;                                      // 2-D boxes widened to 3-D
   axsq_bvh* mesh_bvh = nullptr;                      // its BVH, rebuilt per call
+  // candidate search: the count pass keeps each query's first candidates
+  // (find_stash, find_cnt) and lists the queries with more (find_over,
+  // their number in find_n), so that a fill pass that follows with the same
+  // inputs copies them instead of searching again
+  DevBuf find_stash, find_cnt, find_over, find_n;
+  struct FindKey {
+    const axsq_bvh* bvh = nullptr;
+    uint64_t generation = 0;
+    int kind = -1;
+    const double *x = nullptr, *y = nullptr, *z = nullptr;
+    int64_t stride = 0, nq = 0;
+    double tol = 0.0;
+    int64_t nover = 0;
+    bool valid = false;
+  } find_key;
   int device = -1;
   void release();
 };
@@ -66,6 +81,8 @@ struct ProfRange {
   void stop();  // records the end event once; later calls do nothing
 };
 void prof_flush();  // after a stream sync: folds finished ranges into totals
+
+constexpr int kStash = 8;  // candidate ids a query's count pass keeps for the fill pass
 
 // ---- primitives (axsq_prims.hip) ------------------------------------------
 constexpr int kBlock = 256;
@@ -110,6 +127,7 @@ struct axsq_bvh {
   int32_t* leaf_ids = nullptr;      // n: leaf position -> original id
   uint32_t* mcodes = nullptr;       // n sorted Morton codes
   int32_t* status = nullptr;        // device flag: traversal stack overflow
+  uint64_t generation = 0;          // id of the last build, unique within the process
 };
 
 struct axsq_sdf {

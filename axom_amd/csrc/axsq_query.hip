@@ -3,6 +3,12 @@
 // Reference: LinearBVH::findCandidatesImpl (src/axom/spin/policy/LinearBVH.hpp:273-402),
 // GPU form: count pass, exclusive scan, fill pass (:296-367). One lane per
 // query; candidates of query i are written in traversal order at offsets[i].
+// The count pass also keeps each query's first kStash candidate ids and lists
+// the queries that have more, so the fill pass copies the kept ids and
+// searches again only for the listed queries. The walk runs on the vote
+// schedule (a lane's order of node visits and leaf actions is the one-lane
+// walk's, so the candidate order is too): 11% faster for boxes and 34% for
+// rays than the loop-by-loop schedule on 100,000 boxes, the same for points.
 //
 // FP32 traversal: nodes hold FP32 boxes rounded outward, so every predicate
 // evaluated on them passes whenever it passes on the FP64 box (points and
@@ -34,7 +40,13 @@ struct FindArgs {
   const int32_t* offsets;
   int32_t* cands;
   int32_t* status;
+  int32_t* stash;      // count pass: kStash ids per query (or NULL)
+  int32_t* stash_cnt;  // count pass: a copy of the counts
+  int32_t* over;       // count pass: queries with more than kStash; fill: those to search
+  uint32_t* nover;
 };
+
+enum { kCountPass = 0, kFillAll = 1, kFillListed = 2 };
 
 // Predicates of BVH.hpp:499-501 (contains), :558-560 (intersectsWith),
 // :529-532 (intersect_ray with the BVH tolerance).
@@ -65,83 +77,138 @@ __device__ __forceinline__ bool pred32(const double* prim, const float* lo, cons
   }
 }
 
-template <int KIND, bool FILL, bool MIXED>
+template <int KIND, int MODE, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
+  constexpr bool FILL = MODE != kCountPass;
   __shared__ int32_t s_stack[stack_lds_words(kBlock)];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.nq) return;
-  lds_i32* stk = lane_stack_base(s_stack);
-  double prim[6];
-  if (KIND == AXSQ_QUERY_POINTS) {
-    prim[0] = a.x[i * a.stride];
-    prim[1] = a.y[i * a.stride];
-    prim[2] = (a.dims == 2) ? 0.0 : a.z[i * a.stride];
-    prim[3] = prim[4] = prim[5] = 0.0;
-  } else if (a.dims == 2) {  // box: min x y, max x y; ray: origin x y, direction x y
-    const double* p = a.x + i * a.stride;
-    prim[0] = p[0];
-    prim[1] = p[1];
-    prim[2] = 0.0;
-    prim[3] = p[2];
-    prim[4] = p[3];
-    prim[5] = 0.0;
-  } else {
-    for (int k = 0; k < 6; ++k) prim[k] = a.x[i * a.stride + k];
-  }
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = j < a.nq;
+  if (FILL && !live) return;  // the count pass keeps whole waves for its ballot
   int32_t count = 0;
-  int32_t offset = FILL ? a.offsets[i] : 0;
-  auto emit = [&](int32_t pos) {
-    if (FILL) a.cands[offset++] = a.leaf_ids[pos];
-    else ++count;
-  };
-  auto no_pref = [](const auto*, const auto*) { return false; };
-  int32_t v;
-  if (!MIXED) {
-    v = traverse<Node64, double>(
-        a.nodes, a.ninner, stk, [&](const double* b) { return pred64<KIND>(prim, a.tol, b); }, no_pref, emit);
-  } else {
-    float lo[3], hi[3];
-    const int hoff = (KIND == AXSQ_QUERY_BOXES) ? 3 : 0;
-    for (int d = 0; d < 3; ++d) {
-      lo[d] = __double2float_rd(prim[d]);
-      hi[d] = __double2float_ru(prim[hoff + d]);
+  int64_t i = 0;
+  if (live) {
+    i = (MODE == kFillListed) ? (int64_t)a.over[j] : j;
+    lds_i32* stk = lane_stack_base(s_stack);
+    double prim[6];
+    if (KIND == AXSQ_QUERY_POINTS) {
+      prim[0] = a.x[i * a.stride];
+      prim[1] = a.y[i * a.stride];
+      prim[2] = (a.dims == 2) ? 0.0 : a.z[i * a.stride];
+      prim[3] = prim[4] = prim[5] = 0.0;
+    } else if (a.dims == 2) {  // box: min x y, max x y; ray: origin x y, direction x y
+      const double* p = a.x + i * a.stride;
+      prim[0] = p[0];
+      prim[1] = p[1];
+      prim[2] = 0.0;
+      prim[3] = p[2];
+      prim[4] = p[3];
+      prim[5] = 0.0;
+    } else {
+      for (int k = 0; k < 6; ++k) prim[k] = a.x[i * a.stride + k];
     }
-    v = traverse<Node32, float>(
-        a.nodes32, a.ninner, stk, [&](const float* b) { return pred32<KIND>(prim, lo, hi, a.tol, b); }, no_pref,
-        [&](int32_t pos) {
-          double lb[6];
-          const double2* p = reinterpret_cast<const double2*>(a.leaf_boxes + 6 * (int64_t)pos);
-          const double2 c0 = p[0], c1 = p[1], c2 = p[2];
-          lb[0] = c0.x; lb[1] = c0.y; lb[2] = c1.x; lb[3] = c1.y; lb[4] = c2.x; lb[5] = c2.y;
-          if (valid6(lb) && pred64<KIND>(prim, a.tol, lb)) emit(pos);
-        });
+    int32_t offset = FILL ? a.offsets[i] : 0;
+    int32_t* const stash = (!FILL && a.stash) ? a.stash + i * kStash : nullptr;
+    auto emit = [&](int32_t pos) {
+      if (FILL) {
+        a.cands[offset++] = a.leaf_ids[pos];
+      } else {
+        if (stash && count < kStash) stash[count] = a.leaf_ids[pos];
+        ++count;
+      }
+    };
+    auto no_pref = [](const auto*, const auto*) { return false; };
+    int32_t v;
+    if (!MIXED) {
+      v = traverse_voted<Node64, double>(
+          a.nodes, a.ninner, stk, [&](const double* b) { return pred64<KIND>(prim, a.tol, b); },
+          no_pref, emit);
+    } else {
+      float lo[3], hi[3];
+      const int hoff = (KIND == AXSQ_QUERY_BOXES) ? 3 : 0;
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = __double2float_rd(prim[d]);
+        hi[d] = __double2float_ru(prim[hoff + d]);
+      }
+      v = traverse_voted<Node32, float>(
+          a.nodes32, a.ninner, stk,
+          [&](const float* b) { return pred32<KIND>(prim, lo, hi, a.tol, b); }, no_pref,
+          [&](int32_t pos) {
+            double lb[6];
+            const double2* p = reinterpret_cast<const double2*>(a.leaf_boxes + 6 * (int64_t)pos);
+            const double2 c0 = p[0], c1 = p[1], c2 = p[2];
+            lb[0] = c0.x; lb[1] = c0.y; lb[2] = c1.x; lb[3] = c1.y; lb[4] = c2.x; lb[5] = c2.y;
+            if (valid6(lb) && pred64<KIND>(prim, a.tol, lb)) emit(pos);
+          });
+    }
+    if (v < 0) atomicOr(a.status, 1);
+    if (!FILL) {
+      a.counts[i] = count;
+      if (stash) a.stash_cnt[i] = count;
+    }
   }
-  if (v < 0) atomicOr(a.status, 1);
-  if (!FILL) a.counts[i] = count;
+  if (!FILL && a.stash) {  // queries with more than kStash: one list slot per wave
+    const bool more = live && count > kStash;
+    const uint64_t m = __ballot(more);
+    if (m) {
+      const int lane = (int)__lane_id();
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      uint32_t base = 0;
+      if (lane == leader) base = atomicAdd(a.nover, (uint32_t)__popcll(m));
+      base = __shfl(base, leader);
+      const uint64_t below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
+      if (more) a.over[base + __popcll(below)] = (int32_t)i;
+    }
+  }
 }
 
-template <bool FILL, bool MIXED>
+// The fill pass for queries with at most kStash candidates: the ids the count
+// pass kept, at offsets[i].
+__global__ __launch_bounds__(kBlock) void k_stash_copy(const int32_t* __restrict__ stash,
+                                                        const int32_t* __restrict__ stash_cnt,
+                                                        const int32_t* __restrict__ offsets,
+                                                        int64_t nq, int32_t* __restrict__ cands) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int32_t c = stash_cnt[i];
+  if (c > kStash) return;
+  const int32_t o = offsets[i];
+#pragma unroll
+  for (int k = 0; k < kStash; ++k)
+    if (k < c) cands[o + k] = stash[i * kStash + k];
+}
+
+template <int MODE, bool MIXED>
 hipError_t launch_kind(int kind, const FindArgs& a, hipStream_t s) {
   const unsigned g = grid_for(a.nq);
   switch (kind) {
     case AXSQ_QUERY_POINTS:
-      k_find<AXSQ_QUERY_POINTS, FILL, MIXED><<<g, kBlock, 0, s>>>(a);
+      k_find<AXSQ_QUERY_POINTS, MODE, MIXED><<<g, kBlock, 0, s>>>(a);
       break;
     case AXSQ_QUERY_BOXES:
-      k_find<AXSQ_QUERY_BOXES, FILL, MIXED><<<g, kBlock, 0, s>>>(a);
+      k_find<AXSQ_QUERY_BOXES, MODE, MIXED><<<g, kBlock, 0, s>>>(a);
       break;
     default:
-      k_find<AXSQ_QUERY_RAYS, FILL, MIXED><<<g, kBlock, 0, s>>>(a);
+      k_find<AXSQ_QUERY_RAYS, MODE, MIXED><<<g, kBlock, 0, s>>>(a);
       break;
   }
   return hipGetLastError();
 }
 
+template <bool MIXED>
+hipError_t launch_mode(int mode, int kind, const FindArgs& a, hipStream_t s) {
+  if (mode == kCountPass) return launch_kind<kCountPass, MIXED>(kind, a, s);
+  if (mode == kFillAll) return launch_kind<kFillAll, MIXED>(kind, a, s);
+  return launch_kind<kFillListed, MIXED>(kind, a, s);
+}
+
 }  // namespace
 
-int find_launch(const axsq_bvh* b, int kind, bool fill, const double* x, const double* y,
+// mode 0: count (and keep the first kStash ids when stash is given); 1: fill
+// every query; 2: fill the nq queries listed in `over`.
+int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const double* y,
                 const double* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
-                const int32_t* offsets, int32_t* cands, hipStream_t s) {
+                const int32_t* offsets, int32_t* cands, int32_t* stash, int32_t* stash_cnt,
+                int32_t* over, uint32_t* nover, hipStream_t s) {
   FindArgs a;
   a.nodes = b->nodes;
   a.nodes32 = b->nodes32;
@@ -159,18 +226,27 @@ int find_launch(const axsq_bvh* b, int kind, bool fill, const double* x, const d
   a.offsets = offsets;
   a.cands = cands;
   a.status = b->status;
+  a.stash = stash;
+  a.stash_cnt = stash_cnt;
+  a.over = over;
+  a.nover = nover;
   const bool mixed = (b->fp_mode == AXSQ_FP32_TRAVERSAL) && b->nodes32;
-  ProfRange pr(fill ? "bvh_find_fill" : "bvh_find_count", s);
-  hipError_t e;
-  if (fill)
-    e = mixed ? launch_kind<true, true>(kind, a, s) : launch_kind<true, false>(kind, a, s);
-  else
-    e = mixed ? launch_kind<false, true>(kind, a, s) : launch_kind<false, false>(kind, a, s);
+  ProfRange pr(mode == kCountPass ? "bvh_find_count" : "bvh_find_fill", s);
+  const hipError_t e = mixed ? launch_mode<true>(mode, kind, a, s) : launch_mode<false>(mode, kind, a, s);
   pr.stop();
   if (e != hipSuccess) {
     set_error("find kernel launch failed: %s", hipGetErrorString(e));
     return AXSQ_ERR_HIP;
   }
+  return AXSQ_OK;
+}
+
+int stash_copy_launch(const int32_t* stash, const int32_t* stash_cnt, const int32_t* offsets,
+                      int64_t nq, int32_t* cands, hipStream_t s) {
+  if (nq <= 0) return AXSQ_OK;
+  ProfRange pr("bvh_find_copy", s);
+  k_stash_copy<<<grid_for(nq), kBlock, 0, s>>>(stash, stash_cnt, offsets, nq, cands);
+  AXSQ_CHECK_HIP(hipGetLastError());
   return AXSQ_OK;
 }
 

@@ -319,4 +319,18 @@ __device__ __forceinline__ int32_t traverse(const NodeT* __restrict__ nodes, int
   return st == kWalkDone ? w.visits : -1;
 }
 
+// The same one-shot walk on the vote schedule (every lane of the wave that is
+// still running calls it together).
+template <class NodeT, class Scalar, class Pred, class Pref, class Leaf>
+__device__ __forceinline__ int32_t traverse_voted(const NodeT* __restrict__ nodes, int32_t ninner,
+                                                  lds_i32* lds_stack, Pred&& pred, Pref&& pref,
+                                                  Leaf&& leaf) {
+  LaneStack todo;
+  todo.lds = lds_stack;
+  Walk w = walk_start(todo);
+  const int st =
+      traverse_vote<NodeT, Scalar>(nodes, ninner, todo, w, 0x7fffffff, pred, pref, leaf);
+  return st == kWalkDone ? w.visits : -1;
+}
+
 }  // namespace axsq

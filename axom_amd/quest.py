@@ -61,7 +61,9 @@ class SignedDistance:
         self._h = h
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        # at interpreter exit the module globals may already be gone; the
+        # process then releases the device memory itself
+        if getattr(self, "_h", None) and L is not None:
             L.lib().axsq_sdf_destroy(self._h)
             self._h = None
 

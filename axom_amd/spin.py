@@ -116,7 +116,9 @@ class BVH:
             self.initialize(boxes, numItems)
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        # at interpreter exit the module globals may already be gone; the
+        # process then releases the device memory itself
+        if getattr(self, "_h", None) and L is not None:
             L.lib().axsq_bvh_destroy(self._h)
             self._h = None
 

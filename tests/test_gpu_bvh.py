@@ -4,6 +4,7 @@ Bar: bit-exact tree arrays (inner boxes, child links, leaf ids, Morton codes,
 bounds) and bit-exact candidate lists (offsets, counts, ids in traversal
 order), for the FP64 form and the FP32-traversal form.
 """
+import ctypes
 import json
 import os
 
@@ -93,6 +94,50 @@ def test_initialize_again_rebuilds_in_place():
             cand = g.findPoints(off, cnt, 2000, _t(p)).cpu().numpy()
             ooff, ocnt, ocand = orc.OracleBVH(boxes, 1.000123).find("points", p)
             assert np.array_equal(cnt.cpu().numpy(), ocnt) and np.array_equal(cand, ocand)
+
+
+def test_fill_after_other_count_searches_again():
+    """The fill pass copies the ids the count pass kept only when it follows
+    that count pass with the same inputs; a fill for other inputs (here after
+    a count for other points, and after a rebuild) searches the tree again."""
+    from axom_amd import _lib as L
+    boxes = _boxes(30_000, 8, 0.01, 0.06)  # many candidates per point: the list path too
+    g = spin.BVH(_t(boxes))
+    o = orc.OracleBVH(boxes, 1.000123)
+    pa = np.random.default_rng(1).random((4000, 3))
+    pb = np.random.default_rng(2).random((3000, 3))
+    ta, tb = _t(pa), _t(pb)
+    ooff, ocnt, ocand = o.find("points", pa)
+    assert ocnt.max() > 8 and (ocnt <= 8).any()
+
+    def count(t):
+        n = t.shape[0]
+        off = torch.empty(n, dtype=torch.int32, device=DEV)
+        cnt = torch.empty_like(off)
+        tot = ctypes.c_int64()
+        base = t.data_ptr()
+        L.check(L.lib().axsq_bvh_find_count(g._h, L.QUERY_POINTS, ctypes.c_void_p(base),
+                                            ctypes.c_void_p(base + 8), ctypes.c_void_p(base + 16),
+                                            3, n, g._tol, spin._ptr(off), spin._ptr(cnt),
+                                            ctypes.byref(tot), None))
+        return off, tot.value
+
+    def fill(t, off, total):
+        cand = torch.full((total,), -7, dtype=torch.int32, device=DEV)
+        base = t.data_ptr()
+        L.check(L.lib().axsq_bvh_find_fill(g._h, L.QUERY_POINTS, ctypes.c_void_p(base),
+                                           ctypes.c_void_p(base + 8), ctypes.c_void_p(base + 16),
+                                           3, t.shape[0], g._tol, spin._ptr(off), spin._ptr(cand),
+                                           None))
+        return cand.cpu().numpy()
+
+    off_a, tot_a = count(ta)
+    assert np.array_equal(fill(ta, off_a, tot_a), ocand)  # kept ids + listed queries
+    count(tb)
+    assert np.array_equal(fill(ta, off_a, tot_a), ocand)  # inputs differ: searched again
+    off_a, tot_a = count(ta)
+    g.initialize(_t(boxes))
+    assert np.array_equal(fill(ta, off_a, tot_a), ocand)  # rebuilt: searched again
 
 
 def test_build_is_deterministic():
