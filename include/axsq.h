@@ -114,7 +114,11 @@ int axsq_bvh_export(const axsq_bvh* bvh, double* inner_nodes, int32_t* children,
  *                           y=z=NULL, stride 6; tol = BVH::setTolerance value
  * Phase 1 (axsq_bvh_find_count) writes counts[nq], offsets[nq] (exclusive
  * scan) and *total. Phase 2 (axsq_bvh_find_fill) writes candidates[total]:
- * for query i, ids at [offsets[i], offsets[i]+counts[i]) in traversal order. */
+ * for query i, ids at [offsets[i], offsets[i]+counts[i]) in traversal order.
+ * Phase 1 keeps each query's first 8 ids in the calling thread's workspace; a
+ * phase 2 on the same thread that follows it with the same tree and inputs
+ * copies them and walks again only for queries with more, any other phase 2
+ * walks every query. Results are the same either way. */
 int axsq_bvh_find_count(const axsq_bvh* bvh, int kind, const double* x, const double* y,
                         const double* z, int64_t stride, int64_t nq, double tol,
                         int32_t* offsets, int32_t* counts, int64_t* total, void* stream);
