@@ -1327,17 +1327,20 @@ __global__ __launch_bounds__(kBlock) void k_elem_keys(const int32_t* __restrict_
   vals[i] = (uint32_t)i;
 }
 
-// One lane per triangle: its queries in ascending index order. Writes the
-// query gradients and the per-corner partial sums (9 doubles per triangle).
-__global__ __launch_bounds__(kBlock) void k_bwd_tri(
+// One lane per query, in element order (position p of the sorted list): the
+// query gradient, and the closest point's barycentric weights w and the
+// scaled direction d that the per-triangle sums read (terms[6p .. 6p + 5]).
+__global__ __launch_bounds__(kBlock) void k_bwd_query(
     const double* __restrict__ vx, const double* __restrict__ vy, const double* __restrict__ vz,
-    const int32_t* __restrict__ conn, int64_t nt, const int32_t* __restrict__ seg_start,
-    const int32_t* __restrict__ seg_end, const uint32_t* __restrict__ qidx,
-    const double* __restrict__ qx, const double* __restrict__ qy, const double* __restrict__ qz,
-    int64_t qs, const double* __restrict__ dist, const float* __restrict__ gout,
-    double* __restrict__ grad_q, double* __restrict__ part) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nt) return;
+    const int32_t* __restrict__ conn, int64_t nt, const uint32_t* __restrict__ keys,
+    const uint32_t* __restrict__ qidx, int64_t nq, const double* __restrict__ qx,
+    const double* __restrict__ qy, const double* __restrict__ qz, int64_t qs,
+    const double* __restrict__ dist, const float* __restrict__ gout, double* __restrict__ grad_q,
+    double* __restrict__ terms) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nq) return;
+  const int64_t c = (int64_t)keys[p];
+  if (c >= nt) return;  // flagged by k_segments
   double T[9];
   for (int k = 0; k < 3; ++k) {
     const int32_t v = conn[3 * c + k];
@@ -1345,44 +1348,67 @@ __global__ __launch_bounds__(kBlock) void k_bwd_tri(
     T[3 * k + 1] = vy[v];
     T[3 * k + 2] = vz[v];
   }
-  double acc[9];
-  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
-  const int32_t b = seg_start[c], e = seg_end[c];
-  if (b < e) {
+  const int64_t i = (int64_t)qidx[p];
+  const double P[3] = {qx[i * qs], qy[i * qs], qz[i * qs]};
+  double cp[3], w[3];
+  closest_point<true>(P, T, 1e-12, cp, w);
+  const double sgn = signbit(dist[i]) ? -1.0 : 1.0;
+  const double r0 = P[0] - cp[0], r1 = P[1] - cp[1], r2 = P[2] - cp[2];
+  const double len = sqrt(dot3(r0, r1, r2, r0, r1, r2));
+  double u0, u1, u2;
+  if (len > 0.0) {
+    const double inv = 1.0 / len;
+    u0 = r0 * inv;
+    u1 = r1 * inv;
+    u2 = r2 * inv;
+  } else {  // on the surface: the face normal
     double nrm[3], un[3];
     tri_normal(T, nrm);
     unit3(nrm, un);
-    for (int32_t p = b; p < e; ++p) {
-      const int64_t i = (int64_t)qidx[p];
-      const double P[3] = {qx[i * qs], qy[i * qs], qz[i * qs]};
-      double cp[3], w[3];
-      closest_point<true>(P, T, 1e-12, cp, w);
-      const double sgn = signbit(dist[i]) ? -1.0 : 1.0;
-      const double r0 = P[0] - cp[0], r1 = P[1] - cp[1], r2 = P[2] - cp[2];
-      const double len = sqrt(dot3(r0, r1, r2, r0, r1, r2));
-      double u0, u1, u2;
-      if (len > 0.0) {
-        const double inv = 1.0 / len;
-        u0 = r0 * inv;
-        u1 = r1 * inv;
-        u2 = r2 * inv;
-      } else {
-        u0 = un[0];
-        u1 = un[1];
-        u2 = un[2];
-      }
-      const double g = (double)gout[i] * sgn;
-      const double d0 = g * u0, d1 = g * u1, d2 = g * u2;
-      if (grad_q) {
-        grad_q[3 * i] = d0;
-        grad_q[3 * i + 1] = d1;
-        grad_q[3 * i + 2] = d2;
-      }
-      for (int k = 0; k < 3; ++k) {
-        acc[3 * k] += -(w[k] * d0);
-        acc[3 * k + 1] += -(w[k] * d1);
-        acc[3 * k + 2] += -(w[k] * d2);
-      }
+    u0 = un[0];
+    u1 = un[1];
+    u2 = un[2];
+  }
+  const double g = (double)gout[i] * sgn;
+  const double d0 = g * u0, d1 = g * u1, d2 = g * u2;
+  if (grad_q) {
+    grad_q[3 * i] = d0;
+    grad_q[3 * i + 1] = d1;
+    grad_q[3 * i + 2] = d2;
+  }
+  double* t = terms + 6 * p;
+  t[0] = w[0];
+  t[1] = w[1];
+  t[2] = w[2];
+  t[3] = d0;
+  t[4] = d1;
+  t[5] = d2;
+}
+
+// One lane per triangle: the per-corner partial sums (9 doubles per
+// triangle) over its queries in ascending index order, from k_bwd_query's
+// terms. The work of each query is done by its own lane beforehand, so a
+// triangle reached by many queries costs a streamed sum, not a chain of
+// dependent gathers.
+__global__ __launch_bounds__(kBlock) void k_bwd_tri(int64_t nt,
+                                                     const int32_t* __restrict__ seg_start,
+                                                     const int32_t* __restrict__ seg_end,
+                                                     const double* __restrict__ terms,
+                                                     double* __restrict__ part) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nt) return;
+  double acc[9];
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  const int32_t b = seg_start[c], e = seg_end[c];
+#pragma unroll 4
+  for (int32_t p = b; p < e; ++p) {
+    const double* t = terms + 6 * (int64_t)p;
+    const double w[3] = {t[0], t[1], t[2]};
+    const double d0 = t[3], d1 = t[4], d2 = t[5];
+    for (int k = 0; k < 3; ++k) {
+      acc[3 * k] += -(w[k] * d0);
+      acc[3 * k + 1] += -(w[k] * d1);
+      acc[3 * k + 2] += -(w[k] * d2);
     }
   }
   for (int k = 0; k < 9; ++k) part[9 * c + k] = acc[k];
@@ -1648,12 +1674,15 @@ int sdf_backward_launch(const axsq_sdf* h, const double* x, const double* y, con
   AXSQ_TRY(ws.v0.ensure(sizeof(uint32_t) * nq));
   AXSQ_TRY(ws.k1.ensure(sizeof(uint32_t) * nq));
   AXSQ_TRY(ws.v1.ensure(sizeof(uint32_t) * nq));
-  // triangle segments (2*nt int32) and partials (9*nt doubles)
+  // triangle segments (2*nt int32), partials (9*nt doubles), per-query terms
+  // (6*nq doubles)
   const size_t seg_bytes = (sizeof(int32_t) * 2 * nt + 255) & ~size_t(255);
-  AXSQ_TRY(ws.flag.ensure(seg_bytes + sizeof(double) * 9 * nt));
+  const size_t part_bytes = sizeof(double) * 9 * nt;
+  AXSQ_TRY(ws.flag.ensure(seg_bytes + part_bytes + sizeof(double) * 6 * nq));
   int32_t* seg_start = ws.flag.as<int32_t>();
   int32_t* seg_end = seg_start + nt;
   double* part = reinterpret_cast<double*>(ws.flag.as<char>() + seg_bytes);
+  double* terms = reinterpret_cast<double*>(ws.flag.as<char>() + seg_bytes + part_bytes);
   ProfRange pr("sdf_backward", s);
   k_elem_keys<<<grid_for(nq), kBlock, 0, s>>>(elem, nq, ws.k0.as<uint32_t>(),
                                               ws.v0.as<uint32_t>());
@@ -1664,8 +1693,10 @@ int sdf_backward_launch(const axsq_sdf* h, const double* x, const double* y, con
   AXSQ_TRY(fill_i32(seg_start, 0, 2 * nt, s));
   k_segments<<<grid_for(nq), kBlock, 0, s>>>(sk, nq, nt, seg_start, seg_end, h->bvh->status);
   AXSQ_CHECK_HIP(hipGetLastError());
-  k_bwd_tri<<<grid_for(nt), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, nt, seg_start, seg_end,
-                                            sv, x, y, z, stride, dist, gout, grad_q, part);
+  k_bwd_query<<<grid_for(nq), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, nt, sk, sv, nq, x, y,
+                                              z, stride, dist, gout, grad_q, terms);
+  AXSQ_CHECK_HIP(hipGetLastError());
+  k_bwd_tri<<<grid_for(nt), kBlock, 0, s>>>(nt, seg_start, seg_end, terms, part);
   AXSQ_CHECK_HIP(hipGetLastError());
   if (grad_v) {
     k_bwd_vert<<<grid_for(h->nverts), kBlock, 0, s>>>(h->v2t_start, h->v2t_end, h->v2t_slot,
