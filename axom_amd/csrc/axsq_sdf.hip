@@ -1385,33 +1385,44 @@ __global__ __launch_bounds__(kBlock) void k_bwd_query(
   t[5] = d2;
 }
 
-// One lane per triangle: the per-corner partial sums (9 doubles per
+// One wave per triangle: the per-corner partial sums (9 doubles per
 // triangle) over its queries in ascending index order, from k_bwd_query's
-// terms. The work of each query is done by its own lane beforehand, so a
-// triangle reached by many queries costs a streamed sum, not a chain of
-// dependent gathers.
-__global__ __launch_bounds__(kBlock) void k_bwd_tri(int64_t nt,
-                                                     const int32_t* __restrict__ seg_start,
-                                                     const int32_t* __restrict__ seg_end,
-                                                     const double* __restrict__ terms,
-                                                     double* __restrict__ part) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// terms. The lanes load 64 queries' terms at a time and form their products;
+// lane k < 9 then adds component k in query order. The sums are the
+// one-lane sequential sums, bit for bit, while a triangle reached by many
+// queries (hundreds, where queries share a vertex's normal cone) costs a few
+// coalesced loads instead of a chain of hundreds.
+constexpr int kBwdWave = 64;
+__global__ __launch_bounds__(kBwdWave) void k_bwd_tri(int64_t nt,
+                                                       const int32_t* __restrict__ seg_start,
+                                                       const int32_t* __restrict__ seg_end,
+                                                       const double* __restrict__ terms,
+                                                       double* __restrict__ part) {
+  __shared__ double s_pr[9][kBwdWave];
+  const int64_t c = blockIdx.x;
+  const int lane = (int)threadIdx.x;
   if (c >= nt) return;
-  double acc[9];
-  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
   const int32_t b = seg_start[c], e = seg_end[c];
-#pragma unroll 4
-  for (int32_t p = b; p < e; ++p) {
-    const double* t = terms + 6 * (int64_t)p;
-    const double w[3] = {t[0], t[1], t[2]};
-    const double d0 = t[3], d1 = t[4], d2 = t[5];
-    for (int k = 0; k < 3; ++k) {
-      acc[3 * k] += -(w[k] * d0);
-      acc[3 * k + 1] += -(w[k] * d1);
-      acc[3 * k + 2] += -(w[k] * d2);
+  double acc = 0.0;  // lane k < 9: component k (corner k / 3, axis k % 3)
+  for (int32_t base = b; base < e; base += kBwdWave) {
+    const int32_t p = base + lane;
+    if (p < e) {
+      const double* t = terms + 6 * (int64_t)p;
+      const double w[3] = {t[0], t[1], t[2]};
+      const double d0 = t[3], d1 = t[4], d2 = t[5];
+      for (int k = 0; k < 3; ++k) {
+        s_pr[3 * k][lane] = -(w[k] * d0);
+        s_pr[3 * k + 1][lane] = -(w[k] * d1);
+        s_pr[3 * k + 2][lane] = -(w[k] * d2);
+      }
     }
+    __syncthreads();
+    const int32_t m = (e - base < kBwdWave) ? e - base : kBwdWave;
+    if (lane < 9)
+      for (int32_t l = 0; l < m; ++l) acc += s_pr[lane][l];
+    __syncthreads();
   }
-  for (int k = 0; k < 9; ++k) part[9 * c + k] = acc[k];
+  if (lane < 9) part[9 * c + lane] = acc;
 }
 
 // One lane per vertex: sum of its incident corner partials, ascending triangle.
@@ -1696,7 +1707,7 @@ int sdf_backward_launch(const axsq_sdf* h, const double* x, const double* y, con
   k_bwd_query<<<grid_for(nq), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, nt, sk, sv, nq, x, y,
                                               z, stride, dist, gout, grad_q, terms);
   AXSQ_CHECK_HIP(hipGetLastError());
-  k_bwd_tri<<<grid_for(nt), kBlock, 0, s>>>(nt, seg_start, seg_end, terms, part);
+  k_bwd_tri<<<(unsigned)nt, kBwdWave, 0, s>>>(nt, seg_start, seg_end, terms, part);
   AXSQ_CHECK_HIP(hipGetLastError());
   if (grad_v) {
     k_bwd_vert<<<grid_for(h->nverts), kBlock, 0, s>>>(h->v2t_start, h->v2t_end, h->v2t_slot,
