@@ -15,12 +15,6 @@
 
 namespace axsq {
 
-int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const double* y,
-                const double* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
-                const int32_t* offsets, int32_t* cands, int32_t* stash, int32_t* stash_cnt,
-                int32_t* over, uint32_t* nover, hipStream_t s);
-int stash_copy_launch(const int32_t* stash, const int32_t* stash_cnt, const int32_t* offsets,
-                      int64_t nq, int32_t* cands, hipStream_t s);
 int bvh_export_launch(const axsq_bvh* b, double* inner_nodes, int32_t* children,
                       int32_t* leaf_nodes, uint32_t* mcodes, hipStream_t s);
 int sdf_check_conn(const axsq_sdf* h, hipStream_t s);
@@ -376,7 +370,8 @@ int axsq_bvh_find_count(const axsq_bvh* b, int kind, const double* x, const doub
   if (nq == 0) return AXSQ_OK;
   Workspace& ws = workspace();
   ws.find_key.valid = false;
-  AXSQ_TRY(ws.find_stash.ensure(sizeof(int32_t) * kStash * (size_t)nq));
+  const int kstash = stash_slots(nq);
+  AXSQ_TRY(ws.find_stash.ensure(sizeof(int32_t) * kstash * (size_t)nq));
   AXSQ_TRY(ws.find_cnt.ensure(sizeof(int32_t) * (size_t)nq));
   AXSQ_TRY(ws.find_over.ensure(sizeof(int32_t) * (size_t)nq));
   AXSQ_TRY(ws.find_n.ensure(sizeof(int64_t) * 2));
@@ -385,7 +380,7 @@ int axsq_bvh_find_count(const axsq_bvh* b, int kind, const double* x, const doub
   uint32_t* nover = reinterpret_cast<uint32_t*>(tot_dev + 1);
   AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(nover), 0, 1, s));
   AXSQ_TRY(find_launch(b, kind, 0, x, y, z, stride, nq, tol, counts, nullptr, nullptr,
-                       ws.find_stash.as<int32_t>(), ws.find_cnt.as<int32_t>(),
+                       ws.find_stash.as<int32_t>(), kstash, ws.find_cnt.as<int32_t>(),
                        ws.find_over.as<int32_t>(), nover, s));
   AXSQ_TRY(scan_exclusive_i32(counts, offsets, nq, tot_dev, ws, s));
   int64_t tn[2] = {0, 0};
@@ -403,6 +398,7 @@ int axsq_bvh_find_count(const axsq_bvh* b, int kind, const double* x, const doub
   key.nq = nq;
   key.tol = tol;
   key.nover = (int64_t)(uint32_t)tn[1];
+  key.kstash = kstash;
   key.valid = true;
   if (t > 0x7fffffffll) {
     set_error("candidate total %lld exceeds the int32 index range", (long long)t);
@@ -430,15 +426,15 @@ int axsq_bvh_find_fill(const axsq_bvh* b, int kind, const double* x, const doubl
                     key.stride == stride && key.nq == nq && key.tol == tol;
   key.valid = false;  // the kept ids serve one fill
   if (kept) {  // copy the kept ids; search again only for the listed queries
-    AXSQ_TRY(stash_copy_launch(ws.find_stash.as<int32_t>(), ws.find_cnt.as<int32_t>(), offsets,
-                               nq, candidates, s));
+    AXSQ_TRY(stash_copy_launch(ws.find_stash.as<int32_t>(), key.kstash,
+                               ws.find_cnt.as<int32_t>(), offsets, nq, candidates, s));
     if (key.nover > 0)
       AXSQ_TRY(find_launch(b, kind, 2, x, y, z, stride, key.nover, tol, nullptr, offsets,
-                           candidates, nullptr, nullptr, ws.find_over.as<int32_t>(), nullptr,
+                           candidates, nullptr, 0, nullptr, ws.find_over.as<int32_t>(), nullptr,
                            s));
   } else {
     AXSQ_TRY(find_launch(b, kind, 1, x, y, z, stride, nq, tol, nullptr, offsets, candidates,
-                         nullptr, nullptr, nullptr, nullptr, s));
+                         nullptr, 0, nullptr, nullptr, nullptr, s));
   }
   return finish(b->status, s);
 }

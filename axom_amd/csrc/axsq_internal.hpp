@@ -61,6 +61,7 @@ struct Workspace {
     int64_t stride = 0, nq = 0;
     double tol = 0.0;
     int64_t nover = 0;
+    int kstash = 0;
     bool valid = false;
   } find_key;
   int device = -1;
@@ -82,7 +83,13 @@ struct ProfRange {
 };
 void prof_flush();  // after a stream sync: folds finished ranges into totals
 
-constexpr int kStash = 8;  // candidate ids a query's count pass keeps for the fill pass
+// Candidate ids a query's count pass keeps for the fill pass: 32, fewer when
+// that would take more than 512 MB (at least 4).
+inline int stash_slots(int64_t nq) {
+  int k = 32;
+  while (k > 4 && (int64_t)k * 4 * nq > (int64_t(512) << 20)) k >>= 1;
+  return k;
+}
 
 // ---- primitives (axsq_prims.hip) ------------------------------------------
 constexpr int kBlock = 256;
@@ -163,4 +170,15 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n, double scale, in
 void bvh_release(axsq_bvh* b);
 // n 2-D boxes (4 doubles) -> 3-D boxes with z range [0, 0] (6 doubles)
 int widen_boxes2d(const double* boxes2d, int64_t n, double* out, hipStream_t s);
+// Candidate search launches (axsq_query.hip). mode 0: count pass (keeping the
+// first kstash ids per query when stash is given, listing the queries with
+// more in `over`); 1: fill every query; 2: fill the nq queries listed in
+// `over`.
+int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const double* y,
+                const double* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
+                const int32_t* offsets, int32_t* cands, int32_t* stash, int kstash,
+                int32_t* stash_cnt, int32_t* over, uint32_t* nover, hipStream_t s);
+// The fill for queries with at most kstash candidates, from the kept ids.
+int stash_copy_launch(const int32_t* stash, int kstash, const int32_t* stash_cnt,
+                      const int32_t* offsets, int64_t nq, int32_t* cands, hipStream_t s);
 }  // namespace axsq

@@ -172,6 +172,48 @@ __global__ __launch_bounds__(kBlock) void k_tri_tri(const double* __restrict__ t
 }
 
 // Device allocations of one call, freed on every exit path.
+// Bounding box of each cell of `npc` nodes (primal::compute_bounding_box of
+// the cell's points: BoundingBox::addPoint of each, BoundingBox.hpp:466-482,
+// a coordinate replaces the bound it is strictly below / above). A node index
+// outside [0, nverts) sets bit 1 of *status.
+__global__ __launch_bounds__(kBlock) void k_cell_boxes_n(const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ z,
+                                                         int64_t nverts,
+                                                         const int32_t* __restrict__ conn,
+                                                         int64_t ncells, int npc,
+                                                         double* __restrict__ boxes,
+                                                         int32_t* __restrict__ status) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  double b[6] = {kDblMax, kDblMax, kDblMax, -kDblMax, -kDblMax, -kDblMax};
+  for (int k = 0; k < npc; ++k) {
+    const int32_t v = conn[c * npc + k];
+    if (v < 0 || (int64_t)v >= nverts) {
+      atomicOr(status, 1);
+      continue;
+    }
+    const double p[3] = {x[v], y[v], z[v]};
+    for (int d = 0; d < 3; ++d) {
+      if (p[d] < b[d]) b[d] = p[d];
+      if (p[d] > b[3 + d]) b[3 + d] = p[d];
+    }
+  }
+  double* o = boxes + 6 * c;
+  for (int k = 0; k < 6; ++k) o[k] = b[k];
+}
+
+// The first member of every candidate pair: query cell i at
+// [offsets[i], offsets[i] + counts[i]).
+__global__ __launch_bounds__(kBlock) void k_expand_first(const int32_t* __restrict__ offsets,
+                                                         const int32_t* __restrict__ counts,
+                                                         int64_t nq, int32_t* __restrict__ first) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const int32_t o = offsets[i], c = counts[i];
+  for (int32_t j = 0; j < c; ++j) first[o + j] = (int32_t)i;
+}
+
 struct Scratch {
   std::vector<void*> ptrs;
   template <class T>
@@ -336,5 +378,132 @@ extern "C" int axsq_mesh_self_intersections(const double* x, const double* y, co
   *first = pf;
   *second = ps;
   *degenerate = pd;
+  return AXSQ_OK;
+}
+
+extern "C" int axsq_mesh_candidates(const double* ix, const double* iy, const double* iz,
+                                    int64_t inverts, const int32_t* icells, int64_t nicells,
+                                    const double* qx, const double* qy, const double* qz,
+                                    int64_t qnverts, const int32_t* qcells, int64_t nqcells,
+                                    int nodes_per_cell, int32_t** first, int32_t** second,
+                                    int64_t* npairs, void* stream) {
+  clear_error();
+  if (!first || !second || !npairs) {
+    set_error("axsq_mesh_candidates: NULL output");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  *first = *second = nullptr;
+  *npairs = 0;
+  if (nodes_per_cell < 1 || nodes_per_cell > 8) {
+    set_error("axsq_mesh_candidates: nodes_per_cell %d outside [1, 8]", nodes_per_cell);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (nicells < 0 || nqcells < 0 || nicells > 0x3fffffffll || nqcells > 0x3fffffffll ||
+      inverts < 0 || qnverts < 0 || inverts > 0x7fffffffll || qnverts > 0x7fffffffll ||
+      (nicells > 0 && (!ix || !iy || !iz || !icells)) ||
+      (nqcells > 0 && (!qx || !qy || !qz || !qcells))) {
+    set_error("axsq_mesh_candidates: invalid mesh arguments");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Scratch out;
+  int32_t *pf = nullptr, *ps = nullptr;
+  int64_t total = 0;
+  if (nicells == 0 || nqcells == 0) {
+    AXSQ_TRY(out.alloc(&pf, 1));
+    AXSQ_TRY(out.alloc(&ps, 1));
+  } else {
+    const int64_t ni = nicells, nq = nqcells;
+    Workspace& ws = workspace();
+    AXSQ_TRY(ws.mesh_box.ensure(sizeof(double) * 6 * ni));
+    AXSQ_TRY(ws.mesh_tri.ensure(sizeof(double) * 6 * nq));
+    // mesh_int: totals (2 x int64), status, then offsets and counts
+    AXSQ_TRY(ws.mesh_int.ensure(2 * sizeof(int64_t) + sizeof(int32_t) * (2 * nq + 2)));
+    double* ibox = ws.mesh_box.as<double>();
+    double* qbox = ws.mesh_tri.as<double>();
+    int64_t* tot = reinterpret_cast<int64_t*>(ws.mesh_int.as<char>());
+    int32_t* status = reinterpret_cast<int32_t*>(tot + 2);
+    int32_t* offsets = status + 2;
+    int32_t* counts = offsets + nq;
+    AXSQ_TRY(fill_i32(status, 0, 1, s));
+    k_cell_boxes_n<<<grid_for(ni), kBlock, 0, s>>>(ix, iy, iz, inverts, icells, ni,
+                                                   nodes_per_cell, ibox, status);
+    AXSQ_CHECK_HIP(hipGetLastError());
+    k_cell_boxes_n<<<grid_for(nq), kBlock, 0, s>>>(qx, qy, qz, qnverts, qcells, nq,
+                                                   nodes_per_cell, qbox, status);
+    AXSQ_CHECK_HIP(hipGetLastError());
+    int32_t st = 0;
+    AXSQ_CHECK_HIP(hipMemcpyAsync(&st, status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+    if (st) {
+      set_error("axsq_mesh_candidates: a cell refers to a vertex outside its mesh");
+      return AXSQ_ERR_INVALID_ARG;
+    }
+    // the BVH over the insert mesh's boxes, default scale
+    // (quest_candidates_example.cpp:420-423)
+    if (!ws.mesh_bvh) {
+      ws.mesh_bvh = new axsq_bvh();
+      hipGetDevice(&ws.mesh_bvh->device);
+    }
+    const int rc = bvh_build_into(ws.mesh_bvh, ibox, ni, 1.000123, AXSQ_FP64, s);
+    if (rc != AXSQ_OK) {
+      axsq_bvh_destroy(ws.mesh_bvh);
+      ws.mesh_bvh = nullptr;
+      return rc;
+    }
+    const axsq_bvh* b = ws.mesh_bvh;
+    // findBoundingBoxes for every query box (:426-434): one walk, the first
+    // ids kept, the rest walked again
+    ws.find_key.valid = false;  // the kept-id buffers are reused here
+    const int kstash = stash_slots(nq);
+    AXSQ_TRY(ws.find_stash.ensure(sizeof(int32_t) * kstash * (size_t)nq));
+    AXSQ_TRY(ws.find_cnt.ensure(sizeof(int32_t) * (size_t)nq));
+    AXSQ_TRY(ws.find_over.ensure(sizeof(int32_t) * (size_t)nq));
+    AXSQ_TRY(ws.find_n.ensure(sizeof(int64_t) * 2));
+    uint32_t* nover = reinterpret_cast<uint32_t*>(ws.find_n.as<int64_t>());
+    AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(nover), 0, 1, s));
+    const double tol = 2.220446049250313e-16;  // BVH default tolerance (unused by boxes)
+    AXSQ_TRY(find_launch(b, AXSQ_QUERY_BOXES, 0, qbox, nullptr, nullptr, 6, nq, tol, counts,
+                         nullptr, nullptr, ws.find_stash.as<int32_t>(), kstash,
+                         ws.find_cnt.as<int32_t>(), ws.find_over.as<int32_t>(), nover, s));
+    AXSQ_TRY(scan_exclusive_i32(counts, offsets, nq, tot, ws, s));
+    int64_t tn[2] = {0, 0};
+    int32_t walk_status = 0;
+    AXSQ_CHECK_HIP(hipMemcpyAsync(&tn[0], tot, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    AXSQ_CHECK_HIP(hipMemcpyAsync(&tn[1], nover, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    AXSQ_CHECK_HIP(
+        hipMemcpyAsync(&walk_status, b->status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+    if (walk_status) {
+      AXSQ_TRY(fill_i32(b->status, 0, 1, s));
+      set_error("traversal stopped: stack deeper than %d entries or malformed tree", kStack);
+      return AXSQ_ERR_STACK_OVERFLOW;
+    }
+    total = tn[0];
+    if (total > 0x7fffffffll) {
+      set_error("axsq_mesh_candidates: %lld candidate pairs exceed the int32 index range",
+                (long long)total);
+      return AXSQ_ERR_INVALID_ARG;
+    }
+    const int64_t n_over = (int64_t)(uint32_t)tn[1];
+    AXSQ_TRY(out.alloc(&pf, total));
+    AXSQ_TRY(out.alloc(&ps, total));
+    // the pair list (:440-458): query cell, then its candidates in order
+    AXSQ_TRY(stash_copy_launch(ws.find_stash.as<int32_t>(), kstash, ws.find_cnt.as<int32_t>(),
+                               offsets, nq, ps, s));
+    if (n_over > 0)
+      AXSQ_TRY(find_launch(b, AXSQ_QUERY_BOXES, 2, qbox, nullptr, nullptr, 6, n_over, tol,
+                           nullptr, offsets, ps, nullptr, 0, nullptr, ws.find_over.as<int32_t>(),
+                           nullptr, s));
+    k_expand_first<<<grid_for(nq), kBlock, 0, s>>>(offsets, counts, nq, pf);
+    AXSQ_CHECK_HIP(hipGetLastError());
+    AXSQ_CHECK_HIP(hipStreamSynchronize(s));
+    prof_flush();
+  }
+  out.keep(pf);
+  out.keep(ps);
+  *first = pf;
+  *second = ps;
+  *npairs = total;
   return AXSQ_OK;
 }

@@ -3,7 +3,7 @@
 // Reference: LinearBVH::findCandidatesImpl (src/axom/spin/policy/LinearBVH.hpp:273-402),
 // GPU form: count pass, exclusive scan, fill pass (:296-367). One lane per
 // query; candidates of query i are written in traversal order at offsets[i].
-// The count pass also keeps each query's first kStash candidate ids and lists
+// The count pass also keeps each query's first kstash candidate ids and lists
 // the queries that have more, so the fill pass copies the kept ids and
 // searches again only for the listed queries. The walk runs on the vote
 // schedule (a lane's order of node visits and leaf actions is the one-lane
@@ -40,9 +40,10 @@ struct FindArgs {
   const int32_t* offsets;
   int32_t* cands;
   int32_t* status;
-  int32_t* stash;      // count pass: kStash ids per query (or NULL)
+  int32_t* stash;      // count pass: kstash ids per query (or NULL)
+  int kstash;
   int32_t* stash_cnt;  // count pass: a copy of the counts
-  int32_t* over;       // count pass: queries with more than kStash; fill: those to search
+  int32_t* over;       // count pass: queries with more than kstash; fill: those to search
   uint32_t* nover;
 };
 
@@ -107,12 +108,12 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
       for (int k = 0; k < 6; ++k) prim[k] = a.x[i * a.stride + k];
     }
     int32_t offset = FILL ? a.offsets[i] : 0;
-    int32_t* const stash = (!FILL && a.stash) ? a.stash + i * kStash : nullptr;
+    int32_t* const stash = (!FILL && a.stash) ? a.stash + i * a.kstash : nullptr;
     auto emit = [&](int32_t pos) {
       if (FILL) {
         a.cands[offset++] = a.leaf_ids[pos];
       } else {
-        if (stash && count < kStash) stash[count] = a.leaf_ids[pos];
+        if (stash && count < a.kstash) stash[count] = a.leaf_ids[pos];
         ++count;
       }
     };
@@ -146,8 +147,8 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
       if (stash) a.stash_cnt[i] = count;
     }
   }
-  if (!FILL && a.stash) {  // queries with more than kStash: one list slot per wave
-    const bool more = live && count > kStash;
+  if (!FILL && a.stash) {  // queries with more than kstash: one list slot per wave
+    const bool more = live && count > a.kstash;
     const uint64_t m = __ballot(more);
     if (m) {
       const int lane = (int)__lane_id();
@@ -161,20 +162,20 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
   }
 }
 
-// The fill pass for queries with at most kStash candidates: the ids the count
+// The fill pass for queries with at most kstash candidates: the ids the count
 // pass kept, at offsets[i].
 __global__ __launch_bounds__(kBlock) void k_stash_copy(const int32_t* __restrict__ stash,
+                                                        int kstash,
                                                         const int32_t* __restrict__ stash_cnt,
                                                         const int32_t* __restrict__ offsets,
                                                         int64_t nq, int32_t* __restrict__ cands) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
   const int32_t c = stash_cnt[i];
-  if (c > kStash) return;
+  if (c > kstash) return;
   const int32_t o = offsets[i];
-#pragma unroll
-  for (int k = 0; k < kStash; ++k)
-    if (k < c) cands[o + k] = stash[i * kStash + k];
+  const int32_t* src = stash + i * kstash;
+  for (int32_t k = 0; k < c; ++k) cands[o + k] = src[k];
 }
 
 template <int MODE, bool MIXED>
@@ -203,12 +204,12 @@ hipError_t launch_mode(int mode, int kind, const FindArgs& a, hipStream_t s) {
 
 }  // namespace
 
-// mode 0: count (and keep the first kStash ids when stash is given); 1: fill
+// mode 0: count (and keep the first kstash ids when stash is given); 1: fill
 // every query; 2: fill the nq queries listed in `over`.
 int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const double* y,
                 const double* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
-                const int32_t* offsets, int32_t* cands, int32_t* stash, int32_t* stash_cnt,
-                int32_t* over, uint32_t* nover, hipStream_t s) {
+                const int32_t* offsets, int32_t* cands, int32_t* stash, int kstash,
+                int32_t* stash_cnt, int32_t* over, uint32_t* nover, hipStream_t s) {
   FindArgs a;
   a.nodes = b->nodes;
   a.nodes32 = b->nodes32;
@@ -227,6 +228,7 @@ int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const do
   a.cands = cands;
   a.status = b->status;
   a.stash = stash;
+  a.kstash = kstash;
   a.stash_cnt = stash_cnt;
   a.over = over;
   a.nover = nover;
@@ -241,11 +243,11 @@ int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const do
   return AXSQ_OK;
 }
 
-int stash_copy_launch(const int32_t* stash, const int32_t* stash_cnt, const int32_t* offsets,
-                      int64_t nq, int32_t* cands, hipStream_t s) {
+int stash_copy_launch(const int32_t* stash, int kstash, const int32_t* stash_cnt,
+                      const int32_t* offsets, int64_t nq, int32_t* cands, hipStream_t s) {
   if (nq <= 0) return AXSQ_OK;
   ProfRange pr("bvh_find_copy", s);
-  k_stash_copy<<<grid_for(nq), kBlock, 0, s>>>(stash, stash_cnt, offsets, nq, cands);
+  k_stash_copy<<<grid_for(nq), kBlock, 0, s>>>(stash, kstash, stash_cnt, offsets, nq, cands);
   AXSQ_CHECK_HIP(hipGetLastError());
   return AXSQ_OK;
 }

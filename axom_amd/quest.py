@@ -287,3 +287,30 @@ def findTriMeshIntersectionsBVH(vertices, triangles, intersectionThreshold=1e-8)
     degen = _take_int32(pd.value, ndeg.value, x.device)
     return torch.stack([first, second], dim=1), degen
 
+
+
+def findCandidatesBVH(insertVertices, insertCells, queryVertices, queryCells):
+    """findCandidatesBVH of src/axom/quest/examples/quest_candidates_example.cpp:
+    397-495: candidate pairs between two meshes of cells with the same number of
+    nodes (hexahedra: 8).
+
+    Vertices: (V,3) float64 CUDA tensors or (x, y, z); cells: (C,k) int32 CUDA
+    tensors, k in 1..8. Returns a (K,2) int32 CUDA tensor of (query cell, insert
+    cell) pairs whose bounding boxes overlap (the insert boxes scaled by the
+    BVH's default 1.000123), grouped by query cell ascending and, within one
+    query cell, in BVH traversal order."""
+    ix, iy, iz = _soa(insertVertices)
+    qx, qy, qz = _soa(queryVertices)
+    ic = _dev(insertCells, torch.int32, "insertCells")
+    qc = _dev(queryCells, torch.int32, "queryCells")
+    if ic.dim() != 2 or qc.dim() != 2 or ic.shape[1] != qc.shape[1]:
+        raise ValueError("cells must be (C,k) tensors with the same k for both meshes")
+    pf, ps = ctypes.c_void_p(), ctypes.c_void_p()
+    npairs = ctypes.c_int64()
+    L.check(L.lib().axsq_mesh_candidates(
+        _ptr(ix), _ptr(iy), _ptr(iz), ix.shape[0], _ptr(ic), ic.shape[0],
+        _ptr(qx), _ptr(qy), _ptr(qz), qx.shape[0], _ptr(qc), qc.shape[0], int(ic.shape[1]),
+        ctypes.byref(pf), ctypes.byref(ps), ctypes.byref(npairs), stream_ptr()))
+    first = _take_int32(pf.value, npairs.value, ix.device)
+    second = _take_int32(ps.value, npairs.value, ix.device)
+    return torch.stack([first, second], dim=1)

@@ -245,3 +245,28 @@ def cube_sphere(n, radius=1.0, project=True):
     if project:
         v = radius * v / np.linalg.norm(v, axis=1, keepdims=True)
     return np.ascontiguousarray(v), np.array(quads, dtype=np.int32)
+
+
+def hex_grid(n, origin=(0.0, 0.0, 0.0), h=1.0, jitter=0.0, seed=0, angle=0.0):
+    """A structured mesh of n^3 hexahedra (nodes in the VTK hexahedron order:
+    bottom face counter-clockwise, then the top face), spacing h from origin,
+    each vertex moved by U[-jitter, jitter] per axis, the whole grid turned by
+    `angle` radians about the z axis through its centre. Returns
+    (vertices (V,3) f64, hexes (n^3, 8) i32)."""
+    m = n + 1
+    i, j, k = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    v = np.stack([i, j, k], -1).reshape(-1, 3).astype(np.float64) * h
+    if jitter > 0.0:
+        v += np.random.default_rng(seed).uniform(-jitter, jitter, v.shape)
+    if angle != 0.0:
+        c = 0.5 * n * h
+        x, y = v[:, 0] - c, v[:, 1] - c
+        ca, sa = np.cos(angle), np.sin(angle)
+        v[:, 0], v[:, 1] = c + ca * x - sa * y, c + sa * x + ca * y
+    v += np.asarray(origin, dtype=np.float64)
+    a, b, d = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    base = ((a * m + b) * m + d).reshape(-1)
+    di, dj, dk = m * m, m, 1
+    hexes = np.stack([base, base + di, base + di + dj, base + dj,
+                      base + dk, base + di + dk, base + di + dj + dk, base + dj + dk], 1)
+    return np.ascontiguousarray(v), hexes.astype(np.int32)

@@ -57,7 +57,7 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1_000_000)
     p.add_argument("--workload", default="sdf_l6",
                    choices=["sdf_l6", "find_points", "torus_mixed", "diff_l7", "shapefit_l8",
-                            "self_isect_l8", "find_points_2d", "sdf_quads"],
+                            "self_isect_l8", "find_points_2d", "sdf_quads", "hex_candidates"],
                    help="sdf_l6 is the headline; the others print their own JSON line")
     return p.parse_args()
 
@@ -140,6 +140,32 @@ def other_workload(args, torch, dev):
                             "1,000,000 points, FP64 build + findPoints (spin::BVH<2>)",
                 "build_mprims_per_s": boxes.shape[0] / tb / 1e6, "build_ms": tb * 1e3,
                 "find_points_queries_per_s": p.shape[0] / tq, "find_ms": tq * 1e3}
+    if args.workload == "hex_candidates":
+        iv, ih = synth.hex_grid(64, jitter=0.15, seed=61)
+        qv, qh = synth.hex_grid(60, origin=(0.7, 0.3, 0.5), h=64 / 60 * 1.01, jitter=0.1,
+                                seed=62, angle=0.5)
+        ivd, ihd, qvd, qhd = (torch.as_tensor(a, device=dev) for a in (iv, ih, qv, qh))
+        pairs = quest.findCandidatesBVH(ivd, ihd, qvd, qhd)
+        tq = sync_time(lambda: quest.findCandidatesBVH(ivd, ihd, qvd, qhd), args.steps)
+        out = {"workload": f"hex candidate search: insert 64^3 = {ih.shape[0]:,} hexes "
+                           f"(jittered grid), query 60^3 = {qh.shape[0]:,} hexes (jittered, "
+                           "turned 0.5 rad); cell boxes + BVH + findBoundingBoxes + pairs",
+               "pairs": int(pairs.shape[0]), "ms": tq * 1e3,
+               "query_cells_per_s": qh.shape[0] / tq}
+        if not args.no_cpu:
+            from oracle import oracle as orc
+            nthreads = min(16, os.cpu_count() or 1)
+            m = 20_000
+            t0 = time.perf_counter()
+            want = orc.mesh_candidates(iv, ih, qv, qh[:m], nthreads=nthreads)
+            tc = time.perf_counter() - t0
+            got = pairs.cpu().numpy()
+            out["cpu_baseline"] = {"query_cells_per_s": m / tc, "cores": nthreads, "kind": "port",
+                                   "sample": f"first {m:,} query cells (with the full insert "
+                                             "mesh's BVH build)",
+                                   "matches_gpu_bitwise": bool(np.array_equal(
+                                       got[got[:, 0] < m], want))}
+        return out
     if args.workload == "sdf_quads":
         v, quads = synth.cube_sphere(128)
         conn = quads.reshape(-1)

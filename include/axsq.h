@@ -115,10 +115,11 @@ int axsq_bvh_export(const axsq_bvh* bvh, double* inner_nodes, int32_t* children,
  * Phase 1 (axsq_bvh_find_count) writes counts[nq], offsets[nq] (exclusive
  * scan) and *total. Phase 2 (axsq_bvh_find_fill) writes candidates[total]:
  * for query i, ids at [offsets[i], offsets[i]+counts[i]) in traversal order.
- * Phase 1 keeps each query's first 8 ids in the calling thread's workspace; a
- * phase 2 on the same thread that follows it with the same tree and inputs
- * copies them and walks again only for queries with more, any other phase 2
- * walks every query. Results are the same either way. */
+ * Phase 1 keeps each query's first 32 ids (fewer when that would pass 512 MB)
+ * in the calling thread's workspace; a phase 2 on the same thread that
+ * follows it with the same tree and inputs copies them and walks again only
+ * for queries with more, any other phase 2 walks every query. Results are the
+ * same either way. */
 int axsq_bvh_find_count(const axsq_bvh* bvh, int kind, const double* x, const double* y,
                         const double* z, int64_t stride, int64_t nq, double tol,
                         int32_t* offsets, int32_t* counts, int64_t* total, void* stream);
@@ -235,6 +236,23 @@ int axsq_mesh_self_intersections(const double* x, const double* y, const double*
                                  double threshold, int32_t** first, int32_t** second,
                                  int64_t* npairs, int32_t** degenerate, int64_t* ndegen,
                                  void* stream);
+
+/* Candidate pairs between two meshes of cells with nodes_per_cell nodes each
+ * (hexahedra: 8): findCandidatesBVH of src/axom/quest/examples/
+ * quest_candidates_example.cpp:397-495. Each cell's bounding box
+ * (primal::compute_bounding_box: the box of its nodes), a BVH over the insert
+ * mesh's boxes with the default scale (1.000123), findBoundingBoxes for every
+ * query box, and the pair list: first[k] the query cell, second[k] an insert
+ * cell whose scaled box overlaps it, grouped by query cell ascending and, for
+ * one query cell, in BVH traversal order (the order of the reference's
+ * candidate array). Vertices are device SoA, cells device int32
+ * (nodes_per_cell per cell, 1..8). The library allocates *first and *second
+ * (*npairs entries each, device int32); release each with axsq_free. */
+int axsq_mesh_candidates(const double* ix, const double* iy, const double* iz, int64_t inverts,
+                         const int32_t* icells, int64_t nicells, const double* qx,
+                         const double* qy, const double* qz, int64_t qnverts,
+                         const int32_t* qcells, int64_t nqcells, int nodes_per_cell,
+                         int32_t** first, int32_t** second, int64_t* npairs, void* stream);
 
 #ifdef __cplusplus
 }

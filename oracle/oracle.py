@@ -329,3 +329,31 @@ class OracleSDF:
                                _p(np.ascontiguousarray(gout, np.float32), _f32p),
                                _p(gq, _dp), _p(gv, _dp))
         return gq, gv
+
+
+# ------------------------------------------------- hex candidate search ----
+def cell_boxes(vertices, cells):
+    """primal::compute_bounding_box of each cell's nodes (BoundingBox.hpp:
+    466-482: a coordinate replaces the bound it is strictly below / above),
+    AoS (min xyz, max xyz)."""
+    v = f64(vertices).reshape(-1, 3)
+    c = np.asarray(cells, dtype=np.int64).reshape(len(cells), -1)
+    big = np.finfo(np.float64).max
+    lo = np.full((c.shape[0], 3), big)
+    hi = np.full((c.shape[0], 3), -big)
+    for k in range(c.shape[1]):
+        p = v[c[:, k]]
+        lo = np.where(p < lo, p, lo)
+        hi = np.where(p > hi, p, hi)
+    return np.concatenate([lo, hi], 1)
+
+
+def mesh_candidates(insert_vertices, insert_cells, query_vertices, query_cells, nthreads=1):
+    """findCandidatesBVH (quest/examples/quest_candidates_example.cpp:397-495):
+    (query cell, insert cell) pairs from findBoundingBoxes over a BVH of the
+    insert cells' boxes (default scale), in candidate-array order."""
+    ib = cell_boxes(insert_vertices, insert_cells)
+    qb = cell_boxes(query_vertices, query_cells)
+    off, cnt, cand = OracleBVH(ib).find("boxes", qb, nthreads=nthreads)
+    first = np.repeat(np.arange(qb.shape[0], dtype=np.int32), cnt)
+    return np.stack([first, cand.astype(np.int32)], 1)
