@@ -92,7 +92,7 @@ __device__ __forceinline__ void isect_walk(const IsectArgs& a, int64_t i, lds_i3
   const double* ti = a.tri + 9 * i;
   const V3 a0 = load_v3(ti), a1 = load_v3(ti + 3), a2 = load_v3(ti + 6);
   auto no_pref = [](const auto*, const auto*) { return false; };
-  const int32_t v = traverse<Node64, double>(
+  const int32_t v = traverse_voted<Node64, double>(
       a.nodes, a.ninner, stk,
       [&](const double* b) {
         return (prim[3] >= b[0] && prim[0] <= b[3]) && (prim[4] >= b[1] && prim[1] <= b[4]) &&
