@@ -77,6 +77,7 @@ void Workspace::release() {
   mesh_box.release();
   mesh_int.release();
   mesh_slots.release();
+  hot.release();
   find_stash.release();
   find_cnt.release();
   find_over.release();

@@ -47,6 +47,7 @@ struct Workspace {
   DevBuf k0, v0, k1, v1, hist, tiles, misc, flag, susp, cnt, redo;
   DevBuf mesh_tri, mesh_box, mesh_int, mesh_slots;  // mesh self-intersection
   DevBuf box2d;                                      // 2-D boxes widened to 3-D
+  DevBuf hot;                                        // tables of the longest saved walks
   axsq_bvh* mesh_bvh = nullptr;                      // its BVH, rebuilt per call
   // candidate search: the count pass keeps each query's first candidates
   // (find_stash, find_cnt) and lists the queries with more (find_over,

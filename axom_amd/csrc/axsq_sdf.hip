@@ -42,6 +42,11 @@
 #ifndef AXSQ_RESUME_WAVES_MIXED
 #define AXSQ_RESUME_WAVES_MIXED 2
 #endif
+// second pass: the longest saved walks (at most this many, and at most 2 GB
+// of tables) get their leaf records and node keys computed beforehand
+#ifndef AXSQ_HOT_WALKS
+#define AXSQ_HOT_WALKS 32
+#endif
 #define AXSQ_RESUME_WAVES_ATTR \
   __attribute__((amdgpu_waves_per_eu(MIXED ? AXSQ_RESUME_WAVES_MIXED : AXSQ_RESUME_WAVES)))
 #if defined(AXSQ_SDF_WAVES) && AXSQ_SDF_WAVES > 0
@@ -74,6 +79,9 @@ struct SdfArgs {
   int32_t* redo;       // queries to run again from the start (room for nq)
   uint32_t* nredo;
   const uint32_t* susp_order;  // second pass: saved walks in launch order (NULL: as saved)
+  int64_t nhot;                // second pass: walks 0 .. nhot-1 of that order read tables
+  const void* hot_leaf;        // HotLeaf[nhot][ninner + 1]
+  const void* hot_node;        // HotNode<Scalar>[nhot][ninner]
   double dom[6];
   int watertight;
   int compute_sign;
@@ -491,6 +499,20 @@ struct MemoLane {
   int32_t pl;       // this leaf's parent in the memo, 2 * index + (1 if right child)
 };
 
+// A hot walk's tables: for its query, every leaf's record (leaf_record's
+// values) and every inner node's child-box keys and bits (MemoLane::bits),
+// computed by the same functions as a memo fill, so a fill that reads them
+// holds the same values.
+struct HotLeaf {
+  double sq, p[3];
+  int32_t type, code;
+};
+template <class Scalar>
+struct HotNode {
+  Scalar lk, rk;
+  int32_t bits;
+};
+
 // bits [lo, hi) of a 64-bit mask (0 <= lo <= hi <= 64)
 __device__ __forceinline__ uint64_t bit_range(int32_t lo, int32_t hi) {
   const uint64_t below_hi = hi >= 64 ? ~uint64_t(0) : ((uint64_t(1) << hi) - 1);
@@ -556,6 +578,12 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
   else
     nodes = a.nodes;
   BoxTests<MIXED> bt(q);
+  const HotLeaf* hl = nullptr;
+  const HotNode<Scalar>* hn = nullptr;
+  if (b < a.nhot) {
+    hl = static_cast<const HotLeaf*>(a.hot_leaf) + b * (int64_t)(a.ninner + 1);
+    hn = static_cast<const HotNode<Scalar>*>(a.hot_node) + b * (int64_t)a.ninner;
+  }
 
   auto pop = [&]() {
     cur = lane_get(stk, sp);
@@ -604,17 +632,31 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
     // both loads first (indices clamped into the subtree, so every lane
     // loads), so that their latencies overlap; lanes past the end keep
     // values nothing reads
-    double T[9];
     Scalar lb[6], rb[6];
     int32_t flags;
     const int32_t my_node = nb_new + min(lane, nn_new - 1);
-    load_tri(a.tris, lo + min(lane, hi - lo), T);
-    load_node(nodes, my_node, lb, rb, mem.lc, mem.rc, flags);
+    const int32_t my_leaf = lo + min(lane, hi - lo);
+    if (hl) {  // a hot walk: the tables hold the leaf records and node keys
+      const HotLeaf r = hl[my_leaf];
+      const HotNode<Scalar> k = hn[my_node];
+      load_node(nodes, my_node, lb, rb, mem.lc, mem.rc, flags);  // links only
+      mem.sq = r.sq;
+      for (int d = 0; d < 3; ++d) mem.p[d] = r.p[d];
+      mem.type = r.type;
+      mem.code = r.code;
+      mem.lk = k.lk;
+      mem.rk = k.rk;
+      mem.bits = k.bits;
+    } else {
+      double T[9];
+      load_tri(a.tris, my_leaf, T);
+      load_node(nodes, my_node, lb, rb, mem.lc, mem.rc, flags);
+      leaf_record(q, T, mem.sq, mem.p, mem.type, mem.code);
+      mem.lk = bt.key(lb);
+      mem.rk = bt.key(rb);
+      mem.bits = (flags & 3) | (bt.right_first(lb, rb) ? 4 : 0);
+    }
     const int2 rg = *reinterpret_cast<const int2*>(&a.nodes[my_node].lo);
-    leaf_record(q, T, mem.sq, mem.p, mem.type, mem.code);
-    mem.lk = bt.key(lb);
-    mem.rk = bt.key(rb);
-    mem.bits = (flags & 3) | (bt.right_first(lb, rb) ? 4 : 0);
     mem.nlo = rg.x;
     mem.nhi = rg.y;
     // parent links inside the memo, by scatter through LDS
@@ -658,6 +700,12 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
       for (int d = 0; d < 3; ++d) cp[d] = h_p[d];
       type = h_type;
       code = h_code;
+    } else if (hl) {
+      const HotLeaf r = hl[pos];
+      sq = r.sq;
+      for (int d = 0; d < 3; ++d) cp[d] = r.p[d];
+      type = r.type;
+      code = r.code;
     } else {
       double T[9];
       load_tri(a.tris, pos, T);
@@ -1078,6 +1126,45 @@ __global__ __launch_bounds__(kSdfBlock) void k_sdf_eval_cells(SdfArgs a) {
     atomicAdd((unsigned long long*)&a.stats[1], (unsigned long long)leaves);
   }
   sdf_finish(a, i, q, m, cn);
+}
+
+// The tables of the hot walks: item k < ninner + 1 is leaf k's record for
+// the walk's query, item ninner + 1 + j inner node j's child-box keys and
+// bits (grid y: the walk).
+template <bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_hot_tables(SdfArgs a) {
+  using NodeT = typename std::conditional<MIXED, Node32, Node64>::type;
+  using Scalar = typename std::conditional<MIXED, float, double>::type;
+  const int64_t h = blockIdx.y;
+  const int64_t nleaf = (int64_t)a.ninner + 1;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= a.nhot || k >= nleaf + a.ninner) return;
+  const SdfSuspended& r = a.susp[a.susp_order ? (int64_t)a.susp_order[h] : h];
+  const int64_t i = r.qidx;
+  const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
+  if (k < nleaf) {
+    double T[9];
+    HotLeaf out;
+    load_tri(a.tris, (int32_t)k, T);
+    leaf_record(q, T, out.sq, out.p, out.type, out.code);
+    static_cast<HotLeaf*>(const_cast<void*>(a.hot_leaf))[h * nleaf + k] = out;
+  } else {
+    const int32_t j = (int32_t)(k - nleaf);
+    const NodeT* nodes;
+    if constexpr (MIXED)
+      nodes = a.nodes32;
+    else
+      nodes = a.nodes;
+    Scalar lb[6], rb[6];
+    int32_t lc, rc, flags;
+    load_node(nodes, j, lb, rb, lc, rc, flags);
+    BoxTests<MIXED> bt(q);
+    HotNode<Scalar> out;
+    out.lk = bt.key(lb);
+    out.rk = bt.key(rb);
+    out.bits = (flags & 3) | (bt.right_first(lb, rb) ? 4 : 0);
+    static_cast<HotNode<Scalar>*>(const_cast<void*>(a.hot_node))[h * (int64_t)a.ninner + j] = out;
+  }
 }
 
 // Launch order of the saved walks: the farthest from the surface first (the
@@ -1560,7 +1647,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
                     double* normal, int32_t* elem, bool stats, hipStream_t s) {
   Workspace& ws = workspace();
   const axsq_bvh* b = h->bvh;
-  SdfArgs a;
+  SdfArgs a{};
   a.nodes = b->nodes;
   a.nodes32 = b->nodes32;
   a.tris = h->tris;
@@ -1659,6 +1746,27 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     AXSQ_TRY(radix_sort_pairs(ws.k0.as<uint32_t>(), ws.v0.as<uint32_t>(), ws.k1.as<uint32_t>(),
                               ws.v1.as<uint32_t>(), nres, 32, ws, s, &sk, &sv));
     a.susp_order = sv;
+  }
+  // the first walks of the launch order (the farthest from the surface, which
+  // run longest) read tables of their leaf records and node keys
+  a.nhot = 0;
+  a.hot_leaf = a.hot_node = nullptr;
+  const int64_t nleaf = (int64_t)a.ninner + 1;
+  const size_t hot_bytes = sizeof(HotLeaf) * nleaf +
+                           (mixed ? sizeof(HotNode<float>) : sizeof(HotNode<double>)) * a.ninner;
+  const int64_t nhot = std::min<int64_t>({nres, (int64_t)AXSQ_HOT_WALKS,
+                                          (int64_t)((size_t(2) << 30) / hot_bytes)});
+  if (nhot > 0) {
+    AXSQ_TRY(ws.hot.ensure(hot_bytes * (size_t)nhot));
+    a.nhot = nhot;
+    a.hot_leaf = ws.hot.as<char>();
+    a.hot_node = ws.hot.as<char>() + sizeof(HotLeaf) * nleaf * (size_t)nhot;
+    const dim3 gh(grid_for(nleaf + a.ninner), (unsigned)nhot);
+    if (mixed)
+      k_hot_tables<true><<<gh, kBlock, 0, s>>>(a);
+    else
+      k_hot_tables<false><<<gh, kBlock, 0, s>>>(a);
+    AXSQ_CHECK_HIP(hipGetLastError());
   }
   if (nres + nredo > 0) {
     const unsigned g2 = (unsigned)(nres + nredo);
