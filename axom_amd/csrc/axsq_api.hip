@@ -692,7 +692,8 @@ int axsq_sdf_eval(const axsq_sdf* h, const double* x, const double* y, const dou
     set_error("axsq_sdf_eval: unknown fp_mode %d", fp_mode);
     return AXSQ_ERR_INVALID_ARG;
   }
-  if (fp_mode == AXSQ_FP32_TRAVERSAL && !h->bvh->nodes32) {
+  if (fp_mode == AXSQ_FP32_TRAVERSAL &&
+      (!h->bvh->nodes32 || h->bvh->fp_mode != AXSQ_FP32_TRAVERSAL)) {
     set_error("axsq_sdf_eval: FP32 traversal needs a handle created with AXSQ_FP32_TRAVERSAL");
     return AXSQ_ERR_STATE;
   }

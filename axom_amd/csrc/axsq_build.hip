@@ -437,6 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf
                                                   const int32_t* __restrict__ range,
                                                   Node64* __restrict__ nodes,
                                                   Node32* __restrict__ nodes32,
+                                                  int4* __restrict__ ranges32,
                                                   int32_t* __restrict__ status) {
   const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
   if (i == 0) *status = 0;  // the tree's error flag starts clear
@@ -476,6 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf
     h.flags = (box_valid_f(h.l) ? 1 : 0) | (box_valid_f(h.r) ? 2 : 0);
     h.pad = 0;
     nodes32[i] = h;
+    ranges32[i] = make_int4(nd.lo, nd.hi, nd.mid, 0);
   }
 }
 
@@ -551,12 +553,14 @@ void bvh_release(axsq_bvh* b) {
   if (!b) return;
   hipFree(b->nodes);
   hipFree(b->nodes32);
+  hipFree(b->ranges32);
   hipFree(b->leaf_boxes);
   hipFree(b->leaf_ids);
   hipFree(b->mcodes);
   hipFree(b->status);
   b->nodes = nullptr;
   b->nodes32 = nullptr;
+  b->ranges32 = nullptr;
   b->leaf_boxes = nullptr;
   b->leaf_ids = nullptr;
   b->mcodes = nullptr;
@@ -591,7 +595,10 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   if (b->n != n || (fp_mode == AXSQ_FP32_TRAVERSAL && !b->nodes32) || !b->nodes) {
     bvh_release(b);
     AXSQ_TRY(dmalloc(&b->nodes, (size_t)inner));
-    if (fp_mode == AXSQ_FP32_TRAVERSAL) AXSQ_TRY(dmalloc(&b->nodes32, (size_t)inner));
+    if (fp_mode == AXSQ_FP32_TRAVERSAL) {
+      AXSQ_TRY(dmalloc(&b->nodes32, (size_t)inner));
+      AXSQ_TRY(dmalloc(&b->ranges32, (size_t)inner));
+    }
     AXSQ_TRY(dmalloc(&b->leaf_boxes, 6 * (size_t)n));
     AXSQ_TRY(dmalloc(&b->leaf_ids, (size_t)n));
     AXSQ_TRY(dmalloc(&b->mcodes, (size_t)n));
@@ -673,6 +680,7 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   AXSQ_CHECK_HIP(hipGetLastError());
   k_emit<<<grid_for(inner), kBlock, 0, s>>>(b->leaf_boxes, ib, inner, lch, rch, rng, b->nodes,
                                             fp_mode == AXSQ_FP32_TRAVERSAL ? b->nodes32 : nullptr,
+                                            b->ranges32,
                                             b->status);
   AXSQ_CHECK_HIP(hipGetLastError());
   pr.stop();

@@ -131,6 +131,9 @@ struct axsq_bvh {
   int device = 0;
   axsq::Node64* nodes = nullptr;    // n-1 FP64 nodes, root = 0
   axsq::Node32* nodes32 = nullptr;  // n-1 FP32 nodes (mixed precision)
+  int4* ranges32 = nullptr;         // with nodes32: each node's leaf range and split
+                                    // {lo, hi, mid, 0}, so an FP32 walk reads 16 B, not
+                                    // the FP64 node's 128-B line
   double* leaf_boxes = nullptr;     // n scaled leaf boxes in leaf order (AoS)
   int32_t* leaf_ids = nullptr;      // n: leaf position -> original id
   uint32_t* mcodes = nullptr;       // n sorted Morton codes

@@ -64,6 +64,7 @@ constexpr int kSdfBlock = AXSQ_SDF_BLOCK;
 struct SdfArgs {
   const Node64* nodes;
   const Node32* nodes32;
+  const int4* ranges32;  // with nodes32: {lo, hi, mid, 0} per node
   const TriRec* tris;
   const double* qx;
   const double* qy;
@@ -656,7 +657,13 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
       mem.rk = bt.key(rb);
       mem.bits = (flags & 3) | (bt.right_first(lb, rb) ? 4 : 0);
     }
-    const int2 rg = *reinterpret_cast<const int2*>(&a.nodes[my_node].lo);
+    int2 rg;
+    if constexpr (MIXED) {
+      const int4 r4 = a.ranges32[my_node];
+      rg = make_int2(r4.x, r4.y);
+    } else {
+      rg = *reinterpret_cast<const int2*>(&a.nodes[my_node].lo);
+    }
     mem.nlo = rg.x;
     mem.nhi = rg.y;
     // parent links inside the memo, by scatter through LDS
@@ -981,9 +988,15 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
       const bool in_memo = cur >= nb && cur < nb + nn;
       if (!in_memo) {
         if (cur_lo < 0) {
-          const int2 rg = *reinterpret_cast<const int2*>(&a.nodes[cur].lo);
-          cur_lo = rg.x;
-          cur_hi = rg.y;
+          if constexpr (MIXED) {
+            const int4 r4 = a.ranges32[cur];
+            cur_lo = r4.x;
+            cur_hi = r4.y;
+          } else {
+            const int2 rg = *reinterpret_cast<const int2*>(&a.nodes[cur].lo);
+            cur_lo = rg.x;
+            cur_hi = rg.y;
+          }
         }
         if (cur_lo >= 0 && cur_hi <= a.ninner && cur_lo < cur_hi && cur_hi - cur_lo < kWaveLanes &&
             (cur == cur_lo || cur == cur_hi)) {
@@ -1020,7 +1033,11 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
         in_l = (flags & 1) ? bt.keep(bt.key(lb), m.sq) : false;
         in_r = (flags & 2) ? bt.keep(bt.key(rb), m.sq) : false;
         swap = bt.right_first(lb, rb);
-        const int32_t mid = a.nodes[cur].mid;
+        int32_t mid;
+        if constexpr (MIXED)
+          mid = a.ranges32[cur].z;
+        else
+          mid = a.nodes[cur].mid;
         l_lo = cur_lo;
         l_hi = mid;
         r_lo = mid + 1;
@@ -1650,6 +1667,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   SdfArgs a{};
   a.nodes = b->nodes;
   a.nodes32 = b->nodes32;
+  a.ranges32 = b->ranges32;
   a.tris = h->tris;
   a.qx = x;
   a.qy = y;
