@@ -469,6 +469,18 @@ int axsq_bvh_find(const axsq_bvh* b, int kind, const double* x, const double* y,
 }
 
 // ---- SignedDistance -------------------------------------------------------
+// First-pass step budget. The long walks (queries nearly equidistant from
+// many triangles) grow with the leaf count, and the second pass only pays off
+// for walks well above the typical length: 2048 steps at 81,920 leaves,
+// scaled by (leaves / 81,920)^(3/4), within [2048, 65536]. It changes speed
+// only, never results. AXSQ_SDF_BUDGET overrides it (0: no budget).
+static int32_t sdf_budget(int64_t nleaves) {
+  int32_t b = (int32_t)std::min(
+      65536.0, std::max(2048.0, 2048.0 * std::pow((double)nleaves / 81920.0, 0.75)));
+  if (const char* e = getenv("AXSQ_SDF_BUDGET")) b = atoi(e) > 0 ? atoi(e) : 0x7fffffff;
+  return b;
+}
+
 int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const double* z,
                     int64_t nverts, const int32_t* tris, int64_t ntris, int watertight,
                     int compute_sign, int fp_mode, void* stream) {
@@ -500,13 +512,7 @@ int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const doub
   h->bvh->device = h->device;
   const char* env = getenv("AXSQ_SORT_QUERIES");
   h->sort_queries = !(env && env[0] == '0');
-  // First-pass step budget. The long walks (queries nearly equidistant from
-  // many triangles) grow with the triangle count, and the second pass only
-  // pays off for walks well above the typical length: 2048 steps at 81,920
-  // triangles, scaled by (triangles / 81,920)^(3/4), within [2048, 65536].
-  // It changes speed only, never results.
-  h->budget = (int32_t)std::min(65536.0, std::max(2048.0, 2048.0 * std::pow((double)ntris / 81920.0, 0.75)));
-  if (const char* e = getenv("AXSQ_SDF_BUDGET")) h->budget = atoi(e) > 0 ? atoi(e) : 0x7fffffff;
+  h->budget = sdf_budget(ntris);
   int rc = AXSQ_OK;
   auto fail = [&](int code) {
     axsq_sdf_destroy(h);
@@ -587,6 +593,7 @@ int axsq_sdf_create_cells(axsq_sdf** out, const double* x, const double* y, cons
   h->watertight = watertight ? 1 : 0;
   h->compute_sign = compute_sign ? 1 : 0;
   h->fp_mode = AXSQ_FP64;  // the cell walk runs on FP64 nodes
+  h->budget = sdf_budget(ncells);
   h->bvh = new axsq_bvh();
   h->bvh->device = h->device;
   const char* env = getenv("AXSQ_SORT_QUERIES");

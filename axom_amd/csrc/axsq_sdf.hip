@@ -35,7 +35,8 @@
 #define AXSQ_SDF_VOTE 1  // first pass: vote-scheduled walk (0: loop-by-loop)
 #endif
 // second pass: waves per SIMD, FP64 / FP32-traversal form (the FP32 form,
-// with its two box tests, would spill at 3)
+// with its two box tests, would spill at 3; so would the quad-cell form, with
+// two records per memo lane)
 #ifndef AXSQ_RESUME_WAVES
 #define AXSQ_RESUME_WAVES 3
 #endif
@@ -48,7 +49,7 @@
 #define AXSQ_HOT_WALKS 32
 #endif
 #define AXSQ_RESUME_WAVES_ATTR \
-  __attribute__((amdgpu_waves_per_eu(MIXED ? AXSQ_RESUME_WAVES_MIXED : AXSQ_RESUME_WAVES)))
+  __attribute__((amdgpu_waves_per_eu((MIXED || CELLS) ? AXSQ_RESUME_WAVES_MIXED : AXSQ_RESUME_WAVES)))
 #if defined(AXSQ_SDF_WAVES) && AXSQ_SDF_WAVES > 0
 #define AXSQ_SDF_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(AXSQ_SDF_WAVES)))
 #else
@@ -334,8 +335,11 @@ struct BoxTests {
 // be run again from the start there. Splitting keeps the rare very long walks
 // (queries near the centre of a closed surface, where thousands of triangles
 // are almost equidistant) out of the coherent waves of this pass, which would
-// otherwise all wait for them.
-template <bool MIXED, bool STATS>
+// otherwise all wait for them. CELLS: a mesh with quad cells, whose leaf
+// holds its cell's one or two triangle records at tris[2 * pos + e], checked
+// in order (SignedDistance.hpp:649-658); MinCand::pos and the term list then
+// name records, not leaves.
+template <bool MIXED, bool STATS, bool CELLS = false>
 __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfArgs a) {
   using NodeT = typename std::conditional<MIXED, Node32, Node64>::type;
   using Scalar = typename std::conditional<MIXED, float, double>::type;
@@ -370,8 +374,18 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
   auto pref = [&](const Scalar* l, const Scalar* r) { return bt.right_first(l, r); };
   auto leaf = [&](int32_t pos) {
     double T[9];
-    load_tri(a.tris, pos, T);
-    check_candidate_deferred(m, q, T, pos, cn, terms, nterms);
+    if constexpr (CELLS) {
+      const int32_t ntri = reinterpret_cast<const int32_t*>(a.tris + 2 * (int64_t)pos)[19];
+      for (int e = 0; e < 2; ++e) {
+        if (e < ntri) {
+          load_tri(a.tris, 2 * pos + e, T);
+          check_candidate_deferred(m, q, T, 2 * pos + e, cn, terms, nterms);
+        }
+      }
+    } else {
+      load_tri(a.tris, pos, T);
+      check_candidate_deferred(m, q, T, pos, cn, terms, nterms);
+    }
     if (STATS) ++leaves;
   };
   int st;
@@ -492,6 +506,8 @@ template <class Scalar>
 struct MemoLane {
   double sq, p[3];
   int32_t type, code;
+  double sq2, p2[3];      // quad cells: the cell's second triangle (type2 -2: none)
+  int32_t type2, code2;
   Scalar lk, rk;  // child-box keys (BoxTests::key)
   int32_t lc, rc;
   int32_t bits;     // bit 0/1: child box valid, bit 2: right child first
@@ -521,7 +537,10 @@ __device__ __forceinline__ uint64_t bit_range(int32_t lo, int32_t hi) {
   return below_hi & ~below_lo;
 }
 
-template <bool MIXED, bool STATS>
+// CELLS: quad cells, each leaf holding one or two triangle records (as in
+// k_sdf_eval); a memo lane then holds both records of its leaf, and a leaf
+// changes the running state when either record would.
+template <bool MIXED, bool STATS, bool CELLS = false>
 __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resume_wave(SdfArgs a, int64_t nres,
                                                                  int64_t nredo) {
   static_assert(kStack == kWaveLanes, "one stack entry per lane");
@@ -606,6 +625,32 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
   int32_t h_pos = -1, h_type = 0;
   int32_t h_code = 0;
   double h_sq = 0.0, h_p[3] = {0.0, 0.0, 0.0};
+  int32_t h_type2 = -2, h_code2 = 0;  // quad cells: the held leaf's second record
+  double h_sq2 = 0.0, h_p2[3] = {0.0, 0.0, 0.0};
+  // whether a record (sq, cp, type) would change the running state: a strictly
+  // closer point, or a pseudo-normal term to add (apply_record's two cases)
+  auto rec_eff = [&](double rsq, const double rp[3], int32_t rtype) -> bool {
+    const double e0 = m.p[0] - rp[0], e1 = m.p[1] - rp[1], e2 = m.p[2] - rp[2];
+    return (rsq < m.sq) ||
+           (cn && rtype != 2 && m.type == rtype && near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., kSdfEps));
+  };
+  // a quad cell's records from the geometry (the second: type -2 when absent)
+  auto cell_records = [&](int32_t pos, double& s1, double p1[3], int32_t& t1, int32_t& c1,
+                          double& s2, double p2[3], int32_t& t2, int32_t& c2) {
+    double T[9];
+    const int32_t ntri = reinterpret_cast<const int32_t*>(a.tris + 2 * (int64_t)pos)[19];
+    load_tri(a.tris, 2 * pos, T);
+    leaf_record(q, T, s1, p1, t1, c1);
+    if (ntri >= 2) {
+      load_tri(a.tris, 2 * pos + 1, T);
+      leaf_record(q, T, s2, p2, t2, c2);
+    } else {
+      s2 = kDblMax;
+      p2[0] = p2[1] = p2[2] = 0.0;
+      t2 = -2;
+      c2 = 0;
+    }
+  };
 
   // Wave masks over the memo, bit k for lane k: the node's left / right child
   // passes the pruning test at the current minimum, the node visits its right
@@ -620,10 +665,14 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
     m_r = __ballot(node_ok && (mem.bits & 2) && bt.keep(mem.rk, m.sq));
     bool eff = false;
     if (lane <= chi - clo) {
-      const double e0 = m.p[0] - mem.p[0], e1 = m.p[1] - mem.p[1], e2 = m.p[2] - mem.p[2];
-      eff = (mem.sq < m.sq) ||
-            (cn && mem.type != 2 && m.type == mem.type &&
-             near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., kSdfEps));
+      if constexpr (CELLS) {
+        eff = rec_eff(mem.sq, mem.p, mem.type) || rec_eff(mem.sq2, mem.p2, mem.type2);
+      } else {
+        const double e0 = m.p[0] - mem.p[0], e1 = m.p[1] - mem.p[1], e2 = m.p[2] - mem.p[2];
+        eff = (mem.sq < m.sq) ||
+              (cn && mem.type != 2 && m.type == mem.type &&
+               near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., kSdfEps));
+      }
     }
     m_eff = __ballot(eff);
   };
@@ -637,7 +686,14 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
     int32_t flags;
     const int32_t my_node = nb_new + min(lane, nn_new - 1);
     const int32_t my_leaf = lo + min(lane, hi - lo);
-    if (hl) {  // a hot walk: the tables hold the leaf records and node keys
+    if constexpr (CELLS) {
+      load_node(nodes, my_node, lb, rb, mem.lc, mem.rc, flags);
+      cell_records(my_leaf, mem.sq, mem.p, mem.type, mem.code, mem.sq2, mem.p2, mem.type2,
+                   mem.code2);
+      mem.lk = bt.key(lb);
+      mem.rk = bt.key(rb);
+      mem.bits = (flags & 3) | (bt.right_first(lb, rb) ? 4 : 0);
+    } else if (hl) {  // a hot walk: the tables hold the leaf records and node keys
       const HotLeaf r = hl[my_leaf];
       const HotNode<Scalar> k = hn[my_node];
       load_node(nodes, my_node, lb, rb, mem.lc, mem.rc, flags);  // links only
@@ -695,6 +751,41 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
   auto leaf = [&](int32_t pos) {
     double sq, cp[3];
     int32_t type, code;
+    if constexpr (CELLS) {  // the cell's records in order, from the memo or the geometry
+      double sq2, cp2[3];
+      int32_t type2, code2;
+      if (pos >= clo && pos <= chi) {
+        const int32_t k = pos - clo;
+        if (!((m_eff >> k) & 1)) return;  // no change to the running state
+        sq = lane_get(mem.sq, k);
+        sq2 = lane_get(mem.sq2, k);
+        for (int d = 0; d < 3; ++d) {
+          cp[d] = lane_get(mem.p[d], k);
+          cp2[d] = lane_get(mem.p2[d], k);
+        }
+        type = lane_get(mem.type, k);
+        code = lane_get(mem.code, k);
+        type2 = lane_get(mem.type2, k);
+        code2 = lane_get(mem.code2, k);
+      } else if (pos == h_pos) {
+        sq = h_sq;
+        sq2 = h_sq2;
+        for (int d = 0; d < 3; ++d) {
+          cp[d] = h_p[d];
+          cp2[d] = h_p2[d];
+        }
+        type = h_type;
+        code = h_code;
+        type2 = h_type2;
+        code2 = h_code2;
+      } else {
+        cell_records(pos, sq, cp, type, code, sq2, cp2, type2, code2);
+      }
+      apply_record(m, a.tris, 2 * pos, sq, cp, type, code, cn);
+      if (type2 != -2) apply_record(m, a.tris, 2 * pos + 1, sq2, cp2, type2, code2, cn);
+      remask();
+      return;
+    }
     if (pos >= clo && pos <= chi) {
       const int32_t k = pos - clo;
       if (!((m_eff >> k) & 1)) return;  // no change to the running state
@@ -793,9 +884,13 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
     const bool held = (found < 0 && found != kBarrier);
     bool held_eff = true;
     if (held && -found - 1 == h_pos) {
-      const double e0 = m.p[0] - h_p[0], e1 = m.p[1] - h_p[1], e2 = m.p[2] - h_p[2];
-      held_eff = (h_sq < m.sq) || (cn && h_type != 2 && m.type == h_type &&
-                                   near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., kSdfEps));
+      if constexpr (CELLS) {
+        held_eff = rec_eff(h_sq, h_p, h_type) || rec_eff(h_sq2, h_p2, h_type2);
+      } else {
+        const double e0 = m.p[0] - h_p[0], e1 = m.p[1] - h_p[1], e2 = m.p[2] - h_p[2];
+        held_eff = (h_sq < m.sq) || (cn && h_type != 2 && m.type == h_type &&
+                                     near_eq(dot3(e0, e1, e2, e0, e1, e2), 0., kSdfEps));
+      }
     }
     const bool node_lane = lane < nn;
     const bool leaf_lane = lane <= chi - clo;
@@ -1009,6 +1104,12 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
               for (int d = 0; d < 3; ++d) h_p[d] = lane_get(mem.p[d], k);
               h_type = lane_get(mem.type, k);
               h_code = lane_get(mem.code, k);
+              if constexpr (CELLS) {
+                h_sq2 = lane_get(mem.sq2, k);
+                for (int d = 0; d < 3; ++d) h_p2[d] = lane_get(mem.p2[d], k);
+                h_type2 = lane_get(mem.type2, k);
+                h_code2 = lane_get(mem.code2, k);
+              }
             }
           }
           fill(cur, cur_lo, cur_hi);
@@ -1090,60 +1191,6 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
   sdf_finish(a, i, q, m, cn);
 }
 
-
-// Meshes with quad cells (SignedDistance.hpp:637-737 with the two triangles
-// of a quad, :649-658): one lane per query walks the whole tree in one pass
-// (the vote-scheduled walk, no step budget) and checks each reached cell's
-// one or two triangles in order, adding pseudo-normal terms as they come.
-// The leaf's records are tris[2 * pos + e]; the winner's record index is
-// MinCand::pos, so the finish is the triangle path's. A budgeted first pass
-// with a one-lane second pass over the saved walks gives identical results
-// but runs 25% slower on a 98k-quad sphere (13.2 vs 17.7 M queries/s), so
-// cells keep the one-pass form.
-template <bool STATS>
-__global__ __launch_bounds__(kSdfBlock) void k_sdf_eval_cells(SdfArgs a) {
-  __shared__ int32_t s_stack[stack_lds_words(kSdfBlock)];
-  LaneStack todo;
-  todo.lds = lane_stack_base(s_stack);
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.nq) return;
-  const int64_t i = a.order ? (int64_t)a.order[j] : j;
-  const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
-  const bool cn = a.compute_sign != 0;
-  MinCand m;
-  m.sq = kDblMax;
-  m.p[0] = m.p[1] = m.p[2] = 0.0;
-  m.sn[0] = m.sn[1] = m.sn[2] = 0.0;
-  m.type = -1;
-  m.pos = 0;
-  BoxTests<false> bt(q);
-  int32_t leaves = 0;
-  Walk w = walk_start(todo);
-  const int st = traverse_vote<Node64, double>(
-      a.nodes, a.ninner, todo, w, 0x7fffffff,
-      [&](const double* b) { return bt.keep(bt.key(b), m.sq); },
-      [&](const double* l, const double* r) { return bt.right_first(l, r); },
-      [&](int32_t pos) {
-        const int32_t ntri = reinterpret_cast<const int32_t*>(a.tris + 2 * (int64_t)pos)[19];
-        for (int e = 0; e < 2; ++e) {
-          if (e < ntri) {
-            double T[9], sq, cp[3];
-            int32_t type, code;
-            load_tri(a.tris, 2 * pos + e, T);
-            leaf_record(q, T, sq, cp, type, code);
-            apply_record(m, a.tris, 2 * pos + e, sq, cp, type, code, cn);
-          }
-        }
-        ++leaves;
-      });
-  if (st == kWalkError) atomicOr(a.status, 1);
-  if (STATS) {
-    atomicAdd((unsigned long long*)&a.stats[0],
-              (unsigned long long)(st == kWalkError ? 0 : w.visits));
-    atomicAdd((unsigned long long*)&a.stats[1], (unsigned long long)leaves);
-  }
-  sdf_finish(a, i, q, m, cn);
-}
 
 // The tables of the hot walks: item k < ninner + 1 is leaf k's record for
 // the walk's query, item ninner + 1 + j inner node j's child-box keys and
@@ -1708,17 +1755,8 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     pq.stop();
     a.order = sv;
   }
-  if (h->cells) {  // quad cells: the one-pass walk over FP64 nodes
-    ProfRange pr("sdf_eval_cells", s);
-    if (stats)
-      k_sdf_eval_cells<true><<<grid_for(nq, kSdfBlock), kSdfBlock, 0, s>>>(a);
-    else
-      k_sdf_eval_cells<false><<<grid_for(nq, kSdfBlock), kSdfBlock, 0, s>>>(a);
-    AXSQ_CHECK_HIP(hipGetLastError());
-    const_cast<axsq_sdf*>(h)->last_suspended = 0;
-    return AXSQ_OK;
-  }
-  const bool mixed = (fp_mode == AXSQ_FP32_TRAVERSAL);
+  const bool cells = h->cells;  // quad cells: FP64 nodes, two records per leaf
+  const bool mixed = (fp_mode == AXSQ_FP32_TRAVERSAL) && !cells;
   // room for suspended walks: every query, up to 2^24 records (6.0 GB); a
   // walk with no room left finishes in the first pass
   const int64_t cap = std::min<int64_t>(nq, int64_t(1) << 24);
@@ -1735,7 +1773,12 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   ProfRange pr(mixed ? "sdf_eval_fp32" : "sdf_eval_fp64", s);
   ProfRange p1(mixed ? "sdf_eval_fp32_pass1" : "sdf_eval_fp64_pass1", s);
   const unsigned g = grid_for(nq, kSdfBlock);
-  if (mixed) {
+  if (cells) {
+    if (stats)
+      k_sdf_eval<false, true, true><<<g, kSdfBlock, 0, s>>>(a);
+    else
+      k_sdf_eval<false, false, true><<<g, kSdfBlock, 0, s>>>(a);
+  } else if (mixed) {
     k_sdf_eval<true, false><<<g, kSdfBlock, 0, s>>>(a);
   } else if (stats) {
     k_sdf_eval<false, true><<<g, kSdfBlock, 0, s>>>(a);
@@ -1774,7 +1817,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
                            (mixed ? sizeof(HotNode<float>) : sizeof(HotNode<double>)) * a.ninner;
   const int64_t nhot = std::min<int64_t>({nres, (int64_t)AXSQ_HOT_WALKS,
                                           (int64_t)((size_t(2) << 30) / hot_bytes)});
-  if (nhot > 0) {
+  if (nhot > 0 && !cells) {
     AXSQ_TRY(ws.hot.ensure(hot_bytes * (size_t)nhot));
     a.nhot = nhot;
     a.hot_leaf = ws.hot.as<char>();
@@ -1788,7 +1831,12 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   }
   if (nres + nredo > 0) {
     const unsigned g2 = (unsigned)(nres + nredo);
-    if (mixed) {
+    if (cells) {
+      if (stats)
+        k_sdf_resume_wave<false, true, true><<<g2, kWaveLanes, 0, s>>>(a, nres, nredo);
+      else
+        k_sdf_resume_wave<false, false, true><<<g2, kWaveLanes, 0, s>>>(a, nres, nredo);
+    } else if (mixed) {
       k_sdf_resume_wave<true, false><<<g2, kWaveLanes, 0, s>>>(a, nres, nredo);
     } else if (stats) {
       k_sdf_resume_wave<false, true><<<g2, kWaveLanes, 0, s>>>(a, nres, nredo);

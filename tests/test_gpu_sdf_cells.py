@@ -54,6 +54,27 @@ def test_quads_near_surface_and_without_signs():
     _compare(_eval(g, q), o.eval(q, nthreads=8))
 
 
+@pytest.mark.parametrize("budget", [1, 37])
+def test_quad_suspended_walks_bit_exact(monkeypatch, budget):
+    """Quad-cell walks cut off by a small step budget and finished by the
+    wave-cooperative second pass (two triangle records per memo lane) give the
+    oracle's answers; budget 1 sends every walk to the second pass, and the
+    visit and leaf counts stay the reference traversal's."""
+    monkeypatch.setenv("AXSQ_SDF_BUDGET", str(budget))
+    v, quads = synth.cube_sphere(12)
+    conn, off, _ = mixed_cells(quads)
+    q = np.concatenate([synth.uniform(47, 0, 3 * 30_000, -1.5, 1.5).reshape(-1, 3),
+                        0.05 * synth.uniform(47, 1, 3 * 3000, -1.0, 1.0).reshape(-1, 3)])
+    g, o = _pair(v, conn, off)
+    g.enableStats(True)
+    got = _eval(g, q)
+    assert g.lastSuspended() > (q.shape[0] // 2 if budget == 1 else 0)
+    want = o.eval(q, nthreads=8, want_stats=True)
+    _compare(got, want)
+    visits, leaves = g.lastStats()
+    assert visits == int(want["visits"].sum()) and leaves == int(want["leaves"].sum())
+
+
 def test_triangle_cells_take_the_triangle_path():
     v, t, q = synth.workload2(nq=50_000, level=4)
     off = np.arange(0, 3 * t.shape[0] + 1, 3, dtype=np.int32)
