@@ -115,7 +115,7 @@ int axsq_bvh_export(const axsq_bvh* bvh, double* inner_nodes, int32_t* children,
  * Phase 1 (axsq_bvh_find_count) writes counts[nq], offsets[nq] (exclusive
  * scan) and *total. Phase 2 (axsq_bvh_find_fill) writes candidates[total]:
  * for query i, ids at [offsets[i], offsets[i]+counts[i]) in traversal order.
- * Phase 1 keeps each query's first 32 ids (fewer when that would pass 512 MB)
+ * Phase 1 keeps each query's first 64 ids (fewer when that would pass 512 MB)
  * in the calling thread's workspace; a phase 2 on the same thread that
  * follows it with the same tree and inputs copies them and walks again only
  * for queries with more, any other phase 2 walks every query. Results are the
