@@ -84,8 +84,9 @@ struct ProfRange {
 };
 void prof_flush();  // after a stream sync: folds finished ranges into totals
 
-// Candidate ids a query's count pass keeps for the fill pass: 64, fewer when
-// that would take more than 512 MB (at least 4).
+// Candidate ids a query's count pass keeps for the fill pass: 64 (the copy
+// kernel takes at most one wave's width), fewer when that would take more
+// than 512 MB (at least 4).
 inline int stash_slots(int64_t nq) {
   int k = 64;
   while (k > 4 && (int64_t)k * 4 * nq > (int64_t(512) << 20)) k >>= 1;

@@ -162,20 +162,40 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
   }
 }
 
-// The fill pass for queries with at most kstash candidates: the ids the count
-// pass kept, at offsets[i].
+// The fill pass for queries with at most kstash (<= 64) candidates: the ids
+// the count pass kept, at offsets[i]. A wave whose queries have at most 4 ids
+// each copies one query per lane; otherwise it copies its 64 queries' ids one
+// query at a time, lane l taking id l, so each load and store is one
+// contiguous run, with eight queries' loads issued before their stores.
 __global__ __launch_bounds__(kBlock) void k_stash_copy(const int32_t* __restrict__ stash,
                                                         int kstash,
                                                         const int32_t* __restrict__ stash_cnt,
                                                         const int32_t* __restrict__ offsets,
                                                         int64_t nq, int32_t* __restrict__ cands) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nq) return;
-  const int32_t c = stash_cnt[i];
-  if (c > kstash) return;
-  const int32_t o = offsets[i];
-  const int32_t* src = stash + i * kstash;
-  for (int32_t k = 0; k < c; ++k) cands[o + k] = src[k];
+  const int lane = (int)__lane_id();
+  const int64_t base = (int64_t)blockIdx.x * blockDim.x + threadIdx.x - lane;  // the wave's first
+  if (base >= nq) return;  // the whole wave
+  const int64_t mine = base + lane;
+  const int32_t c_me = (mine < nq) ? stash_cnt[mine] : 0;
+  const int32_t o_me = (mine < nq) ? offsets[mine] : 0;
+  if (__ballot(c_me > 4 && c_me <= kstash) == 0) {  // few ids everywhere: a lane per query
+    if (c_me <= kstash)
+      for (int32_t k = 0; k < c_me; ++k) cands[o_me + k] = stash[mine * kstash + k];
+    return;
+  }
+  for (int j0 = 0; j0 < 64; j0 += 8) {
+    int32_t v[8], cc[8], oo[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      cc[u] = __shfl(c_me, j0 + u);
+      oo[u] = __shfl(o_me, j0 + u);
+      const bool take = cc[u] <= kstash && lane < cc[u];
+      v[u] = take ? stash[(base + j0 + u) * kstash + lane] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (cc[u] <= kstash && lane < cc[u]) cands[oo[u] + lane] = v[u];
+  }
 }
 
 template <int MODE, bool MIXED>
