@@ -260,8 +260,9 @@ def log(msg):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
-              "(every GPU box here has one GPU)", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported: "
+              "splitting the queries across GPUs (tree and mesh replicated on each) is "
+              "not implemented", file=sys.stderr)
         return 2
 
     import torch
