@@ -22,7 +22,7 @@ gradient) over all queries, inputs resident in HBM.
                    (the FP64 kernel follows it exactly; counted on the GPU)
   cpu_baseline     the CPU oracle (restatement of the reference's SEQ path),
                    forward+backward on the first 1,000,000 queries, timed on
-                   this host's cores
+                   this host's cores (and on one thread for the first 100,000)
 """
 import argparse
 import ctypes
@@ -455,6 +455,15 @@ def main():
                                "sample": f"first {ns} of the {nq} queries, forward (dist, cp, "
                                          "elem) + adjoint, FP64",
                                "seconds": el, "matches_gpu_bitwise": same}
+        # the same on one thread (the reference's SEQ execution), on a smaller sample
+        n1 = min(100_000, ns)
+        t0 = time.perf_counter()
+        r1 = o.eval(q[:n1], nthreads=1, want_normal=False)
+        o.backward(q[:n1], r1["dist"], r1["elem"], g[:n1])
+        el1 = time.perf_counter() - t0
+        out["cpu_baseline"]["single_thread"] = {
+            "value": n1 / el1, "unit": "queries/s", "cores": 1,
+            "sample": f"first {n1} queries, forward + adjoint, FP64", "seconds": el1}
     print(json.dumps(out), flush=True)
     return 0
 
