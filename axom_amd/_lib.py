@@ -43,6 +43,8 @@ _SIGS = {
     "axsq_bvh_build2d": (_i32, [ctypes.POINTER(_vp), _vp, _i64, _dbl, _i32, _vp]),
     "axsq_bvh_dimension": (_i32, [_vp]),
     "axsq_bvh_rebuild": (_i32, [_vp, _vp, _i64, _dbl, _i32, _vp]),
+    "axsq_bvh_device_arrays": (_i32, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                      ctypes.POINTER(ctypes.c_int32)]),
     "axsq_bvh_bounds": (_i32, [_vp, _dbl3, _dbl3]),
     "axsq_bvh_num_leaves": (_i64, [_vp]),
     "axsq_bvh_export": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),

@@ -66,8 +66,10 @@ CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_facade")
 
 
 def build_cpp_tests():
-    """Host-only C++ test of include/axom_mi355x.hpp, linked to libaxsq.so."""
-    cmd = [_hipcc(), "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC,
+    """C++ test of include/axom_mi355x.hpp (with kernels of its own that use
+    the device traverser), linked to libaxsq.so."""
+    cmd = [_hipcc(), "-O2", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature",
+           "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include"), CPP_TEST_SRC,
            "-o", CPP_TEST_BIN + ".tmp", "-L", os.path.dirname(LIB), "-laxsq",
            "-Wl,-rpath,$ORIGIN/../../axom_amd/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)

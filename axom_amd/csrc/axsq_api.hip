@@ -9,6 +9,7 @@
 #include <vector>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 
 #include "axsq_internal.hpp"
@@ -322,6 +323,30 @@ int axsq_bvh_rebuild(axsq_bvh* b, const double* boxes, int64_t n, double scale, 
 }
 
 int axsq_bvh_dimension(const axsq_bvh* b) { return b ? b->dims : 0; }
+
+static_assert(sizeof(axsq_node) == sizeof(axsq::Node64), "axsq_node mirrors Node64");
+static_assert(offsetof(axsq_node, lc) == offsetof(axsq::Node64, lc) &&
+                  offsetof(axsq_node, flags) == offsetof(axsq::Node64, flags) &&
+                  offsetof(axsq_node, lo) == offsetof(axsq::Node64, lo) &&
+                  offsetof(axsq_node, mid) == offsetof(axsq::Node64, mid),
+              "axsq_node mirrors Node64");
+
+int axsq_bvh_device_arrays(const axsq_bvh* b, const axsq_node** nodes, const int32_t** leaf_ids,
+                           int32_t* ninner) {
+  clear_error();
+  if (!b || !nodes || !leaf_ids || !ninner) {
+    set_error("axsq_bvh_device_arrays: NULL argument");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (!b->nodes) {
+    set_error("axsq_bvh_device_arrays: the tree is not built");
+    return AXSQ_ERR_STATE;
+  }
+  *nodes = reinterpret_cast<const axsq_node*>(b->nodes);
+  *leaf_ids = b->leaf_ids;
+  *ninner = (int32_t)(b->n - 1);
+  return AXSQ_OK;
+}
 
 int axsq_bvh_destroy(axsq_bvh* b) {
   if (!b) return AXSQ_OK;

@@ -202,6 +202,102 @@ struct BVHTypes<2> {
   using RayType = primal::Ray2;
 };
 
+// The device-side walk of spin::BVH::getTraverser (LinearBVHTraverser,
+// policy/LinearBVH.hpp:57-109, over bvh_traverse, internal/linear_bvh/
+// bvh_traverse.hpp:66-154): for HIP kernels of the caller. The walk keeps a
+// 64-entry stack; at each inner node it tests both valid child boxes with the
+// predicate, goes down the one that passes (both: the preferred one first,
+// the other pushed), holds the first leaf it reaches until a second leaf or
+// the end of a descent, and then calls the leaf action on the held leaves in
+// order: lf(leaf position, leaf id array), the id array mapping a leaf
+// position to the original box index. The overload for the BVH's point type
+// prefers the child whose box centre is nearer (an invalid right box never
+// wins); every other primitive keeps the left-first order. Device code only:
+// the arrays are device memory, valid until the tree is rebuilt or destroyed.
+template <int NDIMS>
+class MI355XTraverser {
+ public:
+  using BoxType = typename BVHTypes<NDIMS>::BoxType;
+  using PointType = typename BVHTypes<NDIMS>::PointType;
+
+  MI355XTraverser() = default;
+  MI355XTraverser(const axsq_node* nodes, const std::int32_t* leaf_ids, std::int32_t ninner)
+      : m_nodes(nodes), m_leaf_ids(leaf_ids), m_ninner(ninner) {}
+
+  template <typename LeafAction, typename Predicate>
+  __device__ void traverse_tree(const PointType& p, LeafAction&& lf, Predicate&& predicate) const {
+    walk(p, lf, predicate, [&p](const BoxType& l, const BoxType& r, bool r_valid) {
+      if (!r_valid) return false;
+      double dl = 0.0, dr = 0.0;
+      for (int d = 0; d < NDIMS; ++d) {
+        const double el = p.x[d] - 0.5 * (l.min[d] + l.max[d]);
+        const double er = p.x[d] - 0.5 * (r.min[d] + r.max[d]);
+        dl += el * el;
+        dr += er * er;
+      }
+      return dl > dr;
+    });
+  }
+
+  template <typename Primitive, typename LeafAction, typename Predicate>
+  __device__ void traverse_tree(const Primitive& p, LeafAction&& lf, Predicate&& predicate) const {
+    walk(p, lf, predicate, [](const BoxType&, const BoxType&, bool) { return false; });
+  }
+
+ private:
+  __device__ static BoxType box_of(const double* b) {
+    BoxType out;
+    for (int d = 0; d < NDIMS; ++d) {
+      out.min[d] = b[d];
+      out.max[d] = b[3 + d];
+    }
+    return out;
+  }
+
+  template <typename Primitive, typename LeafAction, typename Predicate, typename Pref>
+  __device__ void walk(const Primitive& p, LeafAction& lf, Predicate& predicate, Pref&& pref) const {
+    constexpr std::int32_t kDepth = 64;
+    constexpr std::int32_t kEnd = -2000000000;  // below every leaf link
+    std::int32_t stack[kDepth];
+    std::int32_t top = 0;
+    stack[0] = kEnd;
+    std::int32_t node = 0, held = 0;  // held < 0: a leaf waiting for its action
+    const std::int32_t* ids = m_leaf_ids;
+    while (node != kEnd) {
+      while (node >= 0) {  // descend through inner nodes
+        const axsq_node& nd = m_nodes[node];
+        const BoxType lb = box_of(nd.l), rb = box_of(nd.r);
+        const bool go_l = (nd.flags & 1) ? predicate(p, lb) : false;
+        const bool go_r = (nd.flags & 2) ? predicate(p, rb) : false;
+        if (!go_l && !go_r) {
+          node = stack[top--];
+        } else if (go_l && go_r) {
+          const bool right_first = pref(lb, rb, (nd.flags & 2) != 0);
+          node = right_first ? nd.rc : nd.lc;
+          if (top + 1 >= kDepth) return;  // deeper than the reference's stack
+          stack[++top] = right_first ? nd.lc : nd.rc;
+        } else {
+          node = go_l ? nd.lc : nd.rc;
+        }
+        if (node < 0 && held >= 0) {  // hold the first leaf, keep descending
+          held = node;
+          if (node != kEnd) node = stack[top--];
+        }
+      }
+      while (held < 0 && held != kEnd) {  // the held leaves, in order
+        lf(static_cast<std::int32_t>(-held - 1), ids);
+        held = node;
+        if (node < 0 && node != kEnd) node = stack[top--];
+      }
+      held = 0;
+    }
+  }
+
+  const axsq_node* m_nodes = nullptr;
+  const std::int32_t* m_leaf_ids = nullptr;
+  std::int32_t m_ninner = 0;
+};
+
 // The body of spin::BVH<NDIMS, MI355X_EXEC, double> for NDIMS 3 and 2: build
 // over device boxes, find* over device queries. The BVH copies the boxes; it
 // keeps no pointer to them.
@@ -300,6 +396,17 @@ class MI355XLinearBVH {
   }
 
   const axsq_bvh* handle() const { return m_bvh; }
+
+  // BVH.hpp:305-319
+  using TraverserType = MI355XTraverser<NDIMS>;
+  TraverserType getTraverser() const {
+    if (!m_bvh) throw mi355x_error(AXSQ_ERR_STATE, "BVH is not initialized");
+    const axsq_node* nodes = nullptr;
+    const std::int32_t* ids = nullptr;
+    std::int32_t ninner = 0;
+    ::axom::detail::check(axsq_bvh_device_arrays(m_bvh, &nodes, &ids, &ninner));
+    return TraverserType(nodes, ids, ninner);
+  }
 
  private:
   void find(int kind, const double* x, const double* y, const double* z, int64_t stride,

@@ -85,6 +85,28 @@ int axsq_bvh_build2d(axsq_bvh** out, const double* boxes, int64_t n, double scal
                      int fp_mode, void* stream);
 /* 2 or 3: the dimension the handle was built in (0 for NULL). */
 int axsq_bvh_dimension(const axsq_bvh* bvh);
+
+/* One inner node of the tree on the device (128 bytes). Child link >= 0: an
+ * inner node index; < 0: -(leaf position + 1). flags bit 0 / bit 1: the left /
+ * right child box is valid (BoundingBox::isValid). Boxes are the scaled boxes
+ * the BVH stores, min xyz then max xyz (a 2-D tree holds z = [0, 0]). [lo, hi]
+ * is the range of leaf positions under the node, split after mid. */
+typedef struct axsq_node {
+  double l[6];
+  double r[6];
+  int32_t lc, rc;
+  int32_t flags;
+  int32_t lo, hi, mid;
+  int32_t pad[2];
+} axsq_node;
+
+/* The device arrays a device-side walk reads: BVH::getTraverser
+ * (src/axom/spin/BVH.hpp:305-319) -> LinearBVHTraverser
+ * (src/axom/spin/policy/LinearBVH.hpp:57-109). *nodes: the ninner inner
+ * nodes, root 0; *leaf_ids: the original box id of each of the ninner + 1
+ * leaf positions. They stay valid until the tree is rebuilt or destroyed. */
+int axsq_bvh_device_arrays(const axsq_bvh* bvh, const axsq_node** nodes,
+                           const int32_t** leaf_ids, int32_t* ninner);
 /* Builds again over new boxes into an existing tree (3-D or 2-D, as it was
  * built), keeping its device arrays when the box count is unchanged: calling
  * spin::BVH::initialize again on an initialized BVH (src/axom/spin/BVH.hpp:

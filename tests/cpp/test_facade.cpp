@@ -7,6 +7,7 @@
 //   quest/tests/quest_meshtester.cpp:268-305 (caved-in tetrahedron, with and
 //   without degenerate triangles; meshes of quest_test_utilities.hpp:353-442)
 // Exit status 0 when every check passes.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <set>
@@ -288,8 +289,115 @@ static void test_bvh2d_grid() {
   (void)hipFree(dq);
 }
 
+// BVH::getTraverser in a caller's kernel (spin/policy/LinearBVH.hpp:57-109):
+// a box query with the overlap predicate must reach the same leaves in the
+// same order as findBoundingBoxes; a point query (nearer child first) the same
+// set as findPoints.
+constexpr int kTravMax = 64;
+template <typename Trav>
+__global__ void k_trav_boxes(Trav tr, const Box* q, int n, int* counts, int* cand) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = 0;
+  tr.traverse_tree(
+      q[i],
+      [&](std::int32_t pos, const std::int32_t* ids) {
+        if (c < kTravMax) cand[i * kTravMax + c] = ids[pos];
+        ++c;
+      },
+      [](const Box& a, const Box& b) {
+        for (int d = 0; d < 3; ++d)
+          if (a.max[d] < b.min[d] || a.min[d] > b.max[d]) return false;
+        return true;
+      });
+  counts[i] = c;
+}
+template <typename Trav>
+__global__ void k_trav_points(Trav tr, const Pt* q, int n, int* counts, int* cand) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = 0;
+  tr.traverse_tree(
+      q[i],
+      [&](std::int32_t pos, const std::int32_t* ids) {
+        if (c < kTravMax) cand[i * kTravMax + c] = ids[pos];
+        ++c;
+      },
+      [](const Pt& p, const Box& b) {
+        for (int d = 0; d < 3; ++d)
+          if (p.x[d] < b.min[d] || p.x[d] > b.max[d]) return false;
+        return true;
+      });
+  counts[i] = c;
+}
+
+static void test_traverser_matches_find() {
+  // 4,000 boxes and 3,000 box / point queries from a fixed LCG
+  unsigned long long st = 12345;
+  auto rnd = [&]() {
+    st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+    return (double)(st >> 11) / 9007199254740992.0;
+  };
+  std::vector<Box> boxes(4000);
+  for (Box& b : boxes)
+    for (int d = 0; d < 3; ++d) {
+      const double c = rnd(), h = 0.01 + 0.04 * rnd();
+      b.min[d] = c - h;
+      b.max[d] = c + h;
+    }
+  Box* d_boxes = to_device(boxes);
+  BVH bvh;
+  bvh.initialize(d_boxes, (IndexType)boxes.size());
+  const int n = 3000;
+  std::vector<Box> qb(n);
+  std::vector<Pt> qp(n);
+  for (int i = 0; i < n; ++i)
+    for (int d = 0; d < 3; ++d) {
+      const double c = rnd(), h = 0.02 * rnd();
+      qb[i].min[d] = c - h;
+      qb[i].max[d] = c + h;
+      qp[i].x[d] = rnd();
+    }
+  Box* d_qb = to_device(qb);
+  Pt* d_qp = to_device(qp);
+  int *d_cnt = nullptr, *d_cand = nullptr;
+  axom::detail::hip_check(hipMalloc((void**)&d_cnt, sizeof(int) * n));
+  axom::detail::hip_check(hipMalloc((void**)&d_cand, sizeof(int) * n * kTravMax));
+  const auto tr = bvh.getTraverser();
+  // boxes: same leaves, same order
+  k_trav_boxes<<<(n + 255) / 256, 256>>>(tr, d_qb, n, d_cnt, d_cand);
+  axom::detail::hip_check(hipGetLastError());
+  axom::detail::hip_check(hipDeviceSynchronize());
+  std::vector<int> cnt = to_host(d_cnt, n), cand = to_host(d_cand, (size_t)n * kTravMax);
+  Found fb = run_find(n, [&](auto o, auto c, auto& cd) { bvh.findBoundingBoxes(o, c, cd, n, d_qb); });
+  long total = 0;
+  for (int i = 0; i < n; ++i) {
+    EXPECT_EQ(cnt[i], fb.cnt[i]);
+    if (cnt[i] != fb.cnt[i] || cnt[i] > kTravMax) continue;
+    total += cnt[i];
+    for (int k = 0; k < cnt[i]; ++k) EXPECT_EQ(cand[i * kTravMax + k], fb.cand[fb.off[i] + k]);
+  }
+  EXPECT_TRUE(total > 1000);
+  // points: the same set (the nearer child is visited first)
+  k_trav_points<<<(n + 255) / 256, 256>>>(tr, d_qp, n, d_cnt, d_cand);
+  axom::detail::hip_check(hipGetLastError());
+  axom::detail::hip_check(hipDeviceSynchronize());
+  cnt = to_host(d_cnt, n);
+  cand = to_host(d_cand, (size_t)n * kTravMax);
+  Found fp = run_find(n, [&](auto o, auto c, auto& cd) { bvh.findPoints(o, c, cd, n, d_qp); });
+  for (int i = 0; i < n; ++i) {
+    EXPECT_EQ(cnt[i], fp.cnt[i]);
+    if (cnt[i] != fp.cnt[i] || cnt[i] > kTravMax) continue;
+    std::set<IndexType> a(cand.begin() + i * kTravMax, cand.begin() + i * kTravMax + cnt[i]);
+    EXPECT_TRUE(a == set_of(fp, i));
+  }
+  for (void* p : {(void*)d_boxes, (void*)d_qb, (void*)d_qp, (void*)d_cnt, (void*)d_cand})
+    (void)hipFree(p);
+}
+
 int main() {
   test_bvh_grid();
+  test_traverser_matches_find();
   test_bvh2d_grid();
   test_sdf_sphere();
   test_mesh_tester_caved_tet();
