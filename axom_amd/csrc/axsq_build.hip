@@ -50,12 +50,15 @@ __device__ __forceinline__ void invalid_box(double b[6]) {
 
 // Per block: scaled boxes out + min/max of the valid ones. A block with no
 // valid box reports an invalid partial.
+// The same launch clears zero[0, nzero) (the sort's histograms).
 __global__ __launch_bounds__(kBlock) void k_transform(const double* __restrict__ in, int64_t n_in,
                                                        int64_t n, double scale,
                                                        double* __restrict__ out,
-                                                       double* __restrict__ partials) {
+                                                       double* __restrict__ partials,
+                                                       uint32_t* __restrict__ zero, int64_t nzero) {
   __shared__ double s_red[6][kBlock];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t j = i; j < nzero; j += (int64_t)gridDim.x * blockDim.x) zero[j] = 0u;
   double acc[6];
   invalid_box(acc);
   if (i < n) {
@@ -129,10 +132,12 @@ __global__ __launch_bounds__(1024) void k_bounds(const double* __restrict__ part
   }
 }
 
+// hist0 (if given): the sort's first-pass tile histogram (low 8 bits).
 __global__ __launch_bounds__(kBlock) void k_mcodes(const double* __restrict__ boxes, int64_t n,
                                                     const BuildParams* __restrict__ prm, int dims,
                                                     uint32_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals) {
+                                                    uint32_t* __restrict__ vals,
+                                                    uint32_t* __restrict__ hist0) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double b[6];
@@ -143,8 +148,10 @@ __global__ __launch_bounds__(kBlock) void k_mcodes(const double* __restrict__ bo
     minc[d] = prm->minc[d];
     inv[d] = prm->inv[d];
   }
-  keys[i] = (dims == 2) ? morton2_of(c, minc, inv) : morton_of(c, minc, inv);
+  const uint32_t code = (dims == 2) ? morton2_of(c, minc, inv) : morton_of(c, minc, inv);
+  keys[i] = code;
   vals[i] = (uint32_t)i;
+  if (hist0) atomicAdd(&hist0[(i / kSmallSortTile) * 256 + (code & 255u)], 1u);
 }
 
 __global__ __launch_bounds__(kBlock) void k_reorder(const double* __restrict__ boxes,
@@ -650,16 +657,32 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   int32_t* ex1 = ex0 + nchunks * kBlock;
 
   ProfRange pr("bvh_build", s);
-  k_transform<<<(unsigned)nparts, kBlock, 0, s>>>(boxes, n_in, n, scale, scaled, partials);
+  // inputs of up to kSmallSortMax boxes: the first sort pass's histogram is
+  // built with the codes, the others by the scatters
+  const int bits = dims == 2 ? 32 : 30;
+  const bool small_sort = n <= kSmallSortMax;
+  const int64_t sort_tiles = (n + kSmallSortTile - 1) / kSmallSortTile;
+  const int64_t nhist = small_sort ? 4 * 256 * sort_tiles : 0;
+  uint32_t* hists = nullptr;
+  if (small_sort) {
+    AXSQ_TRY(ws.hist.ensure(sizeof(uint32_t) * (size_t)nhist));
+    hists = ws.hist.as<uint32_t>();
+  }
+  k_transform<<<(unsigned)nparts, kBlock, 0, s>>>(boxes, n_in, n, scale, scaled, partials, hists,
+                                                  nhist);
   AXSQ_CHECK_HIP(hipGetLastError());
   k_bounds<<<1, 1024, 0, s>>>(partials, nparts, scaled, n, prm);
   AXSQ_CHECK_HIP(hipGetLastError());
   uint32_t *keys = ws.k0.as<uint32_t>(), *vals = ws.v0.as<uint32_t>();
-  k_mcodes<<<grid_for(n), kBlock, 0, s>>>(scaled, n, prm, dims, keys, vals);
+  k_mcodes<<<grid_for(n), kBlock, 0, s>>>(scaled, n, prm, dims, keys, vals, hists);
   AXSQ_CHECK_HIP(hipGetLastError());
   uint32_t *sk, *sv;
-  AXSQ_TRY(radix_sort_pairs(keys, vals, ws.k1.as<uint32_t>(), ws.v1.as<uint32_t>(), n,
-                            dims == 2 ? 32 : 30, ws, s, &sk, &sv));
+  if (small_sort)
+    AXSQ_TRY(radix_sort_small_prehist(keys, vals, ws.k1.as<uint32_t>(), ws.v1.as<uint32_t>(), n,
+                                      bits, hists, s, &sk, &sv));
+  else
+    AXSQ_TRY(radix_sort_pairs(keys, vals, ws.k1.as<uint32_t>(), ws.v1.as<uint32_t>(), n, bits,
+                              ws, s, &sk, &sv));
   k_reorder<<<grid_for(n), kBlock, 0, s>>>(scaled, sv, sk, n, b->leaf_boxes, b->leaf_ids,
                                            b->mcodes);
   AXSQ_CHECK_HIP(hipGetLastError());

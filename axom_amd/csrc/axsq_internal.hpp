@@ -107,6 +107,16 @@ int fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 // Exclusive scan of n int32 counts into out; *total_dev (int64) receives the sum.
 int scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total_dev,
                        Workspace& ws, hipStream_t s);
+// Stable sort of at most kSmallSortMax (key, val) pairs by the low `bits`
+// (<= 32) bits, for a caller that builds the first pass's histogram itself:
+// hists holds one tile-major histogram per 8-bit pass (256 counts per tile of
+// kSmallSortTile keys, ceil(n / kSmallSortTile) tiles each), the first filled
+// and the others zero; each pass's scatter adds the next pass's counts.
+constexpr int64_t kSmallSortTile = 1024;
+constexpr int64_t kSmallSortMax = 256 * kSmallSortTile;
+int radix_sort_small_prehist(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                             uint32_t* vals_alt, int64_t n, int bits, uint32_t* hists,
+                             hipStream_t s, uint32_t** keys_out, uint32_t** vals_out);
 // Stable ascending sort of (keys, vals) by the low `bits` bits of the key.
 // On return the sorted data is in (*keys_out, *vals_out), which point at
 // either the input arrays or the alternates.

@@ -182,7 +182,8 @@ template <int ITEMS, bool SCANNED>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n, int shift,
     int nbits, const uint32_t* __restrict__ hist, int64_t nblocks,
-    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+    uint32_t* __restrict__ next_hist = nullptr, int next_shift = 0, int next_nbits = 0) {
   __shared__ uint32_t s_base[256];
   __shared__ uint32_t s_wcnt[kWavesPerBlock][256];
   __shared__ uint32_t s_w[kWavesPerBlock];
@@ -246,6 +247,10 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
       for (int w = 0; w < wid; ++w) pos += s_wcnt[w][d];
       keys_out[pos] = key;
       vals_out[pos] = val;
+      if (next_hist)  // the next pass's tile histogram, tiles of kBlock * ITEMS keys
+        atomicAdd(&next_hist[(int64_t)(pos / (kBlock * ITEMS)) * 256 +
+                             ((key >> next_shift) & ((1u << next_nbits) - 1u))],
+                  1u);
     }
     __syncthreads();
     uint32_t add = 0;
@@ -260,6 +265,41 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
 }
 
 }  // namespace
+
+int radix_sort_small_prehist(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                             uint32_t* vals_alt, int64_t n, int bits, uint32_t* hists,
+                             hipStream_t s, uint32_t** keys_out, uint32_t** vals_out) {
+  *keys_out = keys;
+  *vals_out = vals;
+  if (n <= 1) return AXSQ_OK;
+  if (n > kSmallSortMax) {
+    set_error("radix sort: %lld keys exceed the small sort's %lld", (long long)n,
+              (long long)kSmallSortMax);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  const int64_t nb = (n + kSmallSortTile - 1) / kSmallSortTile;
+  uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
+  int pass = 0;
+  for (int shift = 0; shift < bits; shift += 8, ++pass) {
+    const int nbits = (bits - shift < 8) ? bits - shift : 8;
+    const int next = shift + 8;
+    uint32_t* next_hist = (next < bits) ? hists + (int64_t)(pass + 1) * 256 * nb : nullptr;
+    const int next_nbits = (next < bits) ? ((bits - next < 8) ? bits - next : 8) : 0;
+    k_radix_scatter<4, false><<<(unsigned)nb, kBlock, 0, s>>>(
+        ki, vi, n, shift, nbits, hists + (int64_t)pass * 256 * nb, nb, ko, vo, next_hist, next,
+        next_nbits);
+    AXSQ_CHECK_HIP(hipGetLastError());
+    uint32_t* t = ki;
+    ki = ko;
+    ko = t;
+    t = vi;
+    vi = vo;
+    vo = t;
+  }
+  *keys_out = ki;
+  *vals_out = vi;
+  return AXSQ_OK;
+}
 
 int fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
   if (n <= 0) return AXSQ_OK;
