@@ -83,10 +83,19 @@ def other_workload(args, torch, dev):
         cnt = torch.empty(p.shape[0], dtype=torch.int32, device=dev)
         tb = sync_time(lambda: bvh.initialize(bx), args.steps)
         tq = sync_time(lambda: bvh.findPoints(off, cnt, p.shape[0], p), args.steps)
+        c64 = bvh.findPoints(off, cnt, p.shape[0], p)
+        bvh32 = spin.BVH(bx, fp_mode="fp32")
+        off32, cnt32 = torch.empty_like(off), torch.empty_like(cnt)
+        tb32 = sync_time(lambda: bvh32.initialize(bx), args.steps)
+        tq32 = sync_time(lambda: bvh32.findPoints(off32, cnt32, p.shape[0], p), args.steps)
+        c32 = bvh32.findPoints(off32, cnt32, p.shape[0], p)
+        same = bool(torch.equal(c32, c64) and torch.equal(off32, off) and torch.equal(cnt32, cnt))
         return {"workload": "100,000 boxes U[0,1]^3 (half-extents U[0.001,0.01]), 1,000,000 "
                             "points, FP64 build + findPoints",
                 "build_mprims_per_s": boxes.shape[0] / tb / 1e6, "build_ms": tb * 1e3,
-                "find_points_queries_per_s": p.shape[0] / tq, "find_ms": tq * 1e3}
+                "find_points_queries_per_s": p.shape[0] / tq, "find_ms": tq * 1e3,
+                "fp32_traversal": {"build_ms": tb32 * 1e3, "find_ms": tq32 * 1e3,
+                                   "lists_identical_to_fp64": same}}
     if args.workload == "torus_mixed":
         v, t, q = synth.workload3()
         vd, td, qd = (torch.as_tensor(a, device=dev) for a in (v, t, q))
