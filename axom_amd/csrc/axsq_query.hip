@@ -120,9 +120,11 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
     auto no_pref = [](const auto*, const auto*) { return false; };
     int32_t v;
     if (!MIXED) {
+      // an invalid box (min > max on some axis) is never entered
       v = traverse_voted<Node64, double>(
-          a.nodes, a.ninner, stk, [&](const double* b) { return pred64<KIND>(prim, a.tol, b); },
-          no_pref, emit);
+          a.nodes, a.ninner, stk,
+          [&](const double* b) { return valid6(b) && pred64<KIND>(prim, a.tol, b); }, no_pref,
+          emit);
     } else {
       float lo[3], hi[3];
       const int hoff = (KIND == AXSQ_QUERY_BOXES) ? 3 : 0;

@@ -266,3 +266,20 @@ def test_bad_arguments_raise():
     cnt = torch.empty(4, dtype=torch.int32, device=DEV)
     with pytest.raises(ValueError):
         g.findPoints(off, cnt, 4, _t(np.zeros((4, 3))))
+
+
+def test_inverted_leaf_box_is_never_reported():
+    """An inverted leaf box (min > max on x) that the query box would overlap
+    if its bounds were taken as they stand: the traversal tests a box only when
+    it is valid, as the oracle does, so neither the FP64 nor the FP32 form
+    reports it, for boxes or for rays."""
+    boxes = _boxes(1000, 11)
+    boxes[5] = [0.6, 0.5, 0.5, 0.4, 0.6, 0.6]
+    queries = {"boxes": np.array([[0.3, 0.4, 0.4, 0.7, 0.7, 0.7]]),
+               "rays": np.array([[0.0, 0.55, 0.55, 1.0, 0.0, 0.0]])}
+    for kind, q in queries.items():
+        ooff, ocnt, ocand = orc.OracleBVH(boxes, 1.000123).find(kind, q)
+        assert 5 not in ocand
+        for mode in ("fp64", "fp32"):
+            _, cnt, cand = _find(spin.BVH(_t(boxes), fp_mode=mode), kind, q, 1)
+            assert np.array_equal(cnt, ocnt) and np.array_equal(cand, ocand), (kind, mode)
