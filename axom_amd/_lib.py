@@ -9,7 +9,7 @@ import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("AXSQ_LIB_PATH") or os.path.join(HERE, "lib", "libaxsq.so")
-HEADER = os.path.join(os.path.dirname(HERE), "include", "axsq.h")
+HEADERS = [os.path.join(os.path.dirname(HERE), "include", h) for h in ("axsq.h", "axsq_quest.h")]
 
 AXSQ_OK = 0
 AXSQ_FP64 = 0
@@ -18,6 +18,8 @@ QUERY_POINTS, QUERY_BOXES, QUERY_RAYS = 0, 1, 2
 
 ERRORS = {-1: "AXSQ_ERR_INVALID_ARG", -2: "AXSQ_ERR_HIP", -3: "AXSQ_ERR_NOMEM",
           -4: "AXSQ_ERR_STACK_OVERFLOW", -5: "AXSQ_ERR_STATE"}
+ERRORS_BY_NAME = {v: k for k, v in ERRORS.items()}
+AXSQ_ALLOC_DEVICE, AXSQ_ALLOC_MANAGED, AXSQ_ALLOC_PINNED_HOST = 0, 1, 2
 
 _lib = None
 
@@ -50,7 +52,20 @@ _SIGS = {
     "axsq_bvh_export": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "axsq_bvh_find_count": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp,
                                    ctypes.POINTER(_i64), _vp]),
-    "axsq_bvh_find_fill": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp, _vp]),
+    "axsq_bvh_find_fill": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp, _vp,
+                                  _vp]),
+    "axsq_bvh_find_into": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp, _vp,
+                                  _i64, ctypes.POINTER(_i64), _vp]),
+    "axsq_bvh_set_allocator": (_i32, [_vp, _i32]),
+    "axsq_bvh_get_allocator": (_i32, [_vp]),
+    "axsq_bvh_write_vtk": (_i32, [_vp, ctypes.c_char_p]),
+    "axsq_host_free": (_i32, [_vp]),
+    "axsq_release_workspace": (_i32, []),
+    "axsq_stl_read": (_i32, [ctypes.c_char_p, ctypes.POINTER(_vp), ctypes.POINTER(_i64),
+                             ctypes.POINTER(_i64)]),
+    "axsq_sdf_create_from_stl": (_i32, [ctypes.POINTER(_vp), ctypes.c_char_p, _i32, _i32, _i32,
+                                        _vp]),
+    "axsq_sdf_eval_point": (_i32, [_vp, _dbl3, _i32, ctypes.POINTER(_dbl), _dbl3, _dbl3]),
     "axsq_bvh_find": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp,
                              ctypes.POINTER(_vp), ctypes.POINTER(_i64), _vp]),
     "axsq_sdf_create": (_i32, [ctypes.POINTER(_vp), _vp, _vp, _vp, _i64, _vp, _i64, _i32, _i32,
@@ -72,6 +87,24 @@ _SIGS = {
     "axsq_mesh_candidates": (_i32, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _vp,
                                     _i64, _i32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                     ctypes.POINTER(_i64), _vp]),
+    # quest signed-distance interface (include/axsq_quest.h)
+    "QUEST_signed_distance_init_serial": (_i32, [ctypes.c_char_p]),
+    "QUEST_signed_distance_init_serial_bufferify": (_i32, [ctypes.c_char_p, _i32]),
+    "QUEST_signed_distance_initialized": (ctypes.c_bool, []),
+    "QUEST_signed_distance_get_mesh_bounds": (None, [_dbl3, _dbl3]),
+    "QUEST_signed_distance_set_dimension": (None, [_i32]),
+    "QUEST_signed_distance_set_closed_surface": (None, [ctypes.c_bool]),
+    "QUEST_signed_distance_set_compute_signs": (None, [ctypes.c_bool]),
+    "QUEST_signed_distance_set_allocator": (None, [_i32]),
+    "QUEST_signed_distance_set_verbose": (None, [ctypes.c_bool]),
+    "QUEST_signed_distance_use_shared_memory": (None, [ctypes.c_bool]),
+    "QUEST_signed_distance_set_execution_space": (None, [_i32]),
+    "QUEST_signed_distance_evaluate_0": (_dbl, [_dbl, _dbl, _dbl]),
+    "QUEST_signed_distance_evaluate_1": (_dbl, [_dbl, _dbl, _dbl] + [ctypes.POINTER(_dbl)] * 6),
+    "QUEST_signed_distance_finalize": (None, []),
+    "axsq_signed_distance_init_mesh": (_i32, [_vp, _vp, _vp, _i64, _vp, _i64]),
+    "axsq_signed_distance_evaluate_batch": (_i32, [_vp, _vp, _vp, _i32, _vp]),
+    "axsq_signed_distance_set_precision": (None, [_i32]),
     "axsq_mesh_self_intersections": (_i32, [_vp, _vp, _vp, _i64, _vp, _i64, _dbl,
                                             ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                             ctypes.POINTER(_i64), ctypes.POINTER(_vp),
@@ -80,10 +113,12 @@ _SIGS = {
 
 
 def header_symbols():
-    """Names of the entry points declared in include/axsq.h."""
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(axsq_[a-z0-9_]+)\s*\(", text)))
+    """Names of the entry points declared in include/axsq.h and axsq_quest.h."""
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names.update(re.findall(r"\b((?:axsq|QUEST)_[A-Za-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def lib():
@@ -107,3 +142,9 @@ def check(rc):
         msg = lib().axsq_last_error()
         raise AxsqError(rc, msg.decode() if msg else "")
     return rc
+
+
+def last_error():
+    """Text of the calling thread's last error ("" after a call that succeeded)."""
+    msg = lib().axsq_last_error()
+    return msg.decode() if msg else ""

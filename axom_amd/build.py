@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libaxsq.so")
 SOURCES = ["axsq_prims.hip", "axsq_build.hip", "axsq_query.hip", "axsq_sdf.hip", "axsq_mesh.hip",
-           "axsq_api.hip"]
+           "axsq_api.hip", "axsq_host.hip"]
 ARCH = os.environ.get("AXSQ_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -83,7 +83,7 @@ void Workspace::release() {
   find_cnt.release();
   find_over.release();
   find_n.release();
-  find_key.valid = false;
+  one.release();
   if (mesh_bvh) {
     axsq_bvh_destroy(mesh_bvh);
     mesh_bvh = nullptr;
@@ -169,9 +169,15 @@ static int finish(int32_t* status, hipStream_t s) {
     if (h) {
       int32_t z = 0;
       AXSQ_CHECK_HIP(hipMemcpy(status, &z, sizeof(int32_t), hipMemcpyHostToDevice));
-      if (h & 1) {
+      if (h & kStatusStack) {
         set_error("traversal stopped: stack deeper than %d entries or malformed tree", kStack);
         return AXSQ_ERR_STACK_OVERFLOW;
+      }
+      if (h & kStatusFillMismatch) {
+        set_error("candidate fill: the search found another number of candidates than the "
+                  "counts give for some query (queries, tolerance or tree changed since the "
+                  "count); no id was written past a query's count");
+        return AXSQ_ERR_INVALID_ARG;
       }
       set_error("element id out of range in the adjoint input");
       return AXSQ_ERR_INVALID_ARG;
@@ -229,7 +235,14 @@ const char* axsq_last_error(void) { return t_err.c_str(); }
 int axsq_version(void) { return 100; }
 
 int axsq_free(void* p) {
-  if (p) AXSQ_CHECK_HIP(hipFree(p));
+  if (!p) return AXSQ_OK;
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) == hipSuccess && attr.type == hipMemoryTypeHost) {
+    AXSQ_CHECK_HIP(hipHostFree(p));  // a candidate array in pinned host memory
+    return AXSQ_OK;
+  }
+  (void)hipGetLastError();
+  AXSQ_CHECK_HIP(hipFree(p));
   return AXSQ_OK;
 }
 
@@ -324,6 +337,29 @@ int axsq_bvh_rebuild(axsq_bvh* b, const double* boxes, int64_t n, double scale, 
 
 int axsq_bvh_dimension(const axsq_bvh* b) { return b ? b->dims : 0; }
 
+int axsq_bvh_set_allocator(axsq_bvh* b, int allocator_id) {
+  clear_error();
+  if (!b) {
+    set_error("axsq_bvh_set_allocator: bvh is NULL");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (allocator_id != AXSQ_ALLOC_DEVICE && allocator_id != AXSQ_ALLOC_MANAGED &&
+      allocator_id != AXSQ_ALLOC_PINNED_HOST) {
+    set_error("axsq_bvh_set_allocator: unknown allocator id %d", allocator_id);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  b->allocator = allocator_id;
+  return AXSQ_OK;
+}
+
+int axsq_bvh_get_allocator(const axsq_bvh* b) { return b ? b->allocator : AXSQ_ALLOC_DEVICE; }
+
+int axsq_release_workspace(void) {
+  clear_error();
+  workspace().release();
+  return AXSQ_OK;
+}
+
 static_assert(sizeof(axsq_node) == sizeof(axsq::Node64), "axsq_node mirrors Node64");
 static_assert(offsetof(axsq_node, lc) == offsetof(axsq::Node64, lc) &&
                   offsetof(axsq_node, flags) == offsetof(axsq::Node64, flags) &&
@@ -382,6 +418,46 @@ int axsq_bvh_export(const axsq_bvh* b, double* inner_nodes, int32_t* children,
   return finish(nullptr, s);
 }
 
+// The count pass of a candidate search: counts, offsets and *total. With
+// keep, also the first stash_slots(nq) ids of every query and the list of
+// queries with more, for the fill of the same call (axsq_bvh_find).
+static int find_count_impl(const axsq_bvh* b, int kind, const double* x, const double* y,
+                           const double* z, int64_t stride, int64_t nq, double tol,
+                           int32_t* offsets, int32_t* counts, int64_t* total, bool keep,
+                           int64_t* nover_out, int* kstash_out, hipStream_t s) {
+  *total = 0;
+  if (nq == 0) return AXSQ_OK;
+  Workspace& ws = workspace();
+  const int kstash = keep ? stash_slots(nq) : 0;
+  if (keep) {
+    AXSQ_TRY(ws.find_stash.ensure(sizeof(int32_t) * kstash * (size_t)nq));
+    AXSQ_TRY(ws.find_cnt.ensure(sizeof(int32_t) * (size_t)nq));
+    AXSQ_TRY(ws.find_over.ensure(sizeof(int32_t) * (size_t)nq));
+  }
+  AXSQ_TRY(ws.find_n.ensure(sizeof(int64_t) * 2));
+  // find_n: [0] the scanned total (int64), [1] the listed queries (uint32)
+  int64_t* tot_dev = ws.find_n.as<int64_t>();
+  uint32_t* nover = reinterpret_cast<uint32_t*>(tot_dev + 1);
+  AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(nover), 0, 1, s));
+  AXSQ_TRY(find_launch(b, kind, 0, x, y, z, stride, nq, tol, counts, nullptr, nullptr, nullptr,
+                       keep ? ws.find_stash.as<int32_t>() : nullptr, kstash,
+                       keep ? ws.find_cnt.as<int32_t>() : nullptr,
+                       keep ? ws.find_over.as<int32_t>() : nullptr, nover, s));
+  AXSQ_TRY(scan_exclusive_i32(counts, offsets, nq, tot_dev, ws, s));
+  int64_t tn[2] = {0, 0};
+  AXSQ_CHECK_HIP(hipMemcpyAsync(tn, tot_dev, sizeof(tn), hipMemcpyDeviceToHost, s));
+  AXSQ_TRY(finish(b->status, s));
+  const int64_t t = tn[0];
+  if (t > 0x7fffffffll) {
+    set_error("candidate total %lld exceeds the int32 index range", (long long)t);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  *total = t;
+  if (nover_out) *nover_out = (int64_t)(uint32_t)tn[1];
+  if (kstash_out) *kstash_out = kstash;
+  return AXSQ_OK;
+}
+
 int axsq_bvh_find_count(const axsq_bvh* b, int kind, const double* x, const double* y,
                         const double* z, int64_t stride, int64_t nq, double tol,
                         int32_t* offsets, int32_t* counts, int64_t* total, void* stream) {
@@ -391,78 +467,90 @@ int axsq_bvh_find_count(const axsq_bvh* b, int kind, const double* x, const doub
     set_error("axsq_bvh_find_count: NULL output");
     return AXSQ_ERR_INVALID_ARG;
   }
-  hipStream_t s = (hipStream_t)stream;
-  *total = 0;
-  if (nq == 0) return AXSQ_OK;
-  Workspace& ws = workspace();
-  ws.find_key.valid = false;
-  const int kstash = stash_slots(nq);
-  AXSQ_TRY(ws.find_stash.ensure(sizeof(int32_t) * kstash * (size_t)nq));
-  AXSQ_TRY(ws.find_cnt.ensure(sizeof(int32_t) * (size_t)nq));
-  AXSQ_TRY(ws.find_over.ensure(sizeof(int32_t) * (size_t)nq));
-  AXSQ_TRY(ws.find_n.ensure(sizeof(int64_t) * 2));
-  // find_n: [0] the scanned total (int64), [1] the listed queries (uint32)
-  int64_t* tot_dev = ws.find_n.as<int64_t>();
-  uint32_t* nover = reinterpret_cast<uint32_t*>(tot_dev + 1);
-  AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(nover), 0, 1, s));
-  AXSQ_TRY(find_launch(b, kind, 0, x, y, z, stride, nq, tol, counts, nullptr, nullptr,
-                       ws.find_stash.as<int32_t>(), kstash, ws.find_cnt.as<int32_t>(),
-                       ws.find_over.as<int32_t>(), nover, s));
-  AXSQ_TRY(scan_exclusive_i32(counts, offsets, nq, tot_dev, ws, s));
-  int64_t tn[2] = {0, 0};
-  AXSQ_CHECK_HIP(hipMemcpyAsync(tn, tot_dev, sizeof(tn), hipMemcpyDeviceToHost, s));
-  AXSQ_TRY(finish(b->status, s));
-  const int64_t t = tn[0];
-  Workspace::FindKey& key = ws.find_key;
-  key.bvh = b;
-  key.generation = b->generation;
-  key.kind = kind;
-  key.x = x;
-  key.y = y;
-  key.z = z;
-  key.stride = stride;
-  key.nq = nq;
-  key.tol = tol;
-  key.nover = (int64_t)(uint32_t)tn[1];
-  key.kstash = kstash;
-  key.valid = true;
-  if (t > 0x7fffffffll) {
-    set_error("candidate total %lld exceeds the int32 index range", (long long)t);
-    return AXSQ_ERR_INVALID_ARG;
-  }
-  *total = t;
-  return AXSQ_OK;
+  return find_count_impl(b, kind, x, y, z, stride, nq, tol, offsets, counts, total, false,
+                         nullptr, nullptr, (hipStream_t)stream);
 }
 
 int axsq_bvh_find_fill(const axsq_bvh* b, int kind, const double* x, const double* y,
                        const double* z, int64_t stride, int64_t nq, double tol,
-                       const int32_t* offsets, int32_t* candidates, void* stream) {
+                       const int32_t* offsets, const int32_t* counts, int32_t* candidates,
+                       void* stream) {
   clear_error();
   AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq));
-  if (nq > 0 && !offsets) {
-    set_error("axsq_bvh_find_fill: offsets is NULL");
+  if (nq > 0 && (!offsets || !counts)) {
+    set_error("axsq_bvh_find_fill: offsets or counts is NULL");
     return AXSQ_ERR_INVALID_ARG;
   }
   if (nq == 0) return AXSQ_OK;
   hipStream_t s = (hipStream_t)stream;
-  Workspace& ws = workspace();
-  Workspace::FindKey& key = ws.find_key;
-  const bool kept = key.valid && key.bvh == b && key.generation == b->generation &&
-                    key.kind == kind && key.x == x && key.y == y && key.z == z &&
-                    key.stride == stride && key.nq == nq && key.tol == tol;
-  key.valid = false;  // the kept ids serve one fill
-  if (kept) {  // copy the kept ids; search again only for the listed queries
-    AXSQ_TRY(stash_copy_launch(ws.find_stash.as<int32_t>(), key.kstash,
-                               ws.find_cnt.as<int32_t>(), offsets, nq, candidates, s));
-    if (key.nover > 0)
-      AXSQ_TRY(find_launch(b, kind, 2, x, y, z, stride, key.nover, tol, nullptr, offsets,
-                           candidates, nullptr, 0, nullptr, ws.find_over.as<int32_t>(), nullptr,
-                           s));
-  } else {
-    AXSQ_TRY(find_launch(b, kind, 1, x, y, z, stride, nq, tol, nullptr, offsets, candidates,
-                         nullptr, 0, nullptr, nullptr, nullptr, s));
-  }
+  AXSQ_TRY(find_launch(b, kind, 1, x, y, z, stride, nq, tol, nullptr, counts, offsets,
+                       candidates, nullptr, 0, nullptr, nullptr, nullptr, s));
   return finish(b->status, s);
+}
+
+// Candidate array of *total ids (at least one) with the allocator the tree
+// was given (axsq_bvh_set_allocator).
+static int alloc_candidates(const axsq_bvh* b, int64_t total, int32_t** out) {
+  const size_t bytes = sizeof(int32_t) * (size_t)(total > 0 ? total : 1);
+  hipError_t e;
+  if (b->allocator == AXSQ_ALLOC_MANAGED)
+    e = hipMallocManaged((void**)out, bytes);
+  else if (b->allocator == AXSQ_ALLOC_PINNED_HOST)
+    e = hipHostMalloc((void**)out, bytes, hipHostMallocDefault);
+  else
+    e = hipMalloc((void**)out, bytes);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    set_error("candidate allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    return AXSQ_ERR_NOMEM;
+  }
+  return AXSQ_OK;
+}
+
+// The fill of a one-call search: the ids its count pass kept, then a search
+// for the nover listed queries with more.
+static int find_fill_kept(const axsq_bvh* b, int kind, const double* x, const double* y,
+                          const double* z, int64_t stride, int64_t nq, double tol,
+                          const int32_t* offsets, const int32_t* counts, int32_t* cands,
+                          int64_t nover, int kstash, hipStream_t s) {
+  Workspace& ws = workspace();
+  AXSQ_TRY(stash_copy_launch(ws.find_stash.as<int32_t>(), kstash, ws.find_cnt.as<int32_t>(),
+                             offsets, nq, cands, s));
+  if (nover > 0)
+    AXSQ_TRY(find_launch(b, kind, 2, x, y, z, stride, nover, tol, nullptr, counts, offsets,
+                         cands, nullptr, 0, nullptr, ws.find_over.as<int32_t>(), nullptr, s));
+  return finish(b->status, s);
+}
+
+static int check_find_outputs(const char* name, int64_t nq, const int32_t* offsets,
+                              const int32_t* counts, const int64_t* total) {
+  if (!total || (nq > 0 && (!offsets || !counts))) {
+    set_error("%s: NULL offsets, counts or total", name);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  return AXSQ_OK;
+}
+
+int axsq_bvh_find_into(const axsq_bvh* b, int kind, const double* x, const double* y,
+                       const double* z, int64_t stride, int64_t nq, double tol, int32_t* offsets,
+                       int32_t* counts, int32_t* candidates, int64_t capacity, int64_t* total,
+                       void* stream) {
+  clear_error();
+  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq));
+  AXSQ_TRY(check_find_outputs("axsq_bvh_find_into", nq, offsets, counts, total));
+  *total = 0;
+  if (capacity < 0 || (capacity > 0 && !candidates)) {
+    set_error("axsq_bvh_find_into: negative capacity or NULL candidates");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int64_t nover = 0;
+  int kstash = 0;
+  AXSQ_TRY(find_count_impl(b, kind, x, y, z, stride, nq, tol, offsets, counts, total, true,
+                           &nover, &kstash, s));
+  if (*total == 0 || *total > capacity) return AXSQ_OK;  // too small: nothing written
+  return find_fill_kept(b, kind, x, y, z, stride, nq, tol, offsets, counts, candidates, nover,
+                        kstash, s);
 }
 
 int axsq_bvh_find(const axsq_bvh* b, int kind, const double* x, const double* y, const double* z,
@@ -474,18 +562,22 @@ int axsq_bvh_find(const axsq_bvh* b, int kind, const double* x, const double* y,
     return AXSQ_ERR_INVALID_ARG;
   }
   *candidates = nullptr;
-  AXSQ_TRY(axsq_bvh_find_count(b, kind, x, y, z, stride, nq, tol, offsets, counts, total,
-                               stream));
+  *total = 0;
+  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq));
+  AXSQ_TRY(check_find_outputs("axsq_bvh_find", nq, offsets, counts, total));
+  hipStream_t s = (hipStream_t)stream;
+  int64_t nover = 0;
+  int kstash = 0;
+  AXSQ_TRY(find_count_impl(b, kind, x, y, z, stride, nq, tol, offsets, counts, total, true,
+                           &nover, &kstash, s));
   int32_t* c = nullptr;
-  hipError_t e = hipMalloc((void**)&c, sizeof(int32_t) * (size_t)(*total > 0 ? *total : 1));
-  if (e != hipSuccess) {
-    set_error("candidate allocation failed: %s", hipGetErrorString(e));
-    return AXSQ_ERR_NOMEM;
-  }
+  AXSQ_TRY(alloc_candidates(b, *total, &c));
   if (*total > 0) {
-    const int rc = axsq_bvh_find_fill(b, kind, x, y, z, stride, nq, tol, offsets, c, stream);
+    const int rc = find_fill_kept(b, kind, x, y, z, stride, nq, tol, offsets, counts, c, nover,
+                                  kstash, s);
     if (rc != AXSQ_OK) {
-      hipFree(c);
+      (void)axsq_free(c);
+      *total = 0;
       return rc;
     }
   }
@@ -504,6 +596,14 @@ static int32_t sdf_budget(int64_t nleaves) {
       65536.0, std::max(2048.0, 2048.0 * std::pow((double)nleaves / 81920.0, 0.75)));
   if (const char* e = getenv("AXSQ_SDF_BUDGET")) b = atoi(e) > 0 ? atoi(e) : 0x7fffffff;
   return b;
+}
+
+// AXSQ_SDF_SUSPEND_CAP fixes the number of walk records (speed only: a walk
+// over its budget with no record left finishes in the first pass); -1 when
+// unset, so the records are sized from the previous call's suspended walks.
+static int64_t sdf_suspend_cap() {
+  if (const char* e = getenv("AXSQ_SDF_SUSPEND_CAP")) return std::max<int64_t>(atoll(e), 0);
+  return -1;
 }
 
 int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const double* z,
@@ -538,6 +638,7 @@ int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const doub
   const char* env = getenv("AXSQ_SORT_QUERIES");
   h->sort_queries = !(env && env[0] == '0');
   h->budget = sdf_budget(ntris);
+  h->susp_cap_fixed = sdf_suspend_cap();
   int rc = AXSQ_OK;
   auto fail = [&](int code) {
     axsq_sdf_destroy(h);
@@ -619,6 +720,7 @@ int axsq_sdf_create_cells(axsq_sdf** out, const double* x, const double* y, cons
   h->compute_sign = compute_sign ? 1 : 0;
   h->fp_mode = AXSQ_FP64;  // the cell walk runs on FP64 nodes
   h->budget = sdf_budget(ncells);
+  h->susp_cap_fixed = sdf_suspend_cap();
   h->bvh = new axsq_bvh();
   h->bvh->device = h->device;
   const char* env = getenv("AXSQ_SORT_QUERIES");
@@ -740,6 +842,29 @@ int axsq_sdf_eval(const axsq_sdf* h, const double* x, const double* y, const dou
     AXSQ_CHECK_HIP(hipMemcpy(st, h->stats_dev, sizeof(st), hipMemcpyDeviceToHost));
     const_cast<axsq_sdf*>(h)->last_visits = st[0];
     const_cast<axsq_sdf*>(h)->last_leaves = st[1];
+  }
+  return AXSQ_OK;
+}
+
+int axsq_sdf_eval_point(const axsq_sdf* h, const double q[3], int fp_mode, double* dist,
+                        double cp[3], double normal[3]) {
+  clear_error();
+  if (!h || !q || !dist) {
+    set_error("axsq_sdf_eval_point: NULL argument");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  Workspace& ws = workspace();
+  AXSQ_TRY(ws.one.ensure(sizeof(double) * 10));
+  double* d = ws.one.as<double>();  // query, distance, closest point, normal
+  hipStream_t s = nullptr;
+  AXSQ_CHECK_HIP(hipMemcpyAsync(d, q, sizeof(double) * 3, hipMemcpyHostToDevice, s));
+  AXSQ_TRY(axsq_sdf_eval(h, d, d + 1, d + 2, 3, 1, fp_mode, d + 3, d + 4, d + 7, nullptr, s));
+  double out[7];
+  AXSQ_CHECK_HIP(hipMemcpy(out, d + 3, sizeof(out), hipMemcpyDeviceToHost));
+  *dist = out[0];
+  for (int k = 0; k < 3; ++k) {
+    if (cp) cp[k] = out[1 + k];
+    if (normal) normal[k] = out[4 + k];
   }
   return AXSQ_OK;
 }

@@ -12,7 +12,6 @@
 //   k_refit_*     bottom-up boxes; the second thread to reach a node goes on
 //                 (hand-offs kept inside workgroups by tiers of leaf spans)
 //   k_emit        128-byte FP64 nodes (+ 64-byte FP32 nodes, rounded outward)
-#include <atomic>
 
 #include "axsq_internal.hpp"
 
@@ -611,8 +610,6 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
     AXSQ_TRY(dmalloc(&b->mcodes, (size_t)n));
     AXSQ_TRY(dmalloc(&b->status, 1));
   }
-  static std::atomic<uint64_t> builds{0};
-  b->generation = ++builds;  // unique over every tree of the process
   b->n = n;
   b->fp_mode = fp_mode;
   b->scale = scale;

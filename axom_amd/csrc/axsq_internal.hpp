@@ -32,6 +32,14 @@ void clear_error();
     if (rc_ != AXSQ_OK) return rc_; \
   } while (0)
 
+// Bits of a tree's device status word (axsq_bvh::status), raised by kernels
+// and turned into error codes when the call finishes.
+enum : int32_t {
+  kStatusStack = 1,         // a walk's stack went deeper than kStack entries
+  kStatusElem = 2,          // an element id out of range in the adjoint input
+  kStatusFillMismatch = 4,  // a fill found another number of candidates than counted
+};
+
 // ---- device buffers -------------------------------------------------------
 struct DevBuf {
   void* p = nullptr;
@@ -49,22 +57,12 @@ struct Workspace {
   DevBuf box2d;                                      // 2-D boxes widened to 3-D
   DevBuf hot;                                        // tables of the longest saved walks
   axsq_bvh* mesh_bvh = nullptr;                      // its BVH, rebuilt per call
-  // candidate search: the count pass keeps each query's first candidates
-  // (find_stash, find_cnt) and lists the queries with more (find_over,
-  // their number in find_n), so that a fill pass that follows with the same
-  // inputs copies them instead of searching again
+  // one-call candidate search: the count pass keeps each query's first
+  // candidates (find_stash, find_cnt) and lists the queries with more
+  // (find_over, their number in find_n), and the fill of the same call
+  // copies them instead of searching again
   DevBuf find_stash, find_cnt, find_over, find_n;
-  struct FindKey {
-    const axsq_bvh* bvh = nullptr;
-    uint64_t generation = 0;
-    int kind = -1;
-    const double *x = nullptr, *y = nullptr, *z = nullptr;
-    int64_t stride = 0, nq = 0;
-    double tol = 0.0;
-    int64_t nover = 0;
-    int kstash = 0;
-    bool valid = false;
-  } find_key;
+  DevBuf one;  // one query point and its outputs (axsq_sdf_eval_point)
   int device = -1;
   void release();
 };
@@ -148,8 +146,8 @@ struct axsq_bvh {
   double* leaf_boxes = nullptr;     // n scaled leaf boxes in leaf order (AoS)
   int32_t* leaf_ids = nullptr;      // n: leaf position -> original id
   uint32_t* mcodes = nullptr;       // n sorted Morton codes
-  int32_t* status = nullptr;        // device flag: traversal stack overflow
-  uint64_t generation = 0;          // id of the last build, unique within the process
+  int32_t* status = nullptr;        // device status word (kStatus* bits)
+  int allocator = AXSQ_ALLOC_DEVICE;  // memory of the candidate arrays axsq_bvh_find returns
 };
 
 struct axsq_sdf {
@@ -175,6 +173,11 @@ struct axsq_sdf {
   bool sort_queries = true;
   int32_t budget = 2048;   // first-pass step budget per query walk
   int64_t last_suspended = 0;  // walks the last eval finished in its second pass
+  // records for suspended walks: sized from the walks the previous call
+  // suspended (susp_demand, -1 before the first call), or fixed by
+  // AXSQ_SDF_SUSPEND_CAP at creation (susp_cap_fixed, -1 when unset)
+  int64_t susp_demand = -1;
+  int64_t susp_cap_fixed = -1;
 };
 
 namespace axsq {
@@ -188,11 +191,12 @@ int widen_boxes2d(const double* boxes2d, int64_t n, double* out, hipStream_t s);
 // Candidate search launches (axsq_query.hip). mode 0: count pass (keeping the
 // first kstash ids per query when stash is given, listing the queries with
 // more in `over`); 1: fill every query; 2: fill the nq queries listed in
-// `over`.
+// `over`. A fill writes at most fill_cnt[i] ids for query i and raises
+// kStatusFillMismatch when its walk finds another number.
 int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const double* y,
                 const double* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
-                const int32_t* offsets, int32_t* cands, int32_t* stash, int kstash,
-                int32_t* stash_cnt, int32_t* over, uint32_t* nover, hipStream_t s);
+                const int32_t* fill_cnt, const int32_t* offsets, int32_t* cands, int32_t* stash,
+                int kstash, int32_t* stash_cnt, int32_t* over, uint32_t* nover, hipStream_t s);
 // The fill for queries with at most kstash candidates, from the kept ids.
 int stash_copy_launch(const int32_t* stash, int kstash, const int32_t* stash_cnt,
                       const int32_t* offsets, int64_t nq, int32_t* cands, hipStream_t s);

@@ -454,7 +454,6 @@ extern "C" int axsq_mesh_candidates(const double* ix, const double* iy, const do
     const axsq_bvh* b = ws.mesh_bvh;
     // findBoundingBoxes for every query box (:426-434): one walk, the first
     // ids kept, the rest walked again
-    ws.find_key.valid = false;  // the kept-id buffers are reused here
     const int kstash = stash_slots(nq);
     AXSQ_TRY(ws.find_stash.ensure(sizeof(int32_t) * kstash * (size_t)nq));
     AXSQ_TRY(ws.find_cnt.ensure(sizeof(int32_t) * (size_t)nq));
@@ -464,7 +463,7 @@ extern "C" int axsq_mesh_candidates(const double* ix, const double* iy, const do
     AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(nover), 0, 1, s));
     const double tol = 2.220446049250313e-16;  // BVH default tolerance (unused by boxes)
     AXSQ_TRY(find_launch(b, AXSQ_QUERY_BOXES, 0, qbox, nullptr, nullptr, 6, nq, tol, counts,
-                         nullptr, nullptr, ws.find_stash.as<int32_t>(), kstash,
+                         nullptr, nullptr, nullptr, ws.find_stash.as<int32_t>(), kstash,
                          ws.find_cnt.as<int32_t>(), ws.find_over.as<int32_t>(), nover, s));
     AXSQ_TRY(scan_exclusive_i32(counts, offsets, nq, tot, ws, s));
     int64_t tn[2] = {0, 0};
@@ -493,8 +492,8 @@ extern "C" int axsq_mesh_candidates(const double* ix, const double* iy, const do
                                offsets, nq, ps, s));
     if (n_over > 0)
       AXSQ_TRY(find_launch(b, AXSQ_QUERY_BOXES, 2, qbox, nullptr, nullptr, 6, n_over, tol,
-                           nullptr, offsets, ps, nullptr, 0, nullptr, ws.find_over.as<int32_t>(),
-                           nullptr, s));
+                           nullptr, counts, offsets, ps, nullptr, 0, nullptr,
+                           ws.find_over.as<int32_t>(), nullptr, s));
     k_expand_first<<<grid_for(nq), kBlock, 0, s>>>(offsets, counts, nq, pf);
     AXSQ_CHECK_HIP(hipGetLastError());
     AXSQ_CHECK_HIP(hipStreamSynchronize(s));

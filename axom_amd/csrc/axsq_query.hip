@@ -2,9 +2,12 @@
 //
 // Reference: LinearBVH::findCandidatesImpl (src/axom/spin/policy/LinearBVH.hpp:273-402),
 // GPU form: count pass, exclusive scan, fill pass (:296-367). One lane per
-// query; candidates of query i are written in traversal order at offsets[i].
-// The count pass also keeps each query's first kstash candidate ids and lists
-// the queries that have more, so the fill pass copies the kept ids and
+// query; candidates of query i are written in traversal order at offsets[i],
+// at most counts[i] of them (the fill raises kStatusFillMismatch in the status word
+// when its walk finds another number, so a caller whose inputs changed after
+// the count gets an error and no write past its buffer). In the one-call
+// search the count pass also keeps each query's first kstash candidate ids and
+// lists the queries that have more, so the fill copies the kept ids and
 // searches again only for the listed queries. The walk runs on the vote
 // schedule (a lane's order of node visits and leaf actions is the one-lane
 // walk's, so the candidate order is too): 11% faster for boxes and 34% for
@@ -36,7 +39,8 @@ struct FindArgs {
   int dims;  // 2: queries are 2-D (z = 0)
   int32_t ninner;
   double tol;
-  int32_t* counts;
+  int32_t* counts;           // count pass: output
+  const int32_t* fill_cnt;   // fill: each query's room, the counts its count pass wrote
   const int32_t* offsets;
   int32_t* cands;
   int32_t* status;
@@ -107,24 +111,25 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
     } else {
       for (int k = 0; k < 6; ++k) prim[k] = a.x[i * a.stride + k];
     }
-    int32_t offset = FILL ? a.offsets[i] : 0;
+    const int32_t offset = FILL ? a.offsets[i] : 0;
+    const int32_t room = FILL ? a.fill_cnt[i] : 0;
     int32_t* const stash = (!FILL && a.stash) ? a.stash + i * a.kstash : nullptr;
     auto emit = [&](int32_t pos) {
       if (FILL) {
-        a.cands[offset++] = a.leaf_ids[pos];
+        if (count < room) a.cands[offset + count] = a.leaf_ids[pos];
       } else {
         if (stash && count < a.kstash) stash[count] = a.leaf_ids[pos];
-        ++count;
       }
+      ++count;
     };
     auto no_pref = [](const auto*, const auto*) { return false; };
     int32_t v;
     if (!MIXED) {
-      // an invalid box (min > max on some axis) is never entered
+      // the walk tests a child box only when its validity bit is set, so an
+      // invalid box (min > max on some axis) is never entered
       v = traverse_voted<Node64, double>(
-          a.nodes, a.ninner, stk,
-          [&](const double* b) { return valid6(b) && pred64<KIND>(prim, a.tol, b); }, no_pref,
-          emit);
+          a.nodes, a.ninner, stk, [&](const double* b) { return pred64<KIND>(prim, a.tol, b); },
+          no_pref, emit);
     } else {
       float lo[3], hi[3];
       const int hoff = (KIND == AXSQ_QUERY_BOXES) ? 3 : 0;
@@ -143,7 +148,8 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
             if (valid6(lb) && pred64<KIND>(prim, a.tol, lb)) emit(pos);
           });
     }
-    if (v < 0) atomicOr(a.status, 1);
+    if (v < 0) atomicOr(a.status, kStatusStack);
+    if (FILL && count != room) atomicOr(a.status, kStatusFillMismatch);
     if (!FILL) {
       a.counts[i] = count;
       if (stash) a.stash_cnt[i] = count;
@@ -230,8 +236,8 @@ hipError_t launch_mode(int mode, int kind, const FindArgs& a, hipStream_t s) {
 // every query; 2: fill the nq queries listed in `over`.
 int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const double* y,
                 const double* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
-                const int32_t* offsets, int32_t* cands, int32_t* stash, int kstash,
-                int32_t* stash_cnt, int32_t* over, uint32_t* nover, hipStream_t s) {
+                const int32_t* fill_cnt, const int32_t* offsets, int32_t* cands, int32_t* stash,
+                int kstash, int32_t* stash_cnt, int32_t* over, uint32_t* nover, hipStream_t s) {
   FindArgs a;
   a.nodes = b->nodes;
   a.nodes32 = b->nodes32;
@@ -246,6 +252,7 @@ int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const do
   a.ninner = (int32_t)(b->n - 1);
   a.tol = tol;
   a.counts = counts;
+  a.fill_cnt = fill_cnt;
   a.offsets = offsets;
   a.cands = cands;
   a.status = b->status;

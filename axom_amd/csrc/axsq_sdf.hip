@@ -417,7 +417,7 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
     budget = 0x7fffffff;  // no room left: finish here
   }
   if (st == kWalkError) {
-    atomicOr(a.status, 1);
+    atomicOr(a.status, kStatusStack);
   } else if (nterms & kListOverflow) {
     a.redo[atomicAdd(a.nredo, 1u)] = (int32_t)i;  // one slot per query
     return;
@@ -570,7 +570,7 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
     visits = r.visits;
     leaves = r.leaves;
     if (sp < 0 || sp >= kStack || r.nterms < 0 || r.nterms > kNormList) {
-      if (lane == 0) atomicOr(a.status, 1);
+      if (lane == 0) atomicOr(a.status, kStatusStack);
       return;
     }
     stk = (lane == 0) ? kBarrier : (lane <= sp ? r.stack[lane] : 0);
@@ -1183,7 +1183,7 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
     }
   }
   if (lane != 0) return;
-  if (err) atomicOr(a.status, 1);
+  if (err) atomicOr(a.status, kStatusStack);
   if (STATS) {
     atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)(err ? 0 : visits));
     atomicAdd((unsigned long long*)&a.stats[1], (unsigned long long)leaves);
@@ -1437,7 +1437,7 @@ __global__ __launch_bounds__(kBlock) void k_segments(const uint32_t* __restrict_
   if (p >= n) return;
   const uint32_t k = keys[p];
   if ((int64_t)k >= nseg) {
-    atomicOr(status, 2);
+    atomicOr(status, kStatusElem);
     return;
   }
   if (p == 0 || keys[p - 1] != k) start[k] = (int32_t)p;
@@ -1757,9 +1757,20 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   }
   const bool cells = h->cells;  // quad cells: FP64 nodes, two records per leaf
   const bool mixed = (fp_mode == AXSQ_FP32_TRAVERSAL) && !cells;
-  // room for suspended walks: every query, up to 2^24 records (6.0 GB); a
-  // walk with no room left finishes in the first pass
-  const int64_t cap = std::min<int64_t>(nq, int64_t(1) << 24);
+  // room for suspended walks (360 B each): as many as the previous call on
+  // this handle suspended, plus a quarter and 4,096; on the first call an
+  // eighth of the queries (at least 65,536); never more than the queries or
+  // 2^24 (6.0 GB). A walk with no room left finishes in the first pass, and
+  // the next call makes room for it.
+  const int64_t full = std::min<int64_t>(nq, int64_t(1) << 24);
+  int64_t cap;
+  if (h->susp_cap_fixed >= 0)
+    cap = std::min<int64_t>(full, h->susp_cap_fixed);
+  else if (h->susp_demand < 0)
+    cap = std::min<int64_t>(full, std::max<int64_t>(65536, nq / 8));
+  else
+    cap = std::min<int64_t>(full, h->susp_demand + h->susp_demand / 4 + 4096);
+  cap = std::max<int64_t>(cap, 1);
   AXSQ_TRY(ws.susp.ensure(sizeof(SdfSuspended) * (size_t)cap));
   AXSQ_TRY(ws.redo.ensure(sizeof(int32_t) * (size_t)nq));
   AXSQ_TRY(ws.cnt.ensure(sizeof(uint32_t) * 2));
@@ -1793,6 +1804,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   const int64_t nres = std::min<int64_t>(counts[0], cap);
   const int64_t nredo = std::min<int64_t>(counts[1], nq);
   const_cast<axsq_sdf*>(h)->last_suspended = nres + nredo;
+  const_cast<axsq_sdf*>(h)->susp_demand = counts[0];  // saved or not
   ProfRange p2(mixed ? "sdf_eval_fp32_pass2" : "sdf_eval_fp64_pass2", s);
   a.susp_order = nullptr;
   if (nres > 64) {  // the query order is no longer needed: reuse its buffers

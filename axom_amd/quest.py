@@ -60,6 +60,25 @@ class SignedDistance:
                                                   stream_ptr()))
         self._h = h
 
+    @classmethod
+    def fromSTL(cls, path, isWatertight=True, computeSign=True, fp_mode="fp64"):
+        """The query over the triangle soup of an STL file, read as
+        quest::STLReader reads it (axsq_sdf_create_from_stl)."""
+        self = cls.__new__(cls)
+        self._h = None
+        self._fp = FP_MODES[fp_mode]
+        h = ctypes.c_void_p()
+        L.check(L.lib().axsq_sdf_create_from_stl(ctypes.byref(h), str(path).encode(),
+                                                 int(bool(isWatertight)), int(bool(computeSign)),
+                                                 self._fp, stream_ptr()))
+        self._h = h
+        nodes, nn, nf = ctypes.c_void_p(), ctypes.c_int64(), ctypes.c_int64()
+        L.check(L.lib().axsq_stl_read(str(path).encode(), ctypes.byref(nodes), ctypes.byref(nn),
+                                      ctypes.byref(nf)))
+        L.lib().axsq_host_free(nodes)
+        self.nverts, self.ntris = nn.value, nf.value
+        return self
+
     def __del__(self):
         # at interpreter exit the module globals may already be gone; the
         # process then releases the device memory itself
@@ -104,13 +123,14 @@ class SignedDistance:
                                       stream_ptr()))
         return dist
 
-    # -- SignedDistance.hpp:508-524 (one point; batched through the same kernel)
-    def computeDistance(self, pt):
-        q = torch.as_tensor(np.asarray(pt, np.float64).reshape(1, 3), device="cuda")
-        cp = torch.empty((1, 3), dtype=torch.float64, device="cuda")
-        nrm = torch.empty((1, 3), dtype=torch.float64, device="cuda")
-        d = self.computeDistances(1, q, outClosestPts=cp, outNormals=nrm)
-        return float(d[0]), cp[0].cpu().numpy(), nrm[0].cpu().numpy()
+    # -- SignedDistance.hpp:508-524 (one host point; batched through the same kernel)
+    def computeDistance(self, pt, fp_mode=None):
+        """(distance, closest point (3,), unit normal (3,))."""
+        q = L._dbl3(*[float(c) for c in np.asarray(pt, np.float64).reshape(3)])
+        d, cp, nrm = ctypes.c_double(), L._dbl3(), L._dbl3()
+        mode = self._fp if fp_mode is None else FP_MODES[fp_mode]
+        L.check(L.lib().axsq_sdf_eval_point(self._h, q, mode, ctypes.byref(d), cp, nrm))
+        return d.value, np.array(cp[:]), np.array(nrm[:])
 
     def backward(self, queryPts, dist, elem, gout, want_query=True, want_vertices=True):
         """Adjoint: (grad_q (n,3) or None, grad_v (V,3) or None), float64."""
@@ -144,102 +164,115 @@ class SignedDistance:
 
 
 # ---- quest C-style interface (quest/interface/signed_distance.{hpp,cpp}) ----
-class _Params:
-    def __init__(self):
-        self.dimension = 3
-        self.verbose = False
-        self.is_closed_surface = True
-        self.compute_sign = True
-        self.fp_mode = "fp64"
-
-
-_params = _Params()
-_query = None
-
-
-def _must_not_be_initialized(name):
-    if _query is not None:
-        raise RuntimeError(f"{name}: signed distance has already been initialized; "
-                           "call signed_distance_finalize() first")
+# These call the library's process-wide query through its C names
+# (include/axsq_quest.h), so Python, C and C++ callers share one query and
+# one set of options. Misuse raises RuntimeError where the reference aborts.
+def _qcheck():
+    err = L.last_error()
+    if err:
+        raise RuntimeError(err)
 
 
 def signed_distance_initialized():
-    return _query is not None
+    return bool(L.lib().QUEST_signed_distance_initialized())
 
 
 def signed_distance_set_dimension(dim):
-    _must_not_be_initialized("signed_distance_set_dimension")
-    if dim != 3:
-        raise RuntimeError("only 3-D triangle surfaces are supported")
-    _params.dimension = dim
+    L.lib().QUEST_signed_distance_set_dimension(int(dim))
+    _qcheck()
 
 
 def signed_distance_set_closed_surface(status):
-    _must_not_be_initialized("signed_distance_set_closed_surface")
-    _params.is_closed_surface = bool(status)
+    L.lib().QUEST_signed_distance_set_closed_surface(bool(status))
+    _qcheck()
 
 
 def signed_distance_set_compute_signs(compute_sign):
-    _must_not_be_initialized("signed_distance_set_compute_signs")
-    _params.compute_sign = bool(compute_sign)
+    L.lib().QUEST_signed_distance_set_compute_signs(bool(compute_sign))
+    _qcheck()
+
+
+def signed_distance_set_allocator(allocator_id):
+    L.lib().QUEST_signed_distance_set_allocator(int(allocator_id))
+    _qcheck()
 
 
 def signed_distance_set_verbose(status):
-    _must_not_be_initialized("signed_distance_set_verbose")
-    _params.verbose = bool(status)
+    L.lib().QUEST_signed_distance_set_verbose(bool(status))
+    _qcheck()
+
+
+def signed_distance_use_shared_memory(status):
+    L.lib().QUEST_signed_distance_use_shared_memory(bool(status))
+    _qcheck()
+
+
+SignedDistExec_CPU, SignedDistExec_OpenMP, SignedDistExec_GPU = 0, 1, 2
+
+
+def signed_distance_set_execution_space(exec_space):
+    L.lib().QUEST_signed_distance_set_execution_space(int(exec_space))
+    _qcheck()
 
 
 def signed_distance_set_precision(mode):
     """Selects "fp64" (plain) or "fp32" (FP32 traversal, FP64 finish)."""
-    _must_not_be_initialized("signed_distance_set_precision")
     if mode not in ("fp64", "fp32"):
         raise RuntimeError(f"unknown precision {mode!r}")
-    _params.fp_mode = mode
+    L.lib().axsq_signed_distance_set_precision(FP_MODES[mode])
+    _qcheck()
 
 
 def signed_distance_init(mesh_or_file):
-    """Init from an STL path or a (vertices, triangles) pair. Returns 0 on
-    success, -1 on failure (signed_distance.cpp:29-30, 109-227)."""
-    global _query
-    _must_not_be_initialized("signed_distance_init")
-    try:
-        if isinstance(mesh_or_file, str):
-            from .stl import read_stl
-            v, t = read_stl(mesh_or_file)
-            verts = torch.as_tensor(v, device="cuda")
-            tris = torch.as_tensor(t, device="cuda")
-        else:
-            verts, tris = mesh_or_file
-        _query = SignedDistance(verts, tris, _params.is_closed_surface, _params.compute_sign,
-                                fp_mode=_params.fp_mode)
-    except (OSError, ValueError) as e:
-        if _params.verbose:
-            print(f"signed_distance_init failed: {e}")
-        return -1
-    return 0
+    """Init from an STL path or a (vertices, triangles) pair of CUDA tensors.
+    Returns 0 on success, -1 on failure (signed_distance.cpp:29-30, 109-227)."""
+    if isinstance(mesh_or_file, str):
+        return int(L.lib().QUEST_signed_distance_init_serial(mesh_or_file.encode()))
+    verts, tris = mesh_or_file
+    x, y, z = _soa(verts)
+    t = _dev(tris, torch.int32, "triangles", 3)
+    torch.cuda.current_stream().synchronize()
+    return int(L.lib().axsq_signed_distance_init_mesh(_ptr(x), _ptr(y), _ptr(z), x.shape[0],
+                                                      _ptr(t), t.shape[0]))
 
 
 def signed_distance_evaluate(x, y=None, z=None, phi=None, npoints=None):
     """Scalar form evaluate(x, y, z) -> float, or the batched SoA form
-    evaluate(x, y, z, phi) with (n,) CUDA tensors (signed_distance.cpp:346-465)."""
-    if _query is None:
-        raise RuntimeError("signed_distance_evaluate: signed distance is not initialized")
+    evaluate(x, y, z, phi) with (n,) float64 CUDA tensors
+    (signed_distance.cpp:346-465)."""
     if isinstance(x, torch.Tensor):
-        n = x.shape[0] if npoints is None else int(npoints)
-        out = _query.computeDistances(n, (x, y, z), outSgnDist=phi)
+        xs, ys, zs = (_dev(c, torch.float64, "coordinates").contiguous() for c in (x, y, z))
+        n = xs.shape[0] if npoints is None else int(npoints)
+        out = phi if phi is not None else torch.empty(n, dtype=torch.float64, device=xs.device)
+        _dev(out, torch.float64, "phi")
+        torch.cuda.current_stream().synchronize()
+        L.check(L.lib().axsq_signed_distance_evaluate_batch(_ptr(xs), _ptr(ys), _ptr(zs), n,
+                                                            _ptr(out)))
         return out
-    return _query.computeDistance((x, y, z))[0]
+    d = L.lib().QUEST_signed_distance_evaluate_0(float(x), float(y), float(z))
+    _qcheck()
+    return d
+
+
+def signed_distance_evaluate_cp(x, y, z):
+    """evaluate(x, y, z, cp_x, ..., n_z) (signed_distance.hpp:266-274):
+    (distance, closest point (3,), unit normal (3,))."""
+    out = [ctypes.c_double() for _ in range(6)]
+    d = L.lib().QUEST_signed_distance_evaluate_1(float(x), float(y), float(z),
+                                                 *[ctypes.byref(o) for o in out])
+    _qcheck()
+    return d, np.array([o.value for o in out[:3]]), np.array([o.value for o in out[3:]])
 
 
 def signed_distance_get_mesh_bounds():
-    if _query is None:
-        raise RuntimeError("signed_distance_get_mesh_bounds: not initialized")
-    return _query.getMeshBounds()
+    lo, hi = L._dbl3(), L._dbl3()
+    L.lib().QUEST_signed_distance_get_mesh_bounds(lo, hi)
+    _qcheck()
+    return np.array(lo[:]), np.array(hi[:])
 
 
 def signed_distance_finalize():
-    global _query
-    _query = None
+    L.lib().QUEST_signed_distance_finalize()
 
 
 # ---- mesh tester ---------------------------------------------------------------

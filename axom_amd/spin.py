@@ -140,6 +140,8 @@ class BVH:
         build = L.lib().axsq_bvh_build2d if self.NDIMS == 2 else L.lib().axsq_bvh_build
         L.check(build(ctypes.byref(h), _ptr(b), n, self._scale, self._fp, stream_ptr()))
         self._h = h
+        if getattr(self, "_alloc", 0):
+            L.check(L.lib().axsq_bvh_set_allocator(self._h, self._alloc))
         return BVH_BUILD_OK
 
     def isInitialized(self):
@@ -170,6 +172,8 @@ class BVH:
         return int(L.lib().axsq_bvh_num_leaves(self._h)) if self._h else 0
 
     def _find(self, kind, offsets, counts, n, x, y, z, stride):
+        """One search into a candidate tensor sized from the previous call's
+        total; when that is too small, a fill of the exact size follows."""
         if not self._h:
             raise RuntimeError("BVH is not initialized")
         off = _dev(offsets, torch.int32, "offsets")
@@ -179,17 +183,41 @@ class BVH:
             raise ValueError("offsets/counts length not equal to the number of queries")
         total = ctypes.c_int64()
         s = stream_ptr()
-        L.check(L.lib().axsq_bvh_find_count(self._h, kind, x, y, z, stride, n, self._tol,
-                                            _ptr(off), _ptr(cnt), ctypes.byref(total), s))
-        cand = torch.empty(max(total.value, 0), dtype=torch.int32, device=off.device)
-        if total.value > 0:
+        cap = getattr(self, "_cap_hint", 0)
+        cand = torch.empty(max(cap, 1), dtype=torch.int32, device=off.device)
+        L.check(L.lib().axsq_bvh_find_into(self._h, kind, x, y, z, stride, n, self._tol,
+                                           _ptr(off), _ptr(cnt), _ptr(cand), cap,
+                                           ctypes.byref(total), s))
+        t = max(total.value, 0)
+        if t > cap:
+            cand = torch.empty(t, dtype=torch.int32, device=off.device)
             L.check(L.lib().axsq_bvh_find_fill(self._h, kind, x, y, z, stride, n, self._tol,
-                                               _ptr(off), _ptr(cand), s))
+                                               _ptr(off), _ptr(cnt), _ptr(cand), s))
+            self._cap_hint = t + t // 8 + 1024
         if off.data_ptr() != offsets.data_ptr():
             offsets.copy_(off)
         if cnt.data_ptr() != counts.data_ptr():
             counts.copy_(cnt)
-        return cand
+        return cand[:t]
+
+    def setAllocatorID(self, allocator_id):
+        """BVH::setAllocatorID (BVH.hpp:251-261): the memory of the candidate
+        arrays of the C-ABI's one-call search (0 device, 1 managed, 2 pinned
+        host). This mirror returns torch tensors, which live in device memory."""
+        if allocator_id not in (0, 1, 2):
+            raise ValueError(f"unknown allocator id {allocator_id}")
+        self._alloc = int(allocator_id)
+        if self._h:
+            L.check(L.lib().axsq_bvh_set_allocator(self._h, self._alloc))
+
+    def getAllocatorID(self):
+        return getattr(self, "_alloc", 0)
+
+    def writeVtkFile(self, fileName):
+        """BVH::writeVtkFile (BVH.hpp:405,572-577)."""
+        if not self._h:
+            raise RuntimeError("BVH is not initialized")
+        L.check(L.lib().axsq_bvh_write_vtk(self._h, str(fileName).encode()))
 
     # -- BVH.hpp:341-346, 482-505
     def findPoints(self, offsets, counts, numPts, points):

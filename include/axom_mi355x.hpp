@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "axsq.h"
+#include "axsq_quest.h"
 
 namespace axom {
 
@@ -48,6 +49,12 @@ inline void check(int rc) {
 inline void hip_check(hipError_t e) {
   if (e != hipSuccess) throw mi355x_error(AXSQ_ERR_HIP, hipGetErrorString(e));
 }
+// the quest C interface reports misuse through axsq_last_error() (empty after
+// a call that succeeded)
+inline void quest_check() {
+  const char* e = axsq_last_error();
+  if (e && e[0]) throw mi355x_error(AXSQ_ERR_STATE, e);
+}
 }  // namespace detail
 
 // The single MI355X execution space.
@@ -62,8 +69,13 @@ struct execution_space<MI355X_EXEC> {
   static constexpr bool async() noexcept { return false; }  // calls return when done
   static constexpr bool onDevice() noexcept { return true; }
   static constexpr char* name() noexcept { return (char*)"[MI355X_EXEC]"; }
-  static int allocatorID() noexcept { return 0; }  // hipMalloc device memory
-  static bool usesAllocId(int id) noexcept { return id == 0; }
+  // allocator ids: AXSQ_ALLOC_DEVICE (0, the default: hipMalloc),
+  // AXSQ_ALLOC_MANAGED (1) and AXSQ_ALLOC_PINNED_HOST (2); all are memory the
+  // device reads and writes
+  static int allocatorID() noexcept { return AXSQ_ALLOC_DEVICE; }
+  static bool usesAllocId(int id) noexcept {
+    return id == AXSQ_ALLOC_DEVICE || id == AXSQ_ALLOC_MANAGED || id == AXSQ_ALLOC_PINNED_HOST;
+  }
 };
 
 // Non-owning view of device memory (axom::ArrayView on device memory).
@@ -143,6 +155,9 @@ struct BoundingBox3 {
 struct Point3 {
   double x[3];
 };
+struct Vector3 {  // primal::Vector<double,3>
+  double x[3];
+};
 struct Ray3 {
   double origin[3];
   double dir[3];
@@ -214,8 +229,14 @@ struct BVHTypes<2> {
 // prefers the child whose box centre is nearer (an invalid right box never
 // wins); every other primitive keeps the left-first order. Device code only:
 // the arrays are device memory, valid until the tree is rebuilt or destroyed.
-template <int NDIMS>
+// traverse_tree returns false when a push would not fit in the stack (STACK
+// entries, the first the end marker: 63 pending nodes by default): the walk
+// stops there, so the leaf actions it made are a prefix of the full walk's
+// (the reference has no check; its 64-entry stack would overflow).
+template <int NDIMS, int STACK = 64>
 class MI355XTraverser {
+  static_assert(STACK >= 2, "the walk's stack needs room for its end marker and a node");
+
  public:
   using BoxType = typename BVHTypes<NDIMS>::BoxType;
   using PointType = typename BVHTypes<NDIMS>::PointType;
@@ -225,8 +246,8 @@ class MI355XTraverser {
       : m_nodes(nodes), m_leaf_ids(leaf_ids), m_ninner(ninner) {}
 
   template <typename LeafAction, typename Predicate>
-  __device__ void traverse_tree(const PointType& p, LeafAction&& lf, Predicate&& predicate) const {
-    walk(p, lf, predicate, [&p](const BoxType& l, const BoxType& r, bool r_valid) {
+  __device__ bool traverse_tree(const PointType& p, LeafAction&& lf, Predicate&& predicate) const {
+    return walk(p, lf, predicate, [&p](const BoxType& l, const BoxType& r, bool r_valid) {
       if (!r_valid) return false;
       double dl = 0.0, dr = 0.0;
       for (int d = 0; d < NDIMS; ++d) {
@@ -240,8 +261,8 @@ class MI355XTraverser {
   }
 
   template <typename Primitive, typename LeafAction, typename Predicate>
-  __device__ void traverse_tree(const Primitive& p, LeafAction&& lf, Predicate&& predicate) const {
-    walk(p, lf, predicate, [](const BoxType&, const BoxType&, bool) { return false; });
+  __device__ bool traverse_tree(const Primitive& p, LeafAction&& lf, Predicate&& predicate) const {
+    return walk(p, lf, predicate, [](const BoxType&, const BoxType&, bool) { return false; });
   }
 
  private:
@@ -255,8 +276,8 @@ class MI355XTraverser {
   }
 
   template <typename Primitive, typename LeafAction, typename Predicate, typename Pref>
-  __device__ void walk(const Primitive& p, LeafAction& lf, Predicate& predicate, Pref&& pref) const {
-    constexpr std::int32_t kDepth = 64;
+  __device__ bool walk(const Primitive& p, LeafAction& lf, Predicate& predicate, Pref&& pref) const {
+    constexpr std::int32_t kDepth = STACK;
     constexpr std::int32_t kEnd = -2000000000;  // below every leaf link
     std::int32_t stack[kDepth];
     std::int32_t top = 0;
@@ -274,7 +295,7 @@ class MI355XTraverser {
         } else if (go_l && go_r) {
           const bool right_first = pref(lb, rb, (nd.flags & 2) != 0);
           node = right_first ? nd.rc : nd.lc;
-          if (top + 1 >= kDepth) return;  // deeper than the reference's stack
+          if (top + 1 >= kDepth) return false;  // deeper than the reference's stack
           stack[++top] = right_first ? nd.lc : nd.rc;
         } else {
           node = go_l ? nd.lc : nd.rc;
@@ -291,6 +312,7 @@ class MI355XTraverser {
       }
       held = 0;
     }
+    return true;
   }
 
   const axsq_node* m_nodes = nullptr;
@@ -332,6 +354,7 @@ class MI355XLinearBVH {
       ::axom::detail::check(axsq_bvh_build2d(&m_bvh, b, numItems, m_scaleFactor, m_fpMode, nullptr));
     else
       ::axom::detail::check(axsq_bvh_build(&m_bvh, b, numItems, m_scaleFactor, m_fpMode, nullptr));
+    ::axom::detail::check(axsq_bvh_set_allocator(m_bvh, m_allocatorID));
     return BVH_BUILD_OK;
   }
   int initialize(ArrayView<const BoxType> boxes, IndexType numItems) {
@@ -339,8 +362,20 @@ class MI355XLinearBVH {
   }
   bool isInitialized() const { return m_bvh != nullptr; }
 
-  void setAllocatorID(int id) { (void)id; }
-  int getAllocatorID() const { return 0; }
+  // BVH.hpp:251-261: the memory of the candidate arrays find* returns, one of
+  // the ids execution_space<MI355X_EXEC>::usesAllocId accepts
+  void setAllocatorID(int id) {
+    if (!execution_space<MI355X_EXEC>::usesAllocId(id))
+      throw mi355x_error(AXSQ_ERR_INVALID_ARG, "allocator id not usable by MI355X_EXEC");
+    m_allocatorID = id;
+    if (m_bvh) ::axom::detail::check(axsq_bvh_set_allocator(m_bvh, id));
+  }
+  int getAllocatorID() const { return m_allocatorID; }
+  // BVH.hpp:405,572-577: the tree's boxes as a legacy-VTK file
+  void writeVtkFile(const std::string& fileName) const {
+    if (!m_bvh) throw mi355x_error(AXSQ_ERR_STATE, "BVH is not initialized");
+    ::axom::detail::check(axsq_bvh_write_vtk(m_bvh, fileName.c_str()));
+  }
   void setScaleFactor(double s) { m_scaleFactor = s; }
   double getScaleFactor() const { return m_scaleFactor; }
   void setTolerance(double eps) { m_tolerance = eps; }
@@ -399,13 +434,17 @@ class MI355XLinearBVH {
 
   // BVH.hpp:305-319
   using TraverserType = MI355XTraverser<NDIMS>;
-  TraverserType getTraverser() const {
+  TraverserType getTraverser() const { return getTraverserWithStack<64>(); }
+  // The same walker with a stack of STACK entries (an addition: the
+  // reference's walk always has 64)
+  template <int STACK>
+  MI355XTraverser<NDIMS, STACK> getTraverserWithStack() const {
     if (!m_bvh) throw mi355x_error(AXSQ_ERR_STATE, "BVH is not initialized");
     const axsq_node* nodes = nullptr;
     const std::int32_t* ids = nullptr;
     std::int32_t ninner = 0;
     ::axom::detail::check(axsq_bvh_device_arrays(m_bvh, &nodes, &ids, &ninner));
-    return TraverserType(nodes, ids, ninner);
+    return MI355XTraverser<NDIMS, STACK>(nodes, ids, ninner);
   }
 
  private:
@@ -428,6 +467,7 @@ class MI355XLinearBVH {
   double m_tolerance = DEFAULT_TOLERANCE;
   double m_scaleFactor = DEFAULT_SCALE_FACTOR;
   int m_fpMode = AXSQ_FP64;
+  int m_allocatorID = execution_space<MI355X_EXEC>::allocatorID();
 };
 }  // namespace detail
 
@@ -440,9 +480,10 @@ class BVH<NDIMS, MI355X_EXEC, double> : public detail::MI355XLinearBVH<NDIMS> {
  public:
   using typename Base::BoxType;
   BVH() = default;
-  BVH(const BoxType* boxes, IndexType numItems, int allocatorID = 0,
+  BVH(const BoxType* boxes, IndexType numItems,
+      int allocatorID = execution_space<MI355X_EXEC>::allocatorID(),
       double tolerance = Base::DEFAULT_TOLERANCE, double scaleFactor = Base::DEFAULT_SCALE_FACTOR) {
-    (void)allocatorID;
+    this->setAllocatorID(allocatorID);
     this->m_tolerance = tolerance;
     this->m_scaleFactor = scaleFactor;
     this->initialize(boxes, numItems);
@@ -463,6 +504,7 @@ template <>
 class SignedDistance<3, MI355X_EXEC> {
  public:
   using PointType = primal::Point3;
+  using VectorType = primal::Vector3;
 
   SignedDistance(const double* x, const double* y, const double* z, IndexType nnodes,
                  const IndexType* connectivity, IndexType ncells, bool isWatertight,
@@ -479,7 +521,7 @@ class SignedDistance<3, MI355X_EXEC> {
   // (outClosestPts / outNormals may be null).
   void computeDistances(int npts, const PointType* queryPts, double* outSgnDist,
                         PointType* outClosestPts = nullptr,
-                        PointType* outNormals = nullptr) const {
+                        VectorType* outNormals = nullptr) const {
     const double* p = reinterpret_cast<const double*>(queryPts);
     detail::check(axsq_sdf_eval(m_sdf, p, p + 1, p + 2, 3, npts, m_fpMode, outSgnDist,
                                 reinterpret_cast<double*>(outClosestPts),
@@ -491,19 +533,24 @@ class SignedDistance<3, MI355X_EXEC> {
     detail::check(axsq_sdf_eval(m_sdf, x, y, z, 1, npts, m_fpMode, outSgnDist, nullptr, nullptr,
                                 nullptr, nullptr));
   }
-  // SignedDistance.hpp:508-524: one host point, through the same batched kernel.
+  // SignedDistance.hpp:508-512: one host point, through the same batched
+  // kernel (the library stages it in a small device buffer of this thread).
   double computeDistance(const PointType& pt) const {
-    PointType* d_pt = nullptr;
-    double* d_out = nullptr;
-    detail::hip_check(hipMalloc((void**)&d_pt, sizeof(PointType)));
-    detail::hip_check(hipMalloc((void**)&d_out, sizeof(double)));
-    double out = 0.0;
-    detail::hip_check(hipMemcpy(d_pt, &pt, sizeof(PointType), hipMemcpyHostToDevice));
-    computeDistances(1, d_pt, d_out);
-    detail::hip_check(hipMemcpy(&out, d_out, sizeof(double), hipMemcpyDeviceToHost));
-    (void)hipFree(d_pt);
-    (void)hipFree(d_out);
-    return out;
+    double d = 0.0;
+    detail::check(axsq_sdf_eval_point(m_sdf, pt.x, m_fpMode, &d, nullptr, nullptr));
+    return d;
+  }
+  double computeDistance(double x, double y, double z) const {
+    return computeDistance(PointType{{x, y, z}});
+  }
+  // SignedDistance.hpp:517-524: also the closest point and the unit
+  // (pseudo-)normal there
+  double computeDistance(const PointType& queryPnt, PointType& closestPnt,
+                         VectorType& surfaceNormal) const {
+    double d = 0.0;
+    detail::check(axsq_sdf_eval_point(m_sdf, queryPnt.x, m_fpMode, &d, closestPnt.x,
+                                      surfaceNormal.x));
+    return d;
   }
 
   const axsq_sdf* handle() const { return m_sdf; }
@@ -546,6 +593,127 @@ void findTriMeshIntersectionsBVH(const double* x, const double* y, const double*
   for (int64_t k = 0; k < npairs; ++k) intersections[(size_t)k] = {hf[(size_t)k], hg[(size_t)k]};
   degenerateIndices.assign(hd.begin(), hd.end());
 }
+
+// quest::STLReader (src/axom/quest/readers/STLReader.hpp): setFileName,
+// read (0 or -1), the node and face counts, and the mesh as SoA coordinates
+// plus the implicit connectivity of getMesh (STLReader.cpp:207-254). Binary
+// coordinates are widened from float, ASCII ones read as double.
+class STLReader {
+ public:
+  STLReader() = default;
+  explicit STLReader(const std::string& fileName) : m_fileName(fileName) {}
+  void setFileName(const std::string& fileName) { m_fileName = fileName; }
+  const std::string& getFileName() const { return m_fileName; }
+  int read() {
+    clear();
+    double* nodes = nullptr;
+    int64_t nn = 0, nf = 0;
+    if (axsq_stl_read(m_fileName.c_str(), &nodes, &nn, &nf) != AXSQ_OK) return -1;
+    m_nodes.assign(nodes, nodes + 3 * nn);
+    (void)axsq_host_free(nodes);
+    m_numNodes = (IndexType)nn;
+    m_numFaces = (IndexType)nf;
+    return 0;
+  }
+  void clear() {
+    m_nodes.clear();
+    m_numNodes = m_numFaces = 0;
+  }
+  IndexType getNumNodes() const { return m_numNodes; }
+  IndexType getNumFaces() const { return m_numFaces; }
+  // x, y, z of every node (getNumNodes entries each) and 3 * getNumFaces
+  // connectivity entries, on the host
+  void getMesh(std::vector<double>& x, std::vector<double>& y, std::vector<double>& z,
+               std::vector<IndexType>& conn) const {
+    x.resize(m_numNodes);
+    y.resize(m_numNodes);
+    z.resize(m_numNodes);
+    for (IndexType i = 0; i < m_numNodes; ++i) {
+      x[i] = m_nodes[3 * (size_t)i];
+      y[i] = m_nodes[3 * (size_t)i + 1];
+      z[i] = m_nodes[3 * (size_t)i + 2];
+    }
+    conn.resize(3 * (size_t)m_numFaces);
+    for (size_t k = 0; k < conn.size(); ++k) conn[k] = (IndexType)k;
+  }
+
+ private:
+  std::string m_fileName;
+  std::vector<double> m_nodes;
+  IndexType m_numNodes = 0, m_numFaces = 0;
+};
+
+// The signed-distance interface (src/axom/quest/interface/signed_distance.hpp:
+// 88-319) over the library's process-wide query (axsq_quest.h). Misuse throws
+// mi355x_error where the reference aborts through SLIC_ERROR; init returns 0
+// or -1 as the reference does.
+enum class SignedDistExec { CPU = 0, OpenMP = 1, GPU = 2 };
+
+inline int signed_distance_init(const std::string& file) {
+  return QUEST_signed_distance_init_serial(file.c_str());
+}
+// signed_distance_init(const mint::Mesh*) for a triangle mesh held as SoA
+// node coordinates and connectivity, in host or device memory
+inline int signed_distance_init(const double* x, const double* y, const double* z,
+                                IndexType nnodes, const IndexType* connectivity, IndexType ncells) {
+  return axsq_signed_distance_init_mesh(x, y, z, nnodes, connectivity, ncells);
+}
+inline bool signed_distance_initialized() { return QUEST_signed_distance_initialized(); }
+inline void signed_distance_set_dimension(int dim) {
+  QUEST_signed_distance_set_dimension(dim);
+  ::axom::detail::quest_check();
+}
+inline void signed_distance_set_closed_surface(bool status) {
+  QUEST_signed_distance_set_closed_surface(status);
+  ::axom::detail::quest_check();
+}
+inline void signed_distance_set_compute_signs(bool computeSign) {
+  QUEST_signed_distance_set_compute_signs(computeSign);
+  ::axom::detail::quest_check();
+}
+inline void signed_distance_set_allocator(int allocatorID) {
+  QUEST_signed_distance_set_allocator(allocatorID);
+  ::axom::detail::quest_check();
+}
+inline void signed_distance_set_verbose(bool status) {
+  QUEST_signed_distance_set_verbose(status);
+  ::axom::detail::quest_check();
+}
+inline void signed_distance_use_shared_memory(bool status) {
+  QUEST_signed_distance_use_shared_memory(status);
+  ::axom::detail::quest_check();
+}
+inline void signed_distance_set_execution_space(SignedDistExec execSpace) {
+  QUEST_signed_distance_set_execution_space(static_cast<int>(execSpace));
+  ::axom::detail::quest_check();
+}
+// AXSQ_FP64 (default) or AXSQ_FP32_TRAVERSAL (no reference counterpart)
+inline void signed_distance_set_precision(int fp_mode) {
+  axsq_signed_distance_set_precision(fp_mode);
+  ::axom::detail::quest_check();
+}
+inline double signed_distance_evaluate(double x, double y, double z = 0.0) {
+  const double d = QUEST_signed_distance_evaluate_0(x, y, z);
+  ::axom::detail::quest_check();
+  return d;
+}
+inline double signed_distance_evaluate(double x, double y, double z, double& cp_x, double& cp_y,
+                                       double& cp_z, double& n_x, double& n_y, double& n_z) {
+  const double d =
+      QUEST_signed_distance_evaluate_1(x, y, z, &cp_x, &cp_y, &cp_z, &n_x, &n_y, &n_z);
+  ::axom::detail::quest_check();
+  return d;
+}
+// x, y, z and phi in host or device memory
+inline void signed_distance_evaluate(const double* x, const double* y, const double* z,
+                                     int npoints, double* phi) {
+  ::axom::detail::check(axsq_signed_distance_evaluate_batch(x, y, z, npoints, phi));
+}
+inline void signed_distance_get_mesh_bounds(double* lo, double* hi) {
+  QUEST_signed_distance_get_mesh_bounds(lo, hi);
+  ::axom::detail::quest_check();
+}
+inline void signed_distance_finalize() { QUEST_signed_distance_finalize(); }
 
 }  // namespace quest
 }  // namespace axom

@@ -54,12 +54,28 @@ extern "C" {
 typedef struct axsq_bvh axsq_bvh;
 typedef struct axsq_sdf axsq_sdf;
 
+/* Memory that the candidate arrays of axsq_bvh_find are allocated in: the
+ * allocator id a caller gives BVH::setAllocatorID (src/axom/spin/BVH.hpp:
+ * 251-261; the candidate array is allocated with it at
+ * src/axom/spin/policy/LinearBVH.hpp:338-341). */
+#define AXSQ_ALLOC_DEVICE 0      /* hipMalloc: device memory (the default) */
+#define AXSQ_ALLOC_MANAGED 1     /* hipMallocManaged: readable on host and device */
+#define AXSQ_ALLOC_PINNED_HOST 2 /* hipHostMalloc: host memory the device writes */
+
 /* Text of the last error raised on the calling thread ("" if none). */
 const char* axsq_last_error(void);
 /* Library version as 10000*major + 100*minor + patch. */
 int axsq_version(void);
-/* Frees device memory returned by the library (candidate arrays). */
-int axsq_free(void* device_ptr);
+/* Frees memory returned by the library (candidate arrays, pair lists), in
+ * whichever of the AXSQ_ALLOC_* memories it was allocated. */
+int axsq_free(void* ptr);
+/* Frees host memory returned by the library (axsq_stl_read). */
+int axsq_host_free(void* host_ptr);
+/* Frees the calling thread's scratch memory on the current device: the
+ * sort, scan and candidate-search buffers, the signed-distance walk records
+ * and tables, the mesh tester's tree. Later calls allocate again as needed.
+ * Must not run while work of this thread that uses them is in flight. */
+int axsq_release_workspace(void);
 
 /* ---- BVH ----------------------------------------------------------------
  * Builds a linear BVH over n boxes.
@@ -115,6 +131,20 @@ int axsq_bvh_device_arrays(const axsq_bvh* bvh, const axsq_node** nodes,
 int axsq_bvh_rebuild(axsq_bvh* bvh, const double* boxes, int64_t n, double scale, int fp_mode,
                      void* stream);
 
+/* Sets the memory of the candidate arrays axsq_bvh_find returns (one of
+ * AXSQ_ALLOC_*): BVH::setAllocatorID (BVH.hpp:251-261). */
+int axsq_bvh_set_allocator(axsq_bvh* bvh, int allocator_id);
+int axsq_bvh_get_allocator(const axsq_bvh* bvh);
+
+/* Writes the tree as a legacy-VTK unstructured grid of boxes (hexahedra, or
+ * quads for a 2-D tree): the root box at level 0, then every inner node's two
+ * child boxes at their depth, depth first, left before right, with a "level"
+ * cell scalar. BVH::writeVtkFile -> LinearBVH::writeVtkFileImpl
+ * (src/axom/spin/BVH.hpp:405,572-577; policy/LinearBVH.hpp:404-462;
+ * internal/linear_bvh/bvh_vtkio.hpp), numbers written as a C++ stream writes
+ * them by default (6 significant digits). Host file output. */
+int axsq_bvh_write_vtk(const axsq_bvh* bvh, const char* file);
+
 /* Root box of the BVH: BVH::getBounds (BVH.hpp:297-308). Host outputs. */
 int axsq_bvh_bounds(const axsq_bvh* bvh, double lo[3], double hi[3]);
 /* Number of leaves after padding. */
@@ -127,32 +157,48 @@ int64_t axsq_bvh_num_leaves(const axsq_bvh* bvh);
 int axsq_bvh_export(const axsq_bvh* bvh, double* inner_nodes, int32_t* children,
                     int32_t* leaf_nodes, uint32_t* mcodes, void* stream);
 
-/* Candidate search, two-phase form with caller-owned candidate storage.
- * Replaces LinearBVH::findCandidatesImpl (LinearBVH.hpp:273-402) behind
- * BVH::findPoints / findBoundingBoxes / findRays (BVH.hpp:482-568).
+/* Candidate search, two-phase form with caller-owned candidate storage:
+ * the count pass, exclusive scan and fill pass of the reference's GPU path,
+ * LinearBVH::findCandidatesImpl (LinearBVH.hpp:273-402, count :296-330, fill
+ * :332-367) behind BVH::findPoints / findBoundingBoxes / findRays
+ * (BVH.hpp:482-568).
  *   kind AXSQ_QUERY_POINTS: prims = x pointer, y/z/stride as in the header notes
  *   kind AXSQ_QUERY_BOXES : prims = AoS boxes (6 doubles each), y=z=NULL, stride 6
  *   kind AXSQ_QUERY_RAYS  : prims = AoS rays {origin xyz, unit direction xyz},
  *                           y=z=NULL, stride 6; tol = BVH::setTolerance value
  * Phase 1 (axsq_bvh_find_count) writes counts[nq], offsets[nq] (exclusive
- * scan) and *total. Phase 2 (axsq_bvh_find_fill) writes candidates[total]:
- * for query i, ids at [offsets[i], offsets[i]+counts[i]) in traversal order.
- * Phase 1 keeps each query's first 64 ids (fewer when that would pass 512 MB)
- * in the calling thread's workspace; a phase 2 on the same thread that
- * follows it with the same tree and inputs copies them and walks again only
- * for queries with more, any other phase 2 walks every query. Results are the
- * same either way. */
+ * scan) and *total. Phase 2 (axsq_bvh_find_fill) searches again and writes
+ * candidates: for query i, ids at [offsets[i], offsets[i]+counts[i]) in
+ * traversal order, never more than counts[i] of them. When its search finds
+ * another number of candidates for some query than counts[i] (the queries,
+ * the tolerance or the tree changed after phase 1), it returns
+ * AXSQ_ERR_INVALID_ARG; no id is written outside the queries' ranges. */
 int axsq_bvh_find_count(const axsq_bvh* bvh, int kind, const double* x, const double* y,
                         const double* z, int64_t stride, int64_t nq, double tol,
                         int32_t* offsets, int32_t* counts, int64_t* total, void* stream);
 int axsq_bvh_find_fill(const axsq_bvh* bvh, int kind, const double* x, const double* y,
                        const double* z, int64_t stride, int64_t nq, double tol,
-                       const int32_t* offsets, int32_t* candidates, void* stream);
+                       const int32_t* offsets, const int32_t* counts, int32_t* candidates,
+                       void* stream);
+
+/* One call into caller-owned storage: counts, offsets and *total as phase 1,
+ * then, when *total <= capacity, the candidates as phase 2 writes them. The
+ * search runs once: the count pass keeps each query's first 64 ids (fewer
+ * when that would pass 512 MB) in the calling thread's workspace and only
+ * queries with more are searched again. When *total > capacity nothing is
+ * written to candidates (call axsq_bvh_find_fill with room for *total ids, or
+ * this function again). No reference counterpart as one call: the count and
+ * fill of LinearBVH::findCandidatesImpl (LinearBVH.hpp:296-367) in one. */
+int axsq_bvh_find_into(const axsq_bvh* bvh, int kind, const double* x, const double* y,
+                       const double* z, int64_t stride, int64_t nq, double tol, int32_t* offsets,
+                       int32_t* counts, int32_t* candidates, int64_t capacity, int64_t* total,
+                       void* stream);
 
 /* One-call form matching BVH::find* semantics: counts and offsets as above;
- * the library allocates *candidates (device, *total entries; release with
- * axsq_free), as the reference's BVH allocates the candidate array
- * (LinearBVH.hpp:338-341). */
+ * the library allocates *candidates (*total entries, in the memory set by
+ * axsq_bvh_set_allocator, device memory by default; release with axsq_free),
+ * as the reference's BVH allocates the candidate array (LinearBVH.hpp:
+ * 338-341). It searches once, as axsq_bvh_find_into does. */
 int axsq_bvh_find(const axsq_bvh* bvh, int kind, const double* x, const double* y,
                   const double* z, int64_t stride, int64_t nq, double tol, int32_t* offsets,
                   int32_t* counts, int32_t** candidates, int64_t* total, void* stream);
@@ -181,6 +227,14 @@ int axsq_sdf_create_cells(axsq_sdf** out, const double* x, const double* y, cons
                           int64_t nverts, const int32_t* conn, const int32_t* cell_offsets,
                           int64_t ncells, int watertight, int compute_sign, int fp_mode,
                           void* stream);
+/* The same from an STL file read as quest::STLReader reads it (see
+ * axsq_stl_read): one vertex per facet corner, implicit connectivity. This is
+ * signed_distance_init(file) without the singleton
+ * (quest/interface/signed_distance.cpp:109-160 ->
+ * interface/internal/QuestHelpers.cpp:287-325). AXSQ_ERR_INVALID_ARG when the
+ * file cannot be read or holds no triangle. */
+int axsq_sdf_create_from_stl(axsq_sdf** out, const char* file, int watertight,
+                             int compute_sign, int fp_mode, void* stream);
 int axsq_sdf_destroy(axsq_sdf* sdf);
 /* Mesh bounding box (signed_distance_get_mesh_bounds,
  * quest/interface/signed_distance.cpp:468-488). Host outputs. */
@@ -204,6 +258,14 @@ int axsq_sdf_eval(const axsq_sdf* sdf, const double* x, const double* y, const d
                   int64_t stride, int64_t nq, int fp_mode, double* dist, double* cp,
                   double* normal, int32_t* elem, void* stream);
 
+/* One query point, host in and out: SignedDistance::computeDistance(pt)
+ * and computeDistance(pt, closestPnt, surfaceNormal)
+ * (src/axom/quest/SignedDistance.hpp:508-524), through the batched kernel.
+ * cp and normal may be NULL. Uses a small device buffer of the calling
+ * thread, so it allocates nothing after its first call on a device. */
+int axsq_sdf_eval_point(const axsq_sdf* sdf, const double q[3], int fp_mode, double* dist,
+                        double cp[3], double normal[3]);
+
 /* Adjoint of axsq_sdf_eval (no reference counterpart):
  *   grad_q[3*nq]     = g_i * sgn_i * (q_i - cp_i)/|q_i - cp_i|        (AoS)
  *   grad_v[3*nverts] = - sum over queries of w_k * that, w = barycentric
@@ -215,6 +277,17 @@ int axsq_sdf_eval(const axsq_sdf* sdf, const double* x, const double* y, const d
 int axsq_sdf_backward(const axsq_sdf* sdf, const double* x, const double* y, const double* z,
                       int64_t stride, int64_t nq, const double* dist, const int32_t* elem,
                       const float* gout, double* grad_q, double* grad_v, void* stream);
+
+/* ---- STL reader ---------------------------------------------------------
+ * quest::STLReader::read (src/axom/quest/readers/STLReader.cpp:40-205): a file
+ * of at least 84 bytes whose size is 84 + 50 * (the int32 at byte 80) is
+ * binary, whose float coordinates are widened to double; any other is ASCII,
+ * where each token "vertex" is followed by three coordinates read as double.
+ * *nodes receives 3 * *nnodes doubles (x, y, z of each facet corner, AoS, host
+ * memory; release with axsq_host_free); *nfaces = *nnodes / 3, with the
+ * implicit connectivity (3i, 3i+1, 3i+2) of STLReader::getMesh (:207-254).
+ * AXSQ_ERR_INVALID_ARG when the file cannot be opened. */
+int axsq_stl_read(const char* file, double** nodes, int64_t* nnodes, int64_t* nfaces);
 
 /* ---- measurement --------------------------------------------------------
  * When enabled, the library records HIP events around its main kernels on

@@ -611,9 +611,10 @@ typedef void (*leaf_fn)(int32_t pos, const int32_t* leafs, void* ctx);
 typedef int (*pref_fn)(const orc_box* l, const orc_box* r, const void* prim);
 
 /* Returns the number of inner-node visits; -1 on stack overflow (the
- * reference has no bound check, bvh_traverse.hpp:79-120). */
+ * reference has no bound check, bvh_traverse.hpp:79-120). When max_sp is not
+ * NULL it is raised to the deepest stack entry the walk pushed. */
 static inline int32_t traverse(const orc_bvh* t, const void* prim, pred_fn B, void* pctx,
-                               leaf_fn A, void* actx, pref_fn Comp)
+                               leaf_fn A, void* actx, pref_fn Comp, int32_t* max_sp)
 {
   int32_t todo[ORC_STACK];
   int32_t sp = 0;
@@ -651,6 +652,7 @@ static inline int32_t traverse(const orc_bvh* t, const void* prim, pred_fn B, vo
           }
           sp++;
           if(sp >= ORC_STACK) return -1;
+          if(max_sp && sp > *max_sp) *max_sp = sp;
           todo[sp] = rc;
         }
       }
@@ -783,7 +785,7 @@ static void* find_worker(void* arg)
     collect_ctx cc = {0, j->cands ? j->cands + j->offsets[i] : NULL};
     double q[6];
     load_query(j->prims, j->kind, j->t->dims, i, q);
-    if(traverse(j->t, q, B, &fc, leaf_count, &cc, no_pref) < 0)
+    if(traverse(j->t, q, B, &fc, leaf_count, &cc, no_pref, NULL) < 0)
       j->overflow = 1;
     if(!j->cands) j->counts[i] = cc.count;
   }
@@ -842,6 +844,22 @@ int64_t orc_bvh_find(const orc_bvh* t, int kind, const double* prims, int64_t nq
   }
   free(jobs);
   return overflow ? -1 : total;
+}
+
+int32_t orc_bvh_max_depth(const orc_bvh* t, int kind, const double* prims, int64_t nq,
+                          double tol)
+{
+  pred_fn B = (kind == 0) ? pred_point : (kind == 1) ? pred_box : pred_ray;
+  find_ctx fc = {tol, t->dims};
+  int32_t deepest = 0;
+  for(int64_t i = 0; i < nq; ++i)
+  {
+    collect_ctx cc = {0, NULL};
+    double q[6];
+    load_query(prims, kind, t->dims, i, q);
+    if(traverse(t, q, B, &fc, leaf_count, &cc, no_pref, &deepest) < 0) return -1;
+  }
+  return deepest;
 }
 
 int64_t orc_bvh_brute(const orc_bvh* t, int kind, const double* prims, int64_t nq, double tol,
@@ -1162,7 +1180,7 @@ static void* sdf_worker(void* arg)
     m.minElem = -1;
     m.s = s;
     m.q = q;
-    const int32_t v = traverse(s->bvh, q, sdf_pred, &m, sdf_leaf, &m, point_pref);
+    const int32_t v = traverse(s->bvh, q, sdf_pred, &m, sdf_leaf, &m, point_pref, NULL);
     if(v < 0) j->overflow = 1;
     /* computeDistances :580-603 */
     double sgn = 1.0;
@@ -1190,6 +1208,25 @@ static void* sdf_worker(void* arg)
     if(j->leaves) j->leaves[i] = m.leaves;
   }
   return NULL;
+}
+
+int32_t orc_sdf_max_depth(const orc_sdf* s, const double* q, int64_t nq)
+{
+  int32_t deepest = 0;
+  for(int64_t i = 0; i < nq; ++i)
+  {
+    min_cand m;
+    memset(&m, 0, sizeof(m));
+    m.minSqDist = DBL_MAX;
+    m.minType = -1;
+    m.minLoc = 3;
+    m.minElem = -1;
+    m.s = s;
+    m.q = q + 3 * i;
+    if(traverse(s->bvh, q + 3 * i, sdf_pred, &m, sdf_leaf, &m, point_pref, &deepest) < 0)
+      return -1;
+  }
+  return deepest;
 }
 
 int orc_sdf_eval(const orc_sdf* s, const double* q, int64_t nq, double* dist, double* cp,

@@ -83,6 +83,10 @@ void orc_bvh_export(const orc_bvh* t, double* bounds6, uint32_t* mcodes,
 int64_t orc_bvh_find(const orc_bvh* t, int kind, const double* prims, int64_t nq,
                      double tol, int32_t* counts, int32_t* offsets, int32_t* cands,
                      int nthreads);
+/* The deepest traversal stack entry (1 = one pending node) any of the nq
+ * queries' walks pushes; -1 when a walk overflows the 64-entry stack. */
+int32_t orc_bvh_max_depth(const orc_bvh* t, int kind, const double* prims, int64_t nq,
+                          double tol);
 /* O(N*Q) brute force over the scaled leaf boxes, ascending box id. */
 int64_t orc_bvh_brute(const orc_bvh* t, int kind, const double* prims, int64_t nq,
                       double tol, int32_t* counts, int32_t* cands, int64_t cap);
@@ -105,6 +109,9 @@ void orc_sdf_bounds(const orc_sdf* s, double lo[3], double hi[3]);
 int orc_sdf_eval(const orc_sdf* s, const double* q, int64_t nq, double* dist,
                  double* cp, double* normal, int32_t* elem, int32_t* loc,
                  int32_t* visits, int32_t* leaves, int nthreads);
+/* The deepest stack entry any of the nq signed-distance walks pushes (-1 on
+ * overflow). */
+int32_t orc_sdf_max_depth(const orc_sdf* s, const double* q, int64_t nq);
 /* Brute-force min over all triangles of |q - closest_point(q,T)|^2. */
 int orc_sdf_brute(const orc_sdf* s, const double* q, int64_t nq, double* sqdist,
                   int32_t* elem, int nthreads);

@@ -48,6 +48,11 @@ def lib():
         L.orc_bvh_find.restype = ctypes.c_int64
         L.orc_bvh_find.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64,
                                    ctypes.c_double, _i32p, _i32p, _i32p, ctypes.c_int]
+        L.orc_bvh_max_depth.restype = ctypes.c_int32
+        L.orc_bvh_max_depth.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64,
+                                        ctypes.c_double]
+        L.orc_sdf_max_depth.restype = ctypes.c_int32
+        L.orc_sdf_max_depth.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
         L.orc_bvh_brute.restype = ctypes.c_int64
         L.orc_bvh_brute.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64,
                                     ctypes.c_double, _i32p, _i32p, ctypes.c_int64]
@@ -241,6 +246,12 @@ class OracleBVH:
                            _p(offsets, _i32p), _p(cands, _i32p), nthreads)
         return offsets, counts, cands[:total]
 
+    def max_depth(self, kind, prims, tol=np.finfo(np.float64).eps):
+        """The deepest stack entry the queries' walks push (-1: overflow)."""
+        k = KIND[kind]
+        pr = f64(prims).reshape(-1, self._stride(k))
+        return int(lib().orc_bvh_max_depth(self.h, k, _p(pr, _dp), pr.shape[0], tol))
+
     def brute(self, kind, prims, tol=np.finfo(np.float64).eps):
         k = KIND[kind]
         pr = f64(prims).reshape(-1, self._stride(k))
@@ -311,6 +322,11 @@ class OracleSDF:
             out["visits"] = visits
             out["leaves"] = leaves
         return out
+
+    def max_depth(self, q):
+        """The deepest stack entry the queries' walks push (-1: overflow)."""
+        q = f64(q).reshape(-1, 3)
+        return int(lib().orc_sdf_max_depth(self.h, _p(q, _dp), q.shape[0]))
 
     def brute(self, q, nthreads=1):
         q = f64(q).reshape(-1, 3)

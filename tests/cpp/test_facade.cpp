@@ -295,11 +295,12 @@ static void test_bvh2d_grid() {
 // set as findPoints.
 constexpr int kTravMax = 64;
 template <typename Trav>
-__global__ void k_trav_boxes(Trav tr, const Box* q, int n, int* counts, int* cand) {
+__global__ void k_trav_boxes(Trav tr, const Box* q, int n, int* counts, int* cand,
+                             int* complete = nullptr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int c = 0;
-  tr.traverse_tree(
+  const bool done = tr.traverse_tree(
       q[i],
       [&](std::int32_t pos, const std::int32_t* ids) {
         if (c < kTravMax) cand[i * kTravMax + c] = ids[pos];
@@ -311,6 +312,7 @@ __global__ void k_trav_boxes(Trav tr, const Box* q, int n, int* counts, int* can
         return true;
       });
   counts[i] = c;
+  if (complete) complete[i] = done ? 1 : 0;
 }
 template <typename Trav>
 __global__ void k_trav_points(Trav tr, const Pt* q, int n, int* counts, int* cand) {
@@ -395,8 +397,228 @@ static void test_traverser_matches_find() {
     (void)hipFree(p);
 }
 
+// A tree whose walks push 37 entries: Morton codes that differ in successively
+// lower bits (a chain of nodes whose far child is a small subtree) and 256
+// copies of code 0 (split by index into a balanced subtree), each box
+// stretched to a common corner so that a box query around everything passes
+// every node. A 16-entry traverser stops early and says so, having made a
+// prefix of the leaf actions; the 64-entry one finishes with
+// findBoundingBoxes's order.
+static void test_traverser_reports_a_full_stack() {
+  auto cell = [](int qx, int qy, int qz) {
+    return std::vector<double>{(qx + 0.5) / 1024, (qy + 0.5) / 1024, (qz + 0.5) / 1024};
+  };
+  std::vector<std::vector<double>> cents(256, cell(0, 0, 0));
+  for (int k = 0; k < 30; ++k) {
+    int q[3] = {0, 0, 0};
+    q[k % 3] = 1 << (k / 3);
+    cents.push_back(cell(q[0], q[1], q[2]));
+    cents.push_back(cell(q[0], q[1], q[2]));
+  }
+  cents.push_back(cell(1023, 1023, 1023));
+  const double corner = 0.25 / 1024, h = 1e-6;
+  std::vector<Box> boxes;
+  for (const auto& c : cents) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+      b.min[d] = std::min(c[d], corner) - h;
+      b.max[d] = std::max(c[d], corner) + h;
+    }
+    boxes.push_back(b);
+  }
+  Box* d_boxes = to_device(boxes);
+  BVH bvh;
+  bvh.initialize(d_boxes, (IndexType)boxes.size());
+  const int n = 64;
+  std::vector<Box> qb(n, Box{{-1.0, -1.0, -1.0}, {2.0, 2.0, 2.0}});
+  Box* d_qb = to_device(qb);
+  const int cap = 1024;
+  int *d_cnt = nullptr, *d_cand = nullptr, *d_done = nullptr;
+  axom::detail::hip_check(hipMalloc((void**)&d_cnt, sizeof(int) * n));
+  axom::detail::hip_check(hipMalloc((void**)&d_cand, sizeof(int) * n * kTravMax));
+  axom::detail::hip_check(hipMalloc((void**)&d_done, sizeof(int) * n));
+  Found fb = run_find(n, [&](auto o, auto c, auto& cd) { bvh.findBoundingBoxes(o, c, cd, n, d_qb); });
+  EXPECT_EQ(fb.cnt[0], (IndexType)boxes.size());
+  (void)cap;
+  // 16 entries: stops early, reports it; the leaves it reached begin the full order
+  k_trav_boxes<<<1, n>>>(bvh.getTraverserWithStack<16>(), d_qb, n, d_cnt, d_cand, d_done);
+  axom::detail::hip_check(hipGetLastError());
+  axom::detail::hip_check(hipDeviceSynchronize());
+  std::vector<int> cnt = to_host(d_cnt, n), done = to_host(d_done, n),
+                   cand = to_host(d_cand, (size_t)n * kTravMax);
+  for (int i = 0; i < n; ++i) {
+    EXPECT_EQ(done[i], 0);
+    EXPECT_TRUE(cnt[i] < fb.cnt[i]);
+    for (int k = 0; k < std::min(cnt[i], kTravMax); ++k)
+      EXPECT_EQ(cand[i * kTravMax + k], fb.cand[fb.off[i] + k]);
+  }
+  // 64 entries: the whole walk
+  k_trav_boxes<<<1, n>>>(bvh.getTraverser(), d_qb, n, d_cnt, d_cand, d_done);
+  axom::detail::hip_check(hipGetLastError());
+  axom::detail::hip_check(hipDeviceSynchronize());
+  cnt = to_host(d_cnt, n);
+  done = to_host(d_done, n);
+  cand = to_host(d_cand, (size_t)n * kTravMax);
+  for (int i = 0; i < n; ++i) {
+    EXPECT_EQ(done[i], 1);
+    EXPECT_EQ(cnt[i], fb.cnt[i]);
+    for (int k = 0; k < std::min(cnt[i], kTravMax); ++k)
+      EXPECT_EQ(cand[i * kTravMax + k], fb.cand[fb.off[i] + k]);
+  }
+  for (void* p : {(void*)d_boxes, (void*)d_qb, (void*)d_cnt, (void*)d_cand, (void*)d_done})
+    (void)hipFree(p);
+}
+
+// BVH::setAllocatorID: candidate arrays in managed memory, read on the host.
+static void test_allocator_and_vtk() {
+  std::vector<Box> boxes = grid_boxes();
+  Box* d_boxes = to_device(boxes);
+  BVH bvh;
+  bvh.setAllocatorID(AXSQ_ALLOC_MANAGED);
+  EXPECT_EQ(bvh.getAllocatorID(), AXSQ_ALLOC_MANAGED);
+  bvh.initialize(d_boxes, (IndexType)boxes.size());
+  std::vector<Pt> pts;
+  for (const Box& b : boxes)
+    pts.push_back(Pt{{0.5 * (b.min[0] + b.max[0]), 0.5 * (b.min[1] + b.max[1]),
+                      0.5 * (b.min[2] + b.max[2])}});
+  Pt* d_pts = to_device(pts);
+  const IndexType n = (IndexType)pts.size();
+  axom::Array<IndexType> off(n), cnt(n), cand;
+  bvh.findPoints(off.view(), cnt.view(), cand, n, d_pts);
+  axom::detail::hip_check(hipDeviceSynchronize());
+  EXPECT_EQ(cand.size(), n);
+  for (IndexType i = 0; i < n && cand.size() == n; ++i) EXPECT_EQ(cand.data()[i], i);  // host read
+  bool threw = false;
+  try {
+    bvh.setAllocatorID(42);
+  } catch (const axom::mi355x_error&) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);
+  // BVH::writeVtkFile: 1 root + 2 per inner node boxes, 8 points each
+  bvh.writeVtkFile("test_facade_bvh.vtk");
+  FILE* f = std::fopen("test_facade_bvh.vtk", "r");
+  EXPECT_TRUE(f != nullptr);
+  if (f) {
+    char line[128] = {0};
+    EXPECT_TRUE(std::fgets(line, sizeof(line), f) != nullptr);
+    EXPECT_EQ(std::string(line), std::string("# vtk DataFile Version 3.0\n"));
+    int npts = 0;
+    while (std::fgets(line, sizeof(line), f))
+      if (std::sscanf(line, "POINTS %d double", &npts) == 1) break;
+    EXPECT_EQ(npts, 8 * (1 + 2 * (n - 1)));
+    std::fclose(f);
+    std::remove("test_facade_bvh.vtk");
+  }
+  for (void* p : {(void*)d_boxes, (void*)d_pts}) (void)hipFree(p);
+}
+
+// quest::SignedDistance single points and the signed_distance_* interface
+// (signed_distance.hpp:88-319) over an octahedron |x|+|y|+|z| = 1.
+static void test_signed_distance_interface() {
+  std::vector<double> x{1, -1, 0, 0, 0, 0}, y{0, 0, 1, -1, 0, 0}, z{0, 0, 0, 0, 1, -1};
+  std::vector<IndexType> conn{0, 2, 4, 2, 1, 4, 1, 3, 4, 3, 0, 4,
+                              2, 0, 5, 1, 2, 5, 3, 1, 5, 0, 3, 5};
+  double *dx = to_device(x), *dy = to_device(y), *dz = to_device(z);
+  IndexType* dc = to_device(conn);
+  axom::quest::SignedDistance<3, axom::MI355X_EXEC> sd(dx, dy, dz, 6, dc, 8, true, true);
+  // computeDistance(pt, cp, normal) against the batched form
+  std::vector<Pt> q{Pt{{2.0, 2.0, 2.0}}, Pt{{0.1, 0.0, 0.0}}, Pt{{0.0, 0.0, 3.0}}};
+  Pt* dq = to_device(q);
+  double* dd = nullptr;
+  Pt* dcp = nullptr;
+  axom::primal::Vector3* dn = nullptr;
+  axom::detail::hip_check(hipMalloc((void**)&dd, sizeof(double) * 3));
+  axom::detail::hip_check(hipMalloc((void**)&dcp, sizeof(Pt) * 3));
+  axom::detail::hip_check(hipMalloc((void**)&dn, sizeof(axom::primal::Vector3) * 3));
+  sd.computeDistances(3, dq, dd, dcp, dn);
+  std::vector<double> d = to_host(dd, 3);
+  std::vector<Pt> cp = to_host(dcp, 3);
+  std::vector<axom::primal::Vector3> nv = to_host(dn, 3);
+  for (int i = 0; i < 3; ++i) {
+    Pt c;
+    axom::primal::Vector3 nn;
+    EXPECT_EQ(sd.computeDistance(q[i], c, nn), d[i]);
+    EXPECT_EQ(sd.computeDistance(q[i]), d[i]);
+    for (int k = 0; k < 3; ++k) {
+      EXPECT_EQ(c.x[k], cp[i].x[k]);
+      EXPECT_EQ(nn.x[k], nv[i].x[k]);
+    }
+  }
+  EXPECT_NEAR(d[0], (6.0 - 1.0) / std::sqrt(3.0), 1e-15);  // outside, face (1,1,1)
+  EXPECT_NEAR(d[2], 2.0, 1e-15);                           // above the vertex (0,0,1)
+  EXPECT_TRUE(d[1] < 0.0);                                 // inside
+  // the process-wide query, initialized from host arrays
+  namespace q_ = axom::quest;
+  q_::signed_distance_finalize();
+  q_::signed_distance_set_closed_surface(true);
+  q_::signed_distance_set_execution_space(q_::SignedDistExec::GPU);
+  EXPECT_EQ(q_::signed_distance_init(x.data(), y.data(), z.data(), 6, conn.data(), 8), 0);
+  EXPECT_TRUE(q_::signed_distance_initialized());
+  bool threw = false;
+  try {
+    q_::signed_distance_set_compute_signs(false);
+  } catch (const axom::mi355x_error&) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);
+  double cx, cy, cz, nx, ny, nz;
+  EXPECT_EQ(q_::signed_distance_evaluate(2.0, 2.0, 2.0), d[0]);
+  EXPECT_EQ(q_::signed_distance_evaluate(0.1, 0.0, 0.0, cx, cy, cz, nx, ny, nz), d[1]);
+  EXPECT_TRUE(cx == cp[1].x[0] && cy == cp[1].x[1] && cz == cp[1].x[2]);
+  EXPECT_TRUE(nx == nv[1].x[0] && ny == nv[1].x[1] && nz == nv[1].x[2]);
+  std::vector<double> hx{2.0, 0.1, 0.0}, hy{2.0, 0.0, 0.0}, hz{2.0, 0.0, 3.0}, phi(3, 0.0);
+  q_::signed_distance_evaluate(hx.data(), hy.data(), hz.data(), 3, phi.data());  // host arrays
+  for (int i = 0; i < 3; ++i) EXPECT_EQ(phi[i], d[i]);
+  double lo[3], hi[3];
+  q_::signed_distance_get_mesh_bounds(lo, hi);
+  for (int k = 0; k < 3; ++k) {
+    EXPECT_EQ(lo[k], -1.0);
+    EXPECT_EQ(hi[k], 1.0);
+  }
+  q_::signed_distance_finalize();
+  EXPECT_TRUE(!q_::signed_distance_initialized());
+  threw = false;
+  try {
+    (void)q_::signed_distance_evaluate(0.0, 0.0, 0.0);
+  } catch (const axom::mi355x_error&) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);
+  // STLReader (quest_stl_reader.cpp:99-140): the reference's one-triangle file
+  {
+    FILE* f = std::fopen("test_facade_triangle.stl", "w");
+    std::fputs("solid triangle\n\t facet normal 0.0 0.0 1.0\n\t\t outer loop\n"
+               "\t\t\t vertex 0.0 0.0 0.0\n\t\t\t vertex 1.0 0.0 0.0\n"
+               "\t\t\t vertex 0.0 1.0 0.0\n\t\t endloop\n\t endfacet\n"
+               "endsolid triangle\n", f);
+    std::fclose(f);
+    axom::quest::STLReader reader;
+    reader.setFileName("test_facade_triangle.stl");
+    EXPECT_EQ(reader.read(), 0);
+    EXPECT_EQ(reader.getNumNodes(), 3);
+    EXPECT_EQ(reader.getNumFaces(), 1);
+    std::vector<double> mx, my, mz;
+    std::vector<IndexType> mc;
+    reader.getMesh(mx, my, mz, mc);
+    EXPECT_TRUE(mx == std::vector<double>({0.0, 1.0, 0.0}));
+    EXPECT_TRUE(my == std::vector<double>({0.0, 0.0, 1.0}));
+    EXPECT_TRUE(mz == std::vector<double>({0.0, 0.0, 0.0}));
+    EXPECT_TRUE(mc == std::vector<IndexType>({0, 1, 2}));
+    std::remove("test_facade_triangle.stl");
+    axom::quest::STLReader missing("foo.stl");
+    EXPECT_TRUE(missing.read() != 0);
+  }
+  for (void* p : {(void*)dx, (void*)dy, (void*)dz, (void*)dc, (void*)dq, (void*)dd, (void*)dcp,
+                  (void*)dn})
+    (void)hipFree(p);
+}
+
 int main() {
   test_bvh_grid();
+  test_traverser_reports_a_full_stack();
+  test_allocator_and_vtk();
+  test_signed_distance_interface();
   test_traverser_matches_find();
   test_bvh2d_grid();
   test_sdf_sphere();

@@ -96,48 +96,135 @@ def test_initialize_again_rebuilds_in_place():
             assert np.array_equal(cnt.cpu().numpy(), ocnt) and np.array_equal(cand, ocand)
 
 
-def test_fill_after_other_count_searches_again():
-    """The fill pass copies the ids the count pass kept only when it follows
-    that count pass with the same inputs; a fill for other inputs (here after
-    a count for other points, and after a rebuild) searches the tree again."""
+def _abi_count(g, t):
     from axom_amd import _lib as L
-    boxes = _boxes(30_000, 8, 0.02, 0.08)  # many candidates per point: the list path too
+    n = t.shape[0]
+    off = torch.empty(n, dtype=torch.int32, device=DEV)
+    cnt = torch.empty_like(off)
+    tot = ctypes.c_int64()
+    base = t.data_ptr()
+    L.check(L.lib().axsq_bvh_find_count(g._h, L.QUERY_POINTS, ctypes.c_void_p(base),
+                                        ctypes.c_void_p(base + 8), ctypes.c_void_p(base + 16),
+                                        3, n, g._tol, spin._ptr(off), spin._ptr(cnt),
+                                        ctypes.byref(tot), None))
+    return off, cnt, tot.value
+
+
+def _abi_fill(g, t, off, cnt, total, guard=4096):
+    """Fills into a buffer with `guard` words after the total, all -7 first;
+    returns (rc, ids, the guard words)."""
+    from axom_amd import _lib as L
+    buf = torch.full((total + guard,), -7, dtype=torch.int32, device=DEV)
+    base = t.data_ptr()
+    rc = L.lib().axsq_bvh_find_fill(g._h, L.QUERY_POINTS, ctypes.c_void_p(base),
+                                    ctypes.c_void_p(base + 8), ctypes.c_void_p(base + 16), 3,
+                                    t.shape[0], g._tol, spin._ptr(off), spin._ptr(cnt),
+                                    spin._ptr(buf), None)
+    b = buf.cpu().numpy()
+    return rc, b[:total], b[total:]
+
+
+def test_two_phase_fill_is_bounded_by_the_counts():
+    """Phase 2 searches again and writes at most counts[i] ids per query. With
+    the queries unchanged it gives the oracle's lists; with the queries moved
+    in place after phase 1 (so some now have more candidates) it returns
+    AXSQ_ERR_INVALID_ARG, and no id lands past the buffer of phase 1's total:
+    the guard words after it keep their value."""
+    from axom_amd import _lib as L
+    boxes = _boxes(30_000, 8, 0.02, 0.08)  # many candidates per point
     g = spin.BVH(_t(boxes))
     o = orc.OracleBVH(boxes, 1.000123)
     pa = np.random.default_rng(1).random((4000, 3))
-    pb = np.random.default_rng(2).random((3000, 3))
-    ta, tb = _t(pa), _t(pb)
+    ta = _t(pa)
     ooff, ocnt, ocand = o.find("points", pa)
-    assert ocnt.max() > 32 and (ocnt <= 32).any()  # both the kept ids and the list
+    off, cnt, tot = _abi_count(g, ta)
+    assert tot == ocand.size and np.array_equal(cnt.cpu().numpy(), ocnt)
+    rc, ids, guard = _abi_fill(g, ta, off, cnt, tot)
+    assert rc == 0 and np.array_equal(ids, ocand) and np.all(guard == -7)
+    # move the queries in place: the same buffer now holds other points
+    ta.copy_(_t(np.random.default_rng(2).random((4000, 3)) * 0.2 + 0.4))  # a denser region
+    rc, ids, guard = _abi_fill(g, ta, off, cnt, tot)
+    assert rc == L.ERRORS_BY_NAME["AXSQ_ERR_INVALID_ARG"]
+    assert "count" in L.last_error()
+    assert np.all(guard == -7)
+    # after the error the tree's status is clear: a fresh count and fill agree again
+    off2, cnt2, tot2 = _abi_count(g, ta)
+    rc, ids, guard = _abi_fill(g, ta, off2, cnt2, tot2)
+    _, _, want = o.find("points", ta.cpu().numpy())
+    assert rc == 0 and np.array_equal(ids, want) and np.all(guard == -7)
 
-    def count(t):
-        n = t.shape[0]
+
+def test_find_into_capacity():
+    """The one-call search into caller storage writes nothing when the room is
+    too small (only the counts, offsets and total), and the oracle's lists when
+    it is enough; rebuilding the tree in between changes nothing."""
+    from axom_amd import _lib as L
+    boxes = _boxes(30_000, 8, 0.02, 0.08)
+    g = spin.BVH(_t(boxes))
+    pa = np.random.default_rng(3).random((5000, 3))
+    ta = _t(pa)
+    ooff, ocnt, ocand = orc.OracleBVH(boxes, 1.000123).find("points", pa)
+    n = pa.shape[0]
+    base = ta.data_ptr()
+    for cap in (0, ocand.size - 1, ocand.size, ocand.size + 100):
         off = torch.empty(n, dtype=torch.int32, device=DEV)
         cnt = torch.empty_like(off)
+        buf = torch.full((max(cap, 1) + 64,), -7, dtype=torch.int32, device=DEV)
         tot = ctypes.c_int64()
-        base = t.data_ptr()
-        L.check(L.lib().axsq_bvh_find_count(g._h, L.QUERY_POINTS, ctypes.c_void_p(base),
-                                            ctypes.c_void_p(base + 8), ctypes.c_void_p(base + 16),
-                                            3, n, g._tol, spin._ptr(off), spin._ptr(cnt),
-                                            ctypes.byref(tot), None))
-        return off, tot.value
+        g.initialize(_t(boxes))
+        L.check(L.lib().axsq_bvh_find_into(g._h, L.QUERY_POINTS, ctypes.c_void_p(base),
+                                           ctypes.c_void_p(base + 8),
+                                           ctypes.c_void_p(base + 16), 3, n, g._tol,
+                                           spin._ptr(off), spin._ptr(cnt), spin._ptr(buf), cap,
+                                           ctypes.byref(tot), None))
+        b = buf.cpu().numpy()
+        assert tot.value == ocand.size
+        assert np.array_equal(off.cpu().numpy(), ooff) and np.array_equal(cnt.cpu().numpy(), ocnt)
+        if cap < ocand.size:
+            assert np.all(b == -7), cap
+        else:
+            assert np.array_equal(b[:ocand.size], ocand) and np.all(b[ocand.size:] == -7), cap
 
-    def fill(t, off, total):
-        cand = torch.full((total,), -7, dtype=torch.int32, device=DEV)
-        base = t.data_ptr()
-        L.check(L.lib().axsq_bvh_find_fill(g._h, L.QUERY_POINTS, ctypes.c_void_p(base),
-                                           ctypes.c_void_p(base + 8), ctypes.c_void_p(base + 16),
-                                           3, t.shape[0], g._tol, spin._ptr(off), spin._ptr(cand),
-                                           None))
-        return cand.cpu().numpy()
 
-    off_a, tot_a = count(ta)
-    assert np.array_equal(fill(ta, off_a, tot_a), ocand)  # kept ids + listed queries
-    count(tb)
-    assert np.array_equal(fill(ta, off_a, tot_a), ocand)  # inputs differ: searched again
-    off_a, tot_a = count(ta)
-    g.initialize(_t(boxes))
-    assert np.array_equal(fill(ta, off_a, tot_a), ocand)  # rebuilt: searched again
+@pytest.mark.parametrize("allocator", [0, 1, 2])
+def test_one_call_find_allocator(allocator):
+    """axsq_bvh_find allocates the candidate array in the memory the tree's
+    allocator id names (BVH::setAllocatorID): device, managed or pinned host;
+    axsq_free releases each."""
+    from axom_amd import _lib as L
+    boxes = _boxes(5000, 4)
+    g = spin.BVH(_t(boxes))
+    L.check(L.lib().axsq_bvh_set_allocator(g._h, allocator))
+    assert L.lib().axsq_bvh_get_allocator(g._h) == allocator
+    pa = np.random.default_rng(5).random((3000, 3))
+    ta = _t(pa)
+    _, ocnt, ocand = orc.OracleBVH(boxes, 1.000123).find("points", pa)
+    off = torch.empty(3000, dtype=torch.int32, device=DEV)
+    cnt = torch.empty_like(off)
+    ptr, tot = ctypes.c_void_p(), ctypes.c_int64()
+    base = ta.data_ptr()
+    L.check(L.lib().axsq_bvh_find(g._h, L.QUERY_POINTS, ctypes.c_void_p(base),
+                                  ctypes.c_void_p(base + 8), ctypes.c_void_p(base + 16), 3, 3000,
+                                  g._tol, spin._ptr(off), spin._ptr(cnt), ctypes.byref(ptr),
+                                  ctypes.byref(tot), None))
+    assert tot.value == ocand.size
+    host = np.empty(tot.value, np.int32)
+    torch.cuda.synchronize()
+    if allocator == 0:
+        view = torch.as_tensor(_DevI32(ptr.value, tot.value), device=DEV)
+        host[:] = view.cpu().numpy()
+    else:  # managed and pinned host memory are readable from the host
+        ctypes.memmove(host.ctypes.data, ptr.value, 4 * tot.value)
+    assert np.array_equal(host, ocand)
+    L.check(L.lib().axsq_free(ptr))
+    with pytest.raises(L.AxsqError):
+        L.check(L.lib().axsq_bvh_set_allocator(g._h, 9))
+
+
+class _DevI32:
+    def __init__(self, ptr, n):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<i4", "data": (ptr, False),
+                                         "version": 2}
 
 
 def test_build_is_deterministic():

@@ -260,3 +260,34 @@ def test_many_tied_triangles_at_a_vertex(monkeypatch, budget):
     # the pole queries are outside, with the pole as closest point
     assert np.all(g["dist"][:128] > 0)
     assert np.array_equal(g["cp"][:64], np.tile(v[0], (64, 1)))
+
+
+@pytest.mark.parametrize("cap", [0, 16])
+def test_walk_records_cap(monkeypatch, cap):
+    """With room for at most `cap` (at least one) saved walks, the walks over
+    the step budget beyond it finish in the first pass: the results are the
+    oracle's."""
+    v, t, q = synth.workload2(nq=20_000, level=4)
+    monkeypatch.setenv("AXSQ_SDF_SUSPEND_CAP", str(cap))
+    sdf = _sdf_with(monkeypatch, v, t, 37)
+    g = _eval(sdf, q)
+    _compare(g, orc.OracleSDF(v, t).eval(q, nthreads=8))
+    one = _sdf_with(monkeypatch, v, t, 0)  # no budget: one pass, the same answers
+    _compare(_eval(one, q), g)
+
+
+def test_walk_records_grow_from_previous_call(monkeypatch):
+    """Budget 1 suspends every walk. The first call has records for an
+    eighth of the 200,000 queries (65,536 at least), so the rest finish in the
+    first pass; the second call on the handle has room for all of them. Both
+    give the oracle's answers."""
+    v, t, q = synth.workload2(nq=200_000, level=4)
+    sdf = _sdf_with(monkeypatch, v, t, 1)
+    want = orc.OracleSDF(v, t).eval(q, nthreads=8)
+    first = _eval(sdf, q)
+    n_first = sdf.lastSuspended()
+    second = _eval(sdf, q)
+    n_second = sdf.lastSuspended()
+    assert n_first < n_second and n_second >= q.shape[0] - 1000
+    _compare(first, want)
+    _compare(second, want)
