@@ -57,6 +57,8 @@ _SIGS = {
     "axsq_bvh_find_into": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp, _vp,
                                   _i64, ctypes.POINTER(_i64), _vp]),
     "axsq_bvh_set_allocator": (_i32, [_vp, _i32]),
+    "axsq_bvh_stats_enable": (_i32, [_vp, _i32]),
+    "axsq_bvh_stats_get": (_i32, [_vp, ctypes.POINTER(_i64)]),
     "axsq_bvh_get_allocator": (_i32, [_vp]),
     "axsq_bvh_write_vtk": (_i32, [_vp, ctypes.c_char_p]),
     "axsq_host_free": (_i32, [_vp]),

@@ -439,14 +439,23 @@ static int find_count_impl(const axsq_bvh* b, int kind, const double* x, const d
   int64_t* tot_dev = ws.find_n.as<int64_t>();
   uint32_t* nover = reinterpret_cast<uint32_t*>(tot_dev + 1);
   AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(nover), 0, 1, s));
+  if (b->stats) AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(b->find_visits), 0, 2, s));
   AXSQ_TRY(find_launch(b, kind, 0, x, y, z, stride, nq, tol, counts, nullptr, nullptr, nullptr,
                        keep ? ws.find_stash.as<int32_t>() : nullptr, kstash,
                        keep ? ws.find_cnt.as<int32_t>() : nullptr,
                        keep ? ws.find_over.as<int32_t>() : nullptr, nover, s));
-  AXSQ_TRY(scan_exclusive_i32(counts, offsets, nq, tot_dev, ws, s));
+  {
+    ProfRange ps("bvh_find_scan", s);
+    AXSQ_TRY(scan_exclusive_i32(counts, offsets, nq, tot_dev, ws, s));
+  }
   int64_t tn[2] = {0, 0};
   AXSQ_CHECK_HIP(hipMemcpyAsync(tn, tot_dev, sizeof(tn), hipMemcpyDeviceToHost, s));
   AXSQ_TRY(finish(b->status, s));
+  if (b->stats) {
+    unsigned long long v = 0;
+    AXSQ_CHECK_HIP(hipMemcpy(&v, b->find_visits, sizeof(v), hipMemcpyDeviceToHost));
+    const_cast<axsq_bvh*>(b)->last_visits = (int64_t)v;
+  }
   const int64_t t = tn[0];
   if (t > 0x7fffffffll) {
     set_error("candidate total %lld exceeds the int32 index range", (long long)t);
@@ -552,6 +561,7 @@ int axsq_bvh_find_into(const axsq_bvh* b, int kind, const double* x, const doubl
   return find_fill_kept(b, kind, x, y, z, stride, nq, tol, offsets, counts, candidates, nover,
                         kstash, s);
 }
+
 
 int axsq_bvh_find(const axsq_bvh* b, int kind, const double* x, const double* y, const double* z,
                   int64_t stride, int64_t nq, double tol, int32_t* offsets, int32_t* counts,
@@ -922,6 +932,18 @@ int axsq_prof_get(const char* name, int64_t* launches, double* total_ms) {
   auto it = g_totals.find(name);
   *launches = (it == g_totals.end()) ? 0 : it->second.first;
   *total_ms = (it == g_totals.end()) ? 0.0 : it->second.second;
+  return AXSQ_OK;
+}
+
+int axsq_bvh_stats_enable(axsq_bvh* b, int on) {
+  if (!b) return AXSQ_ERR_INVALID_ARG;
+  b->stats = on != 0;
+  return AXSQ_OK;
+}
+
+int axsq_bvh_stats_get(const axsq_bvh* b, int64_t* visits) {
+  if (!b || !visits) return AXSQ_ERR_INVALID_ARG;
+  *visits = b->last_visits;
   return AXSQ_OK;
 }
 

@@ -564,6 +564,7 @@ void bvh_release(axsq_bvh* b) {
   hipFree(b->leaf_ids);
   hipFree(b->mcodes);
   hipFree(b->status);
+  hipFree(b->find_visits);
   b->nodes = nullptr;
   b->nodes32 = nullptr;
   b->ranges32 = nullptr;
@@ -571,6 +572,7 @@ void bvh_release(axsq_bvh* b) {
   b->leaf_ids = nullptr;
   b->mcodes = nullptr;
   b->status = nullptr;
+  b->find_visits = nullptr;
 }
 
 template <class T>
@@ -609,6 +611,7 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
     AXSQ_TRY(dmalloc(&b->leaf_ids, (size_t)n));
     AXSQ_TRY(dmalloc(&b->mcodes, (size_t)n));
     AXSQ_TRY(dmalloc(&b->status, 1));
+    AXSQ_TRY(dmalloc(&b->find_visits, 1));
   }
   b->n = n;
   b->fp_mode = fp_mode;

@@ -148,6 +148,10 @@ struct axsq_bvh {
   uint32_t* mcodes = nullptr;       // n sorted Morton codes
   int32_t* status = nullptr;        // device status word (kStatus* bits)
   int allocator = AXSQ_ALLOC_DEVICE;  // memory of the candidate arrays axsq_bvh_find returns
+  // traversal statistics of the candidate search's count pass
+  bool stats = false;
+  unsigned long long* find_visits = nullptr;  // device: inner-node visits summed over queries
+  int64_t last_visits = 0;                    // host: the last count pass's sum
 };
 
 struct axsq_sdf {

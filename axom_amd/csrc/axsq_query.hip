@@ -49,6 +49,7 @@ struct FindArgs {
   int32_t* stash_cnt;  // count pass: a copy of the counts
   int32_t* over;       // count pass: queries with more than kstash; fill: those to search
   uint32_t* nover;
+  unsigned long long* visits;  // count pass: sum of the walks' inner-node visits (or NULL)
 };
 
 enum { kCountPass = 0, kFillAll = 1, kFillListed = 2 };
@@ -149,6 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
           });
     }
     if (v < 0) atomicOr(a.status, kStatusStack);
+    if (!FILL && a.visits && v > 0) atomicAdd(a.visits, (unsigned long long)v);
     if (FILL && count != room) atomicOr(a.status, kStatusFillMismatch);
     if (!FILL) {
       a.counts[i] = count;
@@ -261,6 +263,7 @@ int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const do
   a.stash_cnt = stash_cnt;
   a.over = over;
   a.nover = nover;
+  a.visits = (mode == kCountPass && b->stats) ? b->find_visits : nullptr;
   const bool mixed = (b->fp_mode == AXSQ_FP32_TRAVERSAL) && b->nodes32;
   ProfRange pr(mode == kCountPass ? "bvh_find_count" : "bvh_find_fill", s);
   const hipError_t e = mixed ? launch_mode<true>(mode, kind, a, s) : launch_mode<false>(mode, kind, a, s);

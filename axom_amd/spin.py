@@ -142,6 +142,8 @@ class BVH:
         self._h = h
         if getattr(self, "_alloc", 0):
             L.check(L.lib().axsq_bvh_set_allocator(self._h, self._alloc))
+        if getattr(self, "_stats", False):
+            L.check(L.lib().axsq_bvh_stats_enable(self._h, 1))
         return BVH_BUILD_OK
 
     def isInitialized(self):
@@ -167,6 +169,19 @@ class BVH:
         lo, hi = L._dbl3(), L._dbl3()
         L.check(L.lib().axsq_bvh_bounds(self._h, lo, hi))
         return np.array(lo[:d]), np.array(hi[:d])
+
+    def enableStats(self, on=True):
+        """Counts inner-node visits in the candidate searches' count passes
+        (no reference counterpart)."""
+        self._stats = bool(on)
+        if self._h:
+            L.check(L.lib().axsq_bvh_stats_enable(self._h, int(self._stats)))
+
+    def lastStats(self):
+        """Inner-node visits of the last count pass, summed over its queries."""
+        v = ctypes.c_int64()
+        L.check(L.lib().axsq_bvh_stats_get(self._h, ctypes.byref(v)))
+        return v.value
 
     def numLeaves(self):
         return int(L.lib().axsq_bvh_num_leaves(self._h)) if self._h else 0

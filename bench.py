@@ -10,19 +10,31 @@ gradient) over all queries, inputs resident in HBM.
   value            FP64-plain forward+backward queries/s
   fp32_traversal   the same step with FP32 traversal (FP64 finish), and its
                    error against the FP64 result
-  bvh_build        BVH build Mprims/s over the 81,920 triangles and over the
-                   100,000-box findPoints workload
-  find_points      findPoints queries/s (100,000 boxes, 1,000,000 points)
   roofline         dominant kernel (FP64 SDF forward) timed with HIP events on
                    its stream; algorithmic bytes per query
                    = 24 (query) + 8 (distance) + 24 (closest point) + 4 (element)
                    + V*104 (inner-node visits: 2 x 48 B boxes + 2 x 4 B links)
                    + L*84 (leaf checks: 3 x 24 B vertices + 3 x 4 B indices),
                    V, L = mean visits / leaf checks of the reference traversal
-                   (the FP64 kernel follows it exactly; counted on the GPU)
+                   (the FP64 kernel follows it exactly; counted on the GPU);
+                   traffic = HBM bytes per launch from the committed counter runs
+  bvh_build        BVH build Mprims/s over the 81,920 triangles and over the
+                   100,000 boxes of the findPoints workload
+  find_points      100,000 boxes / 1,000,000 points: findPoints queries/s (FP64
+                   and FP32 traversal), the count pass's roofline (24 B point +
+                   8 B offset and count + 4 B per candidate + V x 104 B)
+  configs          the three larger workloads at their full sizes: torus
+                   (2,097,152 triangles, 16M queries, forward), wavy L7
+                   icosphere (1M queries, forward + adjoint), L8 shape fit (8M
+                   queries, 20 iterations): ms, queries/s, the forward's
+                   roofline, the FP32 form's error against FP64, per-iteration
+                   build / forward / backward times
   cpu_baseline     the CPU oracle (restatement of the reference's SEQ path),
                    forward+backward on the first 1,000,000 queries, timed on
-                   this host's cores (and on one thread for the first 100,000)
+                   every CPU this process may use (and on one thread for the
+                   first 100,000); each configuration carries its own on a
+                   bounded sample; host gives the thread count, nproc, the
+                   affinity set, the cgroup CPU quota and the CPU model
 """
 import argparse
 import ctypes
@@ -62,10 +74,32 @@ def parse():
     return p.parse_args()
 
 
-def other_workload(args, torch, dev):
-    """The remaining synthetic workloads (see axom_amd/synth.py for their inputs)."""
-    from axom_amd import autograd, quest, shapefit, spin, synth
+def host_cpus():
+    """Threads for the CPU legs and a description of the host: every CPU this
+    process may run on (its affinity set, capped by a cgroup CPU quota when
+    one is set), with nproc and the CPU model."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(-(-int(q) // int(per))))
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = min(aff, quota) if quota else aff
+    return threads, {"threads_used": threads, "nproc": os.cpu_count(), "affinity_cpus": aff,
+                     "cgroup_quota_cpus": quota, "model": model}
 
+
+def sync_timer(torch):
     def sync_time(fn, reps):
         fn()
         torch.cuda.synchronize()
@@ -74,63 +108,243 @@ def other_workload(args, torch, dev):
             fn()
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / reps
+    return sync_time
+
+
+def prof_reader(L, _lib):
+    def prof(name):
+        n, ms = ctypes.c_int64(), ctypes.c_double()
+        _lib.check(L.axsq_prof_get(name.encode(), ctypes.byref(n), ctypes.byref(ms)))
+        return ms.value / max(n.value, 1)
+    return prof
+
+
+def sdf_roofline(kernel, ms, vbar, lbar, node_bytes, nq):
+    """Roofline of one signed-distance forward launch over nq queries: 24 B
+    query + 8 B distance + 24 B closest point + 4 B element + V node visits x
+    node_bytes (104 FP64 / 56 FP32) + L leaf checks x 84 B."""
+    bpq = 24 + 8 + 24 + 4 + node_bytes * vbar + 84.0 * lbar
+    ach = bpq * nq / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "kernel": kernel, "avg_launch_ms": ms,
+            "bytes_per_query": bpq, "mean_inner_visits": vbar, "mean_leaf_checks": lbar}
+
+
+def sdf_config(torch, dev, L, _lib, v, t, q, g, steps, want_bwd, cpu_threads, cpu_sample,
+               label):
+    """Signed distance over one mesh and query set: FP64 and FP32-traversal
+    forward (+ adjoint when want_bwd), ms per step, queries/s, the forward's
+    roofline with the reference traversal's V and L (counted by the FP64
+    kernel), the FP32 form's max / RMS relative error and sign mismatches
+    against FP64; with cpu_threads, the oracle on a bounded sample."""
+    from axom_amd import quest
+    vd, td, qd = (torch.as_tensor(a, device=dev) for a in (v, t, q))
+    gd = torch.as_tensor(g, device=dev) if g is not None else None
+    nq = q.shape[0]
+    sdf = quest.SignedDistance(vd, td, isWatertight=True, computeSign=True, fp_mode="fp32")
+    dist = torch.empty(nq, dtype=torch.float64, device=dev)
+    elem = torch.empty(nq, dtype=torch.int32, device=dev)
+    prof = prof_reader(L, _lib)
+    out = {"workload": label, "queries": nq, "triangles": int(t.shape[0])}
+    res = {}
+    for mode in ("fp64", "fp32"):
+        def step():
+            sdf.computeDistances(nq, qd, outSgnDist=dist, outElem=elem, fp_mode=mode)
+            if want_bwd:
+                sdf.backward(qd, dist, elem, gd)
+        step()
+        torch.cuda.synchronize()
+        L.axsq_prof_reset()
+        L.axsq_prof_enable(1)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        L.axsq_prof_enable(0)
+        res[mode] = dist.clone()
+        key = "fwd_bwd" if want_bwd else "fwd"
+        out[mode] = {"ms_per_step": el * 1e3, f"{key}_queries_per_s": nq / el,
+                     "sdf_eval_ms": prof(f"sdf_eval_{mode}"),
+                     "second_pass_walks": sdf.lastSuspended()}
+        if want_bwd:
+            out[mode]["sdf_backward_ms"] = prof("sdf_backward")
+    sdf.enableStats(True)
+    sdf.computeDistances(nq, qd, outSgnDist=dist, outElem=elem, fp_mode="fp64")
+    visits, leaves = sdf.lastStats()
+    sdf.enableStats(False)
+    vbar, lbar = visits / nq, leaves / nq
+    out["fp64"]["roofline"] = sdf_roofline("k_sdf_eval<FP64>", out["fp64"]["sdf_eval_ms"], vbar,
+                                           lbar, 104.0, nq)
+    out["fp32"]["roofline"] = sdf_roofline("k_sdf_eval<FP32 traversal>",
+                                           out["fp32"]["sdf_eval_ms"], vbar, lbar, 56.0, nq)
+    rel = (res["fp32"] - res["fp64"]).abs() / res["fp64"].abs().clamp_min(1e-300)
+    out["fp32"]["max_rel_err_vs_fp64"] = float(rel.max())
+    out["fp32"]["rms_rel_err_vs_fp64"] = float(torch.sqrt((rel * rel).mean()))
+    out["fp32"]["sign_mismatches"] = int((torch.signbit(res["fp32"]) !=
+                                          torch.signbit(res["fp64"])).sum())
+    if cpu_threads:
+        from oracle import oracle as orc
+        # an even sample over the whole query set (the sets are ordered in parts)
+        idx = np.linspace(0, nq - 1, min(cpu_sample, nq)).astype(np.int64)
+        qs = np.ascontiguousarray(q[idx])
+        t0 = time.perf_counter()
+        o = orc.OracleSDF(v, t)
+        tb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = o.eval(qs, nthreads=cpu_threads, want_normal=False)
+        if want_bwd:
+            o.backward(qs, r["dist"], r["elem"], np.ascontiguousarray(g[idx]))
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "queries_per_s": len(idx) / tc, "cores": cpu_threads, "kind": "port",
+            "sample": f"{len(idx):,} queries evenly spaced over the {nq:,}, FP64 forward" +
+                      (" + adjoint" if want_bwd else ""),
+            "seconds": tc, "build_seconds_1_thread": tb,
+            "matches_gpu_bitwise": bool(np.array_equal(r["dist"],
+                                                       res["fp64"][idx].cpu().numpy()))}
+    return out
+
+
+def shapefit_config(torch, dev, v, t, q, target, cpu_threads, label, iterations=20):
+    """The shape-fitting loop: per-iteration BVH rebuild, FP32-traversal
+    forward and adjoint times, loss; with cpu_threads, the oracle's rebuild
+    (one thread) and forward + adjoint over a bounded sample of the queries."""
+    from axom_amd import shapefit
+    vd, td, qd, tg = (torch.as_tensor(a, device=dev) for a in (v, t, q, target))
+    vf, hist = shapefit.shape_fit(vd, td, qd, tg, iterations=iterations, lr=1e-3,
+                                  fp_mode="fp32")
+    n = len(hist)
+    out = {"workload": label, "queries": int(q.shape[0]), "triangles": int(t.shape[0]),
+           "iterations": n,
+           "mean_build_ms": sum(h["build_ms"] for h in hist[1:]) / max(n - 1, 1),
+           "mean_forward_ms": sum(h["forward_ms"] for h in hist) / n,
+           "mean_backward_ms": sum(h["backward_ms"] for h in hist) / n,
+           "loss_first_last": [hist[0]["loss"], hist[-1]["loss"]],
+           "per_iteration": [{k: (round(x, 4) if isinstance(x, float) and k != "loss" else x)
+                              for k, x in h.items()} for h in hist]}
+    it_ms = out["mean_build_ms"] + out["mean_forward_ms"] + out["mean_backward_ms"]
+    out["ms_per_iteration"] = it_ms
+    out["fwd_bwd_queries_per_s"] = q.shape[0] / (it_ms / 1e3)
+    out["build_mprims_per_s"] = t.shape[0] / (out["mean_build_ms"] / 1e3) / 1e6
+    if cpu_threads:
+        from oracle import oracle as orc
+        m = min(200_000, q.shape[0])
+        idx = np.linspace(0, q.shape[0] - 1, m).astype(np.int64)
+        qs = np.ascontiguousarray(q[idx])
+        t0 = time.perf_counter()
+        o = orc.OracleSDF(v, t)
+        tb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r = o.eval(qs, nthreads=cpu_threads, want_cp=False, want_normal=False)
+        gs = ((r["dist"] - target[idx]) * (2.0 / q.shape[0])).astype(np.float32)
+        o.backward(qs, r["dist"], r["elem"], gs)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"build_ms_1_thread": tb * 1e3,
+                               "fwd_bwd_queries_per_s": m / tc, "cores": cpu_threads,
+                               "kind": "port",
+                               "sample": f"first iteration: {m:,} of the queries, evenly "
+                                         "spaced, FP64 forward + adjoint"}
+    return out
+
+
+def find_points_config(torch, dev, L, _lib, boxes, pts, cpu_threads, reps=10):
+    """BVH build over the boxes and findPoints over the points (FP64 and FP32
+    traversal), with the count pass's roofline: 24 B point + 8 B offset and
+    count + 4 B per candidate + V inner-node visits x 104 B, V counted by the
+    FP64 count pass (the reference walk's own visits)."""
+    from axom_amd import spin
+    prof = prof_reader(L, _lib)
+    sync_time = sync_timer(torch)
+    bx, p = torch.as_tensor(boxes, device=dev), torch.as_tensor(pts, device=dev)
+    n = p.shape[0]
+    out = {"workload": f"{boxes.shape[0]:,} boxes, {n:,} points; BVH build + findPoints",
+           "queries": n, "boxes": int(boxes.shape[0])}
+    bvh = spin.BVH(bx)
+    off = torch.empty(n, dtype=torch.int32, device=dev)
+    cnt = torch.empty(n, dtype=torch.int32, device=dev)
+    tb = sync_time(lambda: bvh.initialize(bx), 20)
+    out["build_ms"] = tb * 1e3
+    out["build_mprims_per_s"] = boxes.shape[0] / tb / 1e6
+    res = {}
+    L.axsq_prof_reset()
+    L.axsq_prof_enable(1)
+    tq = sync_time(lambda: res.__setitem__("c", bvh.findPoints(off, cnt, n, p)), reps)
+    L.axsq_prof_enable(0)
+    cand = res["c"]
+    out["ms"] = tq * 1e3
+    out["queries_per_s"] = n / tq
+    out["candidates"] = int(cand.numel())
+    t_count = prof("bvh_find_count")
+    bvh.enableStats(True)
+    bvh.findPoints(off, cnt, n, p)
+    vbar = bvh.lastStats() / n
+    bvh.enableStats(False)
+    bpq = 24 + 8 + 4.0 * cand.numel() / n + 104.0 * vbar
+    ach = bpq * n / (t_count / 1e3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": ach / HBM_PEAK_GBS, "kernel": "k_find<points, count>",
+                       "avg_launch_ms": t_count, "bytes_per_query": bpq,
+                       "mean_inner_visits": vbar,
+                       "other_kernels_ms": {"scan": prof("bvh_find_scan"),
+                                            "copy": prof("bvh_find_copy"),
+                                            "fill": prof("bvh_find_fill")}}
+    bvh32 = spin.BVH(bx, fp_mode="fp32")
+    off32, cnt32 = torch.empty_like(off), torch.empty_like(cnt)
+    tq32 = sync_time(lambda: res.__setitem__("c32", bvh32.findPoints(off32, cnt32, n, p)), reps)
+    out["fp32_traversal"] = {"ms": tq32 * 1e3, "queries_per_s": n / tq32,
+                             "lists_identical_to_fp64": bool(
+                                 torch.equal(res["c32"], cand) and torch.equal(off32, off) and
+                                 torch.equal(cnt32, cnt))}
+    if cpu_threads:
+        from oracle import oracle as orc
+        t0 = time.perf_counter()
+        ob = orc.OracleBVH(boxes)
+        el_b = time.perf_counter() - t0
+        same_tree = bool(np.array_equal(ob.export()["nodes"], bvh.export()["nodes"].cpu().numpy()))
+        t0 = time.perf_counter()
+        ooff, ocnt, ocand = ob.find("points", pts, nthreads=cpu_threads)
+        el_f = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "build_ms_1_thread": el_b * 1e3, "build_mprims_per_s": boxes.shape[0] / el_b / 1e6,
+            "tree_matches_gpu_bitwise": same_tree,
+            "ms": el_f * 1e3, "queries_per_s": n / el_f, "cores": cpu_threads, "kind": "port",
+            "sample": f"the same {boxes.shape[0]:,} boxes (build, one thread) and {n:,} points",
+            "lists_match_gpu_bitwise": bool(np.array_equal(ocnt, cnt.cpu().numpy()) and
+                                            np.array_equal(ocand, cand.cpu().numpy()))}
+    return out
+
+
+LABEL3 = ("torus R=1, r=0.3+0.02 sin(7u) cos(5v), 1024x1024 quads = 2,097,152 tris; 16M "
+          "queries (half uniform in the box padded by 0.2, half |d|<0.01 of the surface); "
+          "SignedDistance forward, FP64 and FP32 traversal")
+LABEL4 = ("icosphere L7 (327,680 tris), radius 1+0.1 sin(4 theta) cos(3 phi); 1M queries at "
+          "normal offsets U[-0.1,0.1], upstream gradient N(0,1) FP32; forward + adjoint")
+LABEL5 = ("icosphere L8 (1,310,720 tris) fitted to a rounded box (b=(0.8,0.6,0.5), r=0.1); 8M "
+          "queries U[-1.2,1.2]^3; per iteration: BVH rebuild + FP32-traversal forward + "
+          "L=mean((d-t)^2) + adjoint + V -= 1e-3 dL/dV")
+
+
+def other_workload(args, torch, dev):
+    """One workload on its own line (see axom_amd/synth.py for the inputs)."""
+    from axom_amd import _lib, quest, spin, synth
+    L = _lib.lib()
+    sync_time = sync_timer(torch)
+    cpu = 0 if args.no_cpu else host_cpus()[0]
 
     if args.workload == "find_points":
         boxes, pts = synth.workload1()
-        bx, p = torch.as_tensor(boxes, device=dev), torch.as_tensor(pts, device=dev)
-        bvh = spin.BVH(bx)
-        off = torch.empty(p.shape[0], dtype=torch.int32, device=dev)
-        cnt = torch.empty(p.shape[0], dtype=torch.int32, device=dev)
-        tb = sync_time(lambda: bvh.initialize(bx), args.steps)
-        tq = sync_time(lambda: bvh.findPoints(off, cnt, p.shape[0], p), args.steps)
-        c64 = bvh.findPoints(off, cnt, p.shape[0], p)
-        bvh32 = spin.BVH(bx, fp_mode="fp32")
-        off32, cnt32 = torch.empty_like(off), torch.empty_like(cnt)
-        tb32 = sync_time(lambda: bvh32.initialize(bx), args.steps)
-        tq32 = sync_time(lambda: bvh32.findPoints(off32, cnt32, p.shape[0], p), args.steps)
-        c32 = bvh32.findPoints(off32, cnt32, p.shape[0], p)
-        same = bool(torch.equal(c32, c64) and torch.equal(off32, off) and torch.equal(cnt32, cnt))
-        return {"workload": "100,000 boxes U[0,1]^3 (half-extents U[0.001,0.01]), 1,000,000 "
-                            "points, FP64 build + findPoints",
-                "build_mprims_per_s": boxes.shape[0] / tb / 1e6, "build_ms": tb * 1e3,
-                "find_points_queries_per_s": p.shape[0] / tq, "find_ms": tq * 1e3,
-                "fp32_traversal": {"build_ms": tb32 * 1e3, "find_ms": tq32 * 1e3,
-                                   "lists_identical_to_fp64": same}}
+        return find_points_config(torch, dev, L, _lib, boxes, pts, cpu, reps=args.steps)
     if args.workload == "torus_mixed":
         v, t, q = synth.workload3()
-        vd, td, qd = (torch.as_tensor(a, device=dev) for a in (v, t, q))
-        sdf = quest.SignedDistance(vd, td, fp_mode="fp32")
-        n = q.shape[0]
-        d32 = torch.empty(n, dtype=torch.float64, device=dev)
-        d64 = torch.empty(n, dtype=torch.float64, device=dev)
-        t32 = sync_time(lambda: sdf.computeDistances(n, qd, outSgnDist=d32, fp_mode="fp32"),
-                        args.steps)
-        t64 = sync_time(lambda: sdf.computeDistances(n, qd, outSgnDist=d64, fp_mode="fp64"),
-                        max(1, args.steps // 2))
-        rel = (d32 - d64).abs() / d64.abs().clamp_min(1e-300)
-        return {"workload": "torus R=1, r=0.3+0.02 sin(7u) cos(5v), 2,097,152 tris; 16M queries "
-                            "(half box, half |d|<0.01); SDF forward",
-                "fp32_traversal_queries_per_s": n / t32, "fp64_queries_per_s": n / t64,
-                "max_rel_err": float(rel.max()), "rms_rel_err": float(torch.sqrt((rel * rel).mean())),
-                "sign_mismatches": int((torch.signbit(d32) != torch.signbit(d64)).sum())}
+        return sdf_config(torch, dev, L, _lib, v, t, q, None, args.steps, False, cpu, 100_000,
+                          LABEL3)
     if args.workload == "diff_l7":
         v, t, q, g = synth.workload4()
-        vd = torch.as_tensor(v, device=dev).requires_grad_(True)
-        qd = torch.as_tensor(q, device=dev).requires_grad_(True)
-        td = torch.as_tensor(t, device=dev)
-        gd = torch.as_tensor(g, device=dev, dtype=torch.float64)
-        out = {"workload": "icosphere L7 (327,680 tris), radius 1+0.1 sin(4 theta) cos(3 phi); 1M "
-                           "queries at normal offsets U[-0.1,0.1]; SDF forward + adjoint"}
-        for mode in ("fp64", "fp32"):
-            sdf = quest.SignedDistance(vd.detach(), td, fp_mode="fp32")
-
-            def fb():
-                qd.grad = None
-                vd.grad = None
-                autograd.signed_distance(qd, vd, sdf, mode).backward(gd)
-            tt = sync_time(fb, args.steps)
-            out[f"{mode}_fwd_bwd_queries_per_s"] = q.shape[0] / tt
-        return out
+        return sdf_config(torch, dev, L, _lib, v, t, q, g, args.steps, True, cpu, 100_000, LABEL4)
+    if args.workload == "shapefit_l8":
+        v, t, q, target = synth.workload5()
+        return shapefit_config(torch, dev, v, t, q, target, cpu, LABEL5)
     if args.workload == "self_isect_l8":
         return self_isect_workload(args, torch, dev, sync_time)
     if args.workload == "find_points_2d":
@@ -163,7 +377,7 @@ def other_workload(args, torch, dev):
                "query_cells_per_s": qh.shape[0] / tq}
         if not args.no_cpu:
             from oracle import oracle as orc
-            nthreads = min(16, os.cpu_count() or 1)
+            nthreads = cpu
             m = 20_000
             t0 = time.perf_counter()
             want = orc.mesh_candidates(iv, ih, qv, qh[:m], nthreads=nthreads)
@@ -192,7 +406,7 @@ def other_workload(args, torch, dev):
         if not args.no_cpu:
             from oracle import oracle as orc
             m = min(n, 200_000)
-            nthreads = min(16, os.cpu_count() or 1)
+            nthreads = cpu
             o = orc.OracleSDF(v, conn, cell_offsets=off)
             t0 = time.perf_counter()
             r = o.eval(q[:m], nthreads=nthreads)
@@ -202,17 +416,7 @@ def other_workload(args, torch, dev):
                                    "matches_gpu_bitwise": bool(np.array_equal(
                                        r["dist"], d[:m].cpu().numpy()))}
         return out
-    v, t, q, target = synth.workload5()
-    vd, td, qd, tg = (torch.as_tensor(a, device=dev) for a in (v, t, q, target))
-    _, hist = shapefit.shape_fit(vd, td, qd, tg, iterations=20, lr=1e-3, fp_mode="fp32")
-    return {"workload": "icosphere L8 (1,310,720 tris) fitted to a rounded box (b=(0.8,0.6,0.5), "
-                        "r=0.1); 8M queries U[-1.2,1.2]^3; 20 iterations of rebuild + FP32-traversal "
-                        "forward + loss ||d - target||^2 + adjoint + vertex step of length lr=1e-3",
-            "iterations": hist,
-            "rms_residual_first_last": [hist[0]["rms_residual"], hist[-1]["rms_residual"]],
-            "mean_build_ms": sum(h["build_ms"] for h in hist[1:]) / 19,
-            "mean_forward_ms": sum(h["forward_ms"] for h in hist) / 20,
-            "mean_backward_ms": sum(h["backward_ms"] for h in hist) / 20}
+    raise ValueError(f"unknown workload {args.workload}")
 
 
 def self_isect_workload(args, torch, dev, sync_time):
@@ -250,7 +454,7 @@ def self_isect_workload(args, torch, dev, sync_time):
                                                 "mesh_isect_emit")}}
     if not args.no_cpu:
         from oracle import oracle as orc
-        nthreads = min(16, os.cpu_count() or 1)
+        nthreads = host_cpus()[0]
         t0 = time.perf_counter()
         want_p, want_d = orc.mesh_self_intersections(v, t, 1e-8, nthreads=nthreads)
         tc = time.perf_counter() - t0
@@ -285,6 +489,9 @@ def main():
     if args.workload != "sdf_l6":
         print(json.dumps(other_workload(args, torch, dev)), flush=True)
         return 0
+    cpu, host = host_cpus()
+    if args.no_cpu:
+        cpu = 0
 
     log("generating inputs")
     v, t, q = synth.workload2(nq=args.queries)
@@ -348,6 +555,7 @@ def main():
                                 "sdf_query_order": ms_ord / max(n_ord, 1),
                                 "sdf_backward": ms_bwd / max(n_bwd, 1)},
         "second_pass_walks": sdf.lastSuspended(),
+        "host": host,
     }
 
     # roofline of the dominant kernel: visits / leaf checks of the traversal
@@ -382,63 +590,40 @@ def main():
             "rms_rel_err_vs_fp64": float(torch.sqrt((rel * rel).mean())),
             "sign_mismatches": int((torch.signbit(d32) != torch.signbit(d64)).sum())}
 
-        log("timing BVH builds and findPoints")
+        log("timing BVH build over the 81,920 triangles' boxes")
         tb = torch.empty((t.shape[0], 6), dtype=torch.float64, device=dev)
         a, b, c = vd[td[:, 0].long()], vd[td[:, 1].long()], vd[td[:, 2].long()]
         tb[:, :3] = torch.minimum(torch.minimum(a, b), c)
         tb[:, 3:] = torch.maximum(torch.maximum(a, b), c)
+        bvh = spin.BVH(tb)
+        el = sync_timer(torch)(lambda: bvh.initialize(tb), 20)
+        out["bvh_build"] = {"icosphere6_tris": {"prims": int(tb.shape[0]), "ms": el * 1e3,
+                                                "mprims_per_s": tb.shape[0] / el / 1e6}}
+        log("timing BVH build and findPoints (100,000 boxes, 1,000,000 points)")
         boxes1, pts1 = synth.workload1()
-        bx1 = torch.as_tensor(boxes1, device=dev)
-        builds = {}
-        for name, bx in (("icosphere6_tris", tb), ("boxes_100k", bx1)):
-            bvh = spin.BVH(bx)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            reps = 20
-            for _ in range(reps):
-                bvh.initialize(bx)
-            torch.cuda.synchronize()
-            el = (time.perf_counter() - t0) / reps
-            builds[name] = {"prims": int(bx.shape[0]), "ms": el * 1e3,
-                            "mprims_per_s": bx.shape[0] / el / 1e6}
-        out["bvh_build"] = builds
-        bvh = spin.BVH(bx1)
-        p1 = torch.as_tensor(pts1, device=dev)
-        off = torch.empty(p1.shape[0], dtype=torch.int32, device=dev)
-        cnt = torch.empty(p1.shape[0], dtype=torch.int32, device=dev)
-        bvh.findPoints(off, cnt, p1.shape[0], p1)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(10):
-            cand = bvh.findPoints(off, cnt, p1.shape[0], p1)
-        torch.cuda.synchronize()
-        el = (time.perf_counter() - t0) / 10
-        out["find_points"] = {"queries": int(p1.shape[0]), "boxes": int(bx1.shape[0]),
-                              "ms": el * 1e3, "queries_per_s": p1.shape[0] / el,
-                              "candidates": int(cand.numel())}
-        if not args.no_cpu:
-            # the oracle's build, one thread (the reference's SEQ build), and
-            # its findPoints over all host threads, on the same boxes and points
-            from oracle import oracle as orc
-            ncpu = min(os.cpu_count() or 1, 16)
-            t0 = time.perf_counter()
-            ob = orc.OracleBVH(boxes1)
-            el_b = time.perf_counter() - t0
-            ex = bvh.export()
-            same_tree = bool(np.array_equal(ob.export()["nodes"], ex["nodes"].cpu().numpy()))
-            t0 = time.perf_counter()
-            ooff, ocnt, ocand = ob.find("points", pts1, nthreads=ncpu)
-            el_f = time.perf_counter() - t0
-            same_find = bool(np.array_equal(ocnt, cnt.cpu().numpy()) and
-                             np.array_equal(ocand, cand.cpu().numpy()))
-            out["bvh_build"]["boxes_100k"]["cpu_baseline"] = {
-                "ms": el_b * 1e3, "mprims_per_s": boxes1.shape[0] / el_b / 1e6, "cores": 1,
-                "kind": "port", "sample": "the same 100,000 boxes",
-                "matches_gpu_bitwise": same_tree}
-            out["find_points"]["cpu_baseline"] = {
-                "ms": el_f * 1e3, "queries_per_s": pts1.shape[0] / el_f, "cores": ncpu,
-                "kind": "port", "sample": "the same 1,000,000 points",
-                "matches_gpu_bitwise": same_find}
+        fp = find_points_config(torch, dev, L, _lib, boxes1, pts1, cpu)
+        out["bvh_build"]["boxes_100k"] = {"prims": int(boxes1.shape[0]), "ms": fp["build_ms"],
+                                          "mprims_per_s": fp["build_mprims_per_s"]}
+        out["find_points"] = fp
+        # the remaining configurations at their full sizes
+        cfg = {}
+        log("torus: 2,097,152 triangles, 16M queries")
+        v3, t3, q3 = synth.workload3()
+        cfg["torus_mixed_2M_tris_16M_queries"] = sdf_config(
+            torch, dev, L, _lib, v3, t3, q3, None, 3, False, cpu, 100_000, LABEL3)
+        del v3, t3, q3
+        log("icosphere L7: 327,680 triangles, 1M queries, forward + adjoint")
+        v4, t4, q4, g4 = synth.workload4()
+        cfg["differentiable_l7_1M_queries"] = sdf_config(
+            torch, dev, L, _lib, v4, t4, q4, g4, 5, True, cpu, 100_000, LABEL4)
+        del v4, t4, q4, g4
+        log("shape fit: icosphere L8, 8M queries, 20 iterations")
+        v5, t5, q5, tg5 = synth.workload5()
+        cfg["shape_fit_l8_8M_queries_20_iterations"] = shapefit_config(
+            torch, dev, v5, t5, q5, tg5, cpu, LABEL5)
+        del v5, t5, q5, tg5
+        out["configs"] = cfg
+        torch.cuda.empty_cache()
 
     if args.calib_copy:
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
@@ -450,7 +635,7 @@ def main():
 
     if not args.no_cpu:
         from oracle import oracle as orc
-        ncpu = min(os.cpu_count() or 1, 16)
+        ncpu = cpu
         ns = min(args.cpu_sample, nq)
         log(f"CPU baseline: oracle on {ns} queries, {ncpu} threads")
         o = orc.OracleSDF(v, t)

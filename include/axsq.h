@@ -297,6 +297,14 @@ int axsq_prof_enable(int on);
 int axsq_prof_reset(void);
 int axsq_prof_get(const char* name, int64_t* launches, double* total_ms);
 
+/* Traversal statistics of the candidate search when enabled (no reference
+ * counterpart): the inner-node visits of the last count pass (axsq_bvh_find,
+ * axsq_bvh_find_into, axsq_bvh_find_count) summed over its queries. Every
+ * walk follows the reference's order, so this is the reference traversal's
+ * count. */
+int axsq_bvh_stats_enable(axsq_bvh* bvh, int on);
+int axsq_bvh_stats_get(const axsq_bvh* bvh, int64_t* visits);
+
 /* Per-query traversal statistics of the last FP64 axsq_sdf_eval when
  * enabled: summed inner-node visits and leaf (triangle) checks. */
 int axsq_sdf_stats_enable(axsq_sdf* sdf, int on);

@@ -862,6 +862,25 @@ int32_t orc_bvh_max_depth(const orc_bvh* t, int kind, const double* prims, int64
   return deepest;
 }
 
+/* Inner-node visits of the walk (bvh_traverse.hpp:66-154), summed over the
+ * queries; -1 after a stack overflow. */
+int64_t orc_bvh_visits(const orc_bvh* t, int kind, const double* prims, int64_t nq, double tol)
+{
+  pred_fn B = (kind == 0) ? pred_point : (kind == 1) ? pred_box : pred_ray;
+  find_ctx fc = {tol, t->dims};
+  int64_t sum = 0;
+  for(int64_t i = 0; i < nq; ++i)
+  {
+    collect_ctx cc = {0, NULL};
+    double q[6];
+    load_query(prims, kind, t->dims, i, q);
+    const int32_t v = traverse(t, q, B, &fc, leaf_count, &cc, no_pref, NULL);
+    if(v < 0) return -1;
+    sum += v;
+  }
+  return sum;
+}
+
 int64_t orc_bvh_brute(const orc_bvh* t, int kind, const double* prims, int64_t nq, double tol,
                       int32_t* counts, int32_t* cands, int64_t cap)
 {

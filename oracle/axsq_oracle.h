@@ -87,6 +87,8 @@ int64_t orc_bvh_find(const orc_bvh* t, int kind, const double* prims, int64_t nq
  * queries' walks pushes; -1 when a walk overflows the 64-entry stack. */
 int32_t orc_bvh_max_depth(const orc_bvh* t, int kind, const double* prims, int64_t nq,
                           double tol);
+/* Inner-node visits of the nq queries' walks, summed; -1 on an overflow. */
+int64_t orc_bvh_visits(const orc_bvh* t, int kind, const double* prims, int64_t nq, double tol);
 /* O(N*Q) brute force over the scaled leaf boxes, ascending box id. */
 int64_t orc_bvh_brute(const orc_bvh* t, int kind, const double* prims, int64_t nq,
                       double tol, int32_t* counts, int32_t* cands, int64_t cap);

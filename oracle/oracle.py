@@ -51,6 +51,9 @@ def lib():
         L.orc_bvh_max_depth.restype = ctypes.c_int32
         L.orc_bvh_max_depth.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64,
                                         ctypes.c_double]
+        L.orc_bvh_visits.restype = ctypes.c_int64
+        L.orc_bvh_visits.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64,
+                                     ctypes.c_double]
         L.orc_sdf_max_depth.restype = ctypes.c_int32
         L.orc_sdf_max_depth.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
         L.orc_bvh_brute.restype = ctypes.c_int64
@@ -251,6 +254,15 @@ class OracleBVH:
         k = KIND[kind]
         pr = f64(prims).reshape(-1, self._stride(k))
         return int(lib().orc_bvh_max_depth(self.h, k, _p(pr, _dp), pr.shape[0], tol))
+
+    def visits(self, kind, prims, tol=np.finfo(np.float64).eps):
+        """Inner-node visits of the queries' walks, summed."""
+        k = KIND[kind]
+        pr = f64(prims).reshape(-1, self._stride(k))
+        v = int(lib().orc_bvh_visits(self.h, k, _p(pr, _dp), pr.shape[0], tol))
+        if v < 0:
+            raise RuntimeError("oracle traversal stack overflow")
+        return v
 
     def brute(self, kind, prims, tol=np.finfo(np.float64).eps):
         k = KIND[kind]

@@ -370,3 +370,31 @@ def test_inverted_leaf_box_is_never_reported():
         for mode in ("fp64", "fp32"):
             _, cnt, cand = _find(spin.BVH(_t(boxes), fp_mode=mode), kind, q, 1)
             assert np.array_equal(cnt, ocnt) and np.array_equal(cand, ocand), (kind, mode)
+
+
+@pytest.mark.parametrize("kind", ["points", "boxes", "rays"])
+def test_count_pass_visits_match_oracle(kind):
+    """The count pass's visit counter (enableStats) equals the oracle walk's
+    inner-node visits summed over the queries, for the FP64 tree."""
+    boxes = _boxes(40_000, 21, 0.001, 0.03)
+    g = spin.BVH(_t(boxes))
+    g.enableStats(True)
+    rng = np.random.default_rng(22)
+    if kind == "points":
+        q = rng.random((100_000, 3))
+        prims = _t(q)
+    elif kind == "boxes":
+        c = rng.random((50_000, 3))
+        h = rng.uniform(0.001, 0.02, (50_000, 3))
+        q = np.concatenate([c - h, c + h], 1)
+        prims = _t(q)
+    else:
+        q = np.concatenate([rng.random((5_000, 3)), rng.normal(size=(5_000, 3))], 1)
+        prims = spin.make_rays(_t(q[:, :3]), _t(q[:, 3:]))
+        q = prims.cpu().numpy()
+    n = q.shape[0]
+    off = torch.empty(n, dtype=torch.int32, device=DEV)
+    cnt = torch.empty_like(off)
+    find = {"points": g.findPoints, "boxes": g.findBoundingBoxes, "rays": g.findRays}[kind]
+    find(off, cnt, n, prims)
+    assert g.lastStats() == orc.OracleBVH(boxes, 1.000123).visits(kind, q)

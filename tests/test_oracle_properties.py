@@ -281,3 +281,29 @@ def test_adjoint_sum_rule():
     g = synth.normal_f32(94, 9, q.shape[0])
     gq, gv = sdf.backward(q, out["dist"], out["elem"], g)
     np.testing.assert_allclose(gv.sum(0), -gq.sum(0), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["points", "boxes"])
+def test_bvh_visits_closed_form(kind):
+    """The walk visits the root and every inner node whose box, as its parent
+    stores it, passes the predicate (no pruning beyond the predicate): a
+    count from the exported tree alone, independent of the walk's order."""
+    boxes = _boxes(3000, 91)
+    o = orc.OracleBVH(boxes)
+    ex = o.export()
+    nodes, children = ex["nodes"], ex["children"]
+    rng = np.random.default_rng(92)
+    if kind == "points":
+        q = rng.random((400, 3))
+        lo = hi = q
+    else:
+        c = rng.random((400, 3))
+        h = rng.uniform(0.001, 0.03, (400, 3))
+        q = np.concatenate([c - h, c + h], 1)
+        lo, hi = q[:, :3], q[:, 3:]
+    inner = children >= 0  # slots holding an inner child
+    bl, bh = nodes[inner, :3], nodes[inner, 3:]
+    valid = np.all(bl <= bh, axis=1)
+    hit = np.all((hi[:, None, :] >= bl[None]) & (lo[:, None, :] <= bh[None]), axis=2) & valid
+    want = q.shape[0] + int(hit.sum())
+    assert o.visits(kind, q) == want
