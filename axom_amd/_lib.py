@@ -56,6 +56,8 @@ _SIGS = {
                                   _vp]),
     "axsq_bvh_find_into": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp, _vp,
                                   _i64, ctypes.POINTER(_i64), _vp]),
+    "axsq_bvh_find_into_noalloc": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp,
+                                          _vp, _i64, ctypes.POINTER(_i64), _vp]),
     "axsq_bvh_set_allocator": (_i32, [_vp, _i32]),
     "axsq_bvh_stats_enable": (_i32, [_vp, _i32]),
     "axsq_bvh_stats_get": (_i32, [_vp, ctypes.POINTER(_i64)]),

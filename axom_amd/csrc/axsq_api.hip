@@ -562,6 +562,110 @@ int axsq_bvh_find_into(const axsq_bvh* b, int kind, const double* x, const doubl
                         kstash, s);
 }
 
+}  // extern "C"
+
+// ---- the search into caller storage without allocation ----------------------
+// axsq_bvh_find_into_noalloc keeps its scratch (the count pass's kept ids,
+// their counts, the list of queries with more, the scan's tile sums and the
+// totals) in this array, one copy per GPU, held until the process ends.
+// Layout, for nq queries: [0, 16) total (int64) and listed-query count
+// (uint32); [kNoAllocHeader, +8 * scan_tiles(nq)) tile sums; then nq kept
+// counts, nq listed queries and kstash * nq kept ids (int32).
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr size_t kNoAllocHeader = 4096;
+constexpr int64_t kNoAllocMaxQueries = int64_t(1) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static_assert(kNoAllocHeader >= 16 + 8 * ((kNoAllocMaxQueries + kTile - 1) / kTile),
+              "tile sums of the largest search fit the header");
+static_assert(kNoAllocHeader + 12 * size_t(kNoAllocMaxQueries) <= kNoAllocScratchBytes,
+              "the largest search keeps at least one id per query");
+
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kNoAllocMaxDevices] = {};
+
+// The device address of this device's copy of `scratch`, looked up on the
+// device's first call.
+int scratch_on_current_device(unsigned char** out) {
+  int dev = 0;
+  AXSQ_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_error("axsq_bvh_find_into_noalloc: device %d beyond the %d it keeps scratch for", dev,
+              kNoAllocMaxDevices);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("axsq_bvh_find_into_noalloc: hipGetSymbolAddress(scratch) failed: %s",
+                hipGetErrorString(e));
+      return AXSQ_ERR_HIP;
+    }
+    g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return AXSQ_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int axsq_bvh_find_into_noalloc(const axsq_bvh* b, int kind, const double* x, const double* y,
+                               const double* z, int64_t stride, int64_t nq, double tol,
+                               int32_t* offsets, int32_t* counts, int32_t* candidates,
+                               int64_t capacity, int64_t* total, void* stream) {
+  clear_error();
+  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq));
+  AXSQ_TRY(check_find_outputs("axsq_bvh_find_into_noalloc", nq, offsets, counts, total));
+  *total = 0;
+  if (nq > kNoAllocMaxQueries) {
+    set_error("axsq_bvh_find_into_noalloc: %lld queries exceed its maximum of %lld (use "
+              "axsq_bvh_find_into, or split the queries)",
+              (long long)nq, (long long)kNoAllocMaxQueries);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (capacity < 0 || (capacity > 0 && !candidates)) {
+    set_error("axsq_bvh_find_into_noalloc: negative capacity or NULL candidates");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (nq == 0) return AXSQ_OK;
+  unsigned char* base = nullptr;
+  AXSQ_TRY(scratch_on_current_device(&base));
+  hipStream_t s = (hipStream_t)stream;
+  int64_t* tot_dev = reinterpret_cast<int64_t*>(base);
+  uint32_t* nover_dev = reinterpret_cast<uint32_t*>(base + 8);
+  int64_t* sums = reinterpret_cast<int64_t*>(base + 16);
+  int32_t* kept_cnt = reinterpret_cast<int32_t*>(base + kNoAllocHeader);
+  int32_t* over = kept_cnt + nq;
+  int32_t* kept = over + nq;
+  const size_t room = kNoAllocScratchBytes - kNoAllocHeader - 8 * (size_t)nq;
+  const int kstash = (int)std::min<size_t>(64, room / (4 * (size_t)nq));  // >= 1
+  // count pass (keeping the first kstash ids), scan, then the fill from the
+  // kept ids and a search again for the listed queries
+  AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(nover_dev), 0, 1, s));
+  AXSQ_TRY(find_launch(b, kind, 0, x, y, z, stride, nq, tol, counts, nullptr, nullptr, nullptr,
+                       kept, kstash, kept_cnt, over, nover_dev, s));
+  AXSQ_TRY(scan_exclusive_i32_into(counts, offsets, nq, sums, tot_dev, s));
+  int64_t tn[2] = {0, 0};
+  AXSQ_CHECK_HIP(hipMemcpyAsync(tn, tot_dev, sizeof(tn), hipMemcpyDeviceToHost, s));
+  AXSQ_TRY(finish(b->status, s));
+  if (tn[0] > 0x7fffffffll) {
+    set_error("candidate total %lld exceeds the int32 index range", (long long)tn[0]);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  *total = tn[0];
+  if (*total == 0 || *total > capacity) return AXSQ_OK;  // too small: nothing written
+  AXSQ_TRY(stash_copy_launch(kept, kstash, kept_cnt, offsets, nq, candidates, s));
+  const int64_t nover = (int64_t)(uint32_t)tn[1];
+  if (nover > 0)
+    AXSQ_TRY(find_launch(b, kind, 2, x, y, z, stride, nover, tol, nullptr, counts, offsets,
+                         candidates, nullptr, 0, nullptr, over, nullptr, s));
+  return finish(b->status, s);
+}
 
 int axsq_bvh_find(const axsq_bvh* b, int kind, const double* x, const double* y, const double* z,
                   int64_t stride, int64_t nq, double tol, int32_t* offsets, int32_t* counts,

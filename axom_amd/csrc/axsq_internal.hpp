@@ -105,6 +105,13 @@ int fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s);
 // Exclusive scan of n int32 counts into out; *total_dev (int64) receives the sum.
 int scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total_dev,
                        Workspace& ws, hipStream_t s);
+// The same scan with the caller's buffer of scan_tiles(n) int64 tile sums.
+inline int64_t scan_tiles(int64_t n) {
+  const int64_t t = (n + kTile - 1) / kTile;
+  return t < 1 ? 1 : t;
+}
+int scan_exclusive_i32_into(const int32_t* in, int32_t* out, int64_t n, int64_t* sums,
+                            int64_t* total_dev, hipStream_t s);
 // Stable sort of at most kSmallSortMax (key, val) pairs by the low `bits`
 // (<= 32) bits, for a caller that builds the first pass's histogram itself:
 // hists holds one tile-major histogram per 8-bit pass (256 counts per tile of

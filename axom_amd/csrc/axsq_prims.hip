@@ -310,9 +310,13 @@ int fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t s) {
 
 int scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total_dev,
                        Workspace& ws, hipStream_t s) {
+  AXSQ_TRY(ws.tiles.ensure(sizeof(int64_t) * (size_t)scan_tiles(n)));
+  return scan_exclusive_i32_into(in, out, n, ws.tiles.as<int64_t>(), total_dev, s);
+}
+
+int scan_exclusive_i32_into(const int32_t* in, int32_t* out, int64_t n, int64_t* sums,
+                            int64_t* total_dev, hipStream_t s) {
   const int64_t ntiles = (n + kTile - 1) / kTile;
-  AXSQ_TRY(ws.tiles.ensure(sizeof(int64_t) * (size_t)(ntiles < 1 ? 1 : ntiles)));
-  int64_t* sums = ws.tiles.as<int64_t>();
   if (n > 0) {
     k_tile_sums<int32_t><<<(unsigned)ntiles, kBlock, 0, s>>>(in, n, sums);
     AXSQ_CHECK_HIP(hipGetLastError());

@@ -194,6 +194,22 @@ int axsq_bvh_find_into(const axsq_bvh* bvh, int kind, const double* x, const dou
                        int32_t* counts, int32_t* candidates, int64_t capacity, int64_t* total,
                        void* stream);
 
+/* axsq_bvh_find_into without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, results and errors as axsq_bvh_find_into, for at most 1,048,576
+ * queries (more: AXSQ_ERR_INVALID_ARG, nothing written). Its scratch (kept
+ * ids, counts, the scan's sums and totals) is a fixed 32 MiB array in device
+ * memory, one per GPU, held until the process ends; it keeps up to 64 ids per
+ * query from the count pass (fewer above 127,084 queries, at least 5).
+ * Every call on one GPU uses that GPU's one array: calls that overlap on the
+ * same GPU (other threads or streams) race and must be serialised by the
+ * caller. */
+int axsq_bvh_find_into_noalloc(const axsq_bvh* bvh, int kind, const double* x, const double* y,
+                               const double* z, int64_t stride, int64_t nq, double tol,
+                               int32_t* offsets, int32_t* counts, int32_t* candidates,
+                               int64_t capacity, int64_t* total, void* stream);
+
 /* One-call form matching BVH::find* semantics: counts and offsets as above;
  * the library allocates *candidates (*total entries, in the memory set by
  * axsq_bvh_set_allocator, device memory by default; release with axsq_free),
