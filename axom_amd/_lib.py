@@ -27,6 +27,7 @@ _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int
 _dbl = ctypes.c_double
+_flt = ctypes.c_float
 _dbl3 = ctypes.c_double * 3
 
 
@@ -58,6 +59,17 @@ _SIGS = {
                                   _i64, ctypes.POINTER(_i64), _vp]),
     "axsq_bvh_find_into_noalloc": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _dbl, _vp, _vp,
                                           _vp, _i64, ctypes.POINTER(_i64), _vp]),
+    "axsq_bvh_build_f32": (_i32, [ctypes.POINTER(_vp), _vp, _i64, _i32, _flt, _vp]),
+    "axsq_bvh_rebuild_f32": (_i32, [_vp, _vp, _i64, _flt, _vp]),
+    "axsq_bvh_precision": (_i32, [_vp]),
+    "axsq_bvh_find_count_f32": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _flt, _vp, _vp,
+                                       ctypes.POINTER(_i64), _vp]),
+    "axsq_bvh_find_fill_f32": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _flt, _vp, _vp, _vp,
+                                      _vp]),
+    "axsq_bvh_find_into_f32": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _flt, _vp, _vp, _vp,
+                                      _i64, ctypes.POINTER(_i64), _vp]),
+    "axsq_bvh_find_f32": (_i32, [_vp, _i32, _vp, _vp, _vp, _i64, _i64, _flt, _vp, _vp,
+                                 ctypes.POINTER(_vp), ctypes.POINTER(_i64), _vp]),
     "axsq_bvh_set_allocator": (_i32, [_vp, _i32]),
     "axsq_bvh_stats_enable": (_i32, [_vp, _i32]),
     "axsq_bvh_stats_get": (_i32, [_vp, ctypes.POINTER(_i64)]),

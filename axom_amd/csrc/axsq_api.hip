@@ -190,10 +190,18 @@ static bool valid_kind(int k) {
   return k == AXSQ_QUERY_POINTS || k == AXSQ_QUERY_BOXES || k == AXSQ_QUERY_RAYS;
 }
 
-static int check_find_args(const axsq_bvh* b, int kind, const double* x, const double* y,
-                           const double* z, int64_t stride, int64_t nq) {
+// f32q: the queries are floats (the _f32 entry points), which a float tree
+// takes and a double tree refuses, and the other way round.
+static int check_find_args(const axsq_bvh* b, int kind, const void* x, const void* y,
+                           const void* z, int64_t stride, int64_t nq, bool f32q) {
   if (!b || !b->nodes) {
     set_error("BVH handle is NULL or not built");
+    return AXSQ_ERR_STATE;
+  }
+  if (b->f32 != f32q) {
+    set_error(b->f32 ? "a float tree takes float queries (the axsq_bvh_find*_f32 entry points)"
+                     : "a double tree takes double queries (the axsq_bvh_find* entry points "
+                       "without _f32)");
     return AXSQ_ERR_STATE;
   }
   if (!valid_kind(kind)) {
@@ -325,6 +333,10 @@ int axsq_bvh_rebuild(axsq_bvh* b, const double* boxes, int64_t n, double scale, 
               b->device, dev);
     return AXSQ_ERR_INVALID_ARG;
   }
+  if (b->f32) {
+    set_error("axsq_bvh_rebuild: a float tree is rebuilt with axsq_bvh_rebuild_f32");
+    return AXSQ_ERR_STATE;
+  }
   hipStream_t s = (hipStream_t)stream;
   if (b->dims == 2) {
     Workspace& ws = workspace();
@@ -334,6 +346,55 @@ int axsq_bvh_rebuild(axsq_bvh* b, const double* boxes, int64_t n, double scale, 
   }
   return bvh_build_into(b, boxes, n, scale, fp_mode, s, 3);
 }
+
+int axsq_bvh_build_f32(axsq_bvh** out, const float* boxes, int64_t n, int ndims, float scale,
+                       void* stream) {
+  clear_error();
+  if (!out) {
+    set_error("axsq_bvh_build_f32: out is NULL");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  if (ndims != 2 && ndims != 3) {
+    set_error("axsq_bvh_build_f32: ndims must be 2 or 3, not %d", ndims);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  axsq_bvh* b = new axsq_bvh();
+  hipGetDevice(&b->device);
+  const int rc = bvh_build_into(b, nullptr, n, (double)scale, AXSQ_FP32_TRAVERSAL,
+                                (hipStream_t)stream, ndims, true, boxes);
+  if (rc != AXSQ_OK) {
+    bvh_release(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return AXSQ_OK;
+}
+
+int axsq_bvh_rebuild_f32(axsq_bvh* b, const float* boxes, int64_t n, float scale,
+                         void* stream) {
+  clear_error();
+  if (!b) {
+    set_error("axsq_bvh_rebuild_f32: bvh is NULL");
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  if (!b->f32) {
+    set_error("axsq_bvh_rebuild_f32: a double tree is rebuilt with axsq_bvh_rebuild");
+    return AXSQ_ERR_STATE;
+  }
+  int dev = -1;
+  AXSQ_CHECK_HIP(hipGetDevice(&dev));
+  if (dev != b->device) {
+    set_error("axsq_bvh_rebuild_f32: the tree lives on device %d, the current device is %d",
+              b->device, dev);
+    return AXSQ_ERR_INVALID_ARG;
+  }
+  return bvh_build_into(b, nullptr, n, (double)scale, AXSQ_FP32_TRAVERSAL, (hipStream_t)stream,
+                        b->dims, true, boxes);
+}
+
+int axsq_bvh_precision(const axsq_bvh* b) { return b ? (b->f32 ? 32 : 64) : 0; }
 
 int axsq_bvh_dimension(const axsq_bvh* b) { return b ? b->dims : 0; }
 
@@ -421,8 +482,8 @@ int axsq_bvh_export(const axsq_bvh* b, double* inner_nodes, int32_t* children,
 // The count pass of a candidate search: counts, offsets and *total. With
 // keep, also the first stash_slots(nq) ids of every query and the list of
 // queries with more, for the fill of the same call (axsq_bvh_find).
-static int find_count_impl(const axsq_bvh* b, int kind, const double* x, const double* y,
-                           const double* z, int64_t stride, int64_t nq, double tol,
+static int find_count_impl(const axsq_bvh* b, int kind, const void* x, const void* y,
+                           const void* z, int64_t stride, int64_t nq, double tol,
                            int32_t* offsets, int32_t* counts, int64_t* total, bool keep,
                            int64_t* nover_out, int* kstash_out, hipStream_t s) {
   *total = 0;
@@ -467,11 +528,14 @@ static int find_count_impl(const axsq_bvh* b, int kind, const double* x, const d
   return AXSQ_OK;
 }
 
-int axsq_bvh_find_count(const axsq_bvh* b, int kind, const double* x, const double* y,
-                        const double* z, int64_t stride, int64_t nq, double tol,
-                        int32_t* offsets, int32_t* counts, int64_t* total, void* stream) {
+}  // extern "C"
+
+static int find_count_any(const axsq_bvh* b, int kind, const void* x, const void* y,
+                          const void* z, int64_t stride, int64_t nq, double tol,
+                          int32_t* offsets, int32_t* counts, int64_t* total, void* stream,
+                          bool f32q) {
   clear_error();
-  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq));
+  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq, f32q));
   if (!total || (nq > 0 && (!offsets || !counts))) {
     set_error("axsq_bvh_find_count: NULL output");
     return AXSQ_ERR_INVALID_ARG;
@@ -480,12 +544,12 @@ int axsq_bvh_find_count(const axsq_bvh* b, int kind, const double* x, const doub
                          nullptr, nullptr, (hipStream_t)stream);
 }
 
-int axsq_bvh_find_fill(const axsq_bvh* b, int kind, const double* x, const double* y,
-                       const double* z, int64_t stride, int64_t nq, double tol,
-                       const int32_t* offsets, const int32_t* counts, int32_t* candidates,
-                       void* stream) {
+static int find_fill_any(const axsq_bvh* b, int kind, const void* x, const void* y,
+                         const void* z, int64_t stride, int64_t nq, double tol,
+                         const int32_t* offsets, const int32_t* counts, int32_t* candidates,
+                         void* stream, bool f32q) {
   clear_error();
-  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq));
+  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq, f32q));
   if (nq > 0 && (!offsets || !counts)) {
     set_error("axsq_bvh_find_fill: offsets or counts is NULL");
     return AXSQ_ERR_INVALID_ARG;
@@ -518,8 +582,8 @@ static int alloc_candidates(const axsq_bvh* b, int64_t total, int32_t** out) {
 
 // The fill of a one-call search: the ids its count pass kept, then a search
 // for the nover listed queries with more.
-static int find_fill_kept(const axsq_bvh* b, int kind, const double* x, const double* y,
-                          const double* z, int64_t stride, int64_t nq, double tol,
+static int find_fill_kept(const axsq_bvh* b, int kind, const void* x, const void* y,
+                          const void* z, int64_t stride, int64_t nq, double tol,
                           const int32_t* offsets, const int32_t* counts, int32_t* cands,
                           int64_t nover, int kstash, hipStream_t s) {
   Workspace& ws = workspace();
@@ -540,12 +604,12 @@ static int check_find_outputs(const char* name, int64_t nq, const int32_t* offse
   return AXSQ_OK;
 }
 
-int axsq_bvh_find_into(const axsq_bvh* b, int kind, const double* x, const double* y,
-                       const double* z, int64_t stride, int64_t nq, double tol, int32_t* offsets,
-                       int32_t* counts, int32_t* candidates, int64_t capacity, int64_t* total,
-                       void* stream) {
+static int find_into_any(const axsq_bvh* b, int kind, const void* x, const void* y,
+                         const void* z, int64_t stride, int64_t nq, double tol, int32_t* offsets,
+                         int32_t* counts, int32_t* candidates, int64_t capacity, int64_t* total,
+                         void* stream, bool f32q) {
   clear_error();
-  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq));
+  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq, f32q));
   AXSQ_TRY(check_find_outputs("axsq_bvh_find_into", nq, offsets, counts, total));
   *total = 0;
   if (capacity < 0 || (capacity > 0 && !candidates)) {
@@ -561,8 +625,6 @@ int axsq_bvh_find_into(const axsq_bvh* b, int kind, const double* x, const doubl
   return find_fill_kept(b, kind, x, y, z, stride, nq, tol, offsets, counts, candidates, nover,
                         kstash, s);
 }
-
-}  // extern "C"
 
 // ---- the search into caller storage without allocation ----------------------
 // axsq_bvh_find_into_noalloc keeps its scratch (the count pass's kept ids,
@@ -619,7 +681,7 @@ int axsq_bvh_find_into_noalloc(const axsq_bvh* b, int kind, const double* x, con
                                int32_t* offsets, int32_t* counts, int32_t* candidates,
                                int64_t capacity, int64_t* total, void* stream) {
   clear_error();
-  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq));
+  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq, false));
   AXSQ_TRY(check_find_outputs("axsq_bvh_find_into_noalloc", nq, offsets, counts, total));
   *total = 0;
   if (nq > kNoAllocMaxQueries) {
@@ -667,9 +729,12 @@ int axsq_bvh_find_into_noalloc(const axsq_bvh* b, int kind, const double* x, con
   return finish(b->status, s);
 }
 
-int axsq_bvh_find(const axsq_bvh* b, int kind, const double* x, const double* y, const double* z,
-                  int64_t stride, int64_t nq, double tol, int32_t* offsets, int32_t* counts,
-                  int32_t** candidates, int64_t* total, void* stream) {
+}  // extern "C"
+
+static int find_alloc_any(const axsq_bvh* b, int kind, const void* x, const void* y,
+                          const void* z, int64_t stride, int64_t nq, double tol, int32_t* offsets,
+                          int32_t* counts, int32_t** candidates, int64_t* total, void* stream,
+                          bool f32q) {
   clear_error();
   if (!candidates || !total) {
     set_error("axsq_bvh_find: NULL output");
@@ -677,7 +742,7 @@ int axsq_bvh_find(const axsq_bvh* b, int kind, const double* x, const double* y,
   }
   *candidates = nullptr;
   *total = 0;
-  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq));
+  AXSQ_TRY(check_find_args(b, kind, x, y, z, stride, nq, f32q));
   AXSQ_TRY(check_find_outputs("axsq_bvh_find", nq, offsets, counts, total));
   hipStream_t s = (hipStream_t)stream;
   int64_t nover = 0;
@@ -697,6 +762,59 @@ int axsq_bvh_find(const axsq_bvh* b, int kind, const double* x, const double* y,
   }
   *candidates = c;
   return AXSQ_OK;
+}
+
+extern "C" {
+
+int axsq_bvh_find_count(const axsq_bvh* b, int kind, const double* x, const double* y,
+                        const double* z, int64_t stride, int64_t nq, double tol,
+                        int32_t* offsets, int32_t* counts, int64_t* total, void* stream) {
+  return find_count_any(b, kind, x, y, z, stride, nq, tol, offsets, counts, total, stream, false);
+}
+int axsq_bvh_find_count_f32(const axsq_bvh* b, int kind, const float* x, const float* y,
+                            const float* z, int64_t stride, int64_t nq, float tol,
+                            int32_t* offsets, int32_t* counts, int64_t* total, void* stream) {
+  return find_count_any(b, kind, x, y, z, stride, nq, tol, offsets, counts, total, stream, true);
+}
+int axsq_bvh_find_fill(const axsq_bvh* b, int kind, const double* x, const double* y,
+                       const double* z, int64_t stride, int64_t nq, double tol,
+                       const int32_t* offsets, const int32_t* counts, int32_t* candidates,
+                       void* stream) {
+  return find_fill_any(b, kind, x, y, z, stride, nq, tol, offsets, counts, candidates, stream,
+                       false);
+}
+int axsq_bvh_find_fill_f32(const axsq_bvh* b, int kind, const float* x, const float* y,
+                           const float* z, int64_t stride, int64_t nq, float tol,
+                           const int32_t* offsets, const int32_t* counts, int32_t* candidates,
+                           void* stream) {
+  return find_fill_any(b, kind, x, y, z, stride, nq, tol, offsets, counts, candidates, stream,
+                       true);
+}
+int axsq_bvh_find_into(const axsq_bvh* b, int kind, const double* x, const double* y,
+                       const double* z, int64_t stride, int64_t nq, double tol, int32_t* offsets,
+                       int32_t* counts, int32_t* candidates, int64_t capacity, int64_t* total,
+                       void* stream) {
+  return find_into_any(b, kind, x, y, z, stride, nq, tol, offsets, counts, candidates, capacity,
+                       total, stream, false);
+}
+int axsq_bvh_find_into_f32(const axsq_bvh* b, int kind, const float* x, const float* y,
+                           const float* z, int64_t stride, int64_t nq, float tol,
+                           int32_t* offsets, int32_t* counts, int32_t* candidates,
+                           int64_t capacity, int64_t* total, void* stream) {
+  return find_into_any(b, kind, x, y, z, stride, nq, tol, offsets, counts, candidates, capacity,
+                       total, stream, true);
+}
+int axsq_bvh_find(const axsq_bvh* b, int kind, const double* x, const double* y, const double* z,
+                  int64_t stride, int64_t nq, double tol, int32_t* offsets, int32_t* counts,
+                  int32_t** candidates, int64_t* total, void* stream) {
+  return find_alloc_any(b, kind, x, y, z, stride, nq, tol, offsets, counts, candidates, total,
+                        stream, false);
+}
+int axsq_bvh_find_f32(const axsq_bvh* b, int kind, const float* x, const float* y,
+                      const float* z, int64_t stride, int64_t nq, float tol, int32_t* offsets,
+                      int32_t* counts, int32_t** candidates, int64_t* total, void* stream) {
+  return find_alloc_any(b, kind, x, y, z, stride, nq, tol, offsets, counts, candidates, total,
+                        stream, true);
 }
 
 // ---- SignedDistance -------------------------------------------------------

@@ -42,16 +42,20 @@ __device__ __forceinline__ void store_box(double* __restrict__ p, const double b
   q[2] = make_double2(b[4], b[5]);
 }
 
-__device__ __forceinline__ void invalid_box(double b[6]) {
-  b[0] = b[1] = b[2] = kDblMax;
-  b[3] = b[4] = b[5] = -kDblMax;
+__device__ __forceinline__ void invalid_box(double b[6], double big = kDblMax) {
+  b[0] = b[1] = b[2] = big;
+  b[3] = b[4] = b[5] = -big;
 }
 
 // Per block: scaled boxes out + min/max of the valid ones. A block with no
 // valid box reports an invalid partial.
 // The same launch clears zero[0, nzero) (the sort's histograms).
-__global__ __launch_bounds__(kBlock) void k_transform(const double* __restrict__ in, int64_t n_in,
-                                                       int64_t n, double scale,
+// F32: a float tree; `in` holds float boxes of 2 * in_dims floats (min, max),
+// widened to doubles (z = [0, 0] in 2-D), the padding box is
+// BoundingBox<float>'s invalid box and the scaling is in float.
+template <bool F32>
+__global__ __launch_bounds__(kBlock) void k_transform(const void* __restrict__ in_v, int in_dims,
+                                                       int64_t n_in, int64_t n, double scale,
                                                        double* __restrict__ out,
                                                        double* __restrict__ partials,
                                                        uint32_t* __restrict__ zero, int64_t nzero) {
@@ -62,11 +66,23 @@ __global__ __launch_bounds__(kBlock) void k_transform(const double* __restrict__
   invalid_box(acc);
   if (i < n) {
     double b[6];
-    if (n_in > 1 || (n_in == 1 && i == 0))
-      load_box(in + 6 * i, b);
+    if (n_in > 1 || (n_in == 1 && i == 0)) {
+      if constexpr (F32) {
+        const float* p = static_cast<const float*>(in_v) + 2 * in_dims * i;
+        for (int d = 0; d < 3; ++d) {
+          b[d] = d < in_dims ? (double)p[d] : 0.0;
+          b[3 + d] = d < in_dims ? (double)p[in_dims + d] : 0.0;
+        }
+      } else {
+        load_box(static_cast<const double*>(in_v) + 6 * i, b);
+      }
+    } else {
+      invalid_box(b, F32 ? (double)kFltMax : kDblMax);
+    }
+    if constexpr (F32)
+      box_scale_f32(b, (float)scale);
     else
-      invalid_box(b);
-    box_scale(b, scale);
+      box_scale(b, scale);
     store_box(out + 6 * i, b);
     if (box_valid(b))
       for (int k = 0; k < 6; ++k) acc[k] = b[k];
@@ -90,10 +106,13 @@ __global__ __launch_bounds__(kBlock) void k_transform(const double* __restrict__
 
 // Union of the valid scaled boxes; with none valid, the SEQ reduce
 // (addBox from an invalid box, build_radix_tree.hpp:134-142) leaves the last
-// box. Then inv_extent as in get_mcodes (:154-163).
+// box. Then inv_extent as in get_mcodes (:154-163), in float for a float
+// tree (f32: extent and 1 / extent rounded to float, the zero test's
+// threshold 1e-8 as a float).
 __global__ __launch_bounds__(1024) void k_bounds(const double* __restrict__ partials,
                                                   int64_t nparts, const double* __restrict__ boxes,
-                                                  int64_t n, BuildParams* __restrict__ prm) {
+                                                  int64_t n, BuildParams* __restrict__ prm,
+                                                  int f32) {
   __shared__ double s_red[6][1024];
   double acc[6];
   invalid_box(acc);
@@ -124,17 +143,23 @@ __global__ __launch_bounds__(1024) void k_bounds(const double* __restrict__ part
     if (!box_valid(b)) load_box(boxes + 6 * (n - 1), b);
     for (int k = 0; k < 6; ++k) prm->bounds[k] = b[k];
     for (int d = 0; d < 3; ++d) {
-      const double ext = b[3 + d] - b[d];
       prm->minc[d] = b[d];
-      prm->inv[d] = near_eq(ext, 0.0, 1.0e-8) ? 0.0 : 1.0 / ext;
+      if (f32) {
+        const float ext = (float)b[3 + d] - (float)b[d];
+        prm->inv[d] = (fabsf(ext - 0.0f) <= (float)1.0e-8) ? 0.0 : (double)(1.0f / ext);
+      } else {
+        const double ext = b[3 + d] - b[d];
+        prm->inv[d] = near_eq(ext, 0.0, 1.0e-8) ? 0.0 : 1.0 / ext;
+      }
     }
   }
 }
 
 // hist0 (if given): the sort's first-pass tile histogram (low 8 bits).
+// f32: a float tree, the normalised centroid computed in float.
 __global__ __launch_bounds__(kBlock) void k_mcodes(const double* __restrict__ boxes, int64_t n,
                                                     const BuildParams* __restrict__ prm, int dims,
-                                                    uint32_t* __restrict__ keys,
+                                                    int f32, uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ vals,
                                                     uint32_t* __restrict__ hist0) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -146,6 +171,13 @@ __global__ __launch_bounds__(kBlock) void k_mcodes(const double* __restrict__ bo
     c[d] = 0.5 * (b[d] + b[3 + d]);
     minc[d] = prm->minc[d];
     inv[d] = prm->inv[d];
+  }
+  if (f32) {  // (c - min) * inv in float, then codes of that point over [0, 1]
+    mcode_coords_f32(b, minc, inv, c);
+    for (int d = 0; d < 3; ++d) {
+      minc[d] = 0.0;
+      inv[d] = 1.0;
+    }
   }
   const uint32_t code = (dims == 2) ? morton2_of(c, minc, inv) : morton_of(c, minc, inv);
   keys[i] = code;
@@ -586,12 +618,12 @@ static int dmalloc(T** p, size_t count) {
 }
 
 int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale, int fp_mode,
-                   hipStream_t s, int dims) {
+                   hipStream_t s, int dims, bool f32, const float* boxes_f32) {
   if (n_in < 0 || n_in > 0x3fffffff) {
     set_error("axsq_bvh_build: box count %lld out of range", (long long)n_in);
     return AXSQ_ERR_INVALID_ARG;
   }
-  if (n_in > 0 && !boxes) {
+  if (n_in > 0 && (f32 ? !boxes_f32 : !boxes)) {
     set_error("axsq_bvh_build: boxes is NULL");
     return AXSQ_ERR_INVALID_ARG;
   }
@@ -617,6 +649,7 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
   b->fp_mode = fp_mode;
   b->scale = scale;
   b->dims = dims;
+  b->f32 = f32;
 
   // scratch: scaled boxes (6n doubles), keys/vals x2, tree arrays, partials
   const int64_t nparts = grid_for(n);
@@ -668,13 +701,17 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
     AXSQ_TRY(ws.hist.ensure(sizeof(uint32_t) * (size_t)nhist));
     hists = ws.hist.as<uint32_t>();
   }
-  k_transform<<<(unsigned)nparts, kBlock, 0, s>>>(boxes, n_in, n, scale, scaled, partials, hists,
-                                                  nhist);
+  if (f32)
+    k_transform<true><<<(unsigned)nparts, kBlock, 0, s>>>(boxes_f32, dims, n_in, n, scale, scaled,
+                                                          partials, hists, nhist);
+  else
+    k_transform<false><<<(unsigned)nparts, kBlock, 0, s>>>(boxes, 3, n_in, n, scale, scaled,
+                                                           partials, hists, nhist);
   AXSQ_CHECK_HIP(hipGetLastError());
-  k_bounds<<<1, 1024, 0, s>>>(partials, nparts, scaled, n, prm);
+  k_bounds<<<1, 1024, 0, s>>>(partials, nparts, scaled, n, prm, f32 ? 1 : 0);
   AXSQ_CHECK_HIP(hipGetLastError());
   uint32_t *keys = ws.k0.as<uint32_t>(), *vals = ws.v0.as<uint32_t>();
-  k_mcodes<<<grid_for(n), kBlock, 0, s>>>(scaled, n, prm, dims, keys, vals, hists);
+  k_mcodes<<<grid_for(n), kBlock, 0, s>>>(scaled, n, prm, dims, f32 ? 1 : 0, keys, vals, hists);
   AXSQ_CHECK_HIP(hipGetLastError());
   uint32_t *sk, *sv;
   if (small_sort)

@@ -425,4 +425,81 @@ __device__ __forceinline__ bool ray_box(const double r[6], const double b[6], do
   return hit;
 }
 
+// ---- FloatType = float (spin::BVH<NDIMS, Exec, float>) ---------------------
+// A float tree keeps its boxes as doubles holding float values (exact), so
+// unions, comparisons and the point / box predicates give the float results
+// unchanged; the steps below round to float where the reference's float
+// arithmetic does.
+constexpr float kFltMax = 3.40282347e38f;
+
+// BoundingBox<float>::scale (BoundingBox.hpp:547-561): the centroid
+// float(0.5 * (min + max)) with the sum in float, the radius
+// float(range * float(sf * 0.5)) (the product of two floats is exact in
+// double), then min/max in float and checkAndFixBounds.
+__device__ __forceinline__ void box_scale_f32(double b[6], float sf) {
+  if (!box_valid(b)) return;
+  const float s = (float)((double)sf * 0.5);
+  float lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    const float mn = (float)b[d], mx = (float)b[3 + d];
+    const float mid = (float)(0.5 * (double)(mn + mx));
+    const float r = (float)((double)(mx - mn) * (double)s);
+    lo[d] = mid - r;
+    hi[d] = mid + r;
+  }
+  for (int d = 0; d < 3; ++d) {
+    float l = lo[d], h = hi[d];
+    if (l > h) {
+      const float t = l;
+      l = h;
+      h = t;
+    }
+    b[d] = l;
+    b[3 + d] = h;
+  }
+}
+
+// get_mcodes for float (build_radix_tree.hpp:146-176): centroid as above,
+// (c - min) * inv in float, then morton32_encode (:48-65), whose scaling by
+// 2^(32/Dims) and clamps are exact.
+__device__ __forceinline__ void mcode_coords_f32(const double b[6], const double minc[3],
+                                                 const double inv[3], double u[3]) {
+  for (int d = 0; d < 3; ++d) {
+    const float c = (float)(0.5 * (double)((float)b[d] + (float)b[3 + d]));
+    u[d] = (double)((c - (float)minc[d]) * (float)inv[d]);
+  }
+}
+
+// intersect_ray_bbox_test / intersect_ray with T = float
+// (intersect_ray_impl.hpp:150-187,321-351): every step in float, tmin from
+// FLT_MIN, the axis skipped when |n| <= eps.
+__device__ __forceinline__ bool ray_slab_f32(float x0, float n, float bmin, float bmax,
+                                             float& tmin, float& tmax, float eps) {
+  if (fabsf(n) <= eps) return !((x0 < bmin) || (x0 > bmax));
+  const float invn = 1.0f / n;
+  float t1 = (bmin - x0) * invn;
+  float t2 = (bmax - x0) * invn;
+  if (t1 > t2) {
+    const float t = t1;
+    t1 = t2;
+    t2 = t;
+  }
+  tmin = (t1 < tmin) ? tmin : t1;
+  tmax = (t2 < tmax) ? t2 : tmax;
+  return !(tmin > tmax);
+}
+
+// r: the ray {origin, unit direction} and b the box, float values held in
+// doubles; dims 2 tests x and y only.
+__device__ __forceinline__ bool ray_box_f32(const double r[6], const double b[6], float eps,
+                                            int dims) {
+  float tmin = 1.17549435e-38f;  // FLT_MIN
+  float tmax = kFltMax;
+  bool hit = ray_slab_f32((float)r[0], (float)r[3], (float)b[0], (float)b[3], tmin, tmax, eps);
+  hit = hit && ray_slab_f32((float)r[1], (float)r[4], (float)b[1], (float)b[4], tmin, tmax, eps);
+  if (dims == 3)
+    hit = hit && ray_slab_f32((float)r[2], (float)r[5], (float)b[2], (float)b[5], tmin, tmax, eps);
+  return hit;
+}
+
 }  // namespace axsq

@@ -155,6 +155,7 @@ struct axsq_bvh {
   uint32_t* mcodes = nullptr;       // n sorted Morton codes
   int32_t* status = nullptr;        // device status word (kStatus* bits)
   int allocator = AXSQ_ALLOC_DEVICE;  // memory of the candidate arrays axsq_bvh_find returns
+  bool f32 = false;  // a float tree (FloatType = float): boxes are float values, queries float
   // traversal statistics of the candidate search's count pass
   bool stats = false;
   unsigned long long* find_visits = nullptr;  // device: inner-node visits summed over queries
@@ -193,9 +194,11 @@ struct axsq_sdf {
 
 namespace axsq {
 // boxes: AoS of 6 doubles (min xyz, max xyz); dims 2 builds the 2-D tree of
-// boxes whose z range is [0, 0].
+// boxes whose z range is [0, 0]. f32: a float tree over boxes_f32, AoS of
+// 2 * dims floats (boxes unused), with fp_mode AXSQ_FP32_TRAVERSAL.
 int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n, double scale, int fp_mode,
-                   hipStream_t s, int dims = 3);
+                   hipStream_t s, int dims = 3, bool f32 = false,
+                   const float* boxes_f32 = nullptr);
 void bvh_release(axsq_bvh* b);
 // n 2-D boxes (4 doubles) -> 3-D boxes with z range [0, 0] (6 doubles)
 int widen_boxes2d(const double* boxes2d, int64_t n, double* out, hipStream_t s);
@@ -203,9 +206,10 @@ int widen_boxes2d(const double* boxes2d, int64_t n, double* out, hipStream_t s);
 // first kstash ids per query when stash is given, listing the queries with
 // more in `over`); 1: fill every query; 2: fill the nq queries listed in
 // `over`. A fill writes at most fill_cnt[i] ids for query i and raises
-// kStatusFillMismatch when its walk finds another number.
-int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const double* y,
-                const double* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
+// kStatusFillMismatch when its walk finds another number. x, y, z point at
+// doubles, or at floats for a float tree (b->f32).
+int find_launch(const axsq_bvh* b, int kind, int mode, const void* x, const void* y,
+                const void* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
                 const int32_t* fill_cnt, const int32_t* offsets, int32_t* cands, int32_t* stash,
                 int kstash, int32_t* stash_cnt, int32_t* over, uint32_t* nover, hipStream_t s);
 // The fill for queries with at most kstash candidates, from the kept ids.

@@ -31,10 +31,11 @@ struct FindArgs {
   const Node32* nodes32;
   const double* leaf_boxes;
   const int32_t* leaf_ids;
-  const double* x;
-  const double* y;
-  const double* z;
+  const void* x;  // double queries, or float ones for a float tree (f32)
+  const void* y;
+  const void* z;
   int64_t stride;
+  int f32;  // a float tree: float queries, the ray test in float
   int64_t nq;
   int dims;  // 2: queries are 2-D (z = 0)
   int32_t ninner;
@@ -57,7 +58,8 @@ enum { kCountPass = 0, kFillAll = 1, kFillListed = 2 };
 // Predicates of BVH.hpp:499-501 (contains), :558-560 (intersectsWith),
 // :529-532 (intersect_ray with the BVH tolerance).
 template <int KIND>
-__device__ __forceinline__ bool pred64(const double* prim, double tol, const double b[6]) {
+__device__ __forceinline__ bool pred64(const double* prim, double tol, const double b[6],
+                                       int f32 = 0, int dims = 3) {
   if (KIND == AXSQ_QUERY_POINTS) {
     return !(prim[0] < b[0] || prim[0] > b[3] || prim[1] < b[1] || prim[1] > b[4] ||
              prim[2] < b[2] || prim[2] > b[5]);
@@ -65,7 +67,7 @@ __device__ __forceinline__ bool pred64(const double* prim, double tol, const dou
     return (prim[3] >= b[0] && prim[0] <= b[3]) && (prim[4] >= b[1] && prim[1] <= b[4]) &&
            (prim[5] >= b[2] && prim[2] <= b[5]);
   } else {
-    return ray_box(prim, b, tol);
+    return f32 ? ray_box_f32(prim, b, (float)tol, dims) : ray_box(prim, b, tol);
   }
 }
 
@@ -73,10 +75,10 @@ __device__ __forceinline__ bool pred64(const double* prim, double tol, const dou
 // (points: the point; boxes: min rounded down / max rounded up).
 template <int KIND>
 __device__ __forceinline__ bool pred32(const double* prim, const float* lo, const float* hi,
-                                       double tol, const float b[6]) {
+                                       double tol, const float b[6], int f32 = 0, int dims = 3) {
   if (KIND == AXSQ_QUERY_RAYS) {
     const double bd[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
-    return ray_box(prim, bd, tol);
+    return f32 ? ray_box_f32(prim, bd, (float)tol, dims) : ray_box(prim, bd, tol);
   } else {
     return (hi[0] >= b[0] && lo[0] <= b[3]) && (hi[1] >= b[1] && lo[1] <= b[4]) &&
            (hi[2] >= b[2] && lo[2] <= b[5]);
@@ -96,21 +98,25 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
     i = (MODE == kFillListed) ? (int64_t)a.over[j] : j;
     lds_i32* stk = lane_stack_base(s_stack);
     double prim[6];
+    // element k of query array p (float queries widen exactly)
+    auto ld = [&](const void* p, int64_t k) -> double {
+      return a.f32 ? (double)static_cast<const float*>(p)[k] : static_cast<const double*>(p)[k];
+    };
     if (KIND == AXSQ_QUERY_POINTS) {
-      prim[0] = a.x[i * a.stride];
-      prim[1] = a.y[i * a.stride];
-      prim[2] = (a.dims == 2) ? 0.0 : a.z[i * a.stride];
+      prim[0] = ld(a.x, i * a.stride);
+      prim[1] = ld(a.y, i * a.stride);
+      prim[2] = (a.dims == 2) ? 0.0 : ld(a.z, i * a.stride);
       prim[3] = prim[4] = prim[5] = 0.0;
     } else if (a.dims == 2) {  // box: min x y, max x y; ray: origin x y, direction x y
-      const double* p = a.x + i * a.stride;
-      prim[0] = p[0];
-      prim[1] = p[1];
+      const int64_t o = i * a.stride;
+      prim[0] = ld(a.x, o);
+      prim[1] = ld(a.x, o + 1);
       prim[2] = 0.0;
-      prim[3] = p[2];
-      prim[4] = p[3];
+      prim[3] = ld(a.x, o + 2);
+      prim[4] = ld(a.x, o + 3);
       prim[5] = 0.0;
     } else {
-      for (int k = 0; k < 6; ++k) prim[k] = a.x[i * a.stride + k];
+      for (int k = 0; k < 6; ++k) prim[k] = ld(a.x, i * a.stride + k);
     }
     const int32_t offset = FILL ? a.offsets[i] : 0;
     const int32_t room = FILL ? a.fill_cnt[i] : 0;
@@ -129,8 +135,9 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
       // the walk tests a child box only when its validity bit is set, so an
       // invalid box (min > max on some axis) is never entered
       v = traverse_voted<Node64, double>(
-          a.nodes, a.ninner, stk, [&](const double* b) { return pred64<KIND>(prim, a.tol, b); },
-          no_pref, emit);
+          a.nodes, a.ninner, stk,
+          [&](const double* b) { return pred64<KIND>(prim, a.tol, b, a.f32, a.dims); }, no_pref,
+          emit);
     } else {
       float lo[3], hi[3];
       const int hoff = (KIND == AXSQ_QUERY_BOXES) ? 3 : 0;
@@ -140,13 +147,14 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
       }
       v = traverse_voted<Node32, float>(
           a.nodes32, a.ninner, stk,
-          [&](const float* b) { return pred32<KIND>(prim, lo, hi, a.tol, b); }, no_pref,
+          [&](const float* b) { return pred32<KIND>(prim, lo, hi, a.tol, b, a.f32, a.dims); },
+          no_pref,
           [&](int32_t pos) {
             double lb[6];
             const double2* p = reinterpret_cast<const double2*>(a.leaf_boxes + 6 * (int64_t)pos);
             const double2 c0 = p[0], c1 = p[1], c2 = p[2];
             lb[0] = c0.x; lb[1] = c0.y; lb[2] = c1.x; lb[3] = c1.y; lb[4] = c2.x; lb[5] = c2.y;
-            if (valid6(lb) && pred64<KIND>(prim, a.tol, lb)) emit(pos);
+            if (valid6(lb) && pred64<KIND>(prim, a.tol, lb, a.f32, a.dims)) emit(pos);
           });
     }
     if (v < 0) atomicOr(a.status, kStatusStack);
@@ -236,8 +244,8 @@ hipError_t launch_mode(int mode, int kind, const FindArgs& a, hipStream_t s) {
 
 // mode 0: count (and keep the first kstash ids when stash is given); 1: fill
 // every query; 2: fill the nq queries listed in `over`.
-int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const double* y,
-                const double* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
+int find_launch(const axsq_bvh* b, int kind, int mode, const void* x, const void* y,
+                const void* z, int64_t stride, int64_t nq, double tol, int32_t* counts,
                 const int32_t* fill_cnt, const int32_t* offsets, int32_t* cands, int32_t* stash,
                 int kstash, int32_t* stash_cnt, int32_t* over, uint32_t* nover, hipStream_t s) {
   FindArgs a;
@@ -249,6 +257,7 @@ int find_launch(const axsq_bvh* b, int kind, int mode, const double* x, const do
   a.y = y;
   a.z = z;
   a.stride = stride;
+  a.f32 = b->f32 ? 1 : 0;
   a.nq = nq;
   a.dims = b->dims;
   a.ninner = (int32_t)(b->n - 1);

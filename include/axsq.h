@@ -102,6 +102,26 @@ int axsq_bvh_build2d(axsq_bvh** out, const double* boxes, int64_t n, double scal
 /* 2 or 3: the dimension the handle was built in (0 for NULL). */
 int axsq_bvh_dimension(const axsq_bvh* bvh);
 
+/* Builds a float tree over n boxes, AoS of 2 * ndims floats (min, max), ndims
+ * 2 or 3. Replaces spin::BVH<NDIMS, Exec, float>::initialize (BVH.hpp:129-130
+ * FloatType, :426-477; the reference's tests instantiate every scenario with
+ * float, spin/tests/spin_bvh.cpp:1560-1669): the scaling about the centroid,
+ * the bounds, the Morton normalisation and the ray test run in float, and
+ * the scale factor is a float (BVH::setScaleFactor(FloatType), default
+ * 1.000123f). Boxes are kept as their float values; axsq_bvh_bounds,
+ * axsq_bvh_export and axsq_bvh_device_arrays report them as doubles (exact).
+ * Queries go through the axsq_bvh_find*_f32 entry points, with float
+ * coordinates and a float tolerance (FLT_EPSILON by default, BVH.hpp:411-412);
+ * the double ones refuse a float tree (AXSQ_ERR_STATE), and the other way
+ * round. */
+int axsq_bvh_build_f32(axsq_bvh** out, const float* boxes, int64_t n, int ndims, float scale,
+                       void* stream);
+/* axsq_bvh_rebuild for a float tree (the dimension it was built in). */
+int axsq_bvh_rebuild_f32(axsq_bvh* bvh, const float* boxes, int64_t n, float scale,
+                         void* stream);
+/* 32 for a float tree, 64 for a double one, 0 for NULL. */
+int axsq_bvh_precision(const axsq_bvh* bvh);
+
 /* One inner node of the tree on the device (128 bytes). Child link >= 0: an
  * inner node index; < 0: -(leaf position + 1). flags bit 0 / bit 1: the left /
  * right child box is valid (BoundingBox::isValid). Boxes are the scaled boxes
@@ -209,6 +229,24 @@ int axsq_bvh_find_into_noalloc(const axsq_bvh* bvh, int kind, const double* x, c
                                const double* z, int64_t stride, int64_t nq, double tol,
                                int32_t* offsets, int32_t* counts, int32_t* candidates,
                                int64_t capacity, int64_t* total, void* stream);
+
+/* The four searches above for a float tree (spin::BVH<NDIMS, Exec, float>::
+ * findPoints / findBoundingBoxes / findRays, BVH.hpp:482-568): the same
+ * arguments with float queries (strides in floats) and a float tolerance. */
+int axsq_bvh_find_count_f32(const axsq_bvh* bvh, int kind, const float* x, const float* y,
+                            const float* z, int64_t stride, int64_t nq, float tol,
+                            int32_t* offsets, int32_t* counts, int64_t* total, void* stream);
+int axsq_bvh_find_fill_f32(const axsq_bvh* bvh, int kind, const float* x, const float* y,
+                           const float* z, int64_t stride, int64_t nq, float tol,
+                           const int32_t* offsets, const int32_t* counts, int32_t* candidates,
+                           void* stream);
+int axsq_bvh_find_into_f32(const axsq_bvh* bvh, int kind, const float* x, const float* y,
+                           const float* z, int64_t stride, int64_t nq, float tol,
+                           int32_t* offsets, int32_t* counts, int32_t* candidates,
+                           int64_t capacity, int64_t* total, void* stream);
+int axsq_bvh_find_f32(const axsq_bvh* bvh, int kind, const float* x, const float* y,
+                      const float* z, int64_t stride, int64_t nq, float tol, int32_t* offsets,
+                      int32_t* counts, int32_t** candidates, int64_t* total, void* stream);
 
 /* One-call form matching BVH::find* semantics: counts and offsets as above;
  * the library allocates *candidates (*total entries, in the memory set by

@@ -235,6 +235,103 @@ int orc_ray_box(const double o[3], const double dir[3], const orc_box* b, double
   return ray_box_d(o, dir, b, eps, 3);
 }
 
+/* ---- FloatType = float (spin::BVH<NDIMS, Exec, float>) -------------------
+ * A float tree's boxes are float values held in doubles (exact); unions,
+ * comparisons and the point and box predicates then give the float results
+ * as they are. These steps round to float where BoundingBox<float> and the
+ * float instantiation of the build and of intersect_ray do. */
+
+/* BoundingBox<float>::scale (BoundingBox.hpp:547-561): midpoint
+ * float(0.5 * (min + max)) with the sum in float (Point.hpp:279-290), radius
+ * float(range * float(sf * 0.5)) from NumericArray::operator*=(double)
+ * (core/NumericArray.hpp:498-506; a product of floats is exact in double),
+ * then min/max in float and fixBounds. */
+static inline void box_scale_f(orc_box* b, float sf, int nd)
+{
+  if(!box_valid_d(b, nd)) return;
+  const float s = (float)((double)sf * 0.5);
+  float lo[3], hi[3];
+  for(int d = 0; d < nd; ++d)
+  {
+    const float mn = (float)b->min[d];
+    const float mx = (float)b->max[d];
+    const float sum = mn + mx;
+    const float mid = (float)(0.5 * (double)sum);
+    const float range = mx - mn;
+    const float r = (float)((double)range * (double)s);
+    lo[d] = mid - r;
+    hi[d] = mid + r;
+  }
+  for(int d = 0; d < nd; ++d)
+  {
+    if(lo[d] > hi[d])
+    {
+      const float t = lo[d];
+      lo[d] = hi[d];
+      hi[d] = t;
+    }
+    b->min[d] = lo[d];
+    b->max[d] = hi[d];
+  }
+}
+
+/* get_mcodes + morton32_encode with FloatType = float (build_radix_tree.hpp:
+ * 48-65,146-176): (centroid - min) * inv_extent in float. */
+static inline uint32_t box_mcode_f(const orc_box* b, const float minc[3], const float inv[3],
+                                   int nd)
+{
+  const float scale = (nd == 2) ? 65536.0f : 1024.0f;
+  int32_t q[3] = {0, 0, 0};
+  for(int d = 0; d < nd; ++d)
+  {
+    const float sum = (float)b->min[d] + (float)b->max[d];
+    const float c = (float)(0.5 * (double)sum);
+    const float u = (c - minc[d]) * inv[d];
+    q[d] = (int32_t)fminf(fmaxf(u * scale, 0.0f), scale - 1.0f);
+  }
+  return (nd == 2) ? orc_morton2(q[0], q[1]) : orc_morton3(q[0], q[1], q[2]);
+}
+
+/* intersect_ray_bbox_test / intersect_ray with T = float (intersect_ray_impl.hpp:
+ * 150-187,321-351): isNearlyEqual(n, 0, EPS) as |n - 0| <= EPS, 1 / n and the
+ * slab parameters in float, tmin from FLT_MIN. */
+static inline int ray_slab_f(float x0, float n, float bmin, float bmax, float* tmin, float* tmax,
+                             float eps)
+{
+  if(fabsf(n - 0.0f) <= eps)
+  {
+    return ((x0 < bmin) || (x0 > bmax)) ? 0 : 1;
+  }
+  const float invn = 1.0f / n;
+  float t1 = (bmin - x0) * invn;
+  float t2 = (bmax - x0) * invn;
+  if(t1 > t2)
+  {
+    const float t = t1;
+    t1 = t2;
+    t2 = t;
+  }
+  *tmin = (t1 < *tmin) ? *tmin : t1;
+  *tmax = (t2 < *tmax) ? t2 : *tmax;
+  return (*tmin > *tmax) ? 0 : 1;
+}
+
+static inline int ray_box_f(const double o[3], const double dir[3], const orc_box* b, float eps,
+                            int nd)
+{
+  float tmin = FLT_MIN, tmax = FLT_MAX;
+  int hit = 1;
+  for(int d = 0; d < nd; ++d)
+    hit = hit &&
+      ray_slab_f((float)o[d], (float)dir[d], (float)b->min[d], (float)b->max[d], &tmin, &tmax,
+                 eps);
+  return hit;
+}
+int orc_ray_box_f32(const double o[3], const double dir[3], const orc_box* b, float eps)
+{
+  return ray_box_f(o, dir, b, eps, 3);
+}
+
 /* primal/operators/closest_point.hpp:162-290 (Ericson RTCD 5.1.5) with the
  * degenerate-edge guard at :215. Returns the location code:
  * 0,1,2 vertex A,B,C; -1 edge AB; -3 edge AC; -2 edge BC; 3 face. */
@@ -394,11 +491,14 @@ static void stable_sort_codes(uint32_t* keys, int32_t* idx, int32_t n)
   free(i2);
 }
 
-/* boxes_in: AoS of 2 * nd doubles (min, max) per box. */
-static orc_bvh* bvh_new_d(const double* boxes_in, int32_t n_in, double scale, int nd)
+/* boxes_in: AoS of 2 * nd doubles (min, max) per box; f32: a float tree over
+ * boxes_f (2 * nd floats per box) with a float scale factor. */
+static orc_bvh* bvh_new_any(const double* boxes_in, const float* boxes_f, int32_t n_in,
+                            double scale, int nd, int f32)
 {
   orc_bvh* t = (orc_bvh*)calloc(1, sizeof(orc_bvh));
   t->dims = nd;
+  t->f32 = f32;
   /* spin/BVH.hpp:441-464: pad 0 or 1 boxes to 2 with an invalid box. */
   const int32_t n = (n_in <= 1) ? 2 : n_in;
   const int32_t inner = n - 1;
@@ -418,16 +518,37 @@ static orc_bvh* bvh_new_d(const double* boxes_in, int32_t n_in, double scale, in
   for(int32_t i = 0; i < n; ++i)
   {
     box_clear(&aabb[i]);
-    if(n_in > 1 || (n_in == 1 && i == 0))
+    if(f32)
     {
-      const double* b = boxes_in + 2 * nd * (size_t)i;
+      /* BoundingBox<float>'s invalid box (BoundingBox.hpp:72-73) */
       for(int d = 0; d < 3; ++d)
       {
-        aabb[i].min[d] = (d < nd) ? b[d] : 0.0;
-        aabb[i].max[d] = (d < nd) ? b[nd + d] : 0.0;
+        aabb[i].min[d] = FLT_MAX;
+        aabb[i].max[d] = -FLT_MAX;
       }
     }
-    box_scale_d(&aabb[i], scale, nd);
+    if(n_in > 1 || (n_in == 1 && i == 0))
+    {
+      for(int d = 0; d < 3; ++d)
+      {
+        if(f32)
+        {
+          const float* b = boxes_f + 2 * nd * (size_t)i;
+          aabb[i].min[d] = (d < nd) ? (double)b[d] : 0.0;
+          aabb[i].max[d] = (d < nd) ? (double)b[nd + d] : 0.0;
+        }
+        else
+        {
+          const double* b = boxes_in + 2 * nd * (size_t)i;
+          aabb[i].min[d] = (d < nd) ? b[d] : 0.0;
+          aabb[i].max[d] = (d < nd) ? b[nd + d] : 0.0;
+        }
+      }
+    }
+    if(f32)
+      box_scale_f(&aabb[i], (float)scale, nd);
+    else
+      box_scale_d(&aabb[i], scale, nd);
   }
 
   /* reduce :134-142 (SEQ path: addBox from an invalid box) */
@@ -442,9 +563,18 @@ static orc_bvh* bvh_new_d(const double* boxes_in, int32_t n_in, double scale, in
     minc[d] = t->bounds.min[d];
     inv[d] = near_eq(ext, 0.0, 1.0e-8) ? 0.0 : 1.0 / ext;
   }
+  /* the float tree: extent and 1.f / extent in float, the threshold a float */
+  float minc_f[3], inv_f[3];
+  for(int d = 0; d < 3; ++d)
+  {
+    const float ext = (float)t->bounds.max[d] - (float)t->bounds.min[d];
+    minc_f[d] = (float)t->bounds.min[d];
+    inv_f[d] = (fabsf(ext - 0.0f) <= (float)1.0e-8) ? 0.0f : 1.0f / ext;
+  }
   for(int32_t i = 0; i < n; ++i)
   {
-    t->mcodes[i] = box_mcode(&aabb[i], minc, inv, nd);
+    t->mcodes[i] = f32 ? box_mcode_f(&aabb[i], minc_f, inv_f, nd)
+                       : box_mcode(&aabb[i], minc, inv, nd);
     t->leafs[i] = i;
   }
 
@@ -561,12 +691,17 @@ static orc_bvh* bvh_new_d(const double* boxes_in, int32_t n_in, double scale, in
 
 orc_bvh* orc_bvh_new(const double* boxes_in, int32_t n_in, double scale)
 {
-  return bvh_new_d(boxes_in, n_in, scale, 3);
+  return bvh_new_any(boxes_in, NULL, n_in, scale, 3, 0);
 }
 
 orc_bvh* orc_bvh_new2d(const double* boxes_in, int32_t n_in, double scale)
 {
-  return bvh_new_d(boxes_in, n_in, scale, 2);
+  return bvh_new_any(boxes_in, NULL, n_in, scale, 2, 0);
+}
+
+orc_bvh* orc_bvh_new_f32(const float* boxes_in, int32_t n_in, int nd, float scale)
+{
+  return bvh_new_any(NULL, boxes_in, n_in, (double)scale, nd, 1);
 }
 
 void orc_bvh_free(orc_bvh* t)
@@ -710,6 +845,7 @@ typedef struct
 {
   double tol;
   int nd;
+  int f32; /* a float tree: the ray test in float with a float tolerance */
 } find_ctx;
 
 /* prim: a point (3 doubles), a box (orc_box) or a ray (origin, direction),
@@ -725,7 +861,8 @@ static int pred_box(const void* prim, const orc_box* b, void* ctx)
 static int pred_ray(const void* prim, const orc_box* b, void* ctx)
 {
   const double* r = (const double*)prim;
-  return ray_box_d(r, r + 3, b, ((find_ctx*)ctx)->tol, ((find_ctx*)ctx)->nd);
+  const find_ctx* c = (const find_ctx*)ctx;
+  return c->f32 ? ray_box_f(r, r + 3, b, (float)c->tol, c->nd) : ray_box_d(r, r + 3, b, c->tol, c->nd);
 }
 
 /* query i of an AoS array of nd-dimensional points (nd doubles), boxes or
@@ -779,7 +916,7 @@ static void* find_worker(void* arg)
 {
   find_job* j = (find_job*)arg;
   pred_fn B = (j->kind == 0) ? pred_point : (j->kind == 1) ? pred_box : pred_ray;
-  find_ctx fc = {j->tol, j->t->dims};
+  find_ctx fc = {j->tol, j->t->dims, j->t->f32};
   for(int64_t i = j->begin; i < j->end; ++i)
   {
     collect_ctx cc = {0, j->cands ? j->cands + j->offsets[i] : NULL};
@@ -850,7 +987,7 @@ int32_t orc_bvh_max_depth(const orc_bvh* t, int kind, const double* prims, int64
                           double tol)
 {
   pred_fn B = (kind == 0) ? pred_point : (kind == 1) ? pred_box : pred_ray;
-  find_ctx fc = {tol, t->dims};
+  find_ctx fc = {tol, t->dims, t->f32};
   int32_t deepest = 0;
   for(int64_t i = 0; i < nq; ++i)
   {
@@ -867,7 +1004,7 @@ int32_t orc_bvh_max_depth(const orc_bvh* t, int kind, const double* prims, int64
 int64_t orc_bvh_visits(const orc_bvh* t, int kind, const double* prims, int64_t nq, double tol)
 {
   pred_fn B = (kind == 0) ? pred_point : (kind == 1) ? pred_box : pred_ray;
-  find_ctx fc = {tol, t->dims};
+  find_ctx fc = {tol, t->dims, t->f32};
   int64_t sum = 0;
   for(int64_t i = 0; i < nq; ++i)
   {
@@ -888,7 +1025,7 @@ int64_t orc_bvh_brute(const orc_bvh* t, int kind, const double* prims, int64_t n
   /* scaled boxes in original-id order */
   orc_box* orig = (orc_box*)malloc(sizeof(orc_box) * n);
   for(int32_t p = 0; p < n; ++p) orig[t->leafs[p]] = t->leaf_aabbs[p];
-  find_ctx fc = {tol, t->dims};
+  find_ctx fc = {tol, t->dims, t->f32};
   int64_t total = 0;
   for(int64_t i = 0; i < nq; ++i)
   {

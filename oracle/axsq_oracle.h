@@ -48,6 +48,7 @@ typedef struct orc_bvh
   orc_box* nodes;       /* 2(n-1): child boxes of inner node i at 2i, 2i+1 */
   int32_t* children;    /* 2(n-1): 2*child for inner, -(pos+1) for leaf */
   int32_t dims;         /* 3, or 2 for BVH<2>: boxes use only their x, y entries */
+  int32_t f32;          /* a float tree (FloatType = float): float values in the boxes */
 } orc_bvh;
 
 /* ---- primal restatements, exported for unit tests ------------------------ */
@@ -68,6 +69,11 @@ orc_bvh* orc_bvh_new(const double* boxes /* AoS n x 6 */, int32_t n, double scal
 /* spin::BVH<2>: boxes AoS n x 4 (min x y, max x y); queries then are 2-D
  * (points 2 doubles, boxes and rays 4) */
 orc_bvh* orc_bvh_new2d(const double* boxes /* AoS n x 4 */, int32_t n, double scale);
+/* A float tree (spin::BVH<nd, Exec, float>): boxes AoS n x 2nd floats, the
+ * scale, bounds, Morton normalisation and ray test in float. Queries are
+ * passed as doubles holding float values. */
+orc_bvh* orc_bvh_new_f32(const float* boxes, int32_t n, int nd, float scale);
+int orc_ray_box_f32(const double o[3], const double dir[3], const orc_box* b, float eps);
 void orc_bvh_free(orc_bvh* t);
 int32_t orc_bvh_size(const orc_bvh* t);
 void orc_bvh_export(const orc_bvh* t, double* bounds6, uint32_t* mcodes,

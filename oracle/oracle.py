@@ -38,6 +38,11 @@ def lib():
         L.orc_bvh_new.argtypes = [_dp, ctypes.c_int32, ctypes.c_double]
         L.orc_bvh_new2d.restype = ctypes.c_void_p
         L.orc_bvh_new2d.argtypes = [_dp, ctypes.c_int32, ctypes.c_double]
+        L.orc_bvh_new_f32.restype = ctypes.c_void_p
+        L.orc_bvh_new_f32.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int32,
+                                      ctypes.c_int, ctypes.c_float]
+        L.orc_ray_box_f32.restype = ctypes.c_int
+        L.orc_ray_box_f32.argtypes = [_dp, _dp, _dp, ctypes.c_float]
         L.orc_morton2.restype = ctypes.c_uint32
         L.orc_morton2.argtypes = [ctypes.c_int32] * 2
         L.orc_bvh_free.argtypes = [ctypes.c_void_p]
@@ -187,21 +192,34 @@ KIND = {"points": 0, "boxes": 1, "rays": 2}
 
 
 class OracleBVH:
-    """spin::BVH<NDIMS, SEQ_EXEC, double> restated (build + find*); ndims 3 or 2.
-    Boxes are AoS (min, max) of 2 * ndims doubles; 2-D queries are points of 2
-    doubles and boxes / rays of 4."""
+    """spin::BVH<NDIMS, SEQ_EXEC, FloatType> restated (build + find*); ndims 3
+    or 2, dtype float64 (double) or float32 (float). Boxes are AoS (min, max)
+    of 2 * ndims values; 2-D queries are points of 2 values and boxes / rays
+    of 4. A float tree's queries are float32 values (passed on as doubles,
+    exactly), its tolerance FLT_EPSILON by default."""
 
-    def __init__(self, boxes, scale=1.000123, _handle=None, _owner=None, ndims=3):
+    def __init__(self, boxes, scale=1.000123, _handle=None, _owner=None, ndims=3,
+                 dtype=np.float64):
         self.ndims = ndims
+        self.dtype = np.dtype(dtype)
         if _handle is not None:
             self.h = _handle
             self._owner = _owner
+        elif self.dtype == np.float32:
+            b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 2 * ndims)
+            self._boxes = b
+            self.h = lib().orc_bvh_new_f32(b.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                           b.shape[0], ndims, np.float32(scale))
+            self._owner = None
         else:
             b = f64(boxes).reshape(-1, 2 * ndims)
             self._boxes = b
             new = lib().orc_bvh_new2d if ndims == 2 else lib().orc_bvh_new
             self.h = new(_p(b, _dp), b.shape[0], scale)
             self._owner = None
+
+    def _tol(self, tol):
+        return float(np.finfo(self.dtype).eps) if tol is None else float(tol)
 
     def __del__(self):
         if getattr(self, "_owner", 1) is None and getattr(self, "h", None):
@@ -233,8 +251,9 @@ class OracleBVH:
     def _stride(self, k):
         return self.ndims if k == 0 else 2 * self.ndims
 
-    def find(self, kind, prims, tol=np.finfo(np.float64).eps, nthreads=1):
+    def find(self, kind, prims, tol=None, nthreads=1):
         """Returns (offsets, counts, candidates) like LinearBVH::findCandidatesImpl."""
+        tol = self._tol(tol)
         k = KIND[kind]
         pr = f64(prims).reshape(-1, self._stride(k))
         nq = pr.shape[0]
@@ -249,14 +268,16 @@ class OracleBVH:
                            _p(offsets, _i32p), _p(cands, _i32p), nthreads)
         return offsets, counts, cands[:total]
 
-    def max_depth(self, kind, prims, tol=np.finfo(np.float64).eps):
+    def max_depth(self, kind, prims, tol=None):
         """The deepest stack entry the queries' walks push (-1: overflow)."""
+        tol = self._tol(tol)
         k = KIND[kind]
         pr = f64(prims).reshape(-1, self._stride(k))
         return int(lib().orc_bvh_max_depth(self.h, k, _p(pr, _dp), pr.shape[0], tol))
 
-    def visits(self, kind, prims, tol=np.finfo(np.float64).eps):
+    def visits(self, kind, prims, tol=None):
         """Inner-node visits of the queries' walks, summed."""
+        tol = self._tol(tol)
         k = KIND[kind]
         pr = f64(prims).reshape(-1, self._stride(k))
         v = int(lib().orc_bvh_visits(self.h, k, _p(pr, _dp), pr.shape[0], tol))
@@ -264,7 +285,9 @@ class OracleBVH:
             raise RuntimeError("oracle traversal stack overflow")
         return v
 
-    def brute(self, kind, prims, tol=np.finfo(np.float64).eps):
+    def brute(self, kind, prims, tol=None):
+        """O(N*Q) brute force over the scaled leaf boxes: (counts, ids)."""
+        tol = self._tol(tol)
         k = KIND[kind]
         pr = f64(prims).reshape(-1, self._stride(k))
         nq = pr.shape[0]
