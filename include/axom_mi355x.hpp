@@ -192,6 +192,52 @@ inline RayT make_ray_n(const double* o, const double* d) {
 }
 inline Ray3 make_ray(const double o[3], const double d[3]) { return make_ray_n<3, Ray3>(o, d); }
 inline Ray2 make_ray2(const double o[2], const double d[2]) { return make_ray_n<2, Ray2>(o, d); }
+
+// The float layouts (primal::BoundingBox<float,N>, Point<float,N>,
+// Ray<float,N>) of spin::BVH<NDIMS, MI355X_EXEC, float>.
+struct BoundingBox3f {
+  float min[3];
+  float max[3];
+};
+struct Point3f {
+  float x[3];
+};
+struct Ray3f {
+  float origin[3];
+  float dir[3];
+};
+struct BoundingBox2f {
+  float min[2];
+  float max[2];
+};
+struct Point2f {
+  float x[2];
+};
+struct Ray2f {
+  float origin[2];
+  float dir[2];
+};
+// Ray<float>'s unit direction (Vector<float>::unitVector, Vector.hpp:477-490,
+// NumericArray.hpp:498-514): the squared norm summed in float, then each
+// component times 1 / double(float(sqrt)), rounded to float.
+template <int N, class RayT>
+inline RayT make_ray_nf(const float* o, const float* d) {
+  RayT r;
+  float len_sq = 0.0f;
+  for (int k = 0; k < N; ++k) {
+    r.origin[k] = o[k];
+    len_sq += d[k] * d[k];
+    r.dir[k] = (k == 0) ? 1.0f : 0.0f;
+  }
+  if ((double)len_sq >= 1e-50) {
+    const float s = (float)std::sqrt((double)len_sq);
+    const double inv = 1. / (double)s;
+    for (int k = 0; k < N; ++k) r.dir[k] = (float)((double)d[k] * inv);
+  }
+  return r;
+}
+inline Ray3f make_ray(const float o[3], const float d[3]) { return make_ray_nf<3, Ray3f>(o, d); }
+inline Ray2f make_ray2(const float o[2], const float d[2]) { return make_ray_nf<2, Ray2f>(o, d); }
 }  // namespace primal
 
 namespace spin {
@@ -202,19 +248,31 @@ template <int NDIMS, typename ExecSpace, typename FloatType = double>
 class BVH;
 
 namespace detail {
-template <int NDIMS>
+template <int NDIMS, typename T = double>
 struct BVHTypes;
 template <>
-struct BVHTypes<3> {
+struct BVHTypes<3, double> {
   using BoxType = primal::BoundingBox3;
   using PointType = primal::Point3;
   using RayType = primal::Ray3;
 };
 template <>
-struct BVHTypes<2> {
+struct BVHTypes<2, double> {
   using BoxType = primal::BoundingBox2;
   using PointType = primal::Point2;
   using RayType = primal::Ray2;
+};
+template <>
+struct BVHTypes<3, float> {
+  using BoxType = primal::BoundingBox3f;
+  using PointType = primal::Point3f;
+  using RayType = primal::Ray3f;
+};
+template <>
+struct BVHTypes<2, float> {
+  using BoxType = primal::BoundingBox2f;
+  using PointType = primal::Point2f;
+  using RayType = primal::Ray2f;
 };
 
 // The device-side walk of spin::BVH::getTraverser (LinearBVHTraverser,
@@ -233,13 +291,16 @@ struct BVHTypes<2> {
 // entries, the first the end marker: 63 pending nodes by default): the walk
 // stops there, so the leaf actions it made are a prefix of the full walk's
 // (the reference has no check; its 64-entry stack would overflow).
-template <int NDIMS, int STACK = 64>
+// T: the tree's FloatType; a float tree's boxes reach the predicate as float
+// boxes (its node arrays hold float values) and the nearer-first test runs
+// in float, as LinearBVHTraverser<float, NDIMS> computes it.
+template <int NDIMS, int STACK = 64, typename T = double>
 class MI355XTraverser {
   static_assert(STACK >= 2, "the walk's stack needs room for its end marker and a node");
 
  public:
-  using BoxType = typename BVHTypes<NDIMS>::BoxType;
-  using PointType = typename BVHTypes<NDIMS>::PointType;
+  using BoxType = typename BVHTypes<NDIMS, T>::BoxType;
+  using PointType = typename BVHTypes<NDIMS, T>::PointType;
 
   MI355XTraverser() = default;
   MI355XTraverser(const axsq_node* nodes, const std::int32_t* leaf_ids, std::int32_t ninner)
@@ -249,10 +310,13 @@ class MI355XTraverser {
   __device__ bool traverse_tree(const PointType& p, LeafAction&& lf, Predicate&& predicate) const {
     return walk(p, lf, predicate, [&p](const BoxType& l, const BoxType& r, bool r_valid) {
       if (!r_valid) return false;
-      double dl = 0.0, dr = 0.0;
+      T dl = 0, dr = 0;
       for (int d = 0; d < NDIMS; ++d) {
-        const double el = p.x[d] - 0.5 * (l.min[d] + l.max[d]);
-        const double er = p.x[d] - 0.5 * (r.min[d] + r.max[d]);
+        // centroid T(0.5 * (min + max)), the sum in T (Point.hpp:279-290)
+        const T cl = static_cast<T>(0.5 * static_cast<T>(l.min[d] + l.max[d]));
+        const T cr = static_cast<T>(0.5 * static_cast<T>(r.min[d] + r.max[d]));
+        const T el = p.x[d] - cl;
+        const T er = p.x[d] - cr;
         dl += el * el;
         dr += er * er;
       }
@@ -269,8 +333,8 @@ class MI355XTraverser {
   __device__ static BoxType box_of(const double* b) {
     BoxType out;
     for (int d = 0; d < NDIMS; ++d) {
-      out.min[d] = b[d];
-      out.max[d] = b[3 + d];
+      out.min[d] = static_cast<T>(b[d]);
+      out.max[d] = static_cast<T>(b[3 + d]);
     }
     return out;
   }
@@ -320,18 +384,21 @@ class MI355XTraverser {
   std::int32_t m_ninner = 0;
 };
 
-// The body of spin::BVH<NDIMS, MI355X_EXEC, double> for NDIMS 3 and 2: build
-// over device boxes, find* over device queries. The BVH copies the boxes; it
-// keeps no pointer to them.
-template <int NDIMS>
+// The body of spin::BVH<NDIMS, MI355X_EXEC, T> for NDIMS 3 and 2 and T double
+// or float: build over device boxes, find* over device queries. The BVH
+// copies the boxes; it keeps no pointer to them.
+template <int NDIMS, typename T = double>
 class MI355XLinearBVH {
- public:
-  using BoxType = typename BVHTypes<NDIMS>::BoxType;
-  using PointType = typename BVHTypes<NDIMS>::PointType;
-  using RayType = typename BVHTypes<NDIMS>::RayType;
+  static constexpr bool F32 = std::is_same<T, float>::value;
+  static_assert(F32 || std::is_same<T, double>::value, "FloatType is double or float");
 
-  static constexpr double DEFAULT_SCALE_FACTOR = 1.000123;  // BVH.hpp:410
-  static constexpr double DEFAULT_TOLERANCE = std::numeric_limits<double>::epsilon();
+ public:
+  using BoxType = typename BVHTypes<NDIMS, T>::BoxType;
+  using PointType = typename BVHTypes<NDIMS, T>::PointType;
+  using RayType = typename BVHTypes<NDIMS, T>::RayType;
+
+  static constexpr T DEFAULT_SCALE_FACTOR = static_cast<T>(1.000123);  // BVH.hpp:410
+  static constexpr T DEFAULT_TOLERANCE = std::numeric_limits<T>::epsilon();
 
   MI355XLinearBVH() = default;
   MI355XLinearBVH(const MI355XLinearBVH&) = delete;
@@ -340,6 +407,7 @@ class MI355XLinearBVH {
 
   // BVH.hpp:426-477
   int initialize(const BoxType* boxes, IndexType numItems) {
+    if (F32) return initialize_f32(reinterpret_cast<const float*>(boxes), numItems);
     const double* b = reinterpret_cast<const double*>(boxes);
     if (m_bvh) {  // build again in place, keeping the device arrays
       const int rc = axsq_bvh_rebuild(m_bvh, b, numItems, m_scaleFactor, m_fpMode, nullptr);
@@ -360,6 +428,25 @@ class MI355XLinearBVH {
   int initialize(ArrayView<const BoxType> boxes, IndexType numItems) {
     return initialize(boxes.data(), numItems);
   }
+
+ private:
+  int initialize_f32(const float* b, IndexType numItems) {
+    if (m_bvh) {
+      const int rc = axsq_bvh_rebuild_f32(m_bvh, b, numItems, (float)m_scaleFactor, nullptr);
+      if (rc != AXSQ_OK) {
+        (void)axsq_bvh_destroy(m_bvh);
+        m_bvh = nullptr;
+        ::axom::detail::check(rc);
+      }
+      return BVH_BUILD_OK;
+    }
+    ::axom::detail::check(
+        axsq_bvh_build_f32(&m_bvh, b, numItems, NDIMS, (float)m_scaleFactor, nullptr));
+    ::axom::detail::check(axsq_bvh_set_allocator(m_bvh, m_allocatorID));
+    return BVH_BUILD_OK;
+  }
+
+ public:
   bool isInitialized() const { return m_bvh != nullptr; }
 
   // BVH.hpp:251-261: the memory of the candidate arrays find* returns, one of
@@ -376,12 +463,13 @@ class MI355XLinearBVH {
     if (!m_bvh) throw mi355x_error(AXSQ_ERR_STATE, "BVH is not initialized");
     ::axom::detail::check(axsq_bvh_write_vtk(m_bvh, fileName.c_str()));
   }
-  void setScaleFactor(double s) { m_scaleFactor = s; }
-  double getScaleFactor() const { return m_scaleFactor; }
-  void setTolerance(double eps) { m_tolerance = eps; }
-  double getTolerance() const { return m_tolerance; }
-  // Precision of the traversal (MI355X addition): AXSQ_FP64 (default, the
-  // reference's form) or AXSQ_FP32_TRAVERSAL; takes effect at initialize().
+  void setScaleFactor(T s) { m_scaleFactor = s; }
+  T getScaleFactor() const { return m_scaleFactor; }
+  void setTolerance(T eps) { m_tolerance = eps; }
+  T getTolerance() const { return m_tolerance; }
+  // Precision of the traversal of a double tree (MI355X addition): AXSQ_FP64
+  // (default, the reference's form) or AXSQ_FP32_TRAVERSAL; takes effect at
+  // initialize(). A float tree walks its own float nodes.
   void setTraversalPrecision(int fp_mode) { m_fpMode = fp_mode; }
 
   // BVH.hpp:297-308; an invalid box before initialize().
@@ -390,15 +478,15 @@ class MI355XLinearBVH {
     double lo[3], hi[3];
     if (!m_bvh) {
       for (int k = 0; k < 3; ++k) {
-        lo[k] = std::numeric_limits<double>::max();
-        hi[k] = std::numeric_limits<double>::lowest();
+        lo[k] = std::numeric_limits<T>::max();
+        hi[k] = std::numeric_limits<T>::lowest();
       }
     } else {
       ::axom::detail::check(axsq_bvh_bounds(m_bvh, lo, hi));
     }
     for (int k = 0; k < NDIMS; ++k) {
-      b.min[k] = lo[k];
-      b.max[k] = hi[k];
+      b.min[k] = static_cast<T>(lo[k]);  // a float tree's bounds are floats
+      b.max[k] = static_cast<T>(hi[k]);
     }
     return b;
   }
@@ -406,49 +494,49 @@ class MI355XLinearBVH {
   // BVH.hpp:482-505 (AoS points)
   void findPoints(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
                   Array<IndexType>& candidates, IndexType numPts, const PointType* points) const {
-    const double* p = reinterpret_cast<const double*>(points);
+    const T* p = reinterpret_cast<const T*>(points);
     find(AXSQ_QUERY_POINTS, p, p + 1, NDIMS == 3 ? p + 2 : nullptr, NDIMS, offsets, counts,
          candidates, numPts);
   }
   // ZipPoint-style SoA points (z is ignored, and may be null, in 2-D)
   void findPoints(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
-                  Array<IndexType>& candidates, IndexType numPts, const double* x,
-                  const double* y, const double* z = nullptr) const {
+                  Array<IndexType>& candidates, IndexType numPts, const T* x, const T* y,
+                  const T* z = nullptr) const {
     find(AXSQ_QUERY_POINTS, x, y, z, 1, offsets, counts, candidates, numPts);
   }
   // BVH.hpp:510-536
   void findRays(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
                 Array<IndexType>& candidates, IndexType numRays, const RayType* rays) const {
-    find(AXSQ_QUERY_RAYS, reinterpret_cast<const double*>(rays), nullptr, nullptr, 2 * NDIMS,
-         offsets, counts, candidates, numRays);
+    find(AXSQ_QUERY_RAYS, reinterpret_cast<const T*>(rays), nullptr, nullptr, 2 * NDIMS, offsets,
+         counts, candidates, numRays);
   }
   // BVH.hpp:541-568
   void findBoundingBoxes(ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
                          Array<IndexType>& candidates, IndexType numBoxes,
                          const BoxType* boxes) const {
-    find(AXSQ_QUERY_BOXES, reinterpret_cast<const double*>(boxes), nullptr, nullptr, 2 * NDIMS,
+    find(AXSQ_QUERY_BOXES, reinterpret_cast<const T*>(boxes), nullptr, nullptr, 2 * NDIMS,
          offsets, counts, candidates, numBoxes);
   }
 
   const axsq_bvh* handle() const { return m_bvh; }
 
   // BVH.hpp:305-319
-  using TraverserType = MI355XTraverser<NDIMS>;
+  using TraverserType = MI355XTraverser<NDIMS, 64, T>;
   TraverserType getTraverser() const { return getTraverserWithStack<64>(); }
   // The same walker with a stack of STACK entries (an addition: the
   // reference's walk always has 64)
   template <int STACK>
-  MI355XTraverser<NDIMS, STACK> getTraverserWithStack() const {
+  MI355XTraverser<NDIMS, STACK, T> getTraverserWithStack() const {
     if (!m_bvh) throw mi355x_error(AXSQ_ERR_STATE, "BVH is not initialized");
     const axsq_node* nodes = nullptr;
     const std::int32_t* ids = nullptr;
     std::int32_t ninner = 0;
     ::axom::detail::check(axsq_bvh_device_arrays(m_bvh, &nodes, &ids, &ninner));
-    return MI355XTraverser<NDIMS, STACK>(nodes, ids, ninner);
+    return MI355XTraverser<NDIMS, STACK, T>(nodes, ids, ninner);
   }
 
  private:
-  void find(int kind, const double* x, const double* y, const double* z, int64_t stride,
+  void find(int kind, const T* x, const T* y, const T* z, int64_t stride,
             ArrayView<IndexType> offsets, ArrayView<IndexType> counts,
             Array<IndexType>& candidates, IndexType n) const {
     // LinearBVH.hpp:284-285
@@ -457,32 +545,43 @@ class MI355XLinearBVH {
     if (!m_bvh) throw mi355x_error(AXSQ_ERR_STATE, "BVH is not initialized");
     int32_t* cand = nullptr;
     int64_t total = 0;
-    ::axom::detail::check(axsq_bvh_find(m_bvh, kind, x, y, z, stride, n, m_tolerance,
-                                        offsets.data(), counts.data(), &cand, &total, nullptr));
+    if (F32)
+      ::axom::detail::check(axsq_bvh_find_f32(
+          m_bvh, kind, reinterpret_cast<const float*>(x), reinterpret_cast<const float*>(y),
+          reinterpret_cast<const float*>(z), stride, n, (float)m_tolerance, offsets.data(),
+          counts.data(), &cand, &total, nullptr));
+    else
+      ::axom::detail::check(axsq_bvh_find(
+          m_bvh, kind, reinterpret_cast<const double*>(x), reinterpret_cast<const double*>(y),
+          reinterpret_cast<const double*>(z), stride, n, (double)m_tolerance, offsets.data(),
+          counts.data(), &cand, &total, nullptr));
     candidates.adopt(cand, (IndexType)total);
   }
 
  protected:
   axsq_bvh* m_bvh = nullptr;
-  double m_tolerance = DEFAULT_TOLERANCE;
-  double m_scaleFactor = DEFAULT_SCALE_FACTOR;
+  T m_tolerance = DEFAULT_TOLERANCE;
+  T m_scaleFactor = DEFAULT_SCALE_FACTOR;
   int m_fpMode = AXSQ_FP64;
   int m_allocatorID = execution_space<MI355X_EXEC>::allocatorID();
 };
 }  // namespace detail
 
-// spin::BVH<3, MI355X_EXEC, double> and spin::BVH<2, MI355X_EXEC, double>
-template <int NDIMS>
-class BVH<NDIMS, MI355X_EXEC, double> : public detail::MI355XLinearBVH<NDIMS> {
+// spin::BVH<3 or 2, MI355X_EXEC, double> and spin::BVH<3 or 2, MI355X_EXEC,
+// float> (BVH.hpp:129-130: FloatType is the box, point and ray coordinate
+// type, and the type the build and the ray test compute in)
+template <int NDIMS, typename FloatType>
+class BVH<NDIMS, MI355X_EXEC, FloatType> : public detail::MI355XLinearBVH<NDIMS, FloatType> {
   static_assert(NDIMS == 2 || NDIMS == 3, "spin::BVH is 2-D or 3-D");
-  using Base = detail::MI355XLinearBVH<NDIMS>;
+  using Base = detail::MI355XLinearBVH<NDIMS, FloatType>;
 
  public:
   using typename Base::BoxType;
   BVH() = default;
   BVH(const BoxType* boxes, IndexType numItems,
       int allocatorID = execution_space<MI355X_EXEC>::allocatorID(),
-      double tolerance = Base::DEFAULT_TOLERANCE, double scaleFactor = Base::DEFAULT_SCALE_FACTOR) {
+      FloatType tolerance = Base::DEFAULT_TOLERANCE,
+      FloatType scaleFactor = Base::DEFAULT_SCALE_FACTOR) {
     this->setAllocatorID(allocatorID);
     this->m_tolerance = tolerance;
     this->m_scaleFactor = scaleFactor;

@@ -6,6 +6,8 @@
 //   spin/tests/spin_bvh.cpp:409-517, :651-790, :906-1006 (the 2-D versions)
 //   quest/tests/quest_meshtester.cpp:268-305 (caved-in tetrahedron, with and
 //   without degenerate triangles; meshes of quest_test_utilities.hpp:353-442)
+// The BVH tests run with FloatType double and float, as the reference's do
+// (spin_bvh.cpp:1560-1669).
 // Exit status 0 when every check passes.
 #include <algorithm>
 #include <cmath>
@@ -46,12 +48,24 @@ static std::vector<T> to_host(const T* d, size_t n) {
   return h;
 }
 
+template <int N, typename T>
+using TypesOf = axom::spin::detail::BVHTypes<N, T>;
+
 // 3 x 3 x 3 unit cells over [0,3]^3, i fastest (mint::UniformMesh order).
-static std::vector<Box> grid_boxes() {
-  std::vector<Box> b;
+template <typename BoxT = Box>
+static std::vector<BoxT> grid_boxes() {
+  std::vector<BoxT> b;
   for (int k = 0; k < 3; ++k)
     for (int j = 0; j < 3; ++j)
-      for (int i = 0; i < 3; ++i) b.push_back(Box{{1.0 * i, 1.0 * j, 1.0 * k}, {i + 1.0, j + 1.0, k + 1.0}});
+      for (int i = 0; i < 3; ++i) {
+        BoxT x;
+        const int c[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d) {
+          x.min[d] = c[d];
+          x.max[d] = c[d] + 1;
+        }
+        b.push_back(x);
+      }
   return b;
 }
 
@@ -68,8 +82,12 @@ static std::set<IndexType> set_of(const Found& f, int i) {
   return std::set<IndexType>(f.cand.begin() + f.off[i], f.cand.begin() + f.off[i] + f.cnt[i]);
 }
 
+template <typename T>
 static void test_bvh_grid() {
-  std::vector<Box> boxes = grid_boxes();
+  using Box = typename TypesOf<3, T>::BoxType;
+  using Pt = typename TypesOf<3, T>::PointType;
+  using BVH = axom::spin::BVH<3, axom::MI355X_EXEC, T>;
+  std::vector<Box> boxes = grid_boxes<Box>();
   Box* d_boxes = to_device(boxes);
   BVH bvh;
   bvh.setScaleFactor(1.0);
@@ -80,7 +98,8 @@ static void test_bvh_grid() {
     EXPECT_NEAR(b.max[d], 3.0, 1e-6);
   }
   // findBoundingBoxes
-  std::vector<Box> q = {Box{{-1, -1, -1}, {2.5, 2.5, 1.5}}, Box{{-1, -1, -1}, {-0.5, -0.5, -0.5}}};
+  std::vector<Box> q = {Box{{-1, -1, -1}, {T(2.5), T(2.5), T(1.5)}},
+                        Box{{-1, -1, -1}, {T(-0.5), T(-0.5), T(-0.5)}}};
   Box* d_q = to_device(q);
   Found fb = run_find(2, [&](auto o, auto c, auto& cand) { bvh.findBoundingBoxes(o, c, cand, 2, d_q); });
   EXPECT_EQ(fb.cnt[0], 18);
@@ -88,8 +107,9 @@ static void test_bvh_grid() {
   std::set<IndexType> s = set_of(fb, 0);
   for (IndexType i = 0; i < 18; ++i) EXPECT_TRUE(s.count(i) == 1);
   // findRays
-  const double o[3] = {-1, -1, -1}, d1[3] = {1, 1, 1}, d2[3] = {-1, -1, -1};
-  std::vector<axom::primal::Ray3> rays = {axom::primal::make_ray(o, d1), axom::primal::make_ray(o, d2)};
+  const T o[3] = {-1, -1, -1}, d1[3] = {1, 1, 1}, d2[3] = {-1, -1, -1};
+  std::vector<typename TypesOf<3, T>::RayType> rays = {axom::primal::make_ray(o, d1),
+                                                       axom::primal::make_ray(o, d2)};
   auto* d_rays = to_device(rays);
   Found fr = run_find(2, [&](auto oo, auto c, auto& cand) { bvh.findRays(oo, c, cand, 2, d_rays); });
   EXPECT_EQ(fr.cnt[0], 15);
@@ -98,7 +118,8 @@ static void test_bvh_grid() {
   // findPoints: centroids, then shifted by +10
   std::vector<Pt> c, cs;
   for (const Box& bx : boxes) {
-    Pt p{{0.5 * (bx.min[0] + bx.max[0]), 0.5 * (bx.min[1] + bx.max[1]), 0.5 * (bx.min[2] + bx.max[2])}};
+    Pt p{{T(0.5 * (bx.min[0] + bx.max[0])), T(0.5 * (bx.min[1] + bx.max[1])),
+          T(0.5 * (bx.min[2] + bx.max[2]))}};
     c.push_back(p);
     cs.push_back(Pt{{p.x[0] + 10, p.x[1] + 10, p.x[2] + 10}});
   }
@@ -238,19 +259,20 @@ static void test_mesh_tester_caved_tet() {
 // spin::BVH<2>: 3 x 3 unit cells over [0,3]^2 (i fastest). The box
 // [-1,2.5] x [-1,1.5] meets cells 0-5, the ray from (-1,-1) along (1,1) meets
 // all but cells 2 and 6, and each cell centroid finds its own cell.
+template <typename T>
 static void test_bvh2d_grid() {
-  using BVH2 = axom::spin::BVH<2, axom::MI355X_EXEC>;
-  using Box2 = axom::primal::BoundingBox2;
+  using BVH2 = axom::spin::BVH<2, axom::MI355X_EXEC, T>;
+  using Box2 = typename TypesOf<2, T>::BoxType;
   std::vector<Box2> cells;
   for (int j = 0; j < 3; ++j)
-    for (int i = 0; i < 3; ++i) cells.push_back(Box2{{1.0 * i, 1.0 * j}, {i + 1.0, j + 1.0}});
+    for (int i = 0; i < 3; ++i) cells.push_back(Box2{{T(i), T(j)}, {T(i + 1), T(j + 1)}});
   Box2* dcells = to_device(cells);
   BVH2 bvh;
   bvh.setScaleFactor(1.0);
   bvh.initialize(dcells, 9);
   const Box2 bb = bvh.getBounds();
   EXPECT_TRUE(bb.min[0] == 0.0 && bb.min[1] == 0.0 && bb.max[0] == 3.0 && bb.max[1] == 3.0);
-  std::vector<Box2> q = {Box2{{-1.0, -1.0}, {2.5, 1.5}}, Box2{{-1.0, -1.0}, {-0.1, -0.1}}};
+  std::vector<Box2> q = {Box2{{-1, -1}, {T(2.5), T(1.5)}}, Box2{{-1, -1}, {T(-0.1), T(-0.1)}}};
   Box2* dq = to_device(q);
   {
     axom::Array<IndexType> off(2), cnt(2), cand;
@@ -261,9 +283,9 @@ static void test_bvh2d_grid() {
     EXPECT_TRUE((std::set<IndexType>(ids.begin(), ids.end()) == std::set<IndexType>{0, 1, 2, 3, 4, 5}));
   }
   {
-    const double o[2] = {-1.0, -1.0}, d0[2] = {1.0, 1.0}, d1[2] = {-1.0, -1.0};
-    std::vector<axom::primal::Ray2> rays = {axom::primal::make_ray2(o, d0),
-                                            axom::primal::make_ray2(o, d1)};
+    const T o[2] = {-1, -1}, d0[2] = {1, 1}, d1[2] = {-1, -1};
+    std::vector<typename TypesOf<2, T>::RayType> rays = {axom::primal::make_ray2(o, d0),
+                                                         axom::primal::make_ray2(o, d1)};
     auto* dr = to_device(rays);
     axom::Array<IndexType> off(2), cnt(2), cand;
     bvh.findRays(off.view(), cnt.view(), cand, 2, dr);
@@ -274,8 +296,9 @@ static void test_bvh2d_grid() {
     (void)hipFree(dr);
   }
   {
-    std::vector<axom::primal::Point2> pts;
-    for (auto& c : cells) pts.push_back({{0.5 * (c.min[0] + c.max[0]), 0.5 * (c.min[1] + c.max[1])}});
+    std::vector<typename TypesOf<2, T>::PointType> pts;
+    for (auto& c : cells)
+      pts.push_back({{T(0.5 * (c.min[0] + c.max[0])), T(0.5 * (c.min[1] + c.max[1]))}});
     auto* dp = to_device(pts);
     axom::Array<IndexType> off(9), cnt(9), cand;
     bvh.findPoints(off.view(), cnt.view(), cand, 9, dp);
@@ -294,7 +317,7 @@ static void test_bvh2d_grid() {
 // same order as findBoundingBoxes; a point query (nearer child first) the same
 // set as findPoints.
 constexpr int kTravMax = 64;
-template <typename Trav>
+template <typename Trav, typename Box = typename Trav::BoxType>
 __global__ void k_trav_boxes(Trav tr, const Box* q, int n, int* counts, int* cand,
                              int* complete = nullptr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -314,7 +337,8 @@ __global__ void k_trav_boxes(Trav tr, const Box* q, int n, int* counts, int* can
   counts[i] = c;
   if (complete) complete[i] = done ? 1 : 0;
 }
-template <typename Trav>
+template <typename Trav, typename Box = typename Trav::BoxType,
+          typename Pt = typename Trav::PointType>
 __global__ void k_trav_points(Trav tr, const Pt* q, int n, int* counts, int* cand) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -333,7 +357,11 @@ __global__ void k_trav_points(Trav tr, const Pt* q, int n, int* counts, int* can
   counts[i] = c;
 }
 
+template <typename T>
 static void test_traverser_matches_find() {
+  using Box = typename TypesOf<3, T>::BoxType;
+  using Pt = typename TypesOf<3, T>::PointType;
+  using BVH = axom::spin::BVH<3, axom::MI355X_EXEC, T>;
   // 4,000 boxes and 3,000 box / point queries from a fixed LCG
   unsigned long long st = 12345;
   auto rnd = [&]() {
@@ -344,8 +372,8 @@ static void test_traverser_matches_find() {
   for (Box& b : boxes)
     for (int d = 0; d < 3; ++d) {
       const double c = rnd(), h = 0.01 + 0.04 * rnd();
-      b.min[d] = c - h;
-      b.max[d] = c + h;
+      b.min[d] = T(c - h);
+      b.max[d] = T(c + h);
     }
   Box* d_boxes = to_device(boxes);
   BVH bvh;
@@ -356,9 +384,9 @@ static void test_traverser_matches_find() {
   for (int i = 0; i < n; ++i)
     for (int d = 0; d < 3; ++d) {
       const double c = rnd(), h = 0.02 * rnd();
-      qb[i].min[d] = c - h;
-      qb[i].max[d] = c + h;
-      qp[i].x[d] = rnd();
+      qb[i].min[d] = T(c - h);
+      qb[i].max[d] = T(c + h);
+      qp[i].x[d] = T(rnd());
     }
   Box* d_qb = to_device(qb);
   Pt* d_qp = to_device(qp);
@@ -615,12 +643,15 @@ static void test_signed_distance_interface() {
 }
 
 int main() {
-  test_bvh_grid();
+  test_bvh_grid<double>();
+  test_bvh_grid<float>();
   test_traverser_reports_a_full_stack();
   test_allocator_and_vtk();
   test_signed_distance_interface();
-  test_traverser_matches_find();
-  test_bvh2d_grid();
+  test_traverser_matches_find<double>();
+  test_traverser_matches_find<float>();
+  test_bvh2d_grid<double>();
+  test_bvh2d_grid<float>();
   test_sdf_sphere();
   test_mesh_tester_caved_tet();
   std::printf("%s (%d failures)\n", g_fail ? "FAILED" : "PASSED", g_fail);
