@@ -316,10 +316,6 @@ __device__ __forceinline__ int closest_point_ilp(const double P[3], const double
   const double vc = d1 * d4 - d3 * d2;
   const double vb = d5 * d2 - d1 * d6;
   const double va = d3 * d6 - d5 * d4;
-  const double v_ab = d1 / (d1 - d3);
-  const double w_ac = d2 / (d2 - d6);
-  const double w_bc = (d4 - d3) / ((d4 - d3) + (d5 - d6));
-  const double denom = 1.0 / (va + vb + vc);
   int loc;
   if (is_leq(d1, 0.0, EPS) && is_leq(d2, 0.0, EPS)) {
     loc = 0;
@@ -337,6 +333,22 @@ __device__ __forceinline__ int closest_point_ilp(const double P[3], const double
   } else {
     loc = 3;
   }
+  // The one division the region needs, with its own operands: the edge
+  // parameters d1 / (d1 - d3), d2 / (d2 - d6), (d4 - d3) / ((d4 - d3) +
+  // (d5 - d6)), or 1 / (va + vb + vc) for the face (the vertex regions use
+  // none). One division per lane instead of four, with no branch.
+  double num = 1.0, den = va + vb + vc;
+  if (loc == -1) {
+    num = d1;
+    den = d1 - d3;
+  } else if (loc == -3) {
+    num = d2;
+    den = d2 - d6;
+  } else if (loc == -2) {
+    num = d4 - d3;
+    den = (d4 - d3) + (d5 - d6);
+  }
+  const double quo = num / den;
   switch (loc) {
     case 0:
       cp[0] = A[0]; cp[1] = A[1]; cp[2] = A[2];
@@ -348,19 +360,19 @@ __device__ __forceinline__ int closest_point_ilp(const double P[3], const double
       cp[0] = C[0]; cp[1] = C[1]; cp[2] = C[2];
       break;
     case -1:
-      cp[0] = A[0] + ab0 * v_ab; cp[1] = A[1] + ab1 * v_ab; cp[2] = A[2] + ab2 * v_ab;
+      cp[0] = A[0] + ab0 * quo; cp[1] = A[1] + ab1 * quo; cp[2] = A[2] + ab2 * quo;
       break;
     case -3:
-      cp[0] = A[0] + ac0 * w_ac; cp[1] = A[1] + ac1 * w_ac; cp[2] = A[2] + ac2 * w_ac;
+      cp[0] = A[0] + ac0 * quo; cp[1] = A[1] + ac1 * quo; cp[2] = A[2] + ac2 * quo;
       break;
     case -2:
-      cp[0] = B[0] + (C[0] - B[0]) * w_bc;
-      cp[1] = B[1] + (C[1] - B[1]) * w_bc;
-      cp[2] = B[2] + (C[2] - B[2]) * w_bc;
+      cp[0] = B[0] + (C[0] - B[0]) * quo;
+      cp[1] = B[1] + (C[1] - B[1]) * quo;
+      cp[2] = B[2] + (C[2] - B[2]) * quo;
       break;
     default: {
-      const double v = vb * denom;
-      const double ww = vc * denom;
+      const double v = vb * quo;
+      const double ww = vc * quo;
       cp[0] = A[0] + (ab0 * v + ac0 * ww);
       cp[1] = A[1] + (ab1 * v + ac1 * ww);
       cp[2] = A[2] + (ab2 * v + ac2 * ww);
