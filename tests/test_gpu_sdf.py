@@ -193,12 +193,14 @@ def _sdf_with(monkeypatch, v, t, budget, **kw):
     return quest.SignedDistance(_t(v), _t(t), **kw)
 
 
-@pytest.mark.parametrize("budget", [1, 37, 400])
-def test_suspended_walks_resume_bit_exact(monkeypatch, budget):
+@pytest.mark.parametrize("budget,level,inner", [(1, 4, 0.05), (37, 4, 0.05), (400, 4, 0.05),
+                                                (1, 5, 0.3), (37, 5, 0.3)])
+def test_suspended_walks_resume_bit_exact(monkeypatch, budget, level, inner):
     """Walks cut off by a small step budget and finished by the second pass
-    give the oracle's answers: budget 1 sends every walk to the second pass."""
-    v, t, q = synth.workload2(nq=20_000, level=4)
-    q = np.concatenate([q, 0.05 * synth.uniform(11, 0, 6000, -1, 1).reshape(-1, 3)])
+    give the oracle's answers: budget 1 sends every walk to the second pass.
+    Queries inside the sphere (radius `inner`) make the long walks."""
+    v, t, q = synth.workload2(nq=20_000, level=level)
+    q = np.concatenate([q, inner * synth.uniform(11, 0, 9000, -1, 1).reshape(-1, 3)])
     sdf = _sdf_with(monkeypatch, v, t, budget)
     g = _eval(sdf, q)
     assert sdf.lastSuspended() > (q.shape[0] // 2 if budget == 1 else 0)
