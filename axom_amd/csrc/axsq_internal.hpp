@@ -112,16 +112,24 @@ inline int64_t scan_tiles(int64_t n) {
 }
 int scan_exclusive_i32_into(const int32_t* in, int32_t* out, int64_t n, int64_t* sums,
                             int64_t* total_dev, hipStream_t s);
-// Stable sort of at most kSmallSortMax (key, val) pairs by the low `bits`
-// (<= 32) bits, for a caller that builds the first pass's histogram itself:
-// hists holds one tile-major histogram per 8-bit pass (256 counts per tile of
-// kSmallSortTile keys, ceil(n / kSmallSortTile) tiles each), the first filled
-// and the others zero; each pass's scatter adds the next pass's counts.
+// Stable sort of 2 .. kSmallSortMax (key, val) pairs by the low `bits` (9 to
+// 32) bits for the BVH build, in 8-bit passes over tiles of kSmallSortTile
+// keys. hists holds one tile-major histogram per pass (256 * hs words,
+// hs = small_sort_stride(n) tiles; hist[tile * 256 + digit]): the first
+// filled by the caller, the others zero (each pass's scatter adds the next
+// pass's counts). The last pass
+// writes the sorted keys to keys_last and, at each sorted position p, the
+// key's value to vals_last[p] and row vals_last[p] of `rows` (6 doubles) to
+// rows_out[p]; keys/vals and their alternates are scratch.
 constexpr int64_t kSmallSortTile = 1024;
 constexpr int64_t kSmallSortMax = 256 * kSmallSortTile;
-int radix_sort_small_prehist(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
-                             uint32_t* vals_alt, int64_t n, int bits, uint32_t* hists,
-                             hipStream_t s, uint32_t** keys_out, uint32_t** vals_out);
+inline int32_t small_sort_stride(int64_t n) {
+  return (int32_t)((((n + kSmallSortTile - 1) / kSmallSortTile) + 3) & ~int64_t(3));
+}
+int radix_sort_small_build(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                           uint32_t* vals_alt, int64_t n, int bits, uint32_t* hists,
+                           const double* rows, double* rows_out, uint32_t* keys_last,
+                           int32_t* vals_last, hipStream_t s);
 // Stable ascending sort of (keys, vals) by the low `bits` bits of the key.
 // On return the sorted data is in (*keys_out, *vals_out), which point at
 // either the input arrays or the alternates.
@@ -160,6 +168,10 @@ struct axsq_bvh {
   bool stats = false;
   unsigned long long* find_visits = nullptr;  // device: inner-node visits summed over queries
   int64_t last_visits = 0;                    // host: the last count pass's sum
+  // builds of at most kSmallSortMax boxes replay a HIP graph of their
+  // launches while every pointer and size they were captured with holds
+  hipGraphExec_t build_graph = nullptr;
+  std::vector<uint64_t> build_graph_key;
 };
 
 struct axsq_sdf {

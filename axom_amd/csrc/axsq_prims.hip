@@ -264,30 +264,133 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
   }
 }
 
+// The build's small sort (at most kSmallSortMax keys, tiles of
+// kSmallSortTile = 1024 keys, one 1024-thread workgroup each, one key per
+// thread): a stable scatter of one digit. hist is the pass's tile-major
+// histogram (hist[tile * 256 + d], nb tiles): where digit d starts in this
+// tile is the keys of smaller digits everywhere plus those of digit d in
+// earlier tiles, summed by four threads per digit over a quarter of the
+// tiles each (coalesced across digits, 32 loads in flight per thread). Each
+// pass adds the next pass's counts to next_hist as it writes. The last pass
+// (LAST) writes the sorted keys to keys_out and, at each sorted position p,
+// vals_out[p] = the key's value and the six doubles of row vals_out[p] of
+// `rows` to rows_out[p] (the leaves in tree order).
+template <bool LAST>
+__global__ __launch_bounds__(1024) void k_small_scatter(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, int64_t n, int shift,
+    int nbits, const uint32_t* __restrict__ hist, int32_t nb,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+    uint32_t* __restrict__ next_hist, int next_shift, int next_nbits,
+    const double* __restrict__ rows, double* __restrict__ rows_out) {
+  static_assert(kSmallSortTile == 1024, "one key per thread");
+  constexpr int kW = 1024 / kWave;
+  __shared__ uint32_t s_part[2][4][256];  // per quarter of the tiles: all, before this tile
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_wcnt[kW][256];
+  const int t = threadIdx.x;
+  const int wid = t / kWave;
+  const int64_t i = (int64_t)blockIdx.x * 1024 + t;
+  const bool valid = i < n;
+  const uint32_t key = valid ? keys[i] : 0u, val = valid ? vals[i] : 0u;
+  double2 rw[3];
+  if constexpr (LAST) {
+    const double2* src = reinterpret_cast<const double2*>(rows + 6 * (int64_t)val);
+    for (int e = 0; e < 3; ++e) rw[e] = valid ? src[e] : make_double2(0.0, 0.0);
+  }
+  {
+    const int q = t / 256, d = t % 256;
+    const int32_t per = (nb + 3) / 4;  // at most 64: one or two rounds of 32 loads
+    const int32_t b0 = q * per, me = (int32_t)blockIdx.x;
+    uint32_t tot = 0, pre = 0;
+    for (int32_t u0 = 0; u0 < per; u0 += 32) {
+      uint32_t x[32];
+#pragma unroll
+      for (int u = 0; u < 32; ++u) {
+        const int32_t b = b0 + u0 + u;
+        x[u] = (u0 + u < per && b < nb) ? hist[(int64_t)b * 256 + d] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 32; ++u) {
+        tot += x[u];
+        pre += (b0 + u0 + u < me) ? x[u] : 0u;
+      }
+    }
+    s_part[0][q][d] = tot;
+    s_part[1][q][d] = pre;
+  }
+  for (int j = t; j < kW * 256; j += 1024) (&s_wcnt[0][0])[j] = 0u;
+  __syncthreads();
+  uint32_t tot = 0, pre = 0, inc = 0;
+  if (t < 256) {
+    for (int q = 0; q < 4; ++q) {
+      tot += s_part[0][q][t];
+      pre += s_part[1][q][t];
+    }
+    inc = wave_inclusive(tot);
+    if (__lane_id() == kWave - 1) s_wsum[wid] = inc;
+  }
+  __syncthreads();
+  if (t < 256) {
+    uint32_t b = 0;
+    for (int w = 0; w < wid; ++w) b += s_wsum[w];
+    s_base[t] = b + inc - tot + pre;
+  }
+  const uint32_t mask = (1u << nbits) - 1u;
+  const uint32_t d = valid ? (key >> shift) & mask : 0u;
+  uint64_t peers = __ballot(valid);
+  for (int b = 0; b < nbits; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bb = __ballot(valid && bit);
+    peers &= bit ? bb : ~bb;
+  }
+  const uint64_t lt = lanemask_lt();
+  const uint32_t rank = (uint32_t)__popcll(peers & lt);
+  if (valid && (peers & lt) == 0) s_wcnt[wid][d] = (uint32_t)__popcll(peers);
+  __syncthreads();
+  if (valid) {
+    uint32_t pos = s_base[d] + rank;
+    for (int w = 0; w < wid; ++w) pos += s_wcnt[w][d];
+    keys_out[pos] = key;
+    vals_out[pos] = val;
+    if constexpr (LAST) {
+      double2* dst = reinterpret_cast<double2*>(rows_out + 6 * (int64_t)pos);
+      for (int e = 0; e < 3; ++e) dst[e] = rw[e];
+    } else {
+      atomicAdd(&next_hist[(int64_t)(pos / kSmallSortTile) * 256 +
+                           ((key >> next_shift) & ((1u << next_nbits) - 1u))],
+                1u);
+    }
+  }
+}
+
 }  // namespace
 
-int radix_sort_small_prehist(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
-                             uint32_t* vals_alt, int64_t n, int bits, uint32_t* hists,
-                             hipStream_t s, uint32_t** keys_out, uint32_t** vals_out) {
-  *keys_out = keys;
-  *vals_out = vals;
-  if (n <= 1) return AXSQ_OK;
-  if (n > kSmallSortMax) {
-    set_error("radix sort: %lld keys exceed the small sort's %lld", (long long)n,
-              (long long)kSmallSortMax);
+int radix_sort_small_build(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
+                           uint32_t* vals_alt, int64_t n, int bits, uint32_t* hists,
+                           const double* rows, double* rows_out, uint32_t* keys_last,
+                           int32_t* vals_last, hipStream_t s) {
+  if (n < 1 || n > kSmallSortMax || bits <= 8 || bits > 32) {
+    set_error("radix sort: %lld keys of %d bits do not fit the small sort", (long long)n, bits);
     return AXSQ_ERR_INVALID_ARG;
   }
-  const int64_t nb = (n + kSmallSortTile - 1) / kSmallSortTile;
+  const int32_t nb = (int32_t)((n + kSmallSortTile - 1) / kSmallSortTile);
+  const int32_t hs = small_sort_stride(n);
   uint32_t *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
   int pass = 0;
   for (int shift = 0; shift < bits; shift += 8, ++pass) {
     const int nbits = (bits - shift < 8) ? bits - shift : 8;
-    const int next = shift + 8;
-    uint32_t* next_hist = (next < bits) ? hists + (int64_t)(pass + 1) * 256 * nb : nullptr;
-    const int next_nbits = (next < bits) ? ((bits - next < 8) ? bits - next : 8) : 0;
-    k_radix_scatter<4, false><<<(unsigned)nb, kBlock, 0, s>>>(
-        ki, vi, n, shift, nbits, hists + (int64_t)pass * 256 * nb, nb, ko, vo, next_hist, next,
-        next_nbits);
+    const uint32_t* h = hists + (int64_t)pass * 256 * hs;
+    if (shift + 8 < bits) {
+      const int next = shift + 8;
+      k_small_scatter<false><<<(unsigned)nb, 1024, 0, s>>>(
+          ki, vi, n, shift, nbits, h, nb, ko, vo, hists + (int64_t)(pass + 1) * 256 * hs, next,
+          (bits - next < 8) ? bits - next : 8, nullptr, nullptr);
+    } else {
+      k_small_scatter<true><<<(unsigned)nb, 1024, 0, s>>>(
+          ki, vi, n, shift, nbits, h, nb, keys_last, reinterpret_cast<uint32_t*>(vals_last),
+          nullptr, 0, 0, rows, rows_out);
+    }
     AXSQ_CHECK_HIP(hipGetLastError());
     uint32_t* t = ki;
     ki = ko;
@@ -296,8 +399,6 @@ int radix_sort_small_prehist(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt,
     vi = vo;
     vo = t;
   }
-  *keys_out = ki;
-  *vals_out = vi;
   return AXSQ_OK;
 }
 
