@@ -192,17 +192,20 @@ __device__ void finish_params(const double* __restrict__ partials, int64_t npart
   }
 }
 
+// prm: device copy; host_out (pinned host memory): a copy for the host.
 __global__ __launch_bounds__(1024) void k_bounds(const double* __restrict__ partials,
                                                   int64_t nparts, const double* __restrict__ boxes,
                                                   int64_t n, BuildParams* __restrict__ prm,
-                                                  int f32) {
+                                                  BuildParams* __restrict__ host_out, int f32) {
   __shared__ double s_red[6][1024];
   BuildParams p;
   int quirky;
   finish_params(partials, nparts, boxes, n, f32, s_red, p, quirky);
   if (threadIdx.x == 0) {
+    p.quirky = quirky;
+    p.pad = 0;
     *prm = p;
-    prm->quirky = quirky;
+    *host_out = p;
   }
 }
 
@@ -238,7 +241,8 @@ __global__ __launch_bounds__(kBlock) void k_mcodes(const double* __restrict__ bo
 }
 
 // Small builds: the bounds and constants (every workgroup finds the same
-// ones from the transform's partials; workgroup 0 stores them), then the
+// ones from the transform's partials; workgroup 0 writes them to host_out,
+// pinned host memory), then the
 // code of box i = 1024 * workgroup + thread, and the sort's first digit
 // histogram: workgroup g is sort tile g, so its counts go straight to
 // hist0[g * 256 + digit].
@@ -246,7 +250,7 @@ __global__ __launch_bounds__(1024) void k_mcodes_small(const double* __restrict_
                                                         int64_t nparts,
                                                         const double* __restrict__ boxes,
                                                         int64_t n, int dims, int f32,
-                                                        BuildParams* __restrict__ prm,
+                                                        BuildParams* __restrict__ host_out,
                                                         uint32_t* __restrict__ keys,
                                                         uint32_t* __restrict__ vals,
                                                         uint32_t* __restrict__ hist0) {
@@ -261,8 +265,9 @@ __global__ __launch_bounds__(1024) void k_mcodes_small(const double* __restrict_
   finish_params(partials, nparts, boxes, n, f32, s_red, p, quirky);
   if (t == 0) {
     if (blockIdx.x == 0) {
-      *prm = p;
-      prm->quirky = quirky;
+      p.quirky = quirky;
+      p.pad = 0;
+      *host_out = p;
     }
     for (int d = 0; d < 3; ++d) {
       s_prm[d] = p.minc[d];
@@ -1085,6 +1090,8 @@ void bvh_release(axsq_bvh* b) {
   if (b->build_graph) hipGraphExecDestroy(b->build_graph);
   b->build_graph = nullptr;
   b->build_graph_key.clear();
+  if (b->host_params) hipHostFree(b->host_params);
+  b->host_params = b->host_params_dev = nullptr;
   hipFree(b->nodes);
   hipFree(b->nodes32);
   hipFree(b->ranges32);
@@ -1233,6 +1240,13 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
     return AXSQ_OK;
   };
 
+  if (!b->host_params) {
+    void* hp = nullptr;
+    AXSQ_CHECK_HIP(hipHostMalloc(&hp, sizeof(BuildParams), hipHostMallocDefault));
+    b->host_params = hp;
+    AXSQ_CHECK_HIP(hipHostGetDevicePointer(&b->host_params_dev, hp, 0));
+  }
+  BuildParams* host_out = static_cast<BuildParams*>(b->host_params_dev);
   ProfRange pr("bvh_build", s);
   const int bits = dims == 2 ? 32 : 30;
   uint32_t *keys = ws.k0.as<uint32_t>(), *vals = ws.v0.as<uint32_t>();
@@ -1255,8 +1269,8 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
                                                                partials, hists + 256 * hs,
                                                                nhist - 256 * hs);
       AXSQ_CHECK_HIP(hipGetLastError());
-      k_mcodes_small<<<nkb, 1024, 0, q>>>(partials, nparts, scaled, n, dims, f32 ? 1 : 0, prm,
-                                          keys, vals, hists);
+      k_mcodes_small<<<nkb, 1024, 0, q>>>(partials, nparts, scaled, n, dims, f32 ? 1 : 0,
+                                          host_out, keys, vals, hists);
       AXSQ_CHECK_HIP(hipGetLastError());
       AXSQ_TRY(radix_sort_small_build(keys, vals, ws.k1.as<uint32_t>(), ws.v1.as<uint32_t>(), n,
                                       bits, hists, scaled, b->leaf_boxes, b->mcodes, b->leaf_ids,
@@ -1286,7 +1300,7 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
         (uint64_t)(uintptr_t)b->mcodes, (uint64_t)(uintptr_t)b->status,
         (uint64_t)(uintptr_t)base, (uint64_t)(uintptr_t)hists, (uint64_t)(uintptr_t)keys,
         (uint64_t)(uintptr_t)vals, (uint64_t)(uintptr_t)ws.k1.as<uint32_t>(),
-        (uint64_t)(uintptr_t)ws.v1.as<uint32_t>()};
+        (uint64_t)(uintptr_t)ws.v1.as<uint32_t>(), (uint64_t)(uintptr_t)host_out};
     const char* genv = getenv("AXSQ_BUILD_GRAPH");
     const bool use_graph = !(genv && genv[0] == '0');
     if (use_graph && !(b->build_graph && b->build_graph_key == key)) {
@@ -1322,7 +1336,7 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
       k_transform<false><<<(unsigned)nparts, kBlock, 0, s>>>(boxes, 3, n_in, n, scale, scaled,
                                                              partials, nullptr, 0);
     AXSQ_CHECK_HIP(hipGetLastError());
-    k_bounds<<<1, 1024, 0, s>>>(partials, nparts, scaled, n, prm, f32 ? 1 : 0);
+    k_bounds<<<1, 1024, 0, s>>>(partials, nparts, scaled, n, prm, host_out, f32 ? 1 : 0);
     AXSQ_CHECK_HIP(hipGetLastError());
     k_mcodes<<<grid_for(n), kBlock, 0, s>>>(scaled, n, prm, dims, f32 ? 1 : 0, keys, vals);
     AXSQ_CHECK_HIP(hipGetLastError());
@@ -1338,8 +1352,7 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
     AXSQ_TRY(climb_and_emit());
   }
   pr.stop();
-  BuildParams hp;
-  AXSQ_CHECK_HIP(hipMemcpyAsync(&hp, prm, sizeof(BuildParams), hipMemcpyDeviceToHost, s));
+  const BuildParams& hp = *static_cast<const BuildParams*>(b->host_params);
   AXSQ_CHECK_HIP(hipStreamSynchronize(s));
   for (int k = 0; k < 6; ++k) b->bounds[k] = hp.bounds[k];
   if (small && hp.quirky) {

@@ -172,6 +172,10 @@ struct axsq_bvh {
   // launches while every pointer and size they were captured with holds
   hipGraphExec_t build_graph = nullptr;
   std::vector<uint64_t> build_graph_key;
+  // pinned host memory the build's bounds kernel writes its parameters to
+  // (the bounds and the signed-zero flag), read after the stream sync
+  void* host_params = nullptr;       // host pointer
+  void* host_params_dev = nullptr;   // the same memory as the device sees it
 };
 
 struct axsq_sdf {
