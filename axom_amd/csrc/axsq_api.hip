@@ -838,6 +838,14 @@ static int64_t sdf_suspend_cap() {
   return -1;
 }
 
+// AXSQ_SDF_MIXED_PASS2=fp32: the FP32-traversal form's second pass walks the
+// FP32 nodes too (by default it resumes the saved walks on the FP64 nodes).
+// Speed only: either way the pruning is exact.
+static bool sdf_mixed_pass2_fp32() {
+  const char* e = getenv("AXSQ_SDF_MIXED_PASS2");
+  return e && strcmp(e, "fp32") == 0;
+}
+
 int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const double* z,
                     int64_t nverts, const int32_t* tris, int64_t ntris, int watertight,
                     int compute_sign, int fp_mode, void* stream) {
@@ -871,6 +879,7 @@ int axsq_sdf_create(axsq_sdf** out, const double* x, const double* y, const doub
   h->sort_queries = !(env && env[0] == '0');
   h->budget = sdf_budget(ntris);
   h->susp_cap_fixed = sdf_suspend_cap();
+  h->mixed_pass2_fp32 = sdf_mixed_pass2_fp32();
   int rc = AXSQ_OK;
   auto fail = [&](int code) {
     axsq_sdf_destroy(h);

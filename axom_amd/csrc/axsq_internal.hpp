@@ -206,6 +206,9 @@ struct axsq_sdf {
   // AXSQ_SDF_SUSPEND_CAP at creation (susp_cap_fixed, -1 when unset)
   int64_t susp_demand = -1;
   int64_t susp_cap_fixed = -1;
+  // FP32-traversal form: second pass on the FP32 nodes too (AXSQ_SDF_MIXED_PASS2=fp32
+  // at creation); by default it resumes the saved walks on the FP64 nodes
+  bool mixed_pass2_fp32 = false;
 };
 
 namespace axsq {

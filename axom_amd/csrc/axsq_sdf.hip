@@ -48,6 +48,7 @@
 #ifndef AXSQ_HOT_WALKS
 #define AXSQ_HOT_WALKS 32
 #endif
+
 #define AXSQ_RESUME_WAVES_ATTR \
   __attribute__((amdgpu_waves_per_eu((MIXED || CELLS) ? AXSQ_RESUME_WAVES_MIXED : AXSQ_RESUME_WAVES)))
 #if defined(AXSQ_SDF_WAVES) && AXSQ_SDF_WAVES > 0
@@ -1824,9 +1825,14 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   // run longest) read tables of their leaf records and node keys
   a.nhot = 0;
   a.hot_leaf = a.hot_node = nullptr;
+  // The FP32-traversal form's second pass resumes the saved walks on the FP64
+  // nodes unless the handle asks for the FP32 ones: pruning from a saved
+  // state is exact either way, and on gfx950 the FP64 resume kernel runs 3
+  // waves per SIMD where the FP32 one, with its two box tests, fits 2.
+  const bool mixed2 = mixed && h->mixed_pass2_fp32;
   const int64_t nleaf = (int64_t)a.ninner + 1;
   const size_t hot_bytes = sizeof(HotLeaf) * nleaf +
-                           (mixed ? sizeof(HotNode<float>) : sizeof(HotNode<double>)) * a.ninner;
+                           (mixed2 ? sizeof(HotNode<float>) : sizeof(HotNode<double>)) * a.ninner;
   const int64_t nhot = std::min<int64_t>({nres, (int64_t)AXSQ_HOT_WALKS,
                                           (int64_t)((size_t(2) << 30) / hot_bytes)});
   if (nhot > 0 && !cells) {
@@ -1835,7 +1841,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     a.hot_leaf = ws.hot.as<char>();
     a.hot_node = ws.hot.as<char>() + sizeof(HotLeaf) * nleaf * (size_t)nhot;
     const dim3 gh(grid_for(nleaf + a.ninner), (unsigned)nhot);
-    if (mixed)
+    if (mixed2)
       k_hot_tables<true><<<gh, kBlock, 0, s>>>(a);
     else
       k_hot_tables<false><<<gh, kBlock, 0, s>>>(a);
@@ -1848,7 +1854,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
         k_sdf_resume_wave<false, true, true><<<g2, kWaveLanes, 0, s>>>(a, nres, nredo);
       else
         k_sdf_resume_wave<false, false, true><<<g2, kWaveLanes, 0, s>>>(a, nres, nredo);
-    } else if (mixed) {
+    } else if (mixed2) {
       k_sdf_resume_wave<true, false><<<g2, kWaveLanes, 0, s>>>(a, nres, nredo);
     } else if (stats) {
       k_sdf_resume_wave<false, true><<<g2, kWaveLanes, 0, s>>>(a, nres, nredo);

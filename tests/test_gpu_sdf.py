@@ -230,6 +230,28 @@ def test_suspended_walks_fp32_traversal_unchanged(monkeypatch):
         assert np.array_equal(one[k], two[k]), k
 
 
+@pytest.mark.parametrize("pass2", ["fp64", "fp32"])
+def test_fp32_traversal_second_pass_nodes(monkeypatch, pass2):
+    """The FP32-traversal form's saved walks resume on the FP64 nodes (the
+    default) or on the FP32 nodes (AXSQ_SDF_MIXED_PASS2=fp32). Either way the
+    distances equal the FP64 oracle's within 1e-12 relative, with the same
+    signs and closest points."""
+    v, t, q = synth.workload2(nq=30_000, level=4)
+    q = np.concatenate([q, 0.3 * synth.uniform(14, 0, 6000, -1, 1).reshape(-1, 3)])
+    if pass2 == "fp32":
+        monkeypatch.setenv("AXSQ_SDF_MIXED_PASS2", "fp32")
+    else:
+        monkeypatch.delenv("AXSQ_SDF_MIXED_PASS2", raising=False)
+    sdf = _sdf_with(monkeypatch, v, t, 37, fp_mode="fp32")
+    g = _eval(sdf, q, "fp32")
+    assert sdf.lastSuspended() > q.shape[0] // 10
+    o = orc.OracleSDF(v, t).eval(q, nthreads=8)
+    rel = np.abs(g["dist"] - o["dist"]) / np.maximum(np.abs(o["dist"]), 1e-300)
+    assert rel.max() <= 1e-12
+    assert np.array_equal(np.signbit(g["dist"]), np.signbit(o["dist"]))
+    np.testing.assert_allclose(g["cp"], o["cp"], rtol=0, atol=1e-12)
+
+
 def test_suspended_walks_stats_match_oracle(monkeypatch):
     v, t, q = synth.workload2(nq=20_000, level=4)
     sdf = _sdf_with(monkeypatch, v, t, 1)
