@@ -910,30 +910,34 @@ __global__ __launch_bounds__(kSegLeaves) void k_seg_local(
   emit_node((int32_t)k, lc, rc, lb, rb, lo, hi, split, nodes, nodes32, ranges32);
 }
 
-// Order-keeping union over the wave: lane 0 ends with lane 0 ++ lane 1 ++ ... ++ lane 63.
-__device__ __forceinline__ SegBox wave_union(SegBox v) {
-  const int lane = threadIdx.x % kWaveLanes;
-  for (int off = 1; off < kWaveLanes; off <<= 1) {
+// Order-keeping union over each half of the wave: lanes 0 and 32 end with
+// the union of their half's lanes in lane order.
+__device__ __forceinline__ SegBox half_wave_union(SegBox v) {
+  const int hl = threadIdx.x % (kWaveLanes / 2);
+  for (int off = 1; off < kWaveLanes / 2; off <<= 1) {
     const SegBox o = seg_shfl_down(v, off);
-    if ((lane & (2 * off - 1)) == 0) seg_combine(v, o);
+    if ((hl & (2 * off - 1)) == 0) seg_combine(v, o);
   }
   return v;
 }
 
 // The deferred nodes (their leaf range reaches outside one workgroup of
 // k_seg_local), one wave each (the waves split evenly over the workgroups'
-// lists): a child's leaf range [a, b] is the pieces the
-// bottom-up walk takes below level 9 (left side ascending, right side
-// descending; at most 9 each) and the run of level-9 nodes between them.
-// Lane j takes the j-th left piece, the j-th right piece from the right end
-// and up to 8 consecutive level-9 nodes of the run, all loaded at once; the
-// three order-keeping unions over the wave then give left ++ run ++ right.
+// lists), its two halves on the two children at once. A child's leaf range
+// [a, b] is the pieces the bottom-up walk takes below level 9 (left side
+// ascending, right side descending; at most 9 each) and the run of level-9
+// nodes between them. Lane j of a half takes the j-th left piece, the j-th
+// right piece from the right end and up to 16 consecutive level-9 nodes of
+// the run (loaded 8 at a time); three order-keeping unions over the half
+// then give left ++ run ++ right.
 __global__ __launch_bounds__(kBlock) void k_emit_seg(
     const double* __restrict__ leaf_boxes, const double* __restrict__ seg, int64_t P,
     const int32_t* __restrict__ range, const int32_t* __restrict__ deferred,
     const int32_t* __restrict__ ndeferred, int64_t nlists, Node64* __restrict__ nodes,
     Node32* __restrict__ nodes32, int4* __restrict__ ranges32) {
+  constexpr int kHalf = kWaveLanes / 2;
   const int lane = threadIdx.x % kWaveLanes;
+  const int half = lane / kHalf, hl = lane % kHalf;
   const int64_t wave = (int64_t)blockIdx.x * (kBlock / kWaveLanes) + threadIdx.x / kWaveLanes;
   // the waves split evenly over the lists: list g, entries q0, q0 + per, ...
   const int64_t per = ((int64_t)gridDim.x * (kBlock / kWaveLanes)) / nlists;
@@ -944,8 +948,8 @@ __global__ __launch_bounds__(kBlock) void k_emit_seg(
     load_box(x >= P ? leaf_boxes + 6 * (x - P) : seg + 6 * x, o.b);
     o.empty = false;
   };
+  // union of leaves [a, b] over this half of the wave (the result in its lane 0)
   auto range_box = [&](int64_t a, int64_t b, double out[6]) {
-    // the walk's pieces: counts first, then this lane's
     int64_t l = a + P, r = b + 1 + P;
     int nr = 0;
     for (int lev = 0; lev < kSegLog && l < r; ++lev) {
@@ -961,35 +965,37 @@ __global__ __launch_bounds__(kBlock) void k_emit_seg(
     int cl = 0, cr = 0;
     for (int lev = 0; lev < kSegLog && l < r; ++lev) {
       if (l & 1) {
-        if (cl == lane) xl = l;
+        if (cl == hl) xl = l;
         ++cl;
         ++l;
       }
       if (r & 1) {
         --r;
-        if (nr - 1 - cr == lane) xr = r;
+        if (nr - 1 - cr == hl) xr = r;
         ++cr;
       }
       l >>= 1;
       r >>= 1;
     }
-    const int64_t per = (m1 - m0 + kWaveLanes - 1) / kWaveLanes;  // at most 8
-    const int64_t j0 = m0 + per * lane, j1 = (j0 + per < m1) ? j0 + per : m1;
+    const int64_t per_lane = (m1 - m0 + kHalf - 1) / kHalf;  // at most 16
+    const int64_t j0 = m0 + per_lane * hl, j1 = (j0 + per_lane < m1) ? j0 + per_lane : m1;
     SegBox L, M, R, mb[8];
     L.empty = xl < 0;
     R.empty = xr < 0;
     if (xl >= 0) piece(xl, L);
     if (xr >= 0) piece(xr, R);
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (j0 + u < j1) piece(j0 + u, mb[u]);
     M.empty = true;
+    for (int64_t jb = j0; jb < j1; jb += 8) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (j0 + u < j1) seg_combine(M, mb[u]);
-    L = wave_union(L);
-    M = wave_union(M);
-    R = wave_union(R);
+      for (int u = 0; u < 8; ++u)
+        if (jb + u < j1) piece(jb + u, mb[u]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (jb + u < j1) seg_combine(M, mb[u]);
+    }
+    L = half_wave_union(L);
+    M = half_wave_union(M);
+    R = half_wave_union(R);
     seg_combine(L, M);
     seg_combine(L, R);
     for (int c = 0; c < 6; ++c) out[c] = L.b[c];
@@ -997,23 +1003,20 @@ __global__ __launch_bounds__(kBlock) void k_emit_seg(
   for (int64_t q = q0; q < nd; q += per) {
     const int32_t i = deferred[g * kSegLeaves + q];
     const int32_t lo = range[3 * i], hi = range[3 * i + 1], split = range[3 * i + 2];
-    double lb[6], rb[6];
-    int32_t lc, rc;
-    if (lo == split) {
-      load_box(leaf_boxes + 6 * (int64_t)split, lb);
-      lc = -(split + 1);
-    } else {
-      range_box(lo, split, lb);
-      lc = split;
+    // half 0: the left child [lo, split]; half 1: the right child [split + 1, hi]
+    const int32_t ca = half ? split + 1 : lo, cb = half ? hi : split;
+    double bx[6];
+    if (ca == cb)
+      load_box(leaf_boxes + 6 * (int64_t)ca, bx);
+    else
+      range_box(ca, cb, bx);
+    double rb[6];
+    for (int c = 0; c < 6; ++c) rb[c] = __shfl(bx[c], kHalf);
+    if (lane == 0) {
+      const int32_t lc = (lo == split) ? -(split + 1) : split;
+      const int32_t rc = (hi == split + 1) ? -(split + 2) : split + 1;
+      emit_node(i, lc, rc, bx, rb, lo, hi, split, nodes, nodes32, ranges32);
     }
-    if (hi == split + 1) {
-      load_box(leaf_boxes + 6 * (int64_t)(split + 1), rb);
-      rc = -(split + 2);
-    } else {
-      range_box(split + 1, hi, rb);
-      rc = split + 1;
-    }
-    if (lane == 0) emit_node(i, lc, rc, lb, rb, lo, hi, split, nodes, nodes32, ranges32);
   }
 }
 
