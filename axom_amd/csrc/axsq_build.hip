@@ -795,8 +795,8 @@ __global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf
 constexpr int kSegLeaves = 512;  // leaves per workgroup of k_seg_local
 constexpr int kSegLog = 9;
 
-// Small builds, one workgroup per 512 leaves (base = 512 * workgroup),
-// after k_karras: the workgroup puts together the segment tree of its leaves
+// Small builds, one 1,024-thread workgroup per 512 leaves (base = 512 *
+// workgroup), after k_karras: the workgroup puts together the segment tree of its leaves
 // in LDS (local node u in [1, 512) at level l covers leaves
 // base + [(u << l) - 512, ((u + 1) << l) - 512), leaf base + k sits at
 // 512 + k; levels 1-6 within each wave by shuffles, 7-9 in the first wave)
@@ -805,7 +805,7 @@ constexpr int kSegLog = 9;
 // workgroup's leaves is emitted with child boxes taken from LDS; the others
 // are listed for k_emit_seg, kSegLeaves slots per workgroup
 // (deferred[512 * workgroup + j], their number in ndeferred[workgroup]).
-__global__ __launch_bounds__(kSegLeaves) void k_seg_local(
+__global__ __launch_bounds__(2 * kSegLeaves) void k_seg_local(
     int32_t inner, const int32_t* __restrict__ range,
     const double* __restrict__ leaf_boxes, int64_t n, int64_t P, int levels,
     double* __restrict__ seg, Node64* __restrict__ nodes, Node32* __restrict__ nodes32,
@@ -816,32 +816,38 @@ __global__ __launch_bounds__(kSegLeaves) void k_seg_local(
   const int t = threadIdx.x;
   const int lane = t % kWaveLanes, wid = t / kWaveLanes;
   const int64_t base = (int64_t)blockIdx.x * kSegLeaves;
-  const int64_t k = base + t;  // leaf k, inner node k
-  if (k == 0) *status = 0;     // the tree's error flag starts clear
+  if (base + t == 0) *status = 0;  // the tree's error flag starts clear
   if (t == 0) s_ndef = 0;
-  SegBox v;
-  v.empty = k >= n;
-  if (!v.empty)
-    load_box(leaf_boxes + 6 * k, v.b);
-  else
-    invalid_box(v.b);
+  // node pairs: threads 2j and 2j + 1 take inner node base + j, its left and
+  // right child
+  const int64_t k = base + (t >> 1);
+  const int side = t & 1;
   int32_t lo = 0, hi = 0, split = 0;
   if (k < inner) {
     lo = range[3 * k];
     hi = range[3 * k + 1];
     split = range[3 * k + 2];
   }
-  for (int c = 0; c < 6; ++c) s_box[c][kSegLeaves + t] = v.b[c];
   // node u of local level l: stored at s_box[.][u] and, with a leaf, globally
   auto put = [&](const SegBox& x, int l, int u, int64_t first_leaf) {
     for (int c = 0; c < 6; ++c) s_box[c][u] = x.b[c];
     if (l <= levels && !x.empty) store_box(seg + 6 * ((P >> l) + (first_leaf >> l)), x.b);
   };
-  for (int l = 1; l <= 6; ++l) {
-    const SegBox o = seg_shfl_down(v, 1 << (l - 1));
-    if ((lane & ((1 << l) - 1)) == 0) {
-      seg_combine(v, o);
-      put(v, l, (kSegLeaves >> l) + (t >> l), k);
+  if (t < kSegLeaves) {  // leaf base + t (the first 8 waves)
+    const int64_t kl = base + t;
+    SegBox v;
+    v.empty = kl >= n;
+    if (!v.empty)
+      load_box(leaf_boxes + 6 * kl, v.b);
+    else
+      invalid_box(v.b);
+    for (int c = 0; c < 6; ++c) s_box[c][kSegLeaves + t] = v.b[c];
+    for (int l = 1; l <= 6; ++l) {
+      const SegBox o = seg_shfl_down(v, 1 << (l - 1));
+      if ((lane & ((1 << l) - 1)) == 0) {
+        seg_combine(v, o);
+        put(v, l, (kSegLeaves >> l) + (t >> l), kl);
+      }
     }
   }
   __syncthreads();  // the waves' level-6 nodes, local 8 .. 15 (wave w: 8 + w)
@@ -859,10 +865,10 @@ __global__ __launch_bounds__(kSegLeaves) void k_seg_local(
     }
   }
   const bool mine = k < inner && lo >= base && hi < base + kSegLeaves;
-  if (k < inner && !mine) deferred[base + atomicAdd(&s_ndef, 1)] = (int32_t)k;
+  if (side == 0 && k < inner && !mine) deferred[base + atomicAdd(&s_ndef, 1)] = (int32_t)k;
   __syncthreads();
   if (t == 0) ndeferred[blockIdx.x] = s_ndef;
-  if (!mine) return;
+  if (!mine) return;  // both threads of a pair leave together
   // union of local leaves [a, b] from LDS, pieces combined in leaf order
   auto local_box = [&](int32_t a, int32_t b, double out[6]) {
     int32_t l = a + kSegLeaves, r = b + kSegLeaves + 1;
@@ -891,23 +897,21 @@ __global__ __launch_bounds__(kSegLeaves) void k_seg_local(
     for (int c = 0; c < 6; ++c) out[c] = L.b[c];
   };
   const int32_t b0 = (int32_t)base;
-  double lb[6], rb[6];
-  int32_t lc, rc;
-  if (lo == split) {
-    for (int c = 0; c < 6; ++c) lb[c] = s_box[c][kSegLeaves + split - b0];
-    lc = -(split + 1);
+  // this thread's child: [lo, split] (side 0) or [split + 1, hi] (side 1)
+  const int32_t ca = side ? split + 1 : lo, cb = side ? hi : split;
+  double bx[6];
+  if (ca == cb) {
+    for (int c = 0; c < 6; ++c) bx[c] = s_box[c][kSegLeaves + ca - b0];
   } else {
-    local_box(lo - b0, split - b0, lb);
-    lc = split;
+    local_box(ca - b0, cb - b0, bx);
   }
-  if (hi == split + 1) {
-    for (int c = 0; c < 6; ++c) rb[c] = s_box[c][kSegLeaves + split + 1 - b0];
-    rc = -(split + 2);
-  } else {
-    local_box(split + 1 - b0, hi - b0, rb);
-    rc = split + 1;
+  double rb[6];
+  for (int c = 0; c < 6; ++c) rb[c] = __shfl_xor(bx[c], 1);
+  if (side == 0) {
+    const int32_t lc = (lo == split) ? -(split + 1) : split;
+    const int32_t rc = (hi == split + 1) ? -(split + 2) : split + 1;
+    emit_node((int32_t)k, lc, rc, bx, rb, lo, hi, split, nodes, nodes32, ranges32);
   }
-  emit_node((int32_t)k, lc, rc, lb, rb, lo, hi, split, nodes, nodes32, ranges32);
 }
 
 // Order-keeping union over each half of the wave: lanes 0 and 32 end with
@@ -1281,7 +1285,7 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
       k_karras<true><<<grid_for(inner), kBlock, 0, q>>>(b->mcodes, inner, lch, rch, par, rng,
                                                         nullptr, 0, nullptr, 0);
       AXSQ_CHECK_HIP(hipGetLastError());
-      k_seg_local<<<(unsigned)nseg, kSegLeaves, 0, q>>>(inner, rng, b->leaf_boxes, n, P, levels,
+      k_seg_local<<<(unsigned)nseg, 2 * kSegLeaves, 0, q>>>(inner, rng, b->leaf_boxes, n, P, levels,
                                                         seg, b->nodes, n32, b->ranges32,
                                                         b->status, deferred, ndef);
       if (nseg > 1) {  // one workgroup of k_seg_local holds every node
