@@ -12,12 +12,12 @@
 //   k_refit_*     bottom-up boxes; the second thread to reach a node goes on
 //                 (hand-offs kept inside workgroups by tiers of leaf spans)
 //   k_emit        128-byte FP64 nodes (+ 64-byte FP32 nodes, rounded outward)
-// Up to kSmallSortMax boxes the same tree takes nine launches: k_transform;
+// Up to kSmallSortMax boxes the same tree takes eight launches: k_transform;
 // k_mcodes_small (bounds + codes + the sort's first histogram); the sort's
-// passes, the last writing the leaves in tree order; k_karras (wide
-// searches); k_seg_local (a balanced segment tree over each 512 leaves in
-// LDS, whose unions give the child boxes of every node inside those leaves);
-// k_emit_seg (the few nodes that reach across, one wave each) -- no climb.
+// passes, the last writing the leaves in tree order; k_seg_local (the
+// hierarchy by wide searches beside a balanced segment tree over each 512
+// leaves in LDS, whose unions give the child boxes of every node inside
+// those leaves); k_emit_seg (the few nodes that reach across) -- no climb.
 
 #include <mutex>
 
@@ -445,8 +445,6 @@ __device__ __forceinline__ void karras_node_wide(const uint32_t* __restrict__ mc
 }
 
 // The same launch clears the refit's counters (zero_a[0, n_a), zero_b[0, n_b)).
-// WIDE: the searches of karras_node_wide (trees of at most kSmallSortMax leaves).
-template <bool WIDE>
 __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ mc, int32_t inner,
                                                     int32_t* __restrict__ lchild,
                                                     int32_t* __restrict__ rchild,
@@ -458,12 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
   if (i < n_a) zero_a[i] = 0;
   if (i < n_b) zero_b[i] = 0;
   if (i >= inner) return;
-  if constexpr (WIDE) {
-    int32_t lo, hi, split;
-    karras_node_wide(mc, inner, i, lchild, rchild, parents, range, lo, hi, split);
-  } else {
-    karras_node(mc, inner, i, lchild, rchild, parents, range);
-  }
+  karras_node(mc, inner, i, lchild, rchild, parents, range);
 }
 
 // ---- box unions over leaf ranges (small builds) ------------------------------
@@ -795,63 +788,65 @@ __global__ __launch_bounds__(kBlock) void k_emit(const double* __restrict__ leaf
 constexpr int kSegLeaves = 512;  // leaves per workgroup of k_seg_local
 constexpr int kSegLog = 9;
 
-// Small builds, one 1,024-thread workgroup per 512 leaves (base = 512 *
-// workgroup), after k_karras: the workgroup puts together the segment tree of its leaves
-// in LDS (local node u in [1, 512) at level l covers leaves
+// Small builds, one 1,024-thread workgroup per 512 leaves and inner nodes
+// (base = 512 * workgroup). Its first 512 threads find the hierarchy's nodes
+// base .. base + 511 (karras_node_wide) while the other 512 put together the
+// segment tree of its leaves in LDS (local node u in [1, 512) at level l covers leaves
 // base + [(u << l) - 512, ((u + 1) << l) - 512), leaf base + k sits at
-// 512 + k; levels 1-6 within each wave by shuffles, 7-9 in the first wave)
-// and stores its nodes with a leaf at their global places (levels
-// 1 .. min(9, log2 P)). Every inner node whose leaf range lies in the
+// 512 + k; levels 1-6 within each wave by shuffles, 7-9 in the first leaf
+// wave) and stores its nodes with a leaf at their global places (levels
+// 1 .. min(9, log2 P)). Then every inner node whose leaf range lies in the
 // workgroup's leaves is emitted with child boxes taken from LDS; the others
 // are listed for k_emit_seg, kSegLeaves slots per workgroup
 // (deferred[512 * workgroup + j], their number in ndeferred[workgroup]).
 __global__ __launch_bounds__(2 * kSegLeaves) void k_seg_local(
-    int32_t inner, const int32_t* __restrict__ range,
+    const uint32_t* __restrict__ mc, int32_t inner, int32_t* __restrict__ lchild,
+    int32_t* __restrict__ rchild, int32_t* __restrict__ parents, int32_t* __restrict__ range,
     const double* __restrict__ leaf_boxes, int64_t n, int64_t P, int levels,
     double* __restrict__ seg, Node64* __restrict__ nodes, Node32* __restrict__ nodes32,
     int4* __restrict__ ranges32, int32_t* __restrict__ status, int32_t* __restrict__ deferred,
     int32_t* __restrict__ ndeferred) {
   __shared__ double s_box[6][2 * kSegLeaves];
+  __shared__ int32_t s_rng[3][kSegLeaves];
   __shared__ int32_t s_ndef;
   const int t = threadIdx.x;
   const int lane = t % kWaveLanes, wid = t / kWaveLanes;
   const int64_t base = (int64_t)blockIdx.x * kSegLeaves;
   if (base + t == 0) *status = 0;  // the tree's error flag starts clear
   if (t == 0) s_ndef = 0;
-  // node pairs: threads 2j and 2j + 1 take inner node base + j, its left and
-  // right child
-  const int64_t k = base + (t >> 1);
-  const int side = t & 1;
-  int32_t lo = 0, hi = 0, split = 0;
-  if (k < inner) {
-    lo = range[3 * k];
-    hi = range[3 * k + 1];
-    split = range[3 * k + 2];
-  }
   // node u of local level l: stored at s_box[.][u] and, with a leaf, globally
   auto put = [&](const SegBox& x, int l, int u, int64_t first_leaf) {
     for (int c = 0; c < 6; ++c) s_box[c][u] = x.b[c];
     if (l <= levels && !x.empty) store_box(seg + 6 * ((P >> l) + (first_leaf >> l)), x.b);
   };
-  if (t < kSegLeaves) {  // leaf base + t (the first 8 waves)
-    const int64_t kl = base + t;
+  if (t < kSegLeaves) {  // the first 8 waves: the hierarchy's node base + t
+    const int64_t kn = base + t;
+    int32_t nlo = 0, nhi = 0, nsp = 0;
+    if (kn < inner)
+      karras_node_wide(mc, inner, (int32_t)kn, lchild, rchild, parents, range, nlo, nhi, nsp);
+    s_rng[0][t] = nlo;
+    s_rng[1][t] = nhi;
+    s_rng[2][t] = nsp;
+  } else {  // the other 8: leaf base + t - 512 and the tree's levels 1-6
+    const int tl = t - kSegLeaves;
+    const int64_t kl = base + tl;
     SegBox v;
     v.empty = kl >= n;
     if (!v.empty)
       load_box(leaf_boxes + 6 * kl, v.b);
     else
       invalid_box(v.b);
-    for (int c = 0; c < 6; ++c) s_box[c][kSegLeaves + t] = v.b[c];
+    for (int c = 0; c < 6; ++c) s_box[c][kSegLeaves + tl] = v.b[c];
     for (int l = 1; l <= 6; ++l) {
       const SegBox o = seg_shfl_down(v, 1 << (l - 1));
       if ((lane & ((1 << l) - 1)) == 0) {
         seg_combine(v, o);
-        put(v, l, (kSegLeaves >> l) + (t >> l), kl);
+        put(v, l, (kSegLeaves >> l) + (tl >> l), kl);
       }
     }
   }
-  __syncthreads();  // the waves' level-6 nodes, local 8 .. 15 (wave w: 8 + w)
-  if (wid == 0) {
+  __syncthreads();  // the leaf waves' level-6 nodes, local 8 .. 15 (leaf wave w: 8 + w)
+  if (wid == kSegLeaves / kWaveLanes) {  // the first leaf wave: levels 7-9
     const int nw = kSegLeaves / kWaveLanes;  // 8
     SegBox u;
     for (int c = 0; c < 6; ++c) u.b[c] = s_box[c][nw + (lane < nw ? lane : 0)];
@@ -864,6 +859,11 @@ __global__ __launch_bounds__(2 * kSegLeaves) void k_seg_local(
       }
     }
   }
+  // node pairs: threads 2j and 2j + 1 take inner node base + j, its left and
+  // right child
+  const int64_t k = base + (t >> 1);
+  const int side = t & 1;
+  const int32_t lo = s_rng[0][t >> 1], hi = s_rng[1][t >> 1], split = s_rng[2][t >> 1];
   const bool mine = k < inner && lo >= base && hi < base + kSegLeaves;
   if (side == 0 && k < inner && !mine) deferred[base + atomicAdd(&s_ndef, 1)] = (int32_t)k;
   __syncthreads();
@@ -1282,12 +1282,9 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
       AXSQ_TRY(radix_sort_small_build(keys, vals, ws.k1.as<uint32_t>(), ws.v1.as<uint32_t>(), n,
                                       bits, hists, scaled, b->leaf_boxes, b->mcodes, b->leaf_ids,
                                       q));
-      k_karras<true><<<grid_for(inner), kBlock, 0, q>>>(b->mcodes, inner, lch, rch, par, rng,
-                                                        nullptr, 0, nullptr, 0);
-      AXSQ_CHECK_HIP(hipGetLastError());
-      k_seg_local<<<(unsigned)nseg, 2 * kSegLeaves, 0, q>>>(inner, rng, b->leaf_boxes, n, P, levels,
-                                                        seg, b->nodes, n32, b->ranges32,
-                                                        b->status, deferred, ndef);
+      k_seg_local<<<(unsigned)nseg, 2 * kSegLeaves, 0, q>>>(
+          b->mcodes, inner, lch, rch, par, rng, b->leaf_boxes, n, P, levels, seg, b->nodes, n32,
+          b->ranges32, b->status, deferred, ndef);
       if (nseg > 1) {  // one workgroup of k_seg_local holds every node
         k_emit_seg<<<2048, kBlock, 0, q>>>(b->leaf_boxes, seg, P, rng, deferred, ndef, nseg,
                                            b->nodes, n32, b->ranges32);
@@ -1353,8 +1350,8 @@ int bvh_build_into(axsq_bvh* b, const double* boxes, int64_t n_in, double scale,
     k_reorder<<<grid_for(n), kBlock, 0, s>>>(scaled, sv, sk, n, b->leaf_boxes, b->leaf_ids,
                                              b->mcodes);
     AXSQ_CHECK_HIP(hipGetLastError());
-    k_karras<false><<<grid_for(inner), kBlock, 0, s>>>(b->mcodes, inner, lch, rch, par, rng, cnt,
-                                                       inner, n_ex1, 1);
+    k_karras<<<grid_for(inner), kBlock, 0, s>>>(b->mcodes, inner, lch, rch, par, rng, cnt, inner,
+                                                n_ex1, 1);
     AXSQ_CHECK_HIP(hipGetLastError());
     AXSQ_TRY(climb_and_emit());
   }
