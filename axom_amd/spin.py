@@ -29,7 +29,13 @@ FP_MODES = {"fp64": L.AXSQ_FP64, "fp32": L.AXSQ_FP32_TRAVERSAL,
             L.AXSQ_FP64: L.AXSQ_FP64, L.AXSQ_FP32_TRAVERSAL: L.AXSQ_FP32_TRAVERSAL}
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream_ptr():
+    """The current CUDA (HIP) stream of the current device, as a pointer."""
+    if _raw_stream is not None:
+        return ctypes.c_void_p(_raw_stream(torch.cuda.current_device()))
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
