@@ -99,6 +99,7 @@ _SIGS = {
     "axsq_sdf_stats_enable": (_i32, [_vp, _i32]),
     "axsq_sdf_stats_get": (_i32, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "axsq_sdf_last_suspended": (_i32, [_vp, ctypes.POINTER(_i64)]),
+    "axsq_sdf_last_early": (_i32, [_vp, ctypes.POINTER(_i64)]),
     "axsq_tri_tri_intersect": (_i32, [_vp, _vp, _i64, _i32, _dbl, _vp, _vp]),
     "axsq_mesh_candidates": (_i32, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _vp,
                                     _i64, _i32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),

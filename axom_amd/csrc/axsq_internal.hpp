@@ -209,6 +209,16 @@ struct axsq_sdf {
   // FP32-traversal form: second pass on the FP32 nodes too (AXSQ_SDF_MIXED_PASS2=fp32
   // at creation); by default it resumes the saved walks on the FP64 nodes
   bool mixed_pass2_fp32 = false;
+  // the longest walks of the previous call on the same query arrays (early_key:
+  // x, y, z, stride, nq): query indices on the device, walked from the start
+  // beside the next such call's first pass; skip marks them for that pass
+  int32_t* early_q = nullptr;
+  int64_t nearly = 0;
+  uint64_t early_key[5] = {0, 0, 0, 0, 0};
+  int32_t* skip = nullptr;
+  int64_t skip_cap = 0;
+  int64_t last_early = 0;  // walks the last call ran beside its first pass
+  int64_t early_walks = 0;  // walks a call remembers for the next (set at creation)
 };
 
 namespace axsq {

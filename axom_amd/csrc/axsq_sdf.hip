@@ -20,6 +20,7 @@
 //
 // Adjoint (no reference counterpart): see axsq_sdf_backward in axsq.h.
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 
 #include "axsq_internal.hpp"
@@ -75,6 +76,7 @@ struct SdfArgs {
   int64_t nq;
   int32_t ninner;
   const uint32_t* order;
+  const int32_t* skip;  // first pass: queries whose walks run elsewhere (nonzero), or NULL
   int32_t budget;      // first-pass step budget per walk
   int64_t cap;         // capacity of susp
   struct SdfSuspended* susp;
@@ -354,6 +356,7 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nq) return;
   const int64_t i = a.order ? (int64_t)a.order[t] : t;
+  if (a.skip && a.skip[i]) return;  // walked beside this pass from the start
   MinCand m;
   m.sq = kDblMax;
   m.p[0] = m.p[1] = m.p[2] = 0.0;
@@ -1264,6 +1267,26 @@ __global__ __launch_bounds__(kBlock) void k_query_codes(const double* __restrict
   vals[i] = (uint32_t)i;
 }
 
+// skip[q[k]] = v for the n listed queries (indices outside [0, nq) are left out)
+__global__ __launch_bounds__(kBlock) void k_mark_queries(const int32_t* __restrict__ q,
+                                                          int64_t n, int64_t nq, int32_t v,
+                                                          int32_t* __restrict__ skip) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int32_t i = q[k];
+  if (i >= 0 && i < nq) skip[i] = v;
+}
+
+// list[from + j] = the query of the j-th saved walk in launch order, j < n
+__global__ __launch_bounds__(kBlock) void k_early_list(const SdfSuspended* __restrict__ susp,
+                                                        const uint32_t* __restrict__ order,
+                                                        int64_t from, int64_t n,
+                                                        int32_t* __restrict__ list) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  list[from + j] = susp[order ? (int64_t)order[j] : j].qidx;
+}
+
 // ---- mesh set-up -------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_vert_bounds(const double* __restrict__ x,
                                                          const double* __restrict__ y,
@@ -1707,6 +1730,30 @@ int sdf_build_v2t(axsq_sdf* h, hipStream_t s) {
   return AXSQ_OK;
 }
 
+// A side stream and fork/join events for the early walks, one set per host
+// thread and device (default priority, non-blocking).
+struct SideStream {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+static SideStream* side_stream() {
+  constexpr int kMaxDev = 64;
+  thread_local SideStream sides[kMaxDev];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return nullptr;
+  SideStream& ss = sides[dev];
+  if (!ss.stream) {
+    if (hipStreamCreateWithFlags(&ss.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ss.join, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      ss = SideStream{};
+      return nullptr;
+    }
+  }
+  return &ss;
+}
+
 int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const double* z,
                     int64_t stride, int64_t nq, int fp_mode, double* dist, double* cp,
                     double* normal, int32_t* elem, bool stats, hipStream_t s) {
@@ -1783,6 +1830,53 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   a.budget = h->budget;
   AXSQ_TRY(fill_i32(reinterpret_cast<int32_t*>(a.nsusp), 0, 2, s));
   ProfRange pr(mixed ? "sdf_eval_fp32" : "sdf_eval_fp64", s);
+  // Early walks: the previous call on the same query arrays listed its
+  // longest walks (the first of its second pass's launch order). Their
+  // queries are walked here from the start, one wave each as in the second
+  // pass, on a side stream beside the first pass, which skips them; so the
+  // longest walks no longer wait for the first pass to end. Which queries
+  // these are changes speed only: each query is still walked once, in full.
+  axsq_sdf* hm = const_cast<axsq_sdf*>(h);
+  const uint64_t qkey[5] = {(uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)y,
+                            (uint64_t)(uintptr_t)z, (uint64_t)stride, (uint64_t)nq};
+  const int64_t kEarly = cells ? 0 : h->early_walks;
+  if (kEarly == 0 || memcmp(qkey, hm->early_key, sizeof(qkey)) != 0) {
+    hm->nearly = 0;
+    memcpy(hm->early_key, qkey, sizeof(qkey));
+  }
+  const int64_t nearly = std::min<int64_t>(hm->nearly, kEarly);
+  SideStream* side = nearly > 0 ? side_stream() : nullptr;
+  const bool mixed2_early = mixed && h->mixed_pass2_fp32;
+  if (side) {
+    if (hm->skip_cap < nq) {
+      hipFree(hm->skip);
+      hm->skip = nullptr;
+      hm->skip_cap = 0;
+      AXSQ_TRY(dmalloc(&hm->skip, (size_t)nq));
+      hm->skip_cap = nq;
+      AXSQ_TRY(fill_i32(hm->skip, 0, nq, s));
+    }
+    k_mark_queries<<<grid_for(nearly), kBlock, 0, s>>>(hm->early_q, nearly, nq, 1, hm->skip);
+    AXSQ_CHECK_HIP(hipGetLastError());
+    AXSQ_CHECK_HIP(hipEventRecord(side->fork, s));
+    AXSQ_CHECK_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
+    SdfArgs ae = a;
+    ae.redo = hm->early_q;  // walks from the start, like those of overflowed term lists
+    ae.susp = nullptr;
+    ae.susp_order = nullptr;
+    ae.nhot = 0;
+    ae.hot_leaf = ae.hot_node = nullptr;
+    const unsigned ge = (unsigned)nearly;
+    if (mixed2_early)
+      k_sdf_resume_wave<true, false><<<ge, kWaveLanes, 0, side->stream>>>(ae, 0, nearly);
+    else if (stats)
+      k_sdf_resume_wave<false, true><<<ge, kWaveLanes, 0, side->stream>>>(ae, 0, nearly);
+    else
+      k_sdf_resume_wave<false, false><<<ge, kWaveLanes, 0, side->stream>>>(ae, 0, nearly);
+    AXSQ_CHECK_HIP(hipGetLastError());
+    AXSQ_CHECK_HIP(hipEventRecord(side->join, side->stream));
+    a.skip = hm->skip;
+  }
   ProfRange p1(mixed ? "sdf_eval_fp32_pass1" : "sdf_eval_fp64_pass1", s);
   const unsigned g = grid_for(nq, kSdfBlock);
   if (cells) {
@@ -1799,12 +1893,17 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   }
   AXSQ_CHECK_HIP(hipGetLastError());
   p1.stop();
+  if (side) {  // clear the marks for the next call
+    k_mark_queries<<<grid_for(nearly), kBlock, 0, s>>>(hm->early_q, nearly, nq, 0, hm->skip);
+    AXSQ_CHECK_HIP(hipGetLastError());
+  }
   uint32_t counts[2] = {0, 0};
   AXSQ_CHECK_HIP(hipMemcpyAsync(counts, a.nsusp, sizeof(counts), hipMemcpyDeviceToHost, s));
   AXSQ_CHECK_HIP(hipStreamSynchronize(s));
   const int64_t nres = std::min<int64_t>(counts[0], cap);
   const int64_t nredo = std::min<int64_t>(counts[1], nq);
   const_cast<axsq_sdf*>(h)->last_suspended = nres + nredo;
+  hm->last_early = side ? nearly : 0;
   const_cast<axsq_sdf*>(h)->susp_demand = counts[0];  // saved or not
   ProfRange p2(mixed ? "sdf_eval_fp32_pass2" : "sdf_eval_fp64_pass2", s);
   a.susp_order = nullptr;
@@ -1820,6 +1919,19 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     AXSQ_TRY(radix_sort_pairs(ws.k0.as<uint32_t>(), ws.v0.as<uint32_t>(), ws.k1.as<uint32_t>(),
                               ws.v1.as<uint32_t>(), nres, 32, ws, s, &sk, &sv));
     a.susp_order = sv;
+  }
+  // the longest walks of this call (the first of the launch order) join the
+  // list for the next call on these queries
+  if (kEarly > 0) {
+    const int64_t from = side ? nearly : 0;
+    const int64_t nadd = std::min<int64_t>(nres, kEarly - from);
+    if (nadd > 0) {
+      if (!hm->early_q) AXSQ_TRY(dmalloc(&hm->early_q, (size_t)kEarly));
+      k_early_list<<<grid_for(nadd), kBlock, 0, s>>>(a.susp, a.susp_order, from, nadd,
+                                                      hm->early_q);
+      AXSQ_CHECK_HIP(hipGetLastError());
+    }
+    hm->nearly = from + std::max<int64_t>(nadd, 0);
   }
   // the first walks of the launch order (the farthest from the surface, which
   // run longest) read tables of their leaf records and node keys
@@ -1863,6 +1975,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     }
     AXSQ_CHECK_HIP(hipGetLastError());
   }
+  if (side) AXSQ_CHECK_HIP(hipStreamWaitEvent(s, side->join, 0));
   p2.stop();
   pr.stop();
   return AXSQ_OK;

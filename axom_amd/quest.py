@@ -162,6 +162,13 @@ class SignedDistance:
         L.check(L.lib().axsq_sdf_last_suspended(self._h, ctypes.byref(n)))
         return n.value
 
+    def lastEarly(self):
+        """Walks the last computeDistances ran from the start beside its first
+        pass (the previous call's longest, on the same query arrays)."""
+        n = ctypes.c_int64()
+        L.check(L.lib().axsq_sdf_last_early(self._h, ctypes.byref(n)))
+        return n.value
+
 
 # ---- quest C-style interface (quest/interface/signed_distance.{hpp,cpp}) ----
 # These call the library's process-wide query through its C names

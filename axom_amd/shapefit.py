@@ -64,6 +64,7 @@ def shape_fit(vertices, triangles, queries, target, iterations=20, lr=1e-3, fp_m
         torch.cuda.synchronize()
         history.append({"iteration": it, "loss": float(torch.mean(r * r)),
                         "suspended_walks": sdf.lastSuspended(),
+                        "early_walks": sdf.lastEarly(),
                         "build_ms": ev[0].elapsed_time(ev[1]),
                         "forward_ms": ev[1].elapsed_time(ev[2]),
                         "backward_ms": ev[2].elapsed_time(ev[3])})

@@ -369,6 +369,12 @@ int axsq_sdf_stats_get(const axsq_sdf* sdf, int64_t* visits, int64_t* leaves);
  * counterpart: the reference runs every walk in one pass). */
 int axsq_sdf_last_suspended(const axsq_sdf* sdf, int64_t* count);
 
+/* Number of query walks the last axsq_sdf_eval ran from the start beside its
+ * first pass: the longest walks of the previous call on the same query
+ * arrays (same x, y, z pointers, stride and count). Scheduling only; every
+ * query is walked once either way (no reference counterpart). */
+int axsq_sdf_last_early(const axsq_sdf* sdf, int64_t* count);
+
 /* ---- Triangle intersection and mesh self-intersection ----------------------
  * primal::intersect(Triangle3 t1, Triangle3 t2, includeBoundary, EPS)
  * (src/axom/primal/operators/intersect.hpp:65-71 ->

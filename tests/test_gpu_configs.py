@@ -101,6 +101,8 @@ def test_shape_fit_matches_oracle_loop(level, nq):
                                        fp_mode="fp32")
     assert [h["loss"] for h in hist32] == [h["loss"] for h in hist]
     assert np.array_equal(got32.cpu().numpy(), want)
+    if level == 8:  # a large tree: the second iteration walks the first's longest early
+        assert hist[1]["early_walks"] > 0 and hist32[1]["early_walks"] > 0
 
 
 def test_shape_fit_normalized_step_is_opt_in():

@@ -207,6 +207,79 @@ def test_suspended_walks_resume_bit_exact(monkeypatch, budget, level, inner):
     _compare(g, orc.OracleSDF(v, t).eval(q, nthreads=8))
 
 
+@pytest.mark.parametrize("fp_mode", ["fp64", "fp32"])
+def test_repeated_calls_walk_the_longest_early(monkeypatch, fp_mode):
+    """A later call on the same query arrays walks the previous call's longest
+    walks from the start beside its first pass, which skips them. Every call
+    gives the same answers as the first: the oracle's, bit for bit with the
+    same visit and leaf counts in FP64 (FP32 traversal: within 1e-12 of the
+    FP64 oracle, and identical from call to call). So does a call after the
+    queries were rewritten in place, whose remembered walks are then ordinary
+    ones, and a call after they were written back."""
+    v, t, q = synth.workload2(nq=20_000, level=5)
+    q = np.concatenate([q, 0.3 * synth.uniform(11, 0, 9000, -1, 1).reshape(-1, 3)])
+    q2 = np.concatenate([synth.uniform(12, 0, 60_000, -1.5, 1.5).reshape(-1, 3),
+                         0.2 * synth.uniform(13, 0, 9000, -1, 1).reshape(-1, 3)])
+    assert q2.shape == q.shape
+    mode = None if fp_mode == "fp64" else "fp32"
+    monkeypatch.setenv("AXSQ_SDF_EARLY_WALKS", "64")  # on small trees too (read at creation)
+    sdf = _sdf_with(monkeypatch, v, t, 37, fp_mode=fp_mode)
+    sdf.enableStats(fp_mode == "fp64")
+    o = orc.OracleSDF(v, t)
+    qt = _t(q)
+    n = q.shape[0]
+
+    def run():
+        cp = torch.empty((n, 3), dtype=torch.float64, device=DEV)
+        nrm = torch.empty((n, 3), dtype=torch.float64, device=DEV)
+        elem = torch.empty(n, dtype=torch.int32, device=DEV)
+        d = sdf.computeDistances(n, qt, outClosestPts=cp, outNormals=nrm, outElem=elem,
+                                 fp_mode=mode)
+        return dict(dist=d.cpu().numpy(), cp=cp.cpu().numpy(), normal=nrm.cpu().numpy(),
+                    elem=elem.cpu().numpy())
+
+    def check(g, pts, first=None):
+        ref = o.eval(pts, nthreads=8, want_stats=True)
+        if fp_mode == "fp64":
+            _compare(g, ref)
+            assert sdf.lastStats() == (int(ref["visits"].sum()), int(ref["leaves"].sum()))
+        else:
+            np.testing.assert_allclose(g["dist"], ref["dist"], rtol=1e-12, atol=0)
+            assert np.array_equal(np.signbit(g["dist"]), np.signbit(ref["dist"]))
+            if first is not None:
+                for k in ("dist", "cp", "normal", "elem"):
+                    assert np.array_equal(g[k], first[k]), k
+
+    g1 = run()
+    assert sdf.lastEarly() == 0 and sdf.lastSuspended() > 64
+    check(g1, q)
+    for _ in range(2):
+        g = run()
+        assert sdf.lastEarly() == 64
+        check(g, q, g1)
+    qt.copy_(_t(q2))  # same arrays, new queries
+    g = run()
+    assert sdf.lastEarly() == 64
+    check(g, q2)
+    qt.copy_(_t(q))
+    g = run()
+    assert sdf.lastEarly() == 64
+    check(g, q, g1)
+
+
+def test_early_walks_off_on_small_trees(monkeypatch):
+    """By default a tree this small (step budget 2048) runs no early walks."""
+    monkeypatch.delenv("AXSQ_SDF_EARLY_WALKS", raising=False)
+    monkeypatch.delenv("AXSQ_SDF_BUDGET", raising=False)
+    v, t, q = synth.workload2(nq=20_000, level=4)
+    q = np.concatenate([q, 0.3 * synth.uniform(11, 0, 9000, -1, 1).reshape(-1, 3)])
+    sdf = quest.SignedDistance(_t(v), _t(t))
+    qt = _t(q)
+    for _ in range(2):
+        sdf.computeDistances(q.shape[0], qt)
+    assert sdf.lastEarly() == 0
+
+
 @pytest.mark.parametrize("watertight,compute_sign", [(False, True), (True, False)])
 def test_suspended_walks_open_surface_and_unsigned(monkeypatch, watertight, compute_sign):
     v, t = synth.torus(96, 64)
