@@ -1024,6 +1024,7 @@ int axsq_sdf_destroy(axsq_sdf* h) {
   hipFree(h->tris);
   hipFree(h->early_q);
   hipFree(h->skip);
+  if (h->early_done) hipEventDestroy(h->early_done);
   hipFree(h->v2t_start);
   hipFree(h->v2t_end);
   hipFree(h->v2t_slot);

@@ -217,6 +217,7 @@ struct axsq_sdf {
   uint64_t early_key[5] = {0, 0, 0, 0, 0};
   int32_t* skip = nullptr;
   int64_t skip_cap = 0;
+  hipEvent_t early_done = nullptr;  // recorded where a call last touches them
   int64_t last_early = 0;  // walks the last call ran beside its first pass
   int64_t early_walks = 0;  // walks a call remembers for the next (set at creation)
 };

@@ -1848,6 +1848,9 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   SideStream* side = nearly > 0 ? side_stream() : nullptr;
   const bool mixed2_early = mixed && h->mixed_pass2_fp32;
   if (side) {
+    // the previous call may have run on another stream: its last writes of
+    // the list and marks come first
+    if (hm->early_done) AXSQ_CHECK_HIP(hipStreamWaitEvent(s, hm->early_done, 0));
     if (hm->skip_cap < nq) {
       hipFree(hm->skip);
       hm->skip = nullptr;
@@ -1976,6 +1979,11 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     AXSQ_CHECK_HIP(hipGetLastError());
   }
   if (side) AXSQ_CHECK_HIP(hipStreamWaitEvent(s, side->join, 0));
+  if (kEarly > 0) {  // the list and marks are settled once the stream gets here
+    if (!hm->early_done)
+      AXSQ_CHECK_HIP(hipEventCreateWithFlags(&hm->early_done, hipEventDisableTiming));
+    AXSQ_CHECK_HIP(hipEventRecord(hm->early_done, s));
+  }
   p2.stop();
   pr.stop();
   return AXSQ_OK;

@@ -215,7 +215,8 @@ def test_repeated_calls_walk_the_longest_early(monkeypatch, fp_mode):
     same visit and leaf counts in FP64 (FP32 traversal: within 1e-12 of the
     FP64 oracle, and identical from call to call). So does a call after the
     queries were rewritten in place, whose remembered walks are then ordinary
-    ones, and a call after they were written back."""
+    ones, a call after they were written back, and calls on two streams in
+    turn."""
     v, t, q = synth.workload2(nq=20_000, level=5)
     q = np.concatenate([q, 0.3 * synth.uniform(11, 0, 9000, -1, 1).reshape(-1, 3)])
     q2 = np.concatenate([synth.uniform(12, 0, 60_000, -1.5, 1.5).reshape(-1, 3),
@@ -265,6 +266,16 @@ def test_repeated_calls_walk_the_longest_early(monkeypatch, fp_mode):
     g = run()
     assert sdf.lastEarly() == 64
     check(g, q, g1)
+    # calls on two streams in turn, with no synchronisation between them
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    outs = []
+    for st in (s1, s2, s1):
+        with torch.cuda.stream(st):
+            outs.append(run())
+    torch.cuda.synchronize()
+    for g in outs:
+        check(g, q, g1)
 
 
 def test_early_walks_off_on_small_trees(monkeypatch):
