@@ -595,6 +595,7 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
     leaves = 0;
     stk = kBarrier;
   }
+  i = (i >= 0 && i < a.nq) ? i : 0;  // only a malformed record or list is out of range
   const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
   const NodeT* nodes;
   if constexpr (MIXED)
@@ -1267,14 +1268,19 @@ __global__ __launch_bounds__(kBlock) void k_query_codes(const double* __restrict
   vals[i] = (uint32_t)i;
 }
 
-// skip[q[k]] = v for the n listed queries (indices outside [0, nq) are left out)
+// skip[q[k]] = v for the n listed queries; an index outside [0, nq) is left
+// out and raises the walk-error status bit (the resume kernel clamps it)
 __global__ __launch_bounds__(kBlock) void k_mark_queries(const int32_t* __restrict__ q,
                                                           int64_t n, int64_t nq, int32_t v,
-                                                          int32_t* __restrict__ skip) {
+                                                          int32_t* __restrict__ skip,
+                                                          int32_t* __restrict__ status) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int32_t i = q[k];
-  if (i >= 0 && i < nq) skip[i] = v;
+  if (i >= 0 && i < nq)
+    skip[i] = v;
+  else
+    atomicOr(status, kStatusStack);
 }
 
 // list[from + j] = the query of the j-th saved walk in launch order, j < n
@@ -1859,7 +1865,8 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
       hm->skip_cap = nq;
       AXSQ_TRY(fill_i32(hm->skip, 0, nq, s));
     }
-    k_mark_queries<<<grid_for(nearly), kBlock, 0, s>>>(hm->early_q, nearly, nq, 1, hm->skip);
+    k_mark_queries<<<grid_for(nearly), kBlock, 0, s>>>(hm->early_q, nearly, nq, 1, hm->skip,
+                                                            b->status);
     AXSQ_CHECK_HIP(hipGetLastError());
     AXSQ_CHECK_HIP(hipEventRecord(side->fork, s));
     AXSQ_CHECK_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
@@ -1897,7 +1904,8 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   AXSQ_CHECK_HIP(hipGetLastError());
   p1.stop();
   if (side) {  // clear the marks for the next call
-    k_mark_queries<<<grid_for(nearly), kBlock, 0, s>>>(hm->early_q, nearly, nq, 0, hm->skip);
+    k_mark_queries<<<grid_for(nearly), kBlock, 0, s>>>(hm->early_q, nearly, nq, 0, hm->skip,
+                                                            b->status);
     AXSQ_CHECK_HIP(hipGetLastError());
   }
   uint32_t counts[2] = {0, 0};
