@@ -85,10 +85,17 @@ __device__ __forceinline__ bool pred32(const double* prim, const float* lo, cons
   }
 }
 
+// Stack entries per lane in LDS: a point's walk pushes one entry per node
+// where both children hold the point, rarely many, so 16 (deeper ones in the
+// private array) let twice the waves of 32 share a CU's LDS.
+template <int KIND>
+constexpr int find_lds_stack() { return KIND == AXSQ_QUERY_POINTS ? 16 : kLdsStack; }
+
 template <int KIND, int MODE, bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
   constexpr bool FILL = MODE != kCountPass;
-  __shared__ int32_t s_stack[stack_lds_words(kBlock)];
+  constexpr int kLds = find_lds_stack<KIND>();
+  __shared__ int32_t s_stack[stack_lds_words<kLds>(kBlock)];
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = j < a.nq;
   if (FILL && !live) return;  // the count pass keeps whole waves for its ballot
@@ -96,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
   int64_t i = 0;
   if (live) {
     i = (MODE == kFillListed) ? (int64_t)a.over[j] : j;
-    lds_i32* stk = lane_stack_base(s_stack);
+    lds_i32* stk = lane_stack_base<kLds>(s_stack);
     double prim[6];
     // element k of query array p (float queries widen exactly)
     auto ld = [&](const void* p, int64_t k) -> double {
@@ -134,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
     if (!MIXED) {
       // the walk tests a child box only when its validity bit is set, so an
       // invalid box (min > max on some axis) is never entered
-      v = traverse_voted<Node64, double>(
+      v = traverse_voted<Node64, double, kLds>(
           a.nodes, a.ninner, stk,
           [&](const double* b) { return pred64<KIND>(prim, a.tol, b, a.f32, a.dims); }, no_pref,
           emit);
@@ -145,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_find(FindArgs a) {
         lo[d] = __double2float_rd(prim[d]);
         hi[d] = __double2float_ru(prim[hoff + d]);
       }
-      v = traverse_voted<Node32, float>(
+      v = traverse_voted<Node32, float, kLds>(
           a.nodes32, a.ninner, stk,
           [&](const float* b) { return pred32<KIND>(prim, lo, hi, a.tol, b, a.f32, a.dims); },
           no_pref,

@@ -66,30 +66,38 @@ constexpr int kWaveLanes = 64;
 // LDS-qualified word: accesses compile to ds_read/ds_write, not flat ops.
 typedef __attribute__((address_space(3))) int32_t lds_i32;
 
-struct LaneStack {
-  lds_i32* lds;  // &tile[0][lane] of this wave's [kLdsStack][64] tile
-  int32_t spill[kStack - kLdsStack];
+// LDS: entries per lane kept in LDS (the rest of the kStack in the private
+// array). A kernel whose walks stay shallow (point queries) keeps fewer, so
+// that its blocks' LDS lets more waves in.
+template <int LDS = kLdsStack>
+struct LaneStackT {
+  static_assert(LDS > 0 && LDS <= kStack, "LDS entries");
+  lds_i32* lds;  // &tile[0][lane] of this wave's [LDS][64] tile
+  int32_t spill[kStack - LDS > 0 ? kStack - LDS : 1];
   __device__ __forceinline__ int32_t get(int32_t sp) const {
-    if (sp < kLdsStack) return lds[sp * kWaveLanes];
-    return spill[sp - kLdsStack];
+    if (sp < LDS) return lds[sp * kWaveLanes];
+    return spill[sp - LDS];
   }
   __device__ __forceinline__ void set(int32_t sp, int32_t v) {
-    if (sp < kLdsStack)
+    if (sp < LDS)
       lds[sp * kWaveLanes] = v;
     else
-      spill[sp - kLdsStack] = v;
+      spill[sp - LDS] = v;
   }
 };
+using LaneStack = LaneStackT<>;
 
 // LDS words a block of `threads` lanes needs for its stacks.
-constexpr int stack_lds_words(int threads) { return threads * kLdsStack; }
+template <int LDS = kLdsStack>
+constexpr int stack_lds_words(int threads) { return threads * LDS; }
 
 // This lane's stack view into a block-wide __shared__ array of
-// stack_lds_words(blockDim.x) words.
+// stack_lds_words<LDS>(blockDim.x) words.
+template <int LDS = kLdsStack>
 __device__ __forceinline__ lds_i32* lane_stack_base(int32_t* block_stack) {
   const int wave = threadIdx.x / kWaveLanes;
   const int lane = threadIdx.x % kWaveLanes;
-  return (lds_i32*)(block_stack + wave * (kLdsStack * kWaveLanes) + lane);
+  return (lds_i32*)(block_stack + wave * (LDS * kWaveLanes) + lane);
 }
 
 // Resumable walk state: the stack lives in the LaneStack, the rest here.
@@ -101,7 +109,8 @@ struct Walk {
   int32_t steps;   // inner visits + leaf actions so far
 };
 
-__device__ __forceinline__ Walk walk_start(LaneStack& todo) {
+template <class Stack>
+__device__ __forceinline__ Walk walk_start(Stack& todo) {
   todo.set(0, kBarrier);
   return Walk{0, 0, 0, 0};
 }
@@ -115,9 +124,9 @@ enum WalkStatus { kWalkDone = 0, kWalkSuspended = 1, kWalkError = -1 };
 // stack overflow. `ninner` bounds node links and visits: a well-formed tree
 // visits each inner node at most once, so the guards only turn a malformed
 // tree into an error instead of a stray access or an endless walk.
-template <class NodeT, class Scalar, class Pred, class Pref, class Leaf>
+template <class NodeT, class Scalar, class Stack, class Pred, class Pref, class Leaf>
 __device__ __forceinline__ int traverse(const NodeT* __restrict__ nodes, int32_t ninner,
-                                        LaneStack& todo, Walk& w, int32_t budget, Pred&& pred,
+                                        Stack& todo, Walk& w, int32_t budget, Pred&& pred,
                                         Pref&& pref, Leaf&& leaf) {
   int32_t sp = w.sp, cur = w.cur, visits = w.visits, steps = w.steps;
   int32_t found = 0;
@@ -192,9 +201,9 @@ __device__ __forceinline__ int traverse(const NodeT* __restrict__ nodes, int32_t
 // walks of uneven shape in one wave this keeps about half the lanes busy in
 // each step where the loop-by-loop schedule keeps a fifth busy. All lanes of
 // the wave that are still running must call it together.
-template <class NodeT, class Scalar, class Pred, class Pref, class Leaf>
+template <class NodeT, class Scalar, class Stack, class Pred, class Pref, class Leaf>
 __device__ __forceinline__ int traverse_vote(const NodeT* __restrict__ nodes, int32_t ninner,
-                                             LaneStack& todo, Walk& w, int32_t budget, Pred&& pred,
+                                             Stack& todo, Walk& w, int32_t budget, Pred&& pred,
                                              Pref&& pref, Leaf&& leaf) {
   enum { kInner = 0, kLeaf = 1, kIdle = 2 };
   int32_t sp = w.sp, cur = w.cur, visits = w.visits, steps = w.steps;
@@ -307,12 +316,13 @@ __device__ __forceinline__ int traverse_vote(const NodeT* __restrict__ nodes, in
   return status;
 }
 
-// One-shot walk to completion; returns visited inner nodes or -1.
-template <class NodeT, class Scalar, class Pred, class Pref, class Leaf>
+// One-shot walk to completion; returns visited inner nodes or -1. lds_stack
+// is this lane's base from lane_stack_base<LDS>.
+template <class NodeT, class Scalar, int LDS = kLdsStack, class Pred, class Pref, class Leaf>
 __device__ __forceinline__ int32_t traverse(const NodeT* __restrict__ nodes, int32_t ninner,
                                             lds_i32* lds_stack, Pred&& pred, Pref&& pref,
                                             Leaf&& leaf) {
-  LaneStack todo;
+  LaneStackT<LDS> todo;
   todo.lds = lds_stack;
   Walk w = walk_start(todo);
   const int st = traverse<NodeT, Scalar>(nodes, ninner, todo, w, 0x7fffffff, pred, pref, leaf);
@@ -321,11 +331,11 @@ __device__ __forceinline__ int32_t traverse(const NodeT* __restrict__ nodes, int
 
 // The same one-shot walk on the vote schedule (every lane of the wave that is
 // still running calls it together).
-template <class NodeT, class Scalar, class Pred, class Pref, class Leaf>
+template <class NodeT, class Scalar, int LDS = kLdsStack, class Pred, class Pref, class Leaf>
 __device__ __forceinline__ int32_t traverse_voted(const NodeT* __restrict__ nodes, int32_t ninner,
                                                   lds_i32* lds_stack, Pred&& pred, Pref&& pref,
                                                   Leaf&& leaf) {
-  LaneStack todo;
+  LaneStackT<LDS> todo;
   todo.lds = lds_stack;
   Walk w = walk_start(todo);
   const int st =
