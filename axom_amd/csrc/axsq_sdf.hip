@@ -1852,13 +1852,39 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   }
   const int64_t nearly = std::min<int64_t>(hm->nearly, kEarly);
   SideStream* side = nearly > 0 ? side_stream() : nullptr;
+  if (kEarly > 0 && !hm->early_done)
+    AXSQ_CHECK_HIP(hipEventCreateWithFlags(&hm->early_done, hipEventDisableTiming));
+  // On every way out of this call once the early walks are forked (an error
+  // too): clear the marks if pass 1 has not, make the caller's stream wait
+  // for the side stream, and record where this call last touches the list
+  // and marks.
+  struct EarlyExit {
+    hipStream_t s = nullptr;
+    SideStream* side = nullptr;
+    axsq_sdf* hm = nullptr;
+    int64_t n = 0, nq = 0;
+    int32_t* status = nullptr;
+    bool marked = false, record = false;
+    ~EarlyExit() {
+      if (marked)
+        k_mark_queries<<<grid_for(n), kBlock, 0, s>>>(hm->early_q, n, nq, 0, hm->skip, status);
+      if (side) (void)hipStreamWaitEvent(s, side->join, 0);
+      if (record && hm->early_done) (void)hipEventRecord(hm->early_done, s);
+    }
+  } early_exit;
+  early_exit.s = s;
+  early_exit.hm = hm;
+  early_exit.n = nearly;
+  early_exit.nq = nq;
+  early_exit.status = b->status;
+  early_exit.record = kEarly > 0;
   const bool mixed2_early = mixed && h->mixed_pass2_fp32;
   if (side) {
     // the previous call may have run on another stream: its last writes of
     // the list and marks come first
     if (hm->early_done) AXSQ_CHECK_HIP(hipStreamWaitEvent(s, hm->early_done, 0));
     if (hm->skip_cap < nq) {
-      hipFree(hm->skip);
+      (void)hipFree(hm->skip);
       hm->skip = nullptr;
       hm->skip_cap = 0;
       AXSQ_TRY(dmalloc(&hm->skip, (size_t)nq));
@@ -1867,6 +1893,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     }
     k_mark_queries<<<grid_for(nearly), kBlock, 0, s>>>(hm->early_q, nearly, nq, 1, hm->skip,
                                                             b->status);
+    early_exit.marked = true;
     AXSQ_CHECK_HIP(hipGetLastError());
     AXSQ_CHECK_HIP(hipEventRecord(side->fork, s));
     AXSQ_CHECK_HIP(hipStreamWaitEvent(side->stream, side->fork, 0));
@@ -1885,6 +1912,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
       k_sdf_resume_wave<false, false><<<ge, kWaveLanes, 0, side->stream>>>(ae, 0, nearly);
     AXSQ_CHECK_HIP(hipGetLastError());
     AXSQ_CHECK_HIP(hipEventRecord(side->join, side->stream));
+    early_exit.side = side;
     a.skip = hm->skip;
   }
   ProfRange p1(mixed ? "sdf_eval_fp32_pass1" : "sdf_eval_fp64_pass1", s);
@@ -1903,7 +1931,8 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   }
   AXSQ_CHECK_HIP(hipGetLastError());
   p1.stop();
-  if (side) {  // clear the marks for the next call
+  if (early_exit.marked) {  // clear the marks for the next call
+    early_exit.marked = false;
     k_mark_queries<<<grid_for(nearly), kBlock, 0, s>>>(hm->early_q, nearly, nq, 0, hm->skip,
                                                             b->status);
     AXSQ_CHECK_HIP(hipGetLastError());
@@ -1986,10 +2015,12 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     }
     AXSQ_CHECK_HIP(hipGetLastError());
   }
-  if (side) AXSQ_CHECK_HIP(hipStreamWaitEvent(s, side->join, 0));
-  if (kEarly > 0) {  // the list and marks are settled once the stream gets here
-    if (!hm->early_done)
-      AXSQ_CHECK_HIP(hipEventCreateWithFlags(&hm->early_done, hipEventDisableTiming));
+  if (early_exit.side) {
+    early_exit.side = nullptr;
+    AXSQ_CHECK_HIP(hipStreamWaitEvent(s, side->join, 0));
+  }
+  if (early_exit.record) {  // the list and marks are settled once the stream gets here
+    early_exit.record = false;
     AXSQ_CHECK_HIP(hipEventRecord(hm->early_done, s));
   }
   p2.stop();
