@@ -210,16 +210,31 @@ __device__ __forceinline__ int32_t karras_delta(int32_t a, int32_t b, int32_t in
 
 // squared_distance(Point, Box), primal/operators/squared_distance.hpp:77-102.
 // For a point inside the box the clamped vector is exactly zero, so the
-// contains() shortcut of the reference yields the same 0. The clamp
-// (clampVal, core/utilities/Utilities.hpp:147-150) is written as max/min:
-// on a valid box both give the same value except for the sign of a zero
-// coordinate, which the subtraction and squaring erase; a NaN query gives NaN
-// either way.
+// contains() shortcut of the reference yields the same 0. The clamped
+// difference (clampVal, core/utilities/Utilities.hpp:147-150) is taken as
+// the gap max(lo - q, q - hi, 0): on a valid box at most one of lo - q and
+// q - hi is positive, and it equals the clamped difference up to sign, which
+// the squaring erases, so the sum is bit-identical. Max/min of subtraction
+// results need no canonicalisation of loaded operands, which the clamp
+// form costs once per box coordinate.
+// A NaN coordinate gets gap 0 (max drops NaN), as it passes contains()
+// (BoundingBox.hpp:390-401) in that dimension. The reference's distance is
+// then 0 when the other coordinates lie in the box and NaN otherwise; the
+// walks reproduce its pruning by starting such a query at bound 0 (see
+// query_has_nan), so that only a sum of 0 passes.
 __device__ __forceinline__ double sqdist_box(double q0, double q1, double q2, const double b[6]) {
-  const double e0 = fmin(fmax(q0, b[0]), b[3]) - q0;
-  const double e1 = fmin(fmax(q1, b[1]), b[4]) - q1;
-  const double e2 = fmin(fmax(q2, b[2]), b[5]) - q2;
+  const double e0 = fmax(fmax(b[0] - q0, q0 - b[3]), 0.0);
+  const double e1 = fmax(fmax(b[1] - q1, q1 - b[4]), 0.0);
+  const double e2 = fmax(fmax(b[2] - q2, q2 - b[5]), 0.0);
   return dot3(e0, e1, e2, e0, e1, e2);
+}
+
+// A query with a NaN coordinate: every triangle distance is NaN, so no
+// candidate is taken and the running minimum of the reference stays at
+// DBL_MAX, against which a box passes exactly when its distance is 0 (see
+// sqdist_box). Its walk therefore starts at bound 0 in place of DBL_MAX.
+__device__ __forceinline__ bool query_has_nan(const double q[3]) {
+  return __builtin_isnan(q[0]) || __builtin_isnan(q[1]) || __builtin_isnan(q[2]);
 }
 
 // closest_point(Point, Triangle, loc, EPS), primal/operators/closest_point.hpp:162-290.

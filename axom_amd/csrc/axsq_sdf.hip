@@ -238,7 +238,9 @@ __device__ __forceinline__ void sdf_finish(const SdfArgs& a, int64_t i, const do
       sgn = (dp >= 0.0) ? 1.0 : -1.0;
     }
   }
-  a.dist[i] = sqrt(m.sq) * sgn;
+  // a walk that took no triangle reports the initial minimum DBL_MAX (a NaN
+  // query walks with bound 0, see query_has_nan)
+  a.dist[i] = sqrt(m.type < 0 ? kDblMax : m.sq) * sgn;
   if (a.cp) {
     a.cp[3 * i] = m.p[0];
     a.cp[3 * i + 1] = m.p[1];
@@ -251,7 +253,8 @@ __device__ __forceinline__ void sdf_finish(const SdfArgs& a, int64_t i, const do
     a.normal[3 * i + 1] = un[1];
     a.normal[3 * i + 2] = un[2];
   }
-  if (a.elem) a.elem[i] = reinterpret_cast<const int32_t*>(a.tris + m.pos)[18];
+  // -1 when no triangle was taken (a query with a NaN or infinite coordinate)
+  if (a.elem) a.elem[i] = m.type < 0 ? -1 : reinterpret_cast<const int32_t*>(a.tris + m.pos)[18];
 }
 
 // A walk the first pass stopped at its step budget: the whole walk state, so
@@ -367,6 +370,7 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
   int32_t leaves = 0, nterms = 0;
   int32_t budget = a.budget;
   const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
+  if (query_has_nan(q)) m.sq = 0.0;
   const bool cn = a.compute_sign != 0;
   const NodeT* nodes;
   if constexpr (MIXED)
@@ -597,6 +601,7 @@ __global__ __launch_bounds__(kWaveLanes) AXSQ_RESUME_WAVES_ATTR void k_sdf_resum
   }
   i = (i >= 0 && i < a.nq) ? i : 0;  // only a malformed record or list is out of range
   const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
+  if (b >= nres && query_has_nan(q)) m.sq = 0.0;  // a walk from the start
   const NodeT* nodes;
   if constexpr (MIXED)
     nodes = a.nodes32;
@@ -1499,12 +1504,19 @@ __global__ __launch_bounds__(kBlock) void k_copy_u32(const uint32_t* __restrict_
 }
 
 // ---- adjoint -----------------------------------------------------------------
+// Sort keys of the adjoint: the winning triangle, or nt for a query without
+// one (elem -1), which sorts after every triangle; any other id outside
+// [0, nt) sets bit kStatusElem of *status and also takes key nt.
 __global__ __launch_bounds__(kBlock) void k_elem_keys(const int32_t* __restrict__ elem, int64_t nq,
-                                                       uint32_t* __restrict__ keys,
-                                                       uint32_t* __restrict__ vals) {
+                                                       int64_t nt, uint32_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals,
+                                                       int32_t* __restrict__ status) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nq) return;
-  keys[i] = (uint32_t)elem[i];
+  const int32_t e = elem[i];
+  const bool in_range = e >= 0 && (int64_t)e < nt;
+  if (!in_range && e != -1) atomicOr(status, kStatusElem);
+  keys[i] = in_range ? (uint32_t)e : (uint32_t)nt;
   vals[i] = (uint32_t)i;
 }
 
@@ -1521,7 +1533,15 @@ __global__ __launch_bounds__(kBlock) void k_bwd_query(
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nq) return;
   const int64_t c = (int64_t)keys[p];
-  if (c >= nt) return;  // flagged by k_segments
+  if (c >= nt) {  // no closest triangle: the distance is constant in the query
+    if (grad_q) {
+      const int64_t i = (int64_t)qidx[p];
+      grad_q[3 * i] = 0.0;
+      grad_q[3 * i + 1] = 0.0;
+      grad_q[3 * i + 2] = 0.0;
+    }
+    return;
+  }
   double T[9];
   for (int k = 0; k < 3; ++k) {
     const int32_t v = conn[3 * c + k];
@@ -2037,24 +2057,24 @@ int sdf_backward_launch(const axsq_sdf* h, const double* x, const double* y, con
   AXSQ_TRY(ws.v0.ensure(sizeof(uint32_t) * nq));
   AXSQ_TRY(ws.k1.ensure(sizeof(uint32_t) * nq));
   AXSQ_TRY(ws.v1.ensure(sizeof(uint32_t) * nq));
-  // triangle segments (2*nt int32), partials (9*nt doubles), per-query terms
-  // (6*nq doubles)
-  const size_t seg_bytes = (sizeof(int32_t) * 2 * nt + 255) & ~size_t(255);
+  // segments of the nt triangles and of key nt (queries without a triangle;
+  // 2*(nt+1) int32), partials (9*nt doubles), per-query terms (6*nq doubles)
+  const size_t seg_bytes = (sizeof(int32_t) * 2 * (nt + 1) + 255) & ~size_t(255);
   const size_t part_bytes = sizeof(double) * 9 * nt;
   AXSQ_TRY(ws.flag.ensure(seg_bytes + part_bytes + sizeof(double) * 6 * nq));
   int32_t* seg_start = ws.flag.as<int32_t>();
-  int32_t* seg_end = seg_start + nt;
+  int32_t* seg_end = seg_start + nt + 1;
   double* part = reinterpret_cast<double*>(ws.flag.as<char>() + seg_bytes);
   double* terms = reinterpret_cast<double*>(ws.flag.as<char>() + seg_bytes + part_bytes);
   ProfRange pr("sdf_backward", s);
-  k_elem_keys<<<grid_for(nq), kBlock, 0, s>>>(elem, nq, ws.k0.as<uint32_t>(),
-                                              ws.v0.as<uint32_t>());
+  k_elem_keys<<<grid_for(nq), kBlock, 0, s>>>(elem, nq, nt, ws.k0.as<uint32_t>(),
+                                              ws.v0.as<uint32_t>(), h->bvh->status);
   AXSQ_CHECK_HIP(hipGetLastError());
   uint32_t *sk, *sv;
   AXSQ_TRY(radix_sort_pairs(ws.k0.as<uint32_t>(), ws.v0.as<uint32_t>(), ws.k1.as<uint32_t>(),
                             ws.v1.as<uint32_t>(), nq, bits_for((uint64_t)nt), ws, s, &sk, &sv));
-  AXSQ_TRY(fill_i32(seg_start, 0, 2 * nt, s));
-  k_segments<<<grid_for(nq), kBlock, 0, s>>>(sk, nq, nt, seg_start, seg_end, h->bvh->status);
+  AXSQ_TRY(fill_i32(seg_start, 0, 2 * (nt + 1), s));
+  k_segments<<<grid_for(nq), kBlock, 0, s>>>(sk, nq, nt + 1, seg_start, seg_end, h->bvh->status);
   AXSQ_CHECK_HIP(hipGetLastError());
   k_bwd_query<<<grid_for(nq), kBlock, 0, s>>>(h->vx, h->vy, h->vz, h->conn, nt, sk, sv, nq, x, y,
                                               z, stride, dist, gout, grad_q, terms);

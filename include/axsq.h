@@ -307,7 +307,9 @@ int axsq_sdf_update_vertices(axsq_sdf* sdf, const double* x, const double* y,
  *   dist[nq]        signed distance (sign carried also for zero distances)
  *   cp[3*nq]        closest surface point (AoS)
  *   normal[3*nq]    unit (pseudo-)normal at the closest point (AoS)
- *   elem[nq]        winning triangle id (input of axsq_sdf_backward)         */
+ *   elem[nq]        winning triangle id (input of axsq_sdf_backward); -1 when
+ *                   no triangle was taken (a NaN or infinite coordinate,
+ *                   whose dist is then -sqrt(DBL_MAX) or +sqrt(DBL_MAX))    */
 int axsq_sdf_eval(const axsq_sdf* sdf, const double* x, const double* y, const double* z,
                   int64_t stride, int64_t nq, int fp_mode, double* dist, double* cp,
                   double* normal, int32_t* elem, void* stream);
@@ -327,7 +329,9 @@ int axsq_sdf_eval_point(const axsq_sdf* sdf, const double q[3], int fp_mode, dou
  * gout[nq] is the upstream gradient (float). Sums are deterministic: per
  * triangle and corner in ascending query order, then per vertex in
  * ascending triangle order, so repeated calls are bitwise identical.
- * Either output may be NULL. */
+ * A query with elem_i = -1 (no closest triangle) gets grad_q_i = 0 and adds
+ * nothing to grad_v; any other elem_i outside [0, ntris) fails the call with
+ * AXSQ_ERR_INVALID_ARG. Either output may be NULL. */
 int axsq_sdf_backward(const axsq_sdf* sdf, const double* x, const double* y, const double* z,
                       int64_t stride, int64_t nq, const double* dist, const int32_t* elem,
                       const float* gout, double* grad_q, double* grad_v, void* stream);

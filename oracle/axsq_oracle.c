@@ -1553,6 +1553,9 @@ int orc_sdf_backward(const orc_sdf* s, const double* q, int64_t nq, const double
 {
   if(s->off) return -1; /* the adjoint is for triangle meshes */
   const int32_t nt = s->ntris, nv = s->nverts;
+  /* elem -1: no closest triangle (gradient 0); any other id out of range fails */
+  for(int64_t i = 0; i < nq; ++i)
+    if(elem[i] < -1 || elem[i] >= nt) return -1;
   /* per-query direction u = sgn*(q-cp)/|q-cp| scaled by g */
   double* dir = (double*)malloc(sizeof(double) * 3 * (size_t)nq);
   double* wts = (double*)malloc(sizeof(double) * 3 * (size_t)nq);
@@ -1560,6 +1563,16 @@ int orc_sdf_backward(const orc_sdf* s, const double* q, int64_t nq, const double
   {
     const double* P = q + 3 * i;
     const int32_t c = elem[i];
+    if(c < 0)
+    {
+      for(int d = 0; d < 3; ++d)
+      {
+        dir[3 * i + d] = 0.0;
+        wts[3 * i + d] = 0.0;
+        if(grad_q) grad_q[3 * i + d] = 0.0;
+      }
+      continue;
+    }
     double A[3], B[3], C[3], cp[3], w[3];
     int loc;
     tri_points(s, c, 0, A, B, C);  /* the adjoint is for triangle meshes */
@@ -1602,6 +1615,7 @@ int orc_sdf_backward(const orc_sdf* s, const double* q, int64_t nq, const double
     for(int64_t k = 0; k < nq; ++k)
     {
       const int64_t c = pairs[2 * k], i = pairs[2 * k + 1];
+      if(c < 0) continue;
       for(int corner = 0; corner < 3; ++corner)
         for(int d = 0; d < 3; ++d)
           part[9 * c + 3 * corner + d] += -(wts[3 * i + corner] * dir[3 * i + d]);

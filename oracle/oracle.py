@@ -376,9 +376,12 @@ class OracleSDF:
         nq = q.shape[0]
         gq = np.zeros((nq, 3))
         gv = np.zeros((self.nverts, 3))
-        lib().orc_sdf_backward(self.h, _p(q, _dp), nq, _p(f64(dist), _dp), _p(i32(elem), _i32p),
-                               _p(np.ascontiguousarray(gout, np.float32), _f32p),
-                               _p(gq, _dp), _p(gv, _dp))
+        rc = lib().orc_sdf_backward(self.h, _p(q, _dp), nq, _p(f64(dist), _dp),
+                                    _p(i32(elem), _i32p),
+                                    _p(np.ascontiguousarray(gout, np.float32), _f32p),
+                                    _p(gq, _dp), _p(gv, _dp))
+        if rc != 0:
+            raise ValueError("adjoint: a cell mesh, or an element id outside [-1, ntris)")
         return gq, gv
 
 

@@ -93,6 +93,40 @@ def test_sdf_single_triangle():
     _compare(_eval(sdf, q), orc.OracleSDF(v, t, watertight=False).eval(q))
 
 
+@pytest.mark.parametrize("fp_mode", ["fp64", "fp32"])
+def test_non_finite_queries(fp_mode):
+    # a NaN or infinite coordinate gives no box distance that passes the
+    # pruning test against the initial bound, so the result keeps its initial
+    # state (element -1); the traversal counts must agree as well
+    v, t, q = synth.workload2(nq=4096, level=4)
+    q = q.copy()
+    nan, inf = np.nan, np.inf
+    rows = [[nan, 0, 0], [0.1, nan, 0.2], [0, 0, nan], [inf, 0, 0], [0, -inf, 0],
+            [inf, inf, -inf], [nan, inf, 0], [-inf, 0.3, nan]]
+    idx = np.arange(len(rows)) * 509
+    q[idx] = rows
+    sdf = quest.SignedDistance(_t(v), _t(t), fp_mode=fp_mode)
+    sdf.enableStats(True)
+    g_out = _eval(sdf, q, fp_mode=fp_mode)
+    o = orc.OracleSDF(v, t).eval(q, want_stats=True)
+    assert np.all(o["elem"][idx] == -1)
+    _compare(g_out, o)
+    if fp_mode == "fp64":
+        visits, leaves = sdf.lastStats()
+        assert visits == int(o["visits"].sum()) and leaves == int(o["leaves"].sum())
+    # the adjoint: zero query gradient, no vertex contribution for elem -1
+    g = synth.normal_f32(5, 6, q.shape[0])
+    g_dist = g_out["dist"]
+    gq, gv = sdf.backward(_t(q), _t(g_dist), _t(g_out["elem"]), _t(g))
+    rq, rv = orc.OracleSDF(v, t).backward(q, g_dist, g_out["elem"], g)
+    assert np.array_equal(gq.cpu().numpy(), rq)
+    assert np.array_equal(gv.cpu().numpy(), rv)
+    bad = g_out["elem"].copy()
+    bad[1] = t.shape[0]
+    with pytest.raises(RuntimeError):
+        sdf.backward(_t(q), _t(g_dist), _t(bad), _t(g))
+
+
 def test_known_answer_sphere():
     s = GOLD["sdf_sphere"]
     v, t = refmeshes.sphere_uv_mesh(s["center"], s["radius"], s["theta_res"], s["phi_res"])

@@ -283,6 +283,30 @@ def test_adjoint_sum_rule():
     np.testing.assert_allclose(gv.sum(0), -gq.sum(0), rtol=1e-12, atol=1e-12)
 
 
+def test_adjoint_queries_without_a_triangle():
+    # a NaN or infinite query takes no triangle (elem -1): its query gradient
+    # is 0 and the vertex gradient is that of the other queries alone
+    v, t = _ico_mesh(2, 17)
+    sdf = orc.OracleSDF(v, t)
+    q = synth.surface_samples(v, t, 93, 0, 300, -0.1, 0.1)
+    bad = np.array([5, 77, 200])
+    q[bad] = [[np.nan, 0, 0], [np.inf, 0.2, 0], [0, 0, -np.inf]]
+    out = sdf.eval(q)
+    assert np.all(out["elem"][bad] == -1)
+    g = synth.normal_f32(93, 9, q.shape[0])
+    gq, gv = sdf.backward(q, out["dist"], out["elem"], g)
+    assert np.all(gq[bad] == 0.0)
+    keep = np.setdiff1d(np.arange(q.shape[0]), bad)
+    gq_k, gv_k = sdf.backward(q[keep], out["dist"][keep], out["elem"][keep], g[keep])
+    assert np.array_equal(gq[keep], gq_k)
+    assert np.array_equal(gv, gv_k)
+    for e in (-2, t.shape[0]):
+        elem = out["elem"].copy()
+        elem[0] = e
+        with pytest.raises(ValueError):
+            sdf.backward(q, out["dist"], elem, g)
+
+
 @pytest.mark.parametrize("kind", ["points", "boxes"])
 def test_bvh_visits_closed_form(kind):
     """The walk visits the root and every inner node whose box, as its parent
