@@ -79,6 +79,10 @@ __device__ __forceinline__ bool pred32(const double* prim, const float* lo, cons
   if (KIND == AXSQ_QUERY_RAYS) {
     const double bd[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
     return f32 ? ray_box_f32(prim, bd, (float)tol, dims) : ray_box(prim, bd, tol);
+  } else if (KIND == AXSQ_QUERY_POINTS) {
+    // written as pred64's contains(): a NaN coordinate passes, as there
+    return !(hi[0] < b[0] || lo[0] > b[3] || hi[1] < b[1] || lo[1] > b[4] || hi[2] < b[2] ||
+             lo[2] > b[5]);
   } else {
     return (hi[0] >= b[0] && lo[0] <= b[3]) && (hi[1] >= b[1] && lo[1] <= b[4]) &&
            (hi[2] >= b[2] && lo[2] <= b[5]);

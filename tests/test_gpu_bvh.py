@@ -421,6 +421,33 @@ def test_inverted_leaf_box_is_never_reported():
             assert np.array_equal(cnt, ocnt) and np.array_equal(cand, ocand), (kind, mode)
 
 
+@pytest.mark.parametrize("fp_mode", ["fp64", "fp32"])
+def test_non_finite_queries(fp_mode):
+    """NaN and infinite coordinates, next to a finite query of each kind: a
+    NaN point coordinate passes contains() on its axis (an all-NaN point
+    holds every box), a NaN box bound overlaps nothing, and rays follow the
+    slab test; both traversal forms give the oracle's lists."""
+    boxes = synth.workload1(scale=0.2)[0]
+    g, o = _build_pair(boxes, 1.000123, fp_mode)
+    nan, inf = np.nan, np.inf
+    queries = {
+        "points": np.array([[nan, 0.5, 0.5], [0.3, nan, 0.7], [0.2, 0.4, nan], [nan, nan, 0.5],
+                            [nan, nan, nan], [inf, 0.5, 0.5], [0.5, -inf, 0.5],
+                            [0.5, 0.5, 0.5]]),
+        "boxes": np.array([[nan, 0.4, 0.4, 0.6, 0.6, 0.6], [0.4, 0.4, 0.4, nan, 0.6, 0.6],
+                           [-inf, 0.4, 0.4, inf, 0.6, 0.6], [0.4, 0.4, 0.4, 0.6, 0.6, 0.6]]),
+        "rays": np.array([[nan, 0.5, 0.5, 1, 0, 0], [0, 0.5, 0.5, nan, 0, 0],
+                          [-inf, 0.5, 0.5, 1, 0, 0], [0, 0.5, 0.5, inf, 0, 0],
+                          [0, 0.5, 0.5, 1, 0, 0]]),
+    }
+    for kind, q in queries.items():
+        goff, gcnt, gcand = _find(g, kind, q, q.shape[0])
+        ooff, ocnt, ocand = o.find(kind, q)
+        assert np.array_equal(gcnt, ocnt), (kind, gcnt, ocnt)
+        assert np.array_equal(goff, ooff) and np.array_equal(gcand, ocand), kind
+    assert o.find("points", queries["points"][4:5])[1][0] == boxes.shape[0]
+
+
 @pytest.mark.parametrize("kind", ["points", "boxes", "rays"])
 def test_count_pass_visits_match_oracle(kind):
     """The count pass's visit counter (enableStats) equals the oracle walk's
