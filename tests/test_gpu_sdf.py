@@ -127,6 +127,51 @@ def test_non_finite_queries(fp_mode):
         sdf.backward(_t(q), _t(g_dist), _t(bad), _t(g))
 
 
+def _degenerate_mesh():
+    # an icosphere with zero-area triangles on its surface: a repeated corner
+    # on an existing edge, a point triangle at a vertex, and a sliver whose
+    # third corner is the midpoint of an edge (Triangle::degenerate,
+    # Triangle.hpp:327-330, drops them from the pseudo-normal sums)
+    v, t = synth.icosphere(3)
+    nv = v.shape[0]
+    e = t[:40]
+    mid = 0.5 * (v[e[:, 0]] + v[e[:, 1]])
+    v2 = np.concatenate([v, mid])
+    t2 = np.concatenate([t, np.stack([e[:, 0], e[:, 0], e[:, 1]], 1),
+                         np.stack([e[:, 2], e[:, 2], e[:, 2]], 1),
+                         np.stack([e[:, 0], e[:, 1], nv + np.arange(40)], 1)]).astype(np.int32)
+    q = np.concatenate([synth.surface_samples(v, t, 7, 0, 20_000, -0.05, 0.05),
+                        v[e.ravel()] * 1.01, mid * 0.99,
+                        synth.uniform(7, 1, 30_000, -1.3, 1.3).reshape(-1, 3)])
+    return v2, t2, q, t.shape[0]
+
+
+@pytest.mark.parametrize("fp_mode", ["fp64", "fp32"])
+def test_degenerate_triangles(fp_mode):
+    v, t, q, nt0 = _degenerate_mesh()
+    sdf = quest.SignedDistance(_t(v), _t(t), fp_mode=fp_mode)
+    g_out = _eval(sdf, q, fp_mode)
+    o = orc.OracleSDF(v, t).eval(q, nthreads=8)
+    assert np.sum(o["elem"] >= nt0) > 50  # queries won by a degenerate triangle
+    _compare(g_out, o)
+    g = synth.normal_f32(7, 2, q.shape[0])
+    gq, gv = sdf.backward(_t(q), _t(g_out["dist"]), _t(g_out["elem"]), _t(g))
+    rq, rv = orc.OracleSDF(v, t).backward(q, o["dist"], o["elem"], g)
+    assert np.array_equal(gq.cpu().numpy(), rq)
+    assert np.array_equal(gv.cpu().numpy(), rv)
+
+
+def test_uv_sphere_pole_rows_bit_exact():
+    # the reference's test sphere: its pole rows hold zero-area triangles
+    vs, ts = refmeshes.sphere_uv_mesh([0.0, 0.0, 0.0], 1.0, 12, 10)
+    rng = np.random.default_rng(31)
+    cap = np.concatenate([rng.uniform(-0.05, 0.05, (4000, 2)),
+                          rng.choice([-1.0, 1.0], (4000, 1)) * rng.uniform(0.9, 1.1, (4000, 1))], 1)
+    q = np.concatenate([cap, rng.uniform(-1.2, 1.2, (8000, 3))])
+    sdf = quest.SignedDistance(_t(vs), _t(ts))
+    _compare(_eval(sdf, q), orc.OracleSDF(vs, ts).eval(q, nthreads=8))
+
+
 def test_known_answer_sphere():
     s = GOLD["sdf_sphere"]
     v, t = refmeshes.sphere_uv_mesh(s["center"], s["radius"], s["theta_res"], s["phi_res"])
