@@ -56,6 +56,7 @@ struct Workspace {
   DevBuf mesh_tri, mesh_box, mesh_int, mesh_slots;  // mesh self-intersection
   DevBuf box2d;                                      // 2-D boxes widened to 3-D
   DevBuf hot;                                        // tables of the longest saved walks
+  DevBuf wave_cost, wave_order;                      // first pass: launch order of the waves
   axsq_bvh* mesh_bvh = nullptr;                      // its BVH, rebuilt per call
   // one-call candidate search: the count pass keeps each query's first
   // candidates (find_stash, find_cnt) and lists the queries with more
@@ -217,9 +218,17 @@ struct axsq_sdf {
   uint64_t early_key[5] = {0, 0, 0, 0, 0};
   int32_t* skip = nullptr;
   int64_t skip_cap = 0;
-  hipEvent_t early_done = nullptr;  // recorded where a call last touches them
+  hipEvent_t early_done = nullptr;  // recorded where a call last touches them (and wsteps)
   int64_t last_early = 0;  // walks the last call ran beside its first pass
   int64_t early_walks = 0;  // walks a call remembers for the next (set at creation)
+  // first-pass steps of each query of the previous call, at its position in
+  // the query order (wave_key: x, y, z, stride, nq of that call); the next call
+  // on the same arrays launches the waves of least steps last
+  int32_t* wsteps = nullptr;
+  int64_t wsteps_cap = 0;
+  uint64_t wave_key[5] = {0, 0, 0, 0, 0};
+  bool wsteps_valid = false;
+  int32_t wave_tail = 0;  // percent of waves launched last (set at creation; 0: off)
 };
 
 namespace axsq {

@@ -77,6 +77,8 @@ struct SdfArgs {
   int32_t ninner;
   const uint32_t* order;
   const int32_t* skip;  // first pass: queries whose walks run elsewhere (nonzero), or NULL
+  const int32_t* wave_order;  // first pass: wave of the query order per launched wave, or NULL
+  int32_t* wsteps;            // first pass: each walk's steps at its query-order position, or NULL
   int32_t budget;      // first-pass step budget per walk
   int64_t cap;         // capacity of susp
   struct SdfSuspended* susp;
@@ -356,10 +358,19 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
   lds_i32* terms =
       (lds_i32*)(s_terms + (threadIdx.x / kWaveLanes) * (kNormList * kWaveLanes) +
                  threadIdx.x % kWaveLanes);
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = g;  // position in the query order
+  if (a.wave_order) {
+    const int64_t wl = g / kWaveLanes;
+    if (wl >= (a.nq + kWaveLanes - 1) / kWaveLanes) return;
+    t = (int64_t)a.wave_order[wl] * kWaveLanes + g % kWaveLanes;
+  }
   if (t >= a.nq) return;
   const int64_t i = a.order ? (int64_t)a.order[t] : t;
-  if (a.skip && a.skip[i]) return;  // walked beside this pass from the start
+  if (a.skip && a.skip[i]) {  // walked beside this pass from the start
+    if (a.wsteps) a.wsteps[t] = 0;
+    return;
+  }
   MinCand m;
   m.sq = kDblMax;
   m.p[0] = m.p[1] = m.p[2] = 0.0;
@@ -367,6 +378,15 @@ __global__ __launch_bounds__(kSdfBlock) AXSQ_SDF_WAVES_ATTR void k_sdf_eval(SdfA
   m.type = -1;
   m.pos = 0;
   Walk w = walk_start(todo);
+  // this pass's steps of the walk, for the next call's launch order, on every
+  // way out
+  struct StepsOut {
+    int32_t* p;
+    const Walk& w;
+    __device__ ~StepsOut() {
+      if (p) *p = w.steps;
+    }
+  } steps_out{a.wsteps ? a.wsteps + t : nullptr, w};
   int32_t leaves = 0, nterms = 0;
   int32_t budget = a.budget;
   const double q[3] = {a.qx[i * a.qs], a.qy[i * a.qs], a.qz[i * a.qs]};
@@ -1273,6 +1293,73 @@ __global__ __launch_bounds__(kBlock) void k_query_codes(const double* __restrict
   vals[i] = (uint32_t)i;
 }
 
+// Cost of each wave of a first pass (64 consecutive positions of the query
+// order): the most steps one of its walks took. One wave per wave.
+__global__ __launch_bounds__(kBlock) void k_wave_cost(const int32_t* __restrict__ steps,
+                                                       int64_t nq, int64_t nw,
+                                                       int32_t* __restrict__ cost) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t w = g / kWaveLanes;
+  if (w >= nw) return;  // whole waves
+  int32_t c = (g < nq) ? steps[g] : 0;
+  for (int o = kWaveLanes / 2; o > 0; o >>= 1) c = max(c, __shfl_xor(c, o));
+  if (g % kWaveLanes == 0) cost[w] = c;
+}
+
+// Launch order of the first pass's waves: those whose cost falls in the
+// lowest `tail_pct` percent (whole bins of a 1,024-bin histogram over
+// [0, cmax], so at most that share) go last, the others first, each group in
+// query order. One block of 1,024 threads; `order` receives a permutation of
+// 0 .. nw-1.
+constexpr int kOrderThreads = 1024;
+__global__ __launch_bounds__(kOrderThreads) void k_wave_order(const int32_t* __restrict__ cost,
+                                                               int64_t nw, int32_t cmax,
+                                                               int32_t tail_pct,
+                                                               int32_t* __restrict__ order) {
+  __shared__ int32_t hist[kOrderThreads];
+  __shared__ int32_t cnt[kOrderThreads];
+  __shared__ int32_t s_thr;
+  const int tid = (int)threadIdx.x;
+  auto bin = [&](int32_t c) -> int32_t {
+    const int64_t b = (int64_t)max(c, 0) * kOrderThreads / ((int64_t)cmax + 1);
+    return (int32_t)min(b, (int64_t)kOrderThreads - 1);
+  };
+  hist[tid] = 0;
+  __syncthreads();
+  for (int64_t w = tid; w < nw; w += kOrderThreads) atomicAdd(&hist[bin(cost[w])], 1);
+  __syncthreads();
+  if (tid == 0) {
+    const int64_t want = nw * tail_pct / 100;
+    int64_t acc = 0;
+    int32_t b = 0;
+    while (b < kOrderThreads && acc + hist[b] <= want) acc += hist[b++];
+    s_thr = b;  // bins below b go last
+  }
+  __syncthreads();
+  const int32_t thr = s_thr;
+  const int64_t per = (nw + kOrderThreads - 1) / kOrderThreads;
+  const int64_t b0 = min(nw, (int64_t)tid * per), b1 = min(nw, b0 + per);
+  int32_t nshort = 0;
+  for (int64_t w = b0; w < b1; ++w) nshort += bin(cost[w]) < thr;
+  cnt[tid] = nshort;
+  __syncthreads();
+  for (int o = 1; o < kOrderThreads; o <<= 1) {  // inclusive scan of cnt
+    const int32_t v = tid >= o ? cnt[tid - o] : 0;
+    __syncthreads();
+    cnt[tid] += v;
+    __syncthreads();
+  }
+  const int64_t nlong = nw - cnt[kOrderThreads - 1];
+  const int64_t short_before = cnt[tid] - nshort;
+  int64_t ps = nlong + short_before, pl = b0 - short_before;
+  for (int64_t w = b0; w < b1; ++w) {
+    if (bin(cost[w]) < thr)
+      order[ps++] = (int32_t)w;
+    else
+      order[pl++] = (int32_t)w;
+  }
+}
+
 // skip[q[k]] = v for the n listed queries; an index outside [0, nq) is left
 // out and raises the walk-error status bit (the resume kernel clamps it)
 __global__ __launch_bounds__(kBlock) void k_mark_queries(const int32_t* __restrict__ q,
@@ -1872,7 +1959,9 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   }
   const int64_t nearly = std::min<int64_t>(hm->nearly, kEarly);
   SideStream* side = nearly > 0 ? side_stream() : nullptr;
-  if (kEarly > 0 && !hm->early_done)
+  // launch order of the waves from the previous call's steps (in query order)
+  const bool wave_on = h->wave_tail > 0 && a.order != nullptr;
+  if ((kEarly > 0 || wave_on) && !hm->early_done)
     AXSQ_CHECK_HIP(hipEventCreateWithFlags(&hm->early_done, hipEventDisableTiming));
   // On every way out of this call once the early walks are forked (an error
   // too): clear the marks if pass 1 has not, make the caller's stream wait
@@ -1897,7 +1986,7 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
   early_exit.n = nearly;
   early_exit.nq = nq;
   early_exit.status = b->status;
-  early_exit.record = kEarly > 0;
+  early_exit.record = kEarly > 0 || wave_on;
   const bool mixed2_early = mixed && h->mixed_pass2_fp32;
   if (side) {
     // the previous call may have run on another stream: its last writes of
@@ -1934,6 +2023,35 @@ int sdf_eval_launch(const axsq_sdf* h, const double* x, const double* y, const d
     AXSQ_CHECK_HIP(hipEventRecord(side->join, side->stream));
     early_exit.side = side;
     a.skip = hm->skip;
+  }
+  if (wave_on) {
+    // the previous call may have run on another stream: its first pass
+    // wrote the steps read here and rewritten by this call's
+    if (hm->early_done) AXSQ_CHECK_HIP(hipStreamWaitEvent(s, hm->early_done, 0));
+    const bool same = hm->wsteps_valid && memcmp(qkey, hm->wave_key, sizeof(qkey)) == 0;
+    if (hm->wsteps_cap < nq) {
+      (void)hipFree(hm->wsteps);
+      hm->wsteps = nullptr;
+      hm->wsteps_cap = 0;
+      hm->wsteps_valid = false;
+      AXSQ_TRY(dmalloc(&hm->wsteps, (size_t)nq));
+      hm->wsteps_cap = nq;
+    }
+    if (same) {
+      const int64_t nw = (nq + kWaveLanes - 1) / kWaveLanes;
+      AXSQ_TRY(ws.wave_cost.ensure(sizeof(int32_t) * (size_t)nw));
+      AXSQ_TRY(ws.wave_order.ensure(sizeof(int32_t) * (size_t)nw));
+      k_wave_cost<<<grid_for(nw * kWaveLanes), kBlock, 0, s>>>(hm->wsteps, nq, nw,
+                                                               ws.wave_cost.as<int32_t>());
+      AXSQ_CHECK_HIP(hipGetLastError());
+      k_wave_order<<<1, kOrderThreads, 0, s>>>(ws.wave_cost.as<int32_t>(), nw, h->budget,
+                                               h->wave_tail, ws.wave_order.as<int32_t>());
+      AXSQ_CHECK_HIP(hipGetLastError());
+      a.wave_order = ws.wave_order.as<int32_t>();
+    }
+    a.wsteps = hm->wsteps;
+    memcpy(hm->wave_key, qkey, sizeof(qkey));
+    hm->wsteps_valid = true;
   }
   ProfRange p1(mixed ? "sdf_eval_fp32_pass1" : "sdf_eval_fp64_pass1", s);
   const unsigned g = grid_for(nq, kSdfBlock);

@@ -357,6 +357,42 @@ def test_repeated_calls_walk_the_longest_early(monkeypatch, fp_mode):
         check(g, q, g1)
 
 
+@pytest.mark.parametrize("tail", ["0", "50", "90"])
+def test_wave_order_from_the_previous_call(monkeypatch, tail):
+    """A call on the same query arrays as the previous one launches the first
+    pass's waves that took fewest steps then last (AXSQ_SDF_WAVE_TAIL percent
+    of them). Every call gives the oracle's answers and visit and leaf counts:
+    repeated calls, a call after the queries were rewritten in place (the
+    remembered steps are then another set's), and a call on fewer queries of
+    the same storage."""
+    monkeypatch.setenv("AXSQ_SDF_WAVE_TAIL", tail)  # read at creation
+    v, t, q = synth.workload2(nq=50_000, level=5)
+    q2 = synth.uniform(14, 0, 150_000, -1.2, 1.2).reshape(-1, 3)
+    sdf = _sdf_with(monkeypatch, v, t, 400)
+    sdf.enableStats(True)
+    o = orc.OracleSDF(v, t)
+    qt = _t(q)
+
+    def run_check(pts, n):
+        cp = torch.empty((n, 3), dtype=torch.float64, device=DEV)
+        nrm = torch.empty((n, 3), dtype=torch.float64, device=DEV)
+        elem = torch.empty(n, dtype=torch.int32, device=DEV)
+        d = sdf.computeDistances(n, qt[:n], outClosestPts=cp, outNormals=nrm, outElem=elem)
+        g = dict(dist=d.cpu().numpy(), cp=cp.cpu().numpy(), normal=nrm.cpu().numpy(),
+                 elem=elem.cpu().numpy())
+        ref = o.eval(pts[:n], nthreads=8, want_stats=True)
+        _compare(g, ref)
+        assert sdf.lastStats() == (int(ref["visits"].sum()), int(ref["leaves"].sum()))
+
+    n = q.shape[0]
+    for _ in range(3):
+        run_check(q, n)
+    qt.copy_(_t(q2))
+    run_check(q2, n)
+    run_check(q2, n)
+    run_check(q2, 30_000)
+
+
 def test_early_walks_off_on_small_trees(monkeypatch):
     """By default a tree this small (step budget 2048) runs no early walks."""
     monkeypatch.delenv("AXSQ_SDF_EARLY_WALKS", raising=False)
