@@ -1309,17 +1309,19 @@ __global__ __launch_bounds__(kBlock) void k_wave_cost(const int32_t* __restrict_
 // Launch order of the first pass's waves: those whose cost falls in the
 // lowest `tail_pct` percent (whole bins of a 1,024-bin histogram over
 // [0, cmax], so at most that share) go last, the others first, each group in
-// query order. One block of 1,024 threads; `order` receives a permutation of
-// 0 .. nw-1.
+// query order. One block of 1,024 threads: the histogram and its scan give
+// the threshold, then rounds of 1,024 consecutive waves place theirs with
+// ballot ranks. `order` receives a permutation of 0 .. nw-1.
 constexpr int kOrderThreads = 1024;
 __global__ __launch_bounds__(kOrderThreads) void k_wave_order(const int32_t* __restrict__ cost,
                                                                int64_t nw, int32_t cmax,
                                                                int32_t tail_pct,
                                                                int32_t* __restrict__ order) {
+  constexpr int kWaves = kOrderThreads / kWaveLanes;
   __shared__ int32_t hist[kOrderThreads];
-  __shared__ int32_t cnt[kOrderThreads];
-  __shared__ int32_t s_thr;
+  __shared__ int32_t s_short[kWaves], s_long[kWaves];
   const int tid = (int)threadIdx.x;
+  const int lane = tid % kWaveLanes, wv = tid / kWaveLanes;
   auto bin = [&](int32_t c) -> int32_t {
     const int64_t b = (int64_t)max(c, 0) * kOrderThreads / ((int64_t)cmax + 1);
     return (int32_t)min(b, (int64_t)kOrderThreads - 1);
@@ -1328,35 +1330,44 @@ __global__ __launch_bounds__(kOrderThreads) void k_wave_order(const int32_t* __r
   __syncthreads();
   for (int64_t w = tid; w < nw; w += kOrderThreads) atomicAdd(&hist[bin(cost[w])], 1);
   __syncthreads();
-  if (tid == 0) {
-    const int64_t want = nw * tail_pct / 100;
-    int64_t acc = 0;
-    int32_t b = 0;
-    while (b < kOrderThreads && acc + hist[b] <= want) acc += hist[b++];
-    s_thr = b;  // bins below b go last
-  }
-  __syncthreads();
-  const int32_t thr = s_thr;
-  const int64_t per = (nw + kOrderThreads - 1) / kOrderThreads;
-  const int64_t b0 = min(nw, (int64_t)tid * per), b1 = min(nw, b0 + per);
-  int32_t nshort = 0;
-  for (int64_t w = b0; w < b1; ++w) nshort += bin(cost[w]) < thr;
-  cnt[tid] = nshort;
-  __syncthreads();
-  for (int o = 1; o < kOrderThreads; o <<= 1) {  // inclusive scan of cnt
-    const int32_t v = tid >= o ? cnt[tid - o] : 0;
+  for (int o = 1; o < kOrderThreads; o <<= 1) {  // inclusive scan of hist
+    const int32_t v = tid >= o ? hist[tid - o] : 0;
     __syncthreads();
-    cnt[tid] += v;
+    hist[tid] += v;
     __syncthreads();
   }
-  const int64_t nlong = nw - cnt[kOrderThreads - 1];
-  const int64_t short_before = cnt[tid] - nshort;
-  int64_t ps = nlong + short_before, pl = b0 - short_before;
-  for (int64_t w = b0; w < b1; ++w) {
-    if (bin(cost[w]) < thr)
-      order[ps++] = (int32_t)w;
-    else
-      order[pl++] = (int32_t)w;
+  // bins below thr go last: those whose inclusive count stays within the share
+  const int64_t want = nw * tail_pct / 100;
+  const int32_t thr = __syncthreads_count((int64_t)hist[tid] <= want);
+  const int64_t nlong = nw - (thr > 0 ? hist[thr - 1] : 0);
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWaveLanes - lane));
+  int64_t short_base = 0, long_base = 0;
+  for (int64_t base = 0; base < nw; base += kOrderThreads) {
+    const int64_t w = base + tid;
+    const bool valid = w < nw;
+    const bool sh = valid && bin(cost[w]) < thr;
+    const uint64_t ms = __ballot(sh), ml = __ballot(valid && !sh);
+    if (lane == 0) {
+      s_short[wv] = __popcll(ms);
+      s_long[wv] = __popcll(ml);
+    }
+    __syncthreads();
+    int32_t sb = 0, lb = 0, ts = 0, tl = 0;
+    for (int k = 0; k < kWaves; ++k) {
+      if (k < wv) {
+        sb += s_short[k];
+        lb += s_long[k];
+      }
+      ts += s_short[k];
+      tl += s_long[k];
+    }
+    if (sh)
+      order[nlong + short_base + sb + __popcll(ms & below)] = (int32_t)w;
+    else if (valid)
+      order[long_base + lb + __popcll(ml & below)] = (int32_t)w;
+    short_base += ts;
+    long_base += tl;
+    __syncthreads();
   }
 }
 
