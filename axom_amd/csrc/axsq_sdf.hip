@@ -1328,7 +1328,17 @@ __global__ __launch_bounds__(kOrderThreads) void k_wave_order(const int32_t* __r
   };
   hist[tid] = 0;
   __syncthreads();
-  for (int64_t w = tid; w < nw; w += kOrderThreads) atomicAdd(&hist[bin(cost[w])], 1);
+  for (int64_t w0 = tid; w0 < nw; w0 += 4 * kOrderThreads) {  // four loads in flight
+    int32_t c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t w = w0 + (int64_t)u * kOrderThreads;
+      c[u] = w < nw ? cost[w] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (c[u] >= 0) atomicAdd(&hist[bin(c[u])], 1);
+  }
   __syncthreads();
   for (int o = 1; o < kOrderThreads; o <<= 1) {  // inclusive scan of hist
     const int32_t v = tid >= o ? hist[tid - o] : 0;
@@ -1342,10 +1352,13 @@ __global__ __launch_bounds__(kOrderThreads) void k_wave_order(const int32_t* __r
   const int64_t nlong = nw - (thr > 0 ? hist[thr - 1] : 0);
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWaveLanes - lane));
   int64_t short_base = 0, long_base = 0;
+  int32_t c_next = tid < nw ? cost[tid] : 0;  // the next round's cost, loaded a round ahead
   for (int64_t base = 0; base < nw; base += kOrderThreads) {
     const int64_t w = base + tid;
     const bool valid = w < nw;
-    const bool sh = valid && bin(cost[w]) < thr;
+    const int32_t c = c_next;
+    if (w + kOrderThreads < nw) c_next = cost[w + kOrderThreads];
+    const bool sh = valid && bin(c) < thr;
     const uint64_t ms = __ballot(sh), ml = __ballot(valid && !sh);
     if (lane == 0) {
       s_short[wv] = __popcll(ms);
