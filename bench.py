@@ -112,10 +112,15 @@ def sync_timer(torch):
 
 
 def prof_reader(L, _lib):
-    def prof(name):
+    def get(name):
         n, ms = ctypes.c_int64(), ctypes.c_double()
         _lib.check(L.axsq_prof_get(name.encode(), ctypes.byref(n), ctypes.byref(ms)))
-        return ms.value / max(n.value, 1)
+        return n.value, ms.value
+
+    def prof(name):  # ms per launch of the named range
+        n, ms = get(name)
+        return ms / max(n, 1)
+    prof.launches = lambda name: get(name)[0]
     return prof
 
 
@@ -267,10 +272,15 @@ def find_points_config(torch, dev, L, _lib, boxes, pts, cpu_threads, reps=10):
     out["build_ms"] = tb * 1e3
     out["build_mprims_per_s"] = boxes.shape[0] / tb / 1e6
     res = {}
+    # a first call keeps no candidate ids yet (count, scan, fill); the timed
+    # calls and their kernel ranges are those of calls after it
+    bvh.findPoints(off, cnt, n, p)
+    torch.cuda.synchronize()
     L.axsq_prof_reset()
     L.axsq_prof_enable(1)
     tq = sync_time(lambda: res.__setitem__("c", bvh.findPoints(off, cnt, n, p)), reps)
     L.axsq_prof_enable(0)
+    calls = reps + 1  # sync_time's own warm-up call included
     cand = res["c"]
     out["ms"] = tq * 1e3
     out["queries_per_s"] = n / tq
@@ -288,7 +298,8 @@ def find_points_config(torch, dev, L, _lib, boxes, pts, cpu_threads, reps=10):
                        "mean_inner_visits": vbar,
                        "other_kernels_ms": {"scan": prof("bvh_find_scan"),
                                             "copy": prof("bvh_find_copy"),
-                                            "fill": prof("bvh_find_fill")}}
+                                            "fill": prof("bvh_find_fill")},
+                       "fill_launches_per_call": prof.launches("bvh_find_fill") / calls}
     bvh32 = spin.BVH(bx, fp_mode="fp32")
     off32, cnt32 = torch.empty_like(off), torch.empty_like(cnt)
     tq32 = sync_time(lambda: res.__setitem__("c32", bvh32.findPoints(off32, cnt32, n, p)), reps)
